@@ -1,0 +1,163 @@
+/*
+ * sdpx_mi355x.h -- C-ABI of the MI355X (gfx950) Newton-step provider for SDPX.
+ *
+ * One opaque handle owns the device copy of one core-form SDP
+ *   min c'x  s.t.  X_l = sum_i x_i A_i^(l) - C_l >= 0,  l = 1..L   (n_eq = 0)
+ * together with every device workspace buffer the dense Schur route needs,
+ * and one HIP stream.  Each entry point is one phase of SDPX's newton_step!
+ * (step.jl:643-1006) or of the KKT backend seam (kkt_backend.jl:211-530);
+ * the host keeps the interior-point control logic (controller, restarts,
+ * termination) and reads back only the scalars listed per call.
+ *
+ * Conventions
+ *   - every call returns an int: 0 ok, 1 not positive definite, 2 non-finite,
+ *     < 0 error (-1 invalid argument, -2 HIP error, -3 out of memory);
+ *     sdpx_last_error() gives a message for the calling thread's last error;
+ *   - host buffers are caller-owned and only read / written during the call;
+ *   - matrices are column-major FP64; a block X_l is k_l x k_l and the blocks of
+ *     a "concatenated" argument follow each other (offset sum_{l'<l} k_l'^2);
+ *   - Av for block l is the k_l^2 x m panel of DenseCons.Av[l]
+ *     (types/constraints.jl:26-28): column i is vec(A_i^(l));
+ *   - only the lower triangle of the Schur matrix is authoritative
+ *     (workspace.jl:1166-1172, docs/src/providers.md:45-50);
+ *   - a handle is not thread-safe: one handle per SDPX Workspace, called
+ *     sequentially (ext/SDPXMultiFloatLinearAlgebraExt.jl:14-20).
+ */
+#ifndef SDPX_MI355X_H
+#define SDPX_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sdpx_dense sdpx_dense_t;
+
+#define SDPX_OK 0
+#define SDPX_NOT_POSITIVE_DEFINITE 1
+#define SDPX_NON_FINITE 2
+#define SDPX_INVALID_ARGUMENT (-1)
+#define SDPX_HIP_ERROR (-2)
+#define SDPX_OUT_OF_MEMORY (-3)
+
+/* ---- library ------------------------------------------------------------ */
+const char* sdpx_last_error(void);
+/* ABI version (major * 100 + minor). */
+int sdpx_version(void);
+/* Number of visible devices (0 when none). */
+int sdpx_device_count(int* count);
+
+/* ---- lifecycle: replaces instantiate_la_backend (la_backend.jl:508-559) and
+ *      the Workspace constructor's buffer inventory (workspace.jl:779-1268) --- */
+int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out);
+int sdpx_dense_destroy(sdpx_dense_t* h);
+/* Device bytes held by the handle. */
+int sdpx_dense_memory(sdpx_dense_t* h, long long* bytes);
+
+/* ---- problem upload: DenseCons panels (ingest.jl:680-694, 950-962) -------- */
+/* Av: k_l^2 x m column-major (bytes of DenseCons.Av[l]); C: k_l x k_l. */
+int sdpx_dense_set_block(sdpx_dense_t* h, int l, const double* Av, const double* C);
+int sdpx_dense_set_objective(sdpx_dense_t* h, const double* c);
+
+/* ---- iterate I/O (interior_point.jl:1528-1650 starts; Checkpoint problems.jl:27-38) */
+int sdpx_dense_set_iterate(sdpx_dense_t* h, const double* x, const double* X, const double* Y);
+int sdpx_dense_get_iterate(sdpx_dense_t* h, double* x, double* X, double* Y);
+/* dx (m) and concatenated dX, dY blocks of the last direction. */
+int sdpx_dense_get_direction(sdpx_dense_t* h, double* dx, double* dX, double* dY);
+/* Schur matrix S (m x m); only the lower triangle is written by the build. */
+int sdpx_dense_get_schur(sdpx_dense_t* h, double* S);
+/* Overwrite S (test hook of kkt_regressions.jl:927-1000: copy_owned!(ws.S, .)). */
+int sdpx_dense_set_schur(sdpx_dense_t* h, const double* S);
+/* Overwrite the right-hand side r = -(d + v) (m) used by solve / refine. */
+int sdpx_dense_set_rhs(sdpx_dense_t* h, const double* r);
+int sdpx_dense_get_rhs(sdpx_dense_t* h, double* r);
+int sdpx_dense_set_dx(sdpx_dense_t* h, const double* dx);
+
+/* ---- a3: residuals + block factorization -----------------------------------
+ * threaded_compute_residuals! (kernels/threaded.jl:423-505) / compute_residuals!
+ * (step.jl:25-72) + factor_blocks! (step.jl:82-92).
+ * mu: L centering targets; affine != 0 drops the mu*I term (use_affine).
+ * out[0] = p_res, out[1] = d_res, out[2] = 1.0 when every block factor succeeded.
+ * Returns 1 when a block Cholesky failed (factor != 0).                      */
+int sdpx_dense_residuals(sdpx_dense_t* h, const double* mu, int affine, int factor, double* out);
+
+/* ---- a4: dense Schur assembly S = sum_l Phat_l' Phat_l, lower triangle ------
+ * threaded_schur_build! (threaded.jl:1298-1354) -> schur_build! (schur.jl:511-547). */
+int sdpx_dense_schur_build(sdpx_dense_t* h);
+
+/* ---- a6: Schur Cholesky with the sqrt(eps)*10^(a-1) ladder -----------------
+ * factorize!(::DenseCholeskyBackend) (kkt_backend.jl:211-221) ->
+ * _factor_dense_kkt_native! (kkt.jl:1404-1464) with la_chol! (la_backend.jl:1612-1622).
+ * reg_attempts: 0..6. Returns 0 ok, 1 not positive definite after 6 attempts. */
+int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts);
+
+/* ---- a9: right-hand sides --------------------------------------------------
+ * predictor: threaded_predictor_corrector_rhs! (threaded.jl:542-572) and
+ * r = -(d + v) (step.jl:700-706).                                            */
+int sdpx_dense_predictor_rhs(sdpx_dense_t* h);
+/* corrector: mehrotra != 0 -> threaded_mehrotra_corrector_rhs! (threaded.jl:916-1061)
+ * with sigma, mu_global and block_local; otherwise threaded_corrector_rhs!
+ * (threaded.jl:837-888) with the L targets mu and beta. Then r = -(d + v)
+ * (step.jl:863-865).                                                         */
+int sdpx_dense_corrector_rhs(sdpx_dense_t* h, int mehrotra, double sigma, double mu_global, int block_local,
+                             const double* mu, double beta);
+
+/* ---- a7: direction solve dx = L^-T L^-1 r ----------------------------------
+ * solve_direction! (kkt_backend.jl:392-410) -> _solve_dense_kkt_owned!
+ * (kkt.jl:2570-2595) with la_trsv_lower! / la_trsv_transpose! (la_backend.jl:1646-1655). */
+int sdpx_dense_solve(sdpx_dense_t* h);
+
+/* ---- a8: refinement ----------------------------------------------------------
+ * _kkt_direction_residual! (kkt.jl:3689-3708): rho = r - S dx (unfactored S,
+ * schur_mul! kkt.jl:3514-3577). out[0] = ||rho||_inf, out[1] = ||r||_inf.     */
+int sdpx_dense_direction_residual(sdpx_dense_t* h, double* out);
+/* refine! (kkt_backend.jl:476-485) -> _refine_native_direction! (kkt.jl:3891-3963).
+ * fixed_policy != 0: exactly max_steps passes with roll-back; otherwise stop at
+ * abstol, on a decrease smaller than 2x, or after max_steps. */
+int sdpx_dense_refine(sdpx_dense_t* h, double abstol, int max_steps, int fixed_policy, int* steps,
+                      double* residual);
+
+/* ---- a9: direction recovery dX = sum dx_i A_i + P, dY = sym(X^-1 (R - dX Y))
+ * threaded_direction_blocks! (threaded.jl:574-644).                          */
+int sdpx_dense_direction_blocks(sdpx_dense_t* h);
+
+/* ---- a10: complementarity diagnostics (step.jl:373-544) -------------------
+ * out[l] = <X_l, Y_l>, out[L + l] = <X_l + tX dX_l, Y_l + tY dY_l>.         */
+int sdpx_dense_complementarity(sdpx_dense_t* h, double tX, double tY, double* out);
+
+/* ---- a11: step length / PSD checks ----------------------------------------
+ * fraction_to_boundary_search! (step.jl:1081-1098) on fraction_to_boundary_bound!
+ * (kernels/generic.jl:350-420): t = fraction * bound when bound < 1.         */
+int sdpx_dense_fraction_to_boundary(sdpx_dense_t* h, double primal_fraction, double dual_fraction, double* tX,
+                                    double* tY);
+/* line_search! (step.jl:1025-1075) with trial_has_cholesky_margin!
+ * (kernels/generic.jl:318-340): first t = gamma^j (t *= gamma) accepted, or the
+ * first t < min_step. */
+int sdpx_dense_backtrack(sdpx_dense_t* h, double gamma, double min_step, double minimum_ratio, double* tX,
+                         double* tY);
+
+/* ---- a12: accepted-step update ---------------------------------------------
+ * threaded_update_blocks! (threaded.jl:665-747) + x update (interior_point.jl:2228):
+ * out[0] = sum_l <X_l, Y_l>, out[1] = 1.0 when X, Y, x are finite,
+ * out[2 + l] = <X_l, Y_l>.                                                   */
+int sdpx_dense_update(sdpx_dense_t* h, double tX, double tY, double* out);
+/* _accepted_sdp_trial_residuals! (interior_point.jl:1287-1311):
+ * d <- (1-tY) d - tY rho; out[0] = |1-tX| * block_primal_residual, out[1] = ||d||_inf. */
+int sdpx_dense_accepted_residuals(sdpx_dense_t* h, double tX, double tY, double block_primal_residual,
+                                  double* out);
+/* pObj = c'x, dObj = sum_l <C_l, Y_l> (interior_point.jl:7-14, 2302-2303). */
+int sdpx_dense_objectives(sdpx_dense_t* h, double* pobj, double* dobj);
+/* solution_residuals (validation.jl:15-58): recomputed p_res, d_res. */
+int sdpx_dense_solution_residuals(sdpx_dense_t* h, double* p_res, double* d_res);
+/* restart rescale (interior_point.jl:2180-2195): which = 1 -> X, 2 -> Y. */
+int sdpx_dense_scale_blocks(sdpx_dense_t* h, int which, double factor);
+/* Cholesky diagonal qualities (step.jl:233-259, 1229): out[0] primal margin,
+ * out[1] dual margin, out[2] Schur factor quality. */
+int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out);
+/* Wait for the handle's stream. */
+int sdpx_dense_synchronize(sdpx_dense_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDPX_MI355X_H */

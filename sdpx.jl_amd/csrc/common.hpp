@@ -1,0 +1,105 @@
+// Shared definitions for the SDPX MI355X (gfx950) Newton-step library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <algorithm>
+#include <string>
+
+namespace sdpx {
+
+// Status codes returned by every C-ABI entry point (mirrored in include/sdpx_mi355x.h).
+enum Status : int {
+  kOk = 0,
+  kNotPositiveDefinite = 1,
+  kNonFinite = 2,
+  kInvalidArgument = -1,
+  kHipError = -2,
+  kOutOfMemory = -3,
+};
+
+void set_last_error(const std::string& msg);
+const char* last_error();
+
+#define SDPX_HIP(expr)                                                          \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      ::sdpx::set_last_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+      return ::sdpx::kHipError;                                                 \
+    }                                                                           \
+  } while (0)
+
+#define SDPX_CHECK(expr)          \
+  do {                            \
+    int _s = (expr);              \
+    if (_s != ::sdpx::kOk) return _s; \
+  } while (0)
+
+constexpr int kWave = 64;
+constexpr int kNumXcd = 8;  // MI355X: 256 CUs in 8 XCDs, blocks dispatched round-robin
+
+inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM on MFMA (gemm_f64.hip)
+//   C(i,j) = alpha * sum_kk op(A)(i,kk) op(B)(kk,j) + beta * C(i,j)
+// column-major; op(A)(i,kk) = TA ? A[kk + i*lda] : A[i + kk*lda]
+//               op(B)(kk,j) = TB ? B[j + kk*ldb] : B[kk + j*ldb]
+// ---------------------------------------------------------------------------
+enum GemmFlags : unsigned {
+  kGemmLowerOut = 1u << 0,  // only C(i,j) with i >= j is computed / written (M == N)
+  kGemmALower = 1u << 1,    // op(A) is lower triangular: kk <= i
+  kGemmBLower = 1u << 2,    // op(B) is lower triangular: kk >= j
+  kGemmAUpper = 1u << 3,    // op(A) is upper triangular: kk >= i
+  kGemmTransA = 1u << 8,    // op(A) = A'
+  kGemmTransB = 1u << 9,    // op(B) = B'
+};
+
+struct GemmArgs {
+  int M, N, K;
+  double alpha, beta;
+  const double* A;
+  long long lda, strideA;
+  const double* B;
+  long long ldb, strideB;
+  double* C;
+  long long ldc, strideC;
+  int batch;
+  unsigned flags;
+  const int* skip;        // optional per-batch flag: skip the batch entry when *skip != 0
+  long long skip_stride;  // stride between per-batch skip flags
+};
+
+// Launches the GEMM on `stream`. `split_k` > 1 requires `workspace` of
+// split_k * M * N * batch doubles; partial sums are reduced in a fixed order
+// (deterministic). Returns kOk / kHipError.
+int gemm_f64(const GemmArgs& g, hipStream_t stream, int split_k = 1, double* workspace = nullptr);
+
+// ---------------------------------------------------------------------------
+// Dense Cholesky and triangular solves (chol_f64.hip)
+// ---------------------------------------------------------------------------
+constexpr int kCholNb = 64;
+
+// Batched in-place lower Cholesky of `batch` n x n matrices (column-major,
+// leading dim lda, stride between matrices `stride`). The inverse of every
+// diagonal nb x nb factor block is written to dinv (batch * ceil(n/nb) * nb*nb
+// doubles, stride dinv_stride per matrix). info[b] (device) is set to the
+// 1-based column of the first non-positive / non-finite pivot, 0 on success.
+int potrf_f64(double* A, long long lda, long long stride, int n, int batch, double* dinv,
+              long long dinv_stride, int* info, hipStream_t stream);
+
+// Solves L x = b (trans=0) or L' x = b (trans=1) in place for one right-hand
+// side, using the diagonal-block inverses written by potrf_f64.
+int trsv_f64(const double* L, long long lda, int n, const double* dinv, double* x, int trans,
+             hipStream_t stream);
+
+// Inverse of a lower-triangular n x n factor (Linv written to out, ld n),
+// blocked on the diagonal-block inverses written by potrf_f64.
+int trtri_lower_f64(const double* L, long long lda, int n, const double* dinv, double* out,
+                    double* work, hipStream_t stream);
+
+}  // namespace sdpx

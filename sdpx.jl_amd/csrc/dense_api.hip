@@ -1,0 +1,999 @@
+// C-ABI of the dense Schur route: one handle = one core-form SDP resident in
+// HBM plus every workspace buffer of SDPX's dense Newton step, one stream.
+//
+// Device layout (FP64, column-major, per PSD block l of side k):
+//   Av_l   k^2 x m   constraint panel (column i = vec(A_i))       [problem]
+//   T, Ph  k^2 x m   Schur panels L_X^-1 A_i and L_X^-1 A_i M_Y   [max over blocks]
+//   S, Sf  m x m     Schur matrix (lower authoritative) and its Cholesky factor
+//   X, Y, LX, MY, LXinv, Xinv, P, R, dX, dY, W1, W2   k x k blocks
+//   trial  cap x k^2 batched line-search trial matrices
+// Host code only sequences kernels and reads back the scalars of each phase.
+#include <cmath>
+#include <vector>
+
+#include "../../include/sdpx_mi355x.h"
+#include "blas_f64.hpp"
+
+namespace sdpx {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+namespace {
+
+constexpr int kTrialCap = 64;  // batched line-search candidates per launch
+constexpr double kEps = 2.220446049250313e-16;
+
+struct Block {
+  int k = 0;
+  long long k2 = 0;
+  int nblk = 0;  // diagonal blocks of the block Cholesky
+  double *Av = nullptr, *C = nullptr;
+  double *X = nullptr, *Y = nullptr;
+  double *LX = nullptr, *MY = nullptr, *LXinv = nullptr, *Xinv = nullptr;
+  double *dinvX = nullptr, *dinvY = nullptr;
+  double *P = nullptr, *R = nullptr, *dX = nullptr, *dY = nullptr, *W1 = nullptr, *W2 = nullptr;
+  double *trial = nullptr, *trial_dinv = nullptr;
+};
+
+// closed forms for blocks of side <= 2 (kernels/generic.jl:133-154, 292-340, 350-420)
+#pragma clang fp contract(off)
+__device__ bool small_isposdef(const double* X, const double* dX, int k, double t) {
+  if (k == 1) return X[0] + t * dX[0] > 0.0;
+  const double a = X[0] + t * dX[0];
+  if (!(a > 0.0)) return false;
+  const double b = X[1] + t * dX[1];
+  const double d = X[3] + t * dX[3];
+  return a * d > b * b;
+}
+
+__device__ bool small_margin(const double* X, const double* dX, int k, double t, double ratio) {
+  if (!(ratio > 0.0)) return small_isposdef(X, dX, k, t);
+  double mn, mx;
+  if (k == 1) {
+    const double d11 = X[0] + t * dX[0];
+    if (!(d11 > 0.0)) return false;
+    mn = mx = fabs(sqrt(d11));
+  } else {
+    const double d11 = X[0] + t * dX[0];
+    if (!(d11 > 0.0)) return false;
+    const double l11 = sqrt(d11);
+    const double l21 = (X[1] + t * dX[1]) / l11;
+    const double d22 = (X[3] + t * dX[3]) - l21 * l21;
+    if (!(d22 > 0.0)) return false;
+    const double l22 = sqrt(d22);
+    mn = fmin(fabs(l11), fabs(l22));
+    mx = fmax(fabs(l11), fabs(l22));
+  }
+  return isfinite(mn) && isfinite(mx) && mx > 0.0 && mn > ratio * mx;
+}
+
+__device__ double small_ftb(const double* X, const double* dX, int k) {
+  if (k == 1) {
+    const double d = dX[0];
+    return d < 0.0 ? fmin(1.0, -X[0] / d) : 1.0;
+  }
+  const double a = X[0], b = X[2], d = X[3];
+  const double da = dX[0], db = (dX[2] + dX[1]) / 2.0, dd = dX[3];
+  const double a1 = a + da, b1 = b + db, d1 = d + dd;
+  if (a1 > 0.0 && a1 * d1 > b1 * b1) return 1.0;
+  const double c0 = a * d - b * b;
+  const double c1 = da * d + a * dd - 2.0 * b * db;
+  const double c2 = da * dd - db * db;
+  double root = 1.0;
+  bool found = false;
+  if (c2 == 0.0) {
+    if (c1 < 0.0) {
+      const double cand = -c0 / c1;
+      if (0.0 < cand && cand <= 1.0) {
+        root = cand;
+        found = true;
+      }
+    }
+  } else {
+    const double disc = c1 * c1 - 4.0 * c2 * c0;
+    if (disc >= 0.0) {
+      const double sd = sqrt(disc);
+      const double den = 2.0 * c2;
+      const double r1 = (-c1 - sd) / den;
+      const double r2 = (-c1 + sd) / den;
+      if (0.0 < r1 && r1 <= root) {
+        root = r1;
+        found = true;
+      }
+      if (0.0 < r2 && r2 <= root) {
+        root = r2;
+        found = true;
+      }
+    }
+  }
+  if (found) return root;
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    const double mid = (lo + hi) / 2.0;
+    if (small_isposdef(X, dX, k, mid))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+#pragma clang fp contract(on)
+
+__global__ void small_trials_kernel(const double* X, const double* dX, int k, const double* ts, int B, double ratio,
+                                    int* ok) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) ok[b] = small_margin(X, dX, k, ts[b], ratio) ? 1 : 0;
+}
+
+__global__ void small_ftb_kernel(const double* X, const double* dX, int k, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = small_ftb(X, dX, k);
+}
+
+// trial[b] = X + ts[b] * dX
+__global__ void trial_build_kernel(const double* X, const double* dX, long long k2, const double* ts, int B,
+                                   double* trial) {
+  const long long n = k2 * B;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    const int b = (int)(idx / k2);
+    const long long i = idx % k2;
+    trial[idx] = X[i] + ts[b] * dX[i];
+  }
+}
+
+// corrector target (threaded.jl:839-854 / 943-1003) from the block dot in *dotv:
+//   mode 0 (fixed):     max(rl < 1 ? rl^2 : rl, beta) * mu,  rl = dot / mu / k
+//   mode 1 (local):     sigma * dot / k
+//   mode 2 (global):    sigma * mu
+__global__ void corrector_target_kernel(const double* dotv, int mode, double mu, double beta, double sigma, int k,
+                                        double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double target;
+  if (mode == 0) {
+    const double rl = *dotv / mu / (double)k;
+    const double gl = fmax(rl < 1.0 ? rl * rl : rl, beta);
+    target = gl * mu;
+  } else if (mode == 1) {
+    target = sigma * (*dotv) / (double)k;
+  } else {
+    target = sigma * mu;
+  }
+  *out = target;
+}
+
+__global__ void neg_sum_kernel(const double* a, const double* b, double* out, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = -(a[i] + b[i]);
+}
+
+}  // namespace
+}  // namespace sdpx
+
+using namespace sdpx;
+
+struct sdpx_dense {
+  int device = 0;
+  int m = 0;
+  int L = 0;
+  std::vector<Block> blk;
+  hipStream_t stream = nullptr;
+  double *c = nullptr, *x = nullptr, *dx = nullptr, *dx_best = nullptr, *d = nullptr, *v = nullptr, *r = nullptr;
+  double *rho = nullptr, *delta = nullptr, *v2 = nullptr;
+  double *S = nullptr, *Sf = nullptr, *dinvS = nullptr;
+  double *T = nullptr, *Ph = nullptr;  // Schur panels (max over blocks)
+  double* splitws = nullptr;
+  int split = 1;
+  double* gemv_part = nullptr;
+  long long gemv_part_cap = 0;
+  double* symv_part = nullptr;
+  double* red_part = nullptr;  // reduction partials
+  double* dsc = nullptr;       // device scalars
+  double* hsc = nullptr;       // pinned host scalars
+  int* dinfo = nullptr;
+  int* hinfo = nullptr;
+  double* dts = nullptr;  // line-search candidates
+  double* hts = nullptr;
+  int* dok = nullptr;
+  long long bytes = 0;
+  std::vector<void*> allocs;
+  int sinfo_slot = 0;
+  int last_reg_attempts = 0;
+};
+
+namespace {
+
+constexpr int kScalars = 1024;
+
+int dmalloc(sdpx_dense_t* h, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_last_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return kOutOfMemory;
+  }
+  h->allocs.push_back(*p);
+  h->bytes += (long long)bytes;
+  return kOk;
+}
+
+template <typename T>
+int dalloc(sdpx_dense_t* h, T** p, size_t count) {
+  return dmalloc(h, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+int sync_scalars(sdpx_dense_t* h, int ndoubles, int nints = 0) {
+  if (ndoubles > 0)
+    SDPX_HIP(hipMemcpyAsync(h->hsc, h->dsc, sizeof(double) * ndoubles, hipMemcpyDeviceToHost, h->stream));
+  if (nints > 0) SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dinfo, sizeof(int) * nints, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+GemmArgs gemm(int M, int N, int K, double alpha, const double* A, long long lda, const double* B, long long ldb,
+              double beta, double* C, long long ldc, unsigned flags) {
+  GemmArgs g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.C = C;
+  g.ldc = ldc;
+  g.batch = 1;
+  g.flags = flags;
+  return g;
+}
+
+// C = alpha A B + beta C for k x k blocks
+int mm(sdpx_dense_t* h, int k, double alpha, const double* A, bool ta, const double* B, bool tb, double beta,
+       double* C, unsigned extra = 0) {
+  return gemm_f64(gemm(k, k, k, alpha, A, k, B, k, beta, C, k,
+                       (ta ? kGemmTransA : 0u) | (tb ? kGemmTransB : 0u) | extra),
+                  h->stream);
+}
+
+int check_handle(sdpx_dense_t* h) {
+  if (!h) {
+    set_last_error("null handle");
+    return kInvalidArgument;
+  }
+  SDPX_HIP(hipSetDevice(h->device));
+  return kOk;
+}
+
+// Z = X^-1 (P Y - R) for block b into W2 (predictor/corrector RHS, step.jl:95-106)
+int block_rhs_z(sdpx_dense_t* h, Block& b, const double* Y) {
+  SDPX_CHECK(mm(h, b.k, 1.0, b.P, false, Y, false, 0.0, b.W1));
+  SDPX_CHECK(axpbyz_f64(b.k2, 1.0, b.W1, -1.0, b.R, b.W1, h->stream));
+  SDPX_CHECK(mm(h, b.k, 1.0, b.Xinv, false, b.W1, false, 0.0, b.W2));
+  return kOk;
+}
+
+int rhs_from_blocks(sdpx_dense_t* h) {
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(block_rhs_z(h, b, b.Y));
+    SDPX_CHECK(avt_f64(b.Av, b.k2, h->m, b.W2, 1.0, l == 0 ? 0.0 : 1.0, h->v, h->stream));
+  }
+  hipLaunchKernelGGL(neg_sum_kernel, dim3(ceil_div(h->m, 256)), dim3(256), 0, h->stream, h->d, h->v, h->r, h->m);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+// rho = r - S dx; dsc[slot] = ||rho||_inf, dsc[slot+1] = ||r||_inf
+int direction_residual_dev(sdpx_dense_t* h, int slot) {
+  SDPX_CHECK(symv_lower_f64(h->S, h->m, h->dx, -1.0, 1.0, h->r, h->rho, h->symv_part, h->stream));
+  SDPX_CHECK(absmax_f64(h->rho, h->m, h->red_part, h->dsc + slot, h->stream));
+  SDPX_CHECK(absmax_f64(h->r, h->m, h->red_part, h->dsc + slot + 1, h->stream));
+  return kOk;
+}
+
+int solve_into(sdpx_dense_t* h, const double* rhs, double* out) {
+  if (rhs != out) SDPX_HIP(hipMemcpyAsync(out, rhs, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+  SDPX_CHECK(trsv_f64(h->Sf, h->m, h->m, h->dinvS, out, 0, h->stream));
+  SDPX_CHECK(trsv_f64(h->Sf, h->m, h->m, h->dinvS, out, 1, h->stream));
+  return kOk;
+}
+
+// Batched trial test of M + t_j dM for candidates hts[0..B): dok[j] = pass (1) / fail (0).
+int trial_batch(sdpx_dense_t* h, Block& b, const double* M, const double* dM, int B, double ratio) {
+  SDPX_HIP(hipMemcpyAsync(h->dts, h->hts, sizeof(double) * B, hipMemcpyHostToDevice, h->stream));
+  if (b.k <= 2) {
+    hipLaunchKernelGGL(small_trials_kernel, dim3(1), dim3(kTrialCap), 0, h->stream, M, dM, b.k, h->dts, B, ratio,
+                       h->dok);
+    SDPX_HIP(hipGetLastError());
+    SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dok, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    return kOk;
+  }
+  const long long n = b.k2 * B;
+  hipLaunchKernelGGL(trial_build_kernel, dim3(std::min(4096, ceil_div(n, 256))), dim3(256), 0, h->stream, M, dM,
+                     b.k2, h->dts, B, b.trial);
+  SDPX_HIP(hipGetLastError());
+  const long long dstride = (long long)b.nblk * kCholNb * kCholNb;
+  SDPX_CHECK(potrf_f64(b.trial, b.k, b.k2, b.k, B, b.trial_dinv, dstride, h->dok, h->stream));
+  if (ratio > 0.0) SDPX_CHECK(diag_extrema_f64(b.trial, b.k, b.k2, b.k, B, h->dsc, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dok, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+  if (ratio > 0.0) SDPX_HIP(hipMemcpyAsync(h->hsc, h->dsc, sizeof(double) * 2 * B, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  // convert LAPACK info to pass flags (+ margin test, kernels/generic.jl:318-340)
+  for (int j = 0; j < B; ++j) {
+    bool ok = h->hinfo[j] == 0;
+    if (ok && ratio > 0.0) {
+      const double mn = h->hsc[2 * j], mx = h->hsc[2 * j + 1];
+      ok = std::isfinite(mn) && std::isfinite(mx) && mx > 0.0 && mn > ratio * mx;
+    }
+    h->hinfo[j] = ok ? 1 : 0;
+  }
+  return kOk;
+}
+
+// fraction_to_boundary_bound! (kernels/generic.jl:350-420) for one block, k > 2:
+// full-step test, then the 64-step bisection evaluated D levels at a time.  All
+// midpoints of the next D levels are generated with the same FP64 arithmetic
+// as the sequential bisection, tested in one batched launch, and the path is
+// then replayed from the outcomes, so the result equals the sequential one.
+int ftb_bound(sdpx_dense_t* h, Block& b, const double* M, const double* dM, double* bound) {
+  if (b.k <= 2) {
+    hipLaunchKernelGGL(small_ftb_kernel, dim3(1), dim3(64), 0, h->stream, M, dM, b.k, h->dsc);
+    SDPX_HIP(hipGetLastError());
+    SDPX_CHECK(sync_scalars(h, 1));
+    *bound = h->hsc[0];
+    return kOk;
+  }
+  h->hts[0] = 1.0;
+  SDPX_CHECK(trial_batch(h, b, M, dM, 1, 0.0));
+  if (h->hinfo[0]) {
+    *bound = 1.0;
+    return kOk;
+  }
+  double lo = 0.0, hi = 1.0;
+  int remaining = 64;
+  constexpr int D = 6;  // 63 candidates per launch
+  while (remaining > 0) {
+    const int depth = std::min(D, remaining);
+    // heap-ordered tree of midpoints: node 1 = (lo, hi)
+    double nlo[1 << (D + 1)], nhi[1 << (D + 1)];
+    nlo[1] = lo;
+    nhi[1] = hi;
+    const int nodes = (1 << depth) - 1;
+    for (int node = 1; node <= nodes; ++node) {
+      const double mid = (nlo[node] + nhi[node]) / 2.0;
+      h->hts[node - 1] = mid;
+      if (2 * node + 1 < (1 << (D + 1))) {
+        nlo[2 * node] = nlo[node];
+        nhi[2 * node] = mid;
+        nlo[2 * node + 1] = mid;
+        nhi[2 * node + 1] = nhi[node];
+      }
+    }
+    SDPX_CHECK(trial_batch(h, b, M, dM, nodes, 0.0));
+    int node = 1;
+    for (int lvl = 0; lvl < depth; ++lvl) {
+      const double mid = h->hts[node - 1];
+      if (h->hinfo[node - 1]) {
+        lo = mid;
+        node = 2 * node + 1;
+      } else {
+        hi = mid;
+        node = 2 * node;
+      }
+    }
+    remaining -= depth;
+  }
+  *bound = lo;
+  return kOk;
+}
+
+int backtrack_side(sdpx_dense_t* h, bool primal, double gamma, double min_step, double ratio, double* t_out) {
+  double t = 1.0;
+  // candidates are generated exactly as the sequential loop does (t *= gamma)
+  while (true) {
+    int B = 0;
+    double tc = t;
+    std::vector<double> cand;
+    for (; B < kTrialCap; ++B) {
+      cand.push_back(tc);
+      if (tc < min_step) {
+        ++B;
+        break;
+      }
+      tc *= gamma;
+    }
+    B = (int)cand.size();
+    std::vector<int> pass(B, 1);
+    for (int l = 0; l < h->L; ++l) {
+      Block& b = h->blk[l];
+      for (int j = 0; j < B; ++j) h->hts[j] = cand[j];
+      SDPX_CHECK(trial_batch(h, b, primal ? b.X : b.Y, primal ? b.dX : b.dY, B, ratio));
+      for (int j = 0; j < B; ++j) pass[j] = pass[j] && h->hinfo[j];
+    }
+    for (int j = 0; j < B; ++j) {
+      if (pass[j] || cand[j] < min_step) {
+        *t_out = cand[j];
+        return kOk;
+      }
+    }
+    t = cand[B - 1] * gamma;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sdpx_last_error(void) { return sdpx::last_error(); }
+
+int sdpx_version(void) { return 100; }
+
+int sdpx_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  if (count) *count = n;
+  return kOk;
+}
+
+int sdpx_dense_destroy(sdpx_dense_t* h) {
+  if (!h) return kOk;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->hsc) hipHostFree(h->hsc);
+  if (h->hinfo) hipHostFree(h->hinfo);
+  if (h->hts) hipHostFree(h->hts);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return kOk;
+}
+
+int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out) {
+  if (!out || L <= 0 || m <= 0 || !k) {
+    set_last_error("sdpx_dense_create: invalid dimensions");
+    return kInvalidArgument;
+  }
+  *out = nullptr;
+  for (int l = 0; l < L; ++l)
+    if (k[l] <= 0) {
+      set_last_error("sdpx_dense_create: block sizes must be positive");
+      return kInvalidArgument;
+    }
+  SDPX_HIP(hipSetDevice(device));
+  auto* h = new sdpx_dense();
+  h->device = device;
+  h->m = m;
+  h->L = L;
+  h->blk.resize(L);
+  auto fail = [&](int s) {
+    sdpx_dense_destroy(h);
+    return s;
+  };
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
+    set_last_error("hipStreamCreate failed");
+    return fail(kHipError);
+  }
+  long long maxk2 = 0;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    b.k = k[l];
+    b.k2 = (long long)b.k * b.k;
+    b.nblk = ceil_div(b.k, kCholNb);
+    maxk2 = std::max(maxk2, b.k2);
+    const long long dn = (long long)b.nblk * kCholNb * kCholNb;
+    int s = kOk;
+    if ((s = dalloc(h, &b.Av, (size_t)(b.k2 * m)))) return fail(s);
+    double** mats[] = {&b.C, &b.X, &b.Y, &b.LX, &b.MY, &b.LXinv, &b.Xinv, &b.P, &b.R, &b.dX, &b.dY, &b.W1, &b.W2};
+    for (double** p : mats)
+      if ((s = dalloc(h, p, (size_t)b.k2))) return fail(s);
+    if ((s = dalloc(h, &b.dinvX, (size_t)dn))) return fail(s);
+    if ((s = dalloc(h, &b.dinvY, (size_t)dn))) return fail(s);
+    if (b.k > 2) {
+      if ((s = dalloc(h, &b.trial, (size_t)(b.k2 * kTrialCap)))) return fail(s);
+      if ((s = dalloc(h, &b.trial_dinv, (size_t)(dn * kTrialCap)))) return fail(s);
+    }
+  }
+  int s = kOk;
+  double** vecs[] = {&h->c, &h->x, &h->dx, &h->dx_best, &h->d, &h->v, &h->r, &h->rho, &h->delta, &h->v2};
+  for (double** p : vecs)
+    if ((s = dalloc(h, p, (size_t)m))) return fail(s);
+  const long long mm2 = (long long)m * m;
+  if ((s = dalloc(h, &h->S, (size_t)mm2))) return fail(s);
+  if ((s = dalloc(h, &h->Sf, (size_t)mm2))) return fail(s);
+  if ((s = dalloc(h, &h->dinvS, (size_t)ceil_div(m, kCholNb) * kCholNb * kCholNb))) return fail(s);
+  if ((s = dalloc(h, &h->T, (size_t)(maxk2 * m)))) return fail(s);
+  if ((s = dalloc(h, &h->Ph, (size_t)(maxk2 * m)))) return fail(s);
+  // split-K for the Schur Gram product: aim for >= 1024 work-groups
+  {
+    const int bm = (m <= 256 && mm2 < 256LL * 128 * 128) ? 64 : 128;
+    const long long nt = ceil_div(m, bm);
+    const long long tiles = nt * (nt + 1) / 2;
+    const long long stages = ceil_div(maxk2, 16);
+    int split = (int)std::max<long long>(1, std::min<long long>(16, ceil_div(1024, tiles)));
+    split = (int)std::max<long long>(1, std::min<long long>(split, stages / 8));
+    h->split = split;
+    if (split > 1 && (s = dalloc(h, &h->splitws, (size_t)(split * mm2)))) return fail(s);
+  }
+  h->gemv_part_cap = std::max<long long>(maxk2 * 64, 4096);
+  if ((s = dalloc(h, &h->gemv_part, (size_t)h->gemv_part_cap))) return fail(s);
+  if ((s = dalloc(h, &h->symv_part, (size_t)ceil_div(m, 64) * m))) return fail(s);
+  if ((s = dalloc(h, &h->red_part, 1024))) return fail(s);
+  if ((s = dalloc(h, &h->dsc, kScalars))) return fail(s);
+  if ((s = dalloc(h, &h->dinfo, 1024))) return fail(s);
+  if ((s = dalloc(h, &h->dts, kTrialCap))) return fail(s);
+  if ((s = dalloc(h, &h->dok, kTrialCap))) return fail(s);
+  if (hipHostMalloc(reinterpret_cast<void**>(&h->hsc), sizeof(double) * kScalars) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&h->hinfo), sizeof(int) * 1024) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&h->hts), sizeof(double) * kTrialCap) != hipSuccess) {
+    set_last_error("hipHostMalloc failed");
+    return fail(kOutOfMemory);
+  }
+  SDPX_HIP(hipMemsetAsync(h->S, 0, sizeof(double) * mm2, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  *out = h;
+  return kOk;
+}
+
+int sdpx_dense_memory(sdpx_dense_t* h, long long* bytes) {
+  SDPX_CHECK(check_handle(h));
+  if (bytes) *bytes = h->bytes;
+  return kOk;
+}
+
+int sdpx_dense_set_block(sdpx_dense_t* h, int l, const double* Av, const double* C) {
+  SDPX_CHECK(check_handle(h));
+  if (l < 0 || l >= h->L || !Av || !C) {
+    set_last_error("sdpx_dense_set_block: bad block index or null pointer");
+    return kInvalidArgument;
+  }
+  Block& b = h->blk[l];
+  SDPX_HIP(hipMemcpyAsync(b.Av, Av, sizeof(double) * b.k2 * h->m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_objective(sdpx_dense_t* h, const double* c) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(h->c, c, sizeof(double) * h->m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_iterate(sdpx_dense_t* h, const double* x, const double* X, const double* Y) {
+  SDPX_CHECK(check_handle(h));
+  if (x) SDPX_HIP(hipMemcpyAsync(h->x, x, sizeof(double) * h->m, hipMemcpyHostToDevice, h->stream));
+  long long off = 0;
+  for (auto& b : h->blk) {
+    if (X) SDPX_HIP(hipMemcpyAsync(b.X, X + off, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+    if (Y) SDPX_HIP(hipMemcpyAsync(b.Y, Y + off, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+    off += b.k2;
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_get_iterate(sdpx_dense_t* h, double* x, double* X, double* Y) {
+  SDPX_CHECK(check_handle(h));
+  if (x) SDPX_HIP(hipMemcpyAsync(x, h->x, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
+  long long off = 0;
+  for (auto& b : h->blk) {
+    if (X) SDPX_HIP(hipMemcpyAsync(X + off, b.X, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    if (Y) SDPX_HIP(hipMemcpyAsync(Y + off, b.Y, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    off += b.k2;
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_get_direction(sdpx_dense_t* h, double* dx, double* dX, double* dY) {
+  SDPX_CHECK(check_handle(h));
+  if (dx) SDPX_HIP(hipMemcpyAsync(dx, h->dx, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
+  long long off = 0;
+  for (auto& b : h->blk) {
+    if (dX) SDPX_HIP(hipMemcpyAsync(dX + off, b.dX, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    if (dY) SDPX_HIP(hipMemcpyAsync(dY + off, b.dY, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    off += b.k2;
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_get_schur(sdpx_dense_t* h, double* S) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(S, h->S, sizeof(double) * h->m * h->m, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_schur(sdpx_dense_t* h, const double* S) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(h->S, S, sizeof(double) * h->m * h->m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_rhs(sdpx_dense_t* h, const double* r) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(h->r, r, sizeof(double) * h->m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_get_rhs(sdpx_dense_t* h, double* r) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(r, h->r, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_dx(sdpx_dense_t* h, const double* dx) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipMemcpyAsync(h->dx, dx, sizeof(double) * h->m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_synchronize(sdpx_dense_t* h) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+int sdpx_dense_residuals(sdpx_dense_t* h, const double* mu, int affine, int factor, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L;
+  // P_l = Av_l x - X_l - C_l ; slot l = ||P_l||_inf
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avx_f64(b.Av, b.k2, h->m, h->x, 1.0, b.P, -1.0, b.X, -1.0, b.C, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+    SDPX_CHECK(absmax_f64(b.P, b.k2, h->red_part, h->dsc + l, h->stream));
+  }
+  // d = c - sum_l Av_l' vec(Y_l)
+  SDPX_HIP(hipMemcpyAsync(h->d, h->c, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avt_f64(b.Av, b.k2, h->m, b.Y, -1.0, 1.0, h->d, h->stream));
+  }
+  SDPX_CHECK(absmax_f64(h->d, h->m, h->red_part, h->dsc + L, h->stream));
+  // R_l = -X_l Y_l (+ mu_l I)
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(mm(h, b.k, -1.0, b.X, false, b.Y, false, 0.0, b.R));
+    if (!affine) SDPX_CHECK(add_diag_f64(b.R, b.k, b.k, mu[l], nullptr, 0.0, h->stream));
+  }
+  int nint = 0;
+  if (factor) {
+    for (int l = 0; l < L; ++l) {
+      Block& b = h->blk[l];
+      SDPX_HIP(hipMemcpyAsync(b.LX, b.X, sizeof(double) * b.k2, hipMemcpyDeviceToDevice, h->stream));
+      SDPX_HIP(hipMemcpyAsync(b.MY, b.Y, sizeof(double) * b.k2, hipMemcpyDeviceToDevice, h->stream));
+      SDPX_CHECK(potrf_f64(b.LX, b.k, b.k2, b.k, 1, b.dinvX, 0, h->dinfo + 2 * l, h->stream));
+      SDPX_CHECK(potrf_f64(b.MY, b.k, b.k2, b.k, 1, b.dinvY, 0, h->dinfo + 2 * l + 1, h->stream));
+    }
+    nint = 2 * L;
+  }
+  SDPX_CHECK(sync_scalars(h, L + 1, nint));
+  double p_res = 0.0;
+  for (int l = 0; l < L; ++l) p_res = std::max(p_res, h->hsc[l]);
+  for (int l = 0; l < L; ++l)
+    if (std::isnan(h->hsc[l])) p_res = h->hsc[l];
+  const double d_res = h->hsc[L];
+  bool ok = true;
+  for (int i = 0; i < nint; ++i) ok = ok && h->hinfo[i] == 0;
+  if (factor && ok) {
+    // L_X^-1 and X^-1 = L_X^-T L_X^-1 for the Schur panels and X^-1 M products
+    for (int l = 0; l < L; ++l) {
+      Block& b = h->blk[l];
+      SDPX_CHECK(trtri_lower_f64(b.LX, b.k, b.k, b.dinvX, b.LXinv, nullptr, h->stream));
+      SDPX_CHECK(gemm_f64(gemm(b.k, b.k, b.k, 1.0, b.LXinv, b.k, b.LXinv, b.k, 0.0, b.Xinv, b.k,
+                               kGemmTransA | kGemmAUpper | kGemmBLower),
+                          h->stream));
+    }
+  }
+  if (out) {
+    out[0] = p_res;
+    out[1] = d_res;
+    out[2] = ok ? 1.0 : 0.0;
+  }
+  return ok ? kOk : kNotPositiveDefinite;
+}
+
+int sdpx_dense_schur_build(sdpx_dense_t* h) {
+  SDPX_CHECK(check_handle(h));
+  const int m = h->m;
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    const int k = b.k;
+    // T = L_X^-1 [A_1 ... A_m]   (k x km, op(A) lower triangular)
+    SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, b.Av, k, 0.0, h->T, k, kGemmALower), h->stream));
+    // Phat_i = T_i M_Y           (batched k x k, op(B) lower triangular)
+    GemmArgs g = gemm(k, k, k, 1.0, h->T, k, b.MY, k, 0.0, h->Ph, k, kGemmBLower);
+    g.strideA = b.k2;
+    g.strideB = 0;
+    g.strideC = b.k2;
+    g.batch = m;
+    SDPX_CHECK(gemm_f64(g, h->stream));
+    // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
+    GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
+                      kGemmTransA | kGemmLowerOut);
+    SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws));
+  }
+  return kOk;
+}
+
+int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts) {
+  SDPX_CHECK(check_handle(h));
+  const int m = h->m;
+  int attempts = 0;
+  double reg = 0.0;
+  bool ok = false;
+  while (true) {
+    SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
+    SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
+    SDPX_CHECK(sync_scalars(h, 0, 1));
+    ok = h->hinfo[0] == 0;
+    if (ok || attempts >= 6) break;
+    attempts += 1;
+    reg = attempts == 1 ? std::sqrt(kEps) : reg * 10.0;
+  }
+  h->last_reg_attempts = attempts;
+  if (reg_attempts) *reg_attempts = attempts;
+  return ok ? kOk : kNotPositiveDefinite;
+}
+
+int sdpx_dense_predictor_rhs(sdpx_dense_t* h) {
+  SDPX_CHECK(check_handle(h));
+  return rhs_from_blocks(h);
+}
+
+int sdpx_dense_corrector_rhs(sdpx_dense_t* h, int mehrotra, double sigma, double mu_global, int block_local,
+                             const double* mu, double beta) {
+  SDPX_CHECK(check_handle(h));
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    double* dotv = h->dsc + 2 * l;
+    double* target = h->dsc + 2 * l + 1;
+    int mode;
+    if (!mehrotra) {
+      mode = 0;
+      SDPX_CHECK(dot_f64(b.X, b.dX, 1.0, b.Y, b.dY, 1.0, b.k2, h->red_part, dotv, h->stream));
+    } else if (block_local) {
+      mode = 1;
+      SDPX_CHECK(dot_f64(b.X, nullptr, 0.0, b.Y, nullptr, 0.0, b.k2, h->red_part, dotv, h->stream));
+    } else {
+      mode = 2;
+    }
+    hipLaunchKernelGGL(corrector_target_kernel, dim3(1), dim3(64), 0, h->stream, dotv, mode,
+                       mehrotra ? mu_global : mu[l], beta, sigma, b.k, target);
+    SDPX_HIP(hipGetLastError());
+    // R = -X Y - dX dY + target I
+    SDPX_CHECK(mm(h, b.k, -1.0, b.X, false, b.Y, false, 0.0, b.R));
+    SDPX_CHECK(mm(h, b.k, -1.0, b.dX, false, b.dY, false, 1.0, b.R));
+    SDPX_CHECK(add_diag_f64(b.R, b.k, b.k, 0.0, target, 1.0, h->stream));
+  }
+  return rhs_from_blocks(h);
+}
+
+int sdpx_dense_solve(sdpx_dense_t* h) {
+  SDPX_CHECK(check_handle(h));
+  return solve_into(h, h->r, h->dx);
+}
+
+int sdpx_dense_direction_residual(sdpx_dense_t* h, double* out) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_CHECK(direction_residual_dev(h, 0));
+  SDPX_CHECK(sync_scalars(h, 2));
+  if (out) {
+    out[0] = h->hsc[0];
+    out[1] = h->hsc[1];
+  }
+  return kOk;
+}
+
+int sdpx_dense_refine(sdpx_dense_t* h, double abstol, int max_steps, int fixed_policy, int* steps_out,
+                      double* residual_out) {
+  SDPX_CHECK(check_handle(h));
+  int steps = 0;
+  double residual = 0.0;
+  if (max_steps <= 0) {
+    if (steps_out) *steps_out = 0;
+    if (residual_out) *residual_out = 0.0;
+    return kOk;
+  }
+  SDPX_CHECK(direction_residual_dev(h, 0));
+  SDPX_CHECK(sync_scalars(h, 1));
+  residual = h->hsc[0];
+  for (int it = 0; it < max_steps; ++it) {
+    if (!fixed_policy && residual <= abstol) break;
+    // snapshot, correct: dx += S^-1 rho, re-evaluate (kkt.jl:3930-3960)
+    SDPX_HIP(hipMemcpyAsync(h->dx_best, h->dx, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+    SDPX_CHECK(solve_into(h, h->rho, h->delta));
+    SDPX_CHECK(axpbyz_f64(h->m, 1.0, h->dx, 1.0, h->delta, h->dx, h->stream));
+    SDPX_CHECK(direction_residual_dev(h, 0));
+    SDPX_CHECK(sync_scalars(h, 1));
+    const double corrected = h->hsc[0];
+    if (!std::isfinite(corrected) || corrected > residual) {
+      SDPX_HIP(hipMemcpyAsync(h->dx, h->dx_best, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+      // rho must describe the retained direction
+      SDPX_CHECK(direction_residual_dev(h, 0));
+      SDPX_HIP(hipStreamSynchronize(h->stream));
+      break;
+    }
+    steps += 1;
+    const bool small = corrected > residual * 0.5;
+    residual = corrected;
+    if (!fixed_policy && small) break;
+  }
+  if (steps_out) *steps_out = steps;
+  if (residual_out) *residual_out = residual;
+  return kOk;
+}
+
+int sdpx_dense_direction_blocks(sdpx_dense_t* h) {
+  SDPX_CHECK(check_handle(h));
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    // dX = Av dx + P
+    SDPX_CHECK(avx_f64(b.Av, b.k2, h->m, h->dx, 1.0, b.dX, 1.0, b.P, 0.0, nullptr, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+    // dY = sym(X^-1 (R - dX Y))
+    SDPX_CHECK(mm(h, b.k, 1.0, b.dX, false, b.Y, false, 0.0, b.W1));
+    SDPX_CHECK(axpbyz_f64(b.k2, 1.0, b.R, -1.0, b.W1, b.W1, h->stream));
+    SDPX_CHECK(mm(h, b.k, 1.0, b.Xinv, false, b.W1, false, 0.0, b.dY));
+    SDPX_CHECK(symmetrize_f64(b.dY, b.k, h->stream));
+  }
+  return kOk;
+}
+
+int sdpx_dense_complementarity(sdpx_dense_t* h, double tX, double tY, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(dot_f64(b.X, nullptr, 0.0, b.Y, nullptr, 0.0, b.k2, h->red_part, h->dsc + l, h->stream));
+    SDPX_CHECK(dot_f64(b.X, b.dX, tX, b.Y, b.dY, tY, b.k2, h->red_part, h->dsc + L + l, h->stream));
+  }
+  SDPX_CHECK(sync_scalars(h, 2 * L));
+  for (int i = 0; i < 2 * L; ++i) out[i] = h->hsc[i];
+  return kOk;
+}
+
+int sdpx_dense_fraction_to_boundary(sdpx_dense_t* h, double pf, double df, double* tX, double* tY) {
+  SDPX_CHECK(check_handle(h));
+  double bx = 1.0, by = 1.0;
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    double v = 1.0;
+    SDPX_CHECK(ftb_bound(h, b, b.X, b.dX, &v));
+    bx = std::min(bx, v);
+    SDPX_CHECK(ftb_bound(h, b, b.Y, b.dY, &v));
+    by = std::min(by, v);
+  }
+  *tX = bx < 1.0 ? pf * bx : 1.0;
+  *tY = by < 1.0 ? df * by : 1.0;
+  return kOk;
+}
+
+int sdpx_dense_backtrack(sdpx_dense_t* h, double gamma, double min_step, double ratio, double* tX, double* tY) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_CHECK(backtrack_side(h, true, gamma, min_step, ratio, tX));
+  SDPX_CHECK(backtrack_side(h, false, gamma, min_step, ratio, tY));
+  return kOk;
+}
+
+int sdpx_dense_update(sdpx_dense_t* h, double tX, double tY, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(axpbyz_f64(b.k2, 1.0, b.X, tX, b.dX, b.X, h->stream));
+    SDPX_CHECK(axpbyz_f64(b.k2, 1.0, b.Y, tY, b.dY, b.Y, h->stream));
+    SDPX_CHECK(dot_f64(b.X, nullptr, 0.0, b.Y, nullptr, 0.0, b.k2, h->red_part, h->dsc + 2 + l, h->stream));
+    SDPX_CHECK(absmax_f64(b.X, b.k2, h->red_part, h->dsc + 2 + L + 2 * l, h->stream));
+    SDPX_CHECK(absmax_f64(b.Y, b.k2, h->red_part, h->dsc + 2 + L + 2 * l + 1, h->stream));
+  }
+  SDPX_CHECK(axpbyz_f64(h->m, 1.0, h->x, tX, h->dx, h->x, h->stream));
+  SDPX_CHECK(absmax_f64(h->x, h->m, h->red_part, h->dsc + 1, h->stream));
+  SDPX_CHECK(sync_scalars(h, 2 + 3 * L));
+  double comp = 0.0;
+  bool finite = std::isfinite(h->hsc[1]);
+  for (int l = 0; l < L; ++l) {
+    comp += h->hsc[2 + l];
+    out[2 + l] = h->hsc[2 + l];
+    finite = finite && std::isfinite(h->hsc[2 + L + 2 * l]) && std::isfinite(h->hsc[2 + L + 2 * l + 1]);
+  }
+  out[0] = comp;
+  out[1] = finite ? 1.0 : 0.0;
+  return kOk;
+}
+
+int sdpx_dense_accepted_residuals(sdpx_dense_t* h, double tX, double tY, double bpr, double* out) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_CHECK(axpbyz_f64(h->m, 1.0 - tY, h->d, -tY, h->rho, h->d, h->stream));
+  SDPX_CHECK(absmax_f64(h->d, h->m, h->red_part, h->dsc, h->stream));
+  SDPX_CHECK(sync_scalars(h, 1));
+  out[0] = std::fabs(1.0 - tX) * bpr;
+  out[1] = h->hsc[0];
+  return kOk;
+}
+
+int sdpx_dense_objectives(sdpx_dense_t* h, double* pobj, double* dobj) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_CHECK(dot_f64(h->c, nullptr, 0.0, h->x, nullptr, 0.0, h->m, h->red_part, h->dsc, h->stream));
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(dot_f64(b.C, nullptr, 0.0, b.Y, nullptr, 0.0, b.k2, h->red_part, h->dsc + 1 + l, h->stream));
+  }
+  SDPX_CHECK(sync_scalars(h, 1 + h->L));
+  *pobj = h->hsc[0];
+  double d = 0.0;
+  for (int l = 0; l < h->L; ++l) d += h->hsc[1 + l];
+  *dobj = d;
+  return kOk;
+}
+
+int sdpx_dense_solution_residuals(sdpx_dense_t* h, double* p_res, double* d_res) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avx_f64(b.Av, b.k2, h->m, h->x, 1.0, b.W1, -1.0, b.X, -1.0, b.C, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+    SDPX_CHECK(absmax_f64(b.W1, b.k2, h->red_part, h->dsc + l, h->stream));
+  }
+  SDPX_HIP(hipMemcpyAsync(h->v2, h->c, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avt_f64(b.Av, b.k2, h->m, b.Y, -1.0, 1.0, h->v2, h->stream));
+  }
+  SDPX_CHECK(absmax_f64(h->v2, h->m, h->red_part, h->dsc + L, h->stream));
+  SDPX_CHECK(sync_scalars(h, L + 1));
+  double p = 0.0;
+  for (int l = 0; l < L; ++l) p = std::max(p, h->hsc[l]);
+  *p_res = p;
+  *d_res = h->hsc[L];
+  return kOk;
+}
+
+int sdpx_dense_scale_blocks(sdpx_dense_t* h, int which, double factor) {
+  SDPX_CHECK(check_handle(h));
+  for (auto& b : h->blk) {
+    if (which == 1) SDPX_CHECK(axpbyz_f64(b.k2, factor, b.X, 0.0, nullptr, b.X, h->stream));
+    if (which == 2) SDPX_CHECK(axpbyz_f64(b.k2, factor, b.Y, 0.0, nullptr, b.Y, h->stream));
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(diag_extrema_f64(b.LX, b.k, 0, b.k, 1, h->dsc + 4 * l, h->stream));
+    SDPX_CHECK(diag_extrema_f64(b.MY, b.k, 0, b.k, 1, h->dsc + 4 * l + 2, h->stream));
+  }
+  SDPX_CHECK(diag_extrema_f64(h->Sf, h->m, 0, h->m, 1, h->dsc + 4 * L, h->stream));
+  SDPX_CHECK(sync_scalars(h, 4 * L + 2));
+  auto quality = [](double mn, double mx) {
+    return (mx > 0.0 && std::isfinite(mx)) ? std::min(std::max(mn / mx, 0.0), 1.0) : 0.0;
+  };
+  double pm = 1.0, dm = 1.0;
+  for (int l = 0; l < L; ++l) {
+    pm = std::min(pm, quality(h->hsc[4 * l], h->hsc[4 * l + 1]));
+    dm = std::min(dm, quality(h->hsc[4 * l + 2], h->hsc[4 * l + 3]));
+  }
+  out[0] = pm;
+  out[1] = dm;
+  out[2] = quality(h->hsc[4 * L], h->hsc[4 * L + 1]);
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+"""ctypes binding of ``libsdpx_mi355x.so`` (declared in include/sdpx_mi355x.h).
+
+The library is the only compute path: importing a function that needs it
+raises when the shared object is missing, there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libsdpx_mi355x.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_h = C.c_void_p
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "sdpx_last_error": [],
+    "sdpx_version": [],
+    "sdpx_device_count": [_ip],
+    "sdpx_dense_create": [C.c_int, C.c_int, _ip, C.c_int, C.POINTER(_h)],
+    "sdpx_dense_destroy": [_h],
+    "sdpx_dense_memory": [_h, C.POINTER(C.c_longlong)],
+    "sdpx_dense_set_block": [_h, C.c_int, _dp, _dp],
+    "sdpx_dense_set_objective": [_h, _dp],
+    "sdpx_dense_set_iterate": [_h, _dp, _dp, _dp],
+    "sdpx_dense_get_iterate": [_h, _dp, _dp, _dp],
+    "sdpx_dense_get_direction": [_h, _dp, _dp, _dp],
+    "sdpx_dense_get_schur": [_h, _dp],
+    "sdpx_dense_set_schur": [_h, _dp],
+    "sdpx_dense_set_rhs": [_h, _dp],
+    "sdpx_dense_get_rhs": [_h, _dp],
+    "sdpx_dense_set_dx": [_h, _dp],
+    "sdpx_dense_residuals": [_h, _dp, C.c_int, C.c_int, _dp],
+    "sdpx_dense_schur_build": [_h],
+    "sdpx_dense_schur_factor": [_h, _ip],
+    "sdpx_dense_predictor_rhs": [_h],
+    "sdpx_dense_corrector_rhs": [_h, C.c_int, C.c_double, C.c_double, C.c_int, _dp, C.c_double],
+    "sdpx_dense_solve": [_h],
+    "sdpx_dense_direction_residual": [_h, _dp],
+    "sdpx_dense_refine": [_h, C.c_double, C.c_int, C.c_int, _ip, _dp],
+    "sdpx_dense_direction_blocks": [_h],
+    "sdpx_dense_complementarity": [_h, C.c_double, C.c_double, _dp],
+    "sdpx_dense_fraction_to_boundary": [_h, C.c_double, C.c_double, _dp, _dp],
+    "sdpx_dense_backtrack": [_h, C.c_double, C.c_double, C.c_double, _dp, _dp],
+    "sdpx_dense_update": [_h, C.c_double, C.c_double, _dp],
+    "sdpx_dense_accepted_residuals": [_h, C.c_double, C.c_double, C.c_double, _dp],
+    "sdpx_dense_objectives": [_h, _dp, _dp],
+    "sdpx_dense_solution_residuals": [_h, _dp, _dp],
+    "sdpx_dense_scale_blocks": [_h, C.c_int, C.c_double],
+    "sdpx_dense_factor_quality": [_h, _dp],
+    "sdpx_dense_synchronize": [_h],
+}
+_RESTYPES = {"sdpx_last_error": C.c_char_p}
+
+_LIB = None
+
+
+class SDPXError(RuntimeError):
+    """A C-ABI call failed (negative status)."""
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raises when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise SDPXError(
+            f"{LIB_PATH.name} is not built (run __graft_entry__.build()); "
+            "the MI355X provider has no CPU fallback"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _LIB = lib
+    return lib
+
+
+def check(status: int, what: str) -> int:
+    """Raise on negative status; return the non-negative status (0 ok, 1 not PD, 2 non-finite)."""
+    if status < 0:
+        msg = load().sdpx_last_error()
+        raise SDPXError(f"{what} failed ({status}): {msg.decode() if msg else ''}")
+    return status
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None

@@ -1,0 +1,239 @@
+"""The MI355X dense KKT/Schur backend: one device handle per SDP workspace.
+
+``MI355XDenseBackend`` is the host mirror of the provider a Julia extension
+registers behind SDPX's seams (kkt_backend.jl:24-59 ``KKTBackend`` subtypes,
+la_backend.jl:508-559 ``instantiate_la_backend``).  Its methods map one-to-one
+onto the C-ABI (include/sdpx_mi355x.h); the problem data, iterate, Schur
+matrix and factor stay resident in HBM between calls.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+
+
+@dataclass
+class SDPProblem:
+    """Core geometric SDP (types/problems.jl:1-12) with dense panels, n_eq = 0.
+
+    ``Av[l]`` is a C-contiguous (m, k_l^2) array whose row i is vec(A_i^(l)),
+    i.e. the bytes of the reference's k^2 x m column-major ``DenseCons.Av[l]``.
+    """
+
+    c: np.ndarray
+    C: list
+    Av: list
+    k: list
+
+    @property
+    def m(self) -> int:
+        return int(self.c.shape[0])
+
+    @property
+    def L(self) -> int:
+        return len(self.k)
+
+
+def ingest(c, A, C) -> SDPProblem:
+    """Dense ingest (ingest.jl:680-694): ``A[l]`` is (m, k, k) with A[l][i] = A_i^(l)."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    Av, Cs, ks = [], [], []
+    for Al, Cl in zip(A, C):
+        Al = np.asarray(Al, dtype=np.float64)
+        m, k, k2 = Al.shape
+        if k != k2 or m != c.shape[0]:
+            raise ValueError("A blocks must be (m, k, k) with m = len(c)")
+        Av.append(np.ascontiguousarray(Al.transpose(0, 2, 1).reshape(m, k * k)))
+        Cs.append(np.asfortranarray(np.asarray(Cl, dtype=np.float64)))
+        ks.append(int(k))
+    return SDPProblem(c=c, C=Cs, Av=Av, k=ks)
+
+
+def _fortran_concat(blocks) -> np.ndarray:
+    return np.concatenate([np.asarray(b, dtype=np.float64).ravel(order="F") for b in blocks])
+
+
+class MI355XDenseBackend:
+    """Device workspace of the dense Schur route for one problem."""
+
+    name = "mi355x_dense_cholesky"
+
+    def __init__(self, prob: SDPProblem, device: int = 0):
+        self.lib = _lib.load()
+        self.prob = prob
+        self.L, self.m, self.k = prob.L, prob.m, list(prob.k)
+        self.k2 = [k * k for k in self.k]
+        self.offsets = np.cumsum([0] + self.k2)
+        h = C.c_void_p()
+        karr = (C.c_int * self.L)(*self.k)
+        check(self.lib.sdpx_dense_create(device, self.L, karr, self.m, C.byref(h)), "sdpx_dense_create")
+        self.h = h
+        for l in range(self.L):
+            Av = np.ascontiguousarray(prob.Av[l], dtype=np.float64)
+            Cl = np.asfortranarray(prob.C[l], dtype=np.float64)
+            check(self.lib.sdpx_dense_set_block(h, l, dptr(Av), dptr(Cl.ravel(order="F"))), "set_block")
+        check(self.lib.sdpx_dense_set_objective(h, dptr(np.ascontiguousarray(prob.c))), "set_objective")
+        self._o = np.zeros(4 + 4 * self.L)
+
+    # ---- lifecycle -----------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sdpx_dense_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory_bytes(self) -> int:
+        b = C.c_longlong()
+        check(self.lib.sdpx_dense_memory(self.h, C.byref(b)), "memory")
+        return int(b.value)
+
+    # ---- iterate I/O -----------------------------------------------------
+    def set_iterate(self, x, X, Y):
+        xa = None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        Xa = None if X is None else _fortran_concat(X)
+        Ya = None if Y is None else _fortran_concat(Y)
+        check(self.lib.sdpx_dense_set_iterate(self.h, dptr(xa), dptr(Xa), dptr(Ya)), "set_iterate")
+
+    def _split(self, flat):
+        return [flat[self.offsets[l]:self.offsets[l + 1]].reshape(self.k[l], self.k[l], order="F").copy()
+                for l in range(self.L)]
+
+    def get_iterate(self):
+        x = np.empty(self.m)
+        X = np.empty(int(self.offsets[-1]))
+        Y = np.empty(int(self.offsets[-1]))
+        check(self.lib.sdpx_dense_get_iterate(self.h, dptr(x), dptr(X), dptr(Y)), "get_iterate")
+        return x, self._split(X), self._split(Y)
+
+    def get_direction(self):
+        dx = np.empty(self.m)
+        dX = np.empty(int(self.offsets[-1]))
+        dY = np.empty(int(self.offsets[-1]))
+        check(self.lib.sdpx_dense_get_direction(self.h, dptr(dx), dptr(dX), dptr(dY)), "get_direction")
+        return dx, self._split(dX), self._split(dY)
+
+    def get_schur(self) -> np.ndarray:
+        """m x m Schur matrix; only the lower triangle is authoritative."""
+        S = np.empty((self.m, self.m), order="F")
+        check(self.lib.sdpx_dense_get_schur(self.h, dptr(S.ravel(order="K"))), "get_schur")
+        return S
+
+    def set_schur(self, S):
+        Sf = np.asfortranarray(S, dtype=np.float64)
+        check(self.lib.sdpx_dense_set_schur(self.h, dptr(Sf.ravel(order="K"))), "set_schur")
+
+    def set_rhs(self, r):
+        check(self.lib.sdpx_dense_set_rhs(self.h, dptr(np.ascontiguousarray(r, dtype=np.float64))), "set_rhs")
+
+    def get_rhs(self):
+        r = np.empty(self.m)
+        check(self.lib.sdpx_dense_get_rhs(self.h, dptr(r)), "get_rhs")
+        return r
+
+    def set_dx(self, dx):
+        check(self.lib.sdpx_dense_set_dx(self.h, dptr(np.ascontiguousarray(dx, dtype=np.float64))), "set_dx")
+
+    # ---- Newton-step phases -------------------------------------------------
+    def compute_residuals(self, mu, affine: bool, factor: bool):
+        """(p_res, d_res, blocks_ok) -- threaded_compute_residuals! (threaded.jl:423-505)."""
+        mu_a = np.ascontiguousarray(mu, dtype=np.float64)
+        out = np.zeros(3)
+        s = check(self.lib.sdpx_dense_residuals(self.h, dptr(mu_a), int(affine), int(factor), dptr(out)), "residuals")
+        return float(out[0]), float(out[1]), s == 0
+
+    def schur_build(self):
+        """threaded_schur_build! -> schur_build! (schur.jl:511-547)."""
+        check(self.lib.sdpx_dense_schur_build(self.h), "schur_build")
+
+    def factorize(self):
+        """factorize!(::DenseCholeskyBackend) (kkt_backend.jl:211-221) -> (ok, reg_attempts)."""
+        a = C.c_int(0)
+        s = check(self.lib.sdpx_dense_schur_factor(self.h, C.byref(a)), "schur_factor")
+        return s == 0, int(a.value)
+
+    def predictor_rhs(self):
+        check(self.lib.sdpx_dense_predictor_rhs(self.h), "predictor_rhs")
+
+    def corrector_rhs(self, mehrotra: bool, sigma: float, mu_global: float, block_local: bool, mu, beta: float):
+        mu_a = np.ascontiguousarray(mu, dtype=np.float64)
+        check(self.lib.sdpx_dense_corrector_rhs(self.h, int(mehrotra), sigma, mu_global, int(block_local),
+                                                dptr(mu_a), beta), "corrector_rhs")
+
+    def solve_direction(self) -> bool:
+        """solve_direction! (kkt_backend.jl:392-410)."""
+        check(self.lib.sdpx_dense_solve(self.h), "solve")
+        return True
+
+    def direction_residual(self):
+        out = np.zeros(2)
+        check(self.lib.sdpx_dense_direction_residual(self.h, dptr(out)), "direction_residual")
+        return float(out[0]), float(out[1])
+
+    def refine(self, abstol: float, max_steps: int, fixed_policy: bool):
+        """refine! (kkt_backend.jl:476-485) -> (steps, residual)."""
+        steps = C.c_int(0)
+        res = C.c_double(0.0)
+        check(self.lib.sdpx_dense_refine(self.h, abstol, int(max_steps), int(fixed_policy), C.byref(steps),
+                                         C.byref(res)), "refine")
+        return int(steps.value), float(res.value)
+
+    def direction_blocks(self):
+        check(self.lib.sdpx_dense_direction_blocks(self.h), "direction_blocks")
+
+    def complementarity(self, tX: float, tY: float):
+        out = np.zeros(2 * self.L)
+        check(self.lib.sdpx_dense_complementarity(self.h, tX, tY, dptr(out)), "complementarity")
+        return out[: self.L].copy(), out[self.L:].copy()
+
+    def fraction_to_boundary(self, pf: float, df: float):
+        tx, ty = C.c_double(), C.c_double()
+        check(self.lib.sdpx_dense_fraction_to_boundary(self.h, pf, df, C.byref(tx), C.byref(ty)), "ftb")
+        return float(tx.value), float(ty.value)
+
+    def backtrack(self, gamma: float, min_step: float, ratio: float):
+        tx, ty = C.c_double(), C.c_double()
+        check(self.lib.sdpx_dense_backtrack(self.h, gamma, min_step, ratio, C.byref(tx), C.byref(ty)), "backtrack")
+        return float(tx.value), float(ty.value)
+
+    def update(self, tX: float, tY: float):
+        out = np.zeros(2 + self.L)
+        check(self.lib.sdpx_dense_update(self.h, tX, tY, dptr(out)), "update")
+        return float(out[0]), bool(out[1] != 0.0), out[2:].copy()
+
+    def accepted_residuals(self, tX: float, tY: float, block_primal_residual: float):
+        out = np.zeros(2)
+        check(self.lib.sdpx_dense_accepted_residuals(self.h, tX, tY, block_primal_residual, dptr(out)),
+              "accepted_residuals")
+        return float(out[0]), float(out[1])
+
+    def objectives(self):
+        p, d = C.c_double(), C.c_double()
+        check(self.lib.sdpx_dense_objectives(self.h, C.byref(p), C.byref(d)), "objectives")
+        return float(p.value), float(d.value)
+
+    def solution_residuals(self):
+        p, d = C.c_double(), C.c_double()
+        check(self.lib.sdpx_dense_solution_residuals(self.h, C.byref(p), C.byref(d)), "solution_residuals")
+        return float(p.value), float(d.value)
+
+    def scale_blocks(self, which: int, factor: float):
+        check(self.lib.sdpx_dense_scale_blocks(self.h, which, factor), "scale_blocks")
+
+    def factor_quality(self):
+        out = np.zeros(3)
+        check(self.lib.sdpx_dense_factor_quality(self.h, dptr(out)), "factor_quality")
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def synchronize(self):
+        check(self.lib.sdpx_dense_synchronize(self.h), "synchronize")

@@ -1,0 +1,127 @@
+"""Seeded synthetic SDP generators for the dense and sparse workloads.
+
+All generators build the standard-form problem
+
+    min <C, X>  s.t.  <A_i, X> = b_i,  X >= 0
+
+with a strictly feasible primal-dual start X0 = I, y0 ~ N(0,1), Z0 = I, via
+b_i = tr(A_i) and C = sum_i y0_i A_i + I, and return it in SDPX's core
+geometric form (the lowering of the dual-form model, ir/lower_sdp.jl:244-300):
+
+    core x = y,  A_i^core = -A_i,  C^core = -C,  c = -b,
+    start: x0 = y0, X0 = Z0 = I, Y0 = X0 = I.
+
+Layouts follow the reference's dense panel (types/constraints.jl:26-28): the
+per-block panel ``Av`` is k^2 x m column-major with column i = vec(A_i).  Here
+it is returned as a C-contiguous (m, k*k) array whose row i is vec(A_i), which
+is the same bytes.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class DenseSDP:
+    """One core-form SDP with a single dense PSD block."""
+
+    k: int
+    m: int
+    Av_rows: np.ndarray  # (m, k*k) float64, row i = vec(A_i^core)
+    C: np.ndarray        # (k, k) core C
+    c: np.ndarray        # (m,) core c = -b
+    x0: np.ndarray       # (m,) = y0
+    X0: np.ndarray       # (k, k) = I
+    Y0: np.ndarray       # (k, k) = I
+
+    def A_blocks(self) -> np.ndarray:
+        """Core A as an (m, k, k) array (the layout SDPX.ingest accepts)."""
+        return self.Av_rows.reshape(self.m, self.k, self.k)
+
+
+def _symmetric_gaussian(rng: np.random.Generator, m: int, k: int, scale: float) -> np.ndarray:
+    """m symmetric k x k matrices: upper-triangle iid N(0,1)*scale, mirrored."""
+    iu = np.triu_indices(k)
+    out = np.zeros((m, k, k))
+    vals = rng.standard_normal((m, iu[0].size)) * scale
+    out[:, iu[0], iu[1]] = vals
+    out[:, iu[1], iu[0]] = vals
+    return out
+
+
+def dense_feasible(k: int, m: int, seed: int = 0) -> DenseSDP:
+    """Dense Gaussian generator of the plain FP64 workloads (k=n PSD block, m constraints).
+
+    A_i symmetric with upper-triangle entries iid N(0,1)/sqrt(n), mirrored;
+    X0 = I, b_i = tr(A_i), y0 ~ N(0,1), C = sum y0_i A_i + I.
+    """
+    rng = np.random.default_rng(seed)
+    A = _symmetric_gaussian(rng, m, k, 1.0 / np.sqrt(k))
+    y0 = rng.standard_normal(m)
+    b = np.trace(A, axis1=1, axis2=2).copy()
+    Cstd = np.tensordot(y0, A, axes=1) + np.eye(k)
+    return DenseSDP(
+        k=k,
+        m=m,
+        Av_rows=np.ascontiguousarray((-A).reshape(m, k * k)),
+        C=-Cstd,
+        c=-b,
+        x0=y0.copy(),
+        X0=np.eye(k),
+        Y0=np.eye(k),
+    )
+
+
+@dataclass
+class SparseSDP:
+    """One core-form SDP whose A_i are sparse (COO of the upper triangle)."""
+
+    k: int
+    m: int
+    rows: list   # per constraint: int32 row indices (upper triangle, row <= col)
+    cols: list
+    vals: list
+    C: np.ndarray
+    c: np.ndarray
+    x0: np.ndarray
+
+    def dense_rows(self) -> np.ndarray:
+        out = np.zeros((self.m, self.k, self.k))
+        for i in range(self.m):
+            out[i, self.rows[i], self.cols[i]] = self.vals[i]
+            out[i, self.cols[i], self.rows[i]] = self.vals[i]
+        return out.reshape(self.m, self.k * self.k)
+
+
+def sparse_feasible(k: int, m: int, seed: int) -> SparseSDP:
+    """Sparse generator of the batched workload: 3k nonzero positions per A_i.
+
+    Positions are uniform over the upper triangle (distinct), values N(0,1);
+    the mirrored matrix is symmetric. Feasible construction as in the dense
+    generator.
+    """
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(k)
+    npos = iu[0].size
+    nnz = min(3 * k, npos)
+    rows, cols, vals = [], [], []
+    b = np.zeros(m)
+    y0 = rng.standard_normal(m)
+    Cstd = np.eye(k)
+    for i in range(m):
+        pick = rng.choice(npos, size=nnz, replace=False)
+        r = iu[0][pick].astype(np.int32)
+        cc = iu[1][pick].astype(np.int32)
+        v = rng.standard_normal(nnz)
+        rows.append(r)
+        cols.append(cc)
+        vals.append(-v)  # core sign
+        diag = r == cc
+        b[i] = float(v[diag].sum())
+        np.add.at(Cstd, (r, cc), y0[i] * v)
+        off = ~diag
+        np.add.at(Cstd, (cc[off], r[off]), y0[i] * v[off])
+    return SparseSDP(k=k, m=m, rows=rows, cols=cols, vals=vals, C=-Cstd, c=-b, x0=y0)

@@ -1,0 +1,222 @@
+"""GPU parity of the dense Newton-step phases against the CPU oracle (through the C-ABI)."""
+
+import math
+
+import numpy as np
+import pytest
+
+import sdpx_oracle as O
+from sdpx_mi355x import MI355XDenseBackend, SolverOptions, ingest, solve
+from sdpx_mi355x import generators as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel_fro_lower(A, B):
+    La, Lb = np.tril(A), np.tril(B)
+    return np.linalg.norm(La - Lb) / max(np.linalg.norm(Lb), 1e-300)
+
+
+def _random_pd(rng, k, cond=10.0):
+    Q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    ev = np.exp(rng.uniform(0, math.log(cond), k))
+    return (Q * ev) @ Q.T
+
+
+def _dense_case(k, m, seed):
+    g = G.dense_feasible(k, m, seed=seed)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    return g, prob, oprob
+
+
+@pytest.mark.parametrize("k,m", [(7, 11), (50, 200), (65, 130), (130, 300)])
+def test_schur_build_matches_oracle(gpu_lib, k, m):
+    rng = np.random.default_rng(k * 1000 + m)
+    g, prob, oprob = _dense_case(k, m, seed=3)
+    X, Y = _random_pd(rng, k), _random_pd(rng, k)
+    b = MI355XDenseBackend(prob)
+    b.set_iterate(g.x0, [X], [Y])
+    _, _, ok = b.compute_residuals([0.1], affine=True, factor=True)
+    assert ok
+    b.schur_build()
+    S = b.get_schur()
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    Sref = O.schur_build(oprob, [LX], [MY])
+    assert _rel_fro_lower(S, Sref) <= 1e-12
+    b.close()
+
+
+def test_schur_identity_is_gram_and_upper_untouched(gpu_lib):
+    k, m = 40, 150
+    g, prob, _ = _dense_case(k, m, seed=5)
+    b = MI355XDenseBackend(prob)
+    sentinel = np.full((m, m), -17.0)
+    b.set_schur(sentinel)
+    b.set_iterate(g.x0, [np.eye(k)], [np.eye(k)])
+    b.compute_residuals([0.1], affine=True, factor=True)
+    b.schur_build()
+    S = b.get_schur()
+    A = g.Av_rows
+    gram = A @ A.T
+    assert _rel_fro_lower(S, gram) <= 1e-13
+    iu = np.triu_indices(m, 1)
+    assert np.all(S[iu] == -17.0), "upper triangle must not be written"
+    # determinism: a second build is bitwise identical
+    b.schur_build()
+    S2 = b.get_schur()
+    assert np.array_equal(np.tril(S), np.tril(S2))
+    b.close()
+
+
+def test_factor_solve_refine(gpu_lib):
+    k, m = 30, 257
+    rng = np.random.default_rng(0)
+    g, prob, _ = _dense_case(k, m, seed=1)
+    b = MI355XDenseBackend(prob)
+    M = rng.standard_normal((m, m))
+    S = M @ M.T + m * np.eye(m)
+    b.set_schur(S)
+    ok, attempts = b.factorize()
+    assert ok and attempts == 0
+    r = rng.standard_normal(m)
+    b.set_rhs(r)
+    b.solve_direction()
+    dx, _, _ = b.get_direction()
+    assert np.linalg.norm(S @ dx - r, np.inf) <= 1e-12 * np.linalg.norm(r, np.inf) * m
+    res, rn = b.direction_residual()
+    assert res <= 1e-11 and abs(rn - np.max(np.abs(r))) == 0.0
+    steps, res2 = b.refine(64 * 2.2e-16 * max(rn, 1.0), 8, False)
+    assert res2 <= res
+    b.close()
+
+
+def test_regularization_contract(gpu_lib):
+    """kkt_regressions.jl:927-1000: PD never shifted, -1e-6 repaired, -1 fails after 6."""
+    m = 40
+    rng = np.random.default_rng(2)
+    g, prob, _ = _dense_case(4, m, seed=2)
+    b = MI355XDenseBackend(prob)
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    base = np.linspace(1.0, 2.0, m)
+    b.set_schur((Q * base) @ Q.T)
+    assert b.factorize() == (True, 0)
+    ev = base.copy()
+    ev[0] = -1e-6
+    b.set_schur((Q * ev) @ Q.T)
+    ok, attempts = b.factorize()
+    assert ok and 1 <= attempts <= 6
+    Lo, ok_o, att_o = O.factor_dense_kkt((Q * ev) @ Q.T)
+    assert ok_o and att_o == attempts
+    ev[0] = -1.0
+    b.set_schur((Q * ev) @ Q.T)
+    assert b.factorize() == (False, 6)
+    b.close()
+
+
+def test_step_phases_match_oracle(gpu_lib):
+    """One full predictor/corrector step: residuals, rhs, directions, line search vs the oracle."""
+    k, m = 24, 60
+    g, prob, oprob = _dense_case(k, m, seed=11)
+    rng = np.random.default_rng(5)
+    X, Y = _random_pd(rng, k, 5.0), _random_pd(rng, k, 5.0)
+    x = g.x0 + 0.01 * rng.standard_normal(m)
+    mu = [0.1 * float(np.sum(X * Y)) / k]
+    b = MI355XDenseBackend(prob)
+    b.set_iterate(x, [X], [Y])
+    p_res, d_res, ok = b.compute_residuals(mu, affine=False, factor=True)
+    st = O.StepState()
+    o = O.Options(parameter_strategy="fixed")
+    p_o, d_o, ok_o = O.compute_residuals(oprob, st, x, [X], [Y], mu, o, factor=True)
+    assert ok and ok_o
+    assert abs(p_res - p_o) <= 1e-13 * max(1, p_o) and abs(d_res - d_o) <= 1e-13 * max(1, d_o)
+    b.schur_build()
+    assert b.factorize() == (True, 0)
+    st.S = O.schur_build(oprob, st.LX, st.MY)
+    st.Lf, _, _ = O.factor_dense_kkt(st.S)
+    b.predictor_rhs()
+    r = b.get_rhs()
+    r_o = -(st.d + O.rhs_v(oprob, st, [Y]))
+    assert np.max(np.abs(r - r_o)) <= 1e-11 * max(1.0, np.max(np.abs(r_o)))
+    b.solve_direction()
+    st.dx = O.solve_dense_kkt(st.Lf, r_o)
+    b.direction_blocks()
+    dx, dX, dY = b.get_direction()
+    O.direction_blocks(oprob, st, [Y])
+    assert np.max(np.abs(dx - st.dx)) <= 1e-9 * max(1.0, np.max(np.abs(st.dx)))
+    assert np.max(np.abs(dX[0] - st.dX[0])) <= 1e-9 * max(1.0, np.max(np.abs(st.dX[0])))
+    assert np.max(np.abs(dY[0] - st.dY[0])) <= 1e-9 * max(1.0, np.max(np.abs(st.dY[0])))
+    assert np.array_equal(dY[0], dY[0].T)
+    # fixed corrector
+    b.corrector_rhs(False, 0.0, 0.0, False, mu, 0.1)
+    O.corrector_R_fixed(oprob, st, [X], [Y], mu, 0.1)
+    r2 = b.get_rhs()
+    r2_o = -(st.d + O.rhs_v(oprob, st, [Y]))
+    assert np.max(np.abs(r2 - r2_o)) <= 1e-10 * max(1.0, np.max(np.abs(r2_o)))
+    # line searches on the recovered direction
+    b.solve_direction()
+    b.direction_blocks()
+    st.dx = O.solve_dense_kkt(st.Lf, r2_o)
+    O.direction_blocks(oprob, st, [Y])
+    tX, tY = b.backtrack(0.9, 1e-10, 0.0)
+    tX_o, tY_o = O.backtracking_line_search(oprob, [X], [Y], st.dX, st.dY, 0.9, 1e-10, 0.0)
+    assert (tX, tY) == (tX_o, tY_o)
+    fX, fY = b.fraction_to_boundary(1.0, 1.0)
+    fX_o, fY_o = O.fraction_line_search(oprob, [X], [Y], st.dX, st.dY, 1.0, 1.0)
+    assert abs(fX - fX_o) <= 1e-12 and abs(fY - fY_o) <= 1e-12
+    b.close()
+
+
+def test_t1_known_answer(gpu_lib):
+    """test/correctness.jl:71-94, 230-243: p* = 2 sqrt 6, 19 +- 2 iterations (fixed policy)."""
+    A = np.zeros((2, 2, 2))
+    A[0, 0, 0] = 1
+    A[1, 1, 1] = 1
+    Cm = np.array([[0.0, 1.0], [1.0, 0.0]])
+    prob = ingest([2.0, 3.0], [A], [Cm])
+    res = solve(prob, SolverOptions(parameter_strategy="fixed"))
+    assert res.status == "Optimal"
+    assert abs(res.pobj - 2 * math.sqrt(6)) < 1e-8
+    assert abs(res.iterations - 19) <= 2
+
+
+@pytest.mark.parametrize("k,m,strategy", [(50, 200, "adaptive"), (50, 200, "fixed"), (32, 128, "adaptive")])
+def test_end_to_end_matches_oracle(gpu_lib, k, m, strategy):
+    """Plain FP64 GPU path vs the oracle: status, objective 1e-8, certificate, iterations +-2."""
+    g, prob, oprob = _dense_case(k, m, seed=0)
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, parameter_strategy=strategy)
+    res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol, parameter_strategy=strategy)
+    ref = O.solve(oprob, oo, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    assert ref.status == "Optimal"
+    assert res.status == ref.status
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    assert abs(res.iterations - ref.iterations) <= 2
+    cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
+                                         res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
+    assert cert["valid"], cert
+
+
+def test_multi_block(gpu_lib):
+    """L = 3 blocks of different sizes (the shape of correctness.jl T4)."""
+    rng = np.random.default_rng(4)
+    m, ks = 30, [5, 12, 3]
+    A, Cs = [], []
+    y0 = rng.standard_normal(m)
+    for k in ks:
+        Ab = rng.standard_normal((m, k, k))
+        Ab = (Ab + Ab.transpose(0, 2, 1)) / 2
+        A.append(Ab)
+        Cs.append(np.tensordot(y0, Ab, axes=1) - np.eye(k))
+    c = np.array([sum(np.trace(A[l][i]) for l in range(3)) for i in range(m)])
+    prob = ingest(c, A, Cs)
+    oprob = O.ingest(c, A, Cs)
+    o = SolverOptions(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8)
+    X0 = [np.eye(k) for k in ks]
+    res = solve(prob, o, x0=y0, X0=X0, Y0=X0)
+    ref = O.solve(oprob, O.Options(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8), x0=y0, X0=X0, Y0=X0)
+    assert res.status == ref.status == "Optimal"
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    assert abs(res.iterations - ref.iterations) <= 2
