@@ -153,6 +153,15 @@ int sdpx_dense_scale_blocks(sdpx_dense_t* h, int which, double factor);
 /* Cholesky diagonal qualities (step.jl:233-259, 1229): out[0] primal margin,
  * out[1] dual margin, out[2] Schur factor quality. */
 int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out);
+/* Device-event timing of the Schur assembly and factor phases on the handle's
+ * stream (adds one event synchronization per timed call). */
+int sdpx_dense_set_profiling(sdpx_dense_t* h, int enabled);
+/* out[0] Schur Gram kernel ms (sum), out[1] its launch count, out[2] panel
+ * (L_X^-1 A_i M_Y) ms, out[3] split-K reduction ms, out[4] factor ms (all
+ * attempts), out[5] factor calls; reset != 0 clears the counters. */
+int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out);
+/* split-K factor used by the Schur Gram kernel of this handle. */
+int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 

@@ -57,6 +57,7 @@ enum GemmFlags : unsigned {
   kGemmAUpper = 1u << 3,    // op(A) is upper triangular: kk >= i
   kGemmTransA = 1u << 8,    // op(A) = A'
   kGemmTransB = 1u << 9,    // op(B) = B'
+  kGemmSchurGram = 1u << 10,  // route to the dedicated Schur Gram kernel (TN, 128 x 128, lower output)
 };
 
 struct GemmArgs {
@@ -77,7 +78,8 @@ struct GemmArgs {
 // Launches the GEMM on `stream`. `split_k` > 1 requires `workspace` of
 // split_k * M * N * batch doubles; partial sums are reduced in a fixed order
 // (deterministic). Returns kOk / kHipError.
-int gemm_f64(const GemmArgs& g, hipStream_t stream, int split_k = 1, double* workspace = nullptr);
+int gemm_f64(const GemmArgs& g, hipStream_t stream, int split_k = 1, double* workspace = nullptr,
+             hipEvent_t after_main = nullptr);
 
 // ---------------------------------------------------------------------------
 // Dense Cholesky and triangular solves (chol_f64.hip)

@@ -197,8 +197,12 @@ struct sdpx_dense {
   int* dok = nullptr;
   long long bytes = 0;
   std::vector<void*> allocs;
-  int sinfo_slot = 0;
   int last_reg_attempts = 0;
+  // device-event kernel timing (sdpx_dense_set_profiling)
+  bool prof = false;
+  hipEvent_t ev[6] = {};
+  double t_panels = 0.0, t_gram = 0.0, t_reduce = 0.0, t_factor = 0.0;
+  long long n_gram = 0, n_factor = 0;
 };
 
 namespace {
@@ -443,6 +447,8 @@ int sdpx_dense_destroy(sdpx_dense_t* h) {
   if (!h) return kOk;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& e : h->ev)
+    if (e) hipEventDestroy(e);
   for (void* p : h->allocs) hipFree(p);
   if (h->hsc) hipHostFree(h->hsc);
   if (h->hinfo) hipHostFree(h->hinfo);
@@ -509,8 +515,7 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
   if ((s = dalloc(h, &h->Ph, (size_t)(maxk2 * m)))) return fail(s);
   // split-K for the Schur Gram product: aim for >= 1024 work-groups
   {
-    const int bm = (m <= 256 && mm2 < 256LL * 128 * 128) ? 64 : 128;
-    const long long nt = ceil_div(m, bm);
+    const long long nt = ceil_div(m, 128);
     const long long tiles = nt * (nt + 1) / 2;
     const long long stages = ceil_div(maxk2, 16);
     int split = (int)std::max<long long>(1, std::min<long long>(16, ceil_div(1024, tiles)));
@@ -711,6 +716,7 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
   for (int l = 0; l < h->L; ++l) {
     Block& b = h->blk[l];
     const int k = b.k;
+    if (h->prof) SDPX_HIP(hipEventRecord(h->ev[0], h->stream));
     // T = L_X^-1 [A_1 ... A_m]   (k x km, op(A) lower triangular)
     SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, b.Av, k, 0.0, h->T, k, kGemmALower), h->stream));
     // Phat_i = T_i M_Y           (batched k x k, op(B) lower triangular)
@@ -722,8 +728,21 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     SDPX_CHECK(gemm_f64(g, h->stream));
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
-                      kGemmTransA | kGemmLowerOut);
-    SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws));
+                      kGemmTransA | kGemmLowerOut | kGemmSchurGram);
+    if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
+    SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws, h->prof ? h->ev[2] : nullptr));
+    if (h->prof) {
+      SDPX_HIP(hipEventRecord(h->ev[3], h->stream));
+      SDPX_HIP(hipEventSynchronize(h->ev[3]));
+      float a = 0.f, g = 0.f, r = 0.f;
+      SDPX_HIP(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+      SDPX_HIP(hipEventElapsedTime(&g, h->ev[1], h->ev[2]));
+      SDPX_HIP(hipEventElapsedTime(&r, h->ev[2], h->ev[3]));
+      h->t_panels += a;
+      h->t_gram += g;
+      h->t_reduce += r;
+      h->n_gram += 1;
+    }
   }
   return kOk;
 }
@@ -734,6 +753,7 @@ int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts) {
   int attempts = 0;
   double reg = 0.0;
   bool ok = false;
+  if (h->prof) SDPX_HIP(hipEventRecord(h->ev[4], h->stream));
   while (true) {
     SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
     SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
@@ -745,7 +765,47 @@ int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts) {
   }
   h->last_reg_attempts = attempts;
   if (reg_attempts) *reg_attempts = attempts;
+  if (h->prof) {
+    SDPX_HIP(hipEventRecord(h->ev[5], h->stream));
+    SDPX_HIP(hipEventSynchronize(h->ev[5]));
+    float f = 0.f;
+    SDPX_HIP(hipEventElapsedTime(&f, h->ev[4], h->ev[5]));
+    h->t_factor += f;
+    h->n_factor += 1;
+  }
   return ok ? kOk : kNotPositiveDefinite;
+}
+
+int sdpx_dense_set_profiling(sdpx_dense_t* h, int enabled) {
+  SDPX_CHECK(check_handle(h));
+  if (enabled && !h->ev[0]) {
+    for (auto& e : h->ev) SDPX_HIP(hipEventCreate(&e));
+  }
+  h->prof = enabled != 0;
+  return kOk;
+}
+
+int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out) {
+  SDPX_CHECK(check_handle(h));
+  if (out) {
+    out[0] = h->t_gram;
+    out[1] = (double)h->n_gram;
+    out[2] = h->t_panels;
+    out[3] = h->t_reduce;
+    out[4] = h->t_factor;
+    out[5] = (double)h->n_factor;
+  }
+  if (reset) {
+    h->t_gram = h->t_panels = h->t_reduce = h->t_factor = 0.0;
+    h->n_gram = h->n_factor = 0;
+  }
+  return kOk;
+}
+
+int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
+  SDPX_CHECK(check_handle(h));
+  if (split) *split = h->split;
+  return kOk;
 }
 
 int sdpx_dense_predictor_rhs(sdpx_dense_t* h) {

@@ -23,7 +23,7 @@ constexpr int PAD = 16;  // row stride = BM + 16 doubles: conflict-free 64-bit f
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g, int split, double* ws) {
+__device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
@@ -196,7 +196,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g, int split, do
     }
   }
 
-  // Epilogue: acc[i][j][r] = C(row = m0+wm0+16i+lane%16, col = n0+wn0+16j+4*(lane/16)+r)
+  // Epilogue.  The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in
+  // register r of lane `lane`; with the operands swapped D = C', so
+  // acc[i][j][r] = C(row = m0+wm0+16i+lane%16, col = n0+wn0+16j+lane/16+4r).
   double* C = g.C + (long long)b * g.strideC;
   double* W = (split > 1) ? ws + ((long long)b * split + s) * (long long)g.M * g.N : nullptr;
 #pragma unroll
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g, int split, do
     for (int j = 0; j < FN; ++j) {
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int col = n0 + wn0 + j * 16 + 4 * (lane >> 4) + rr;
+        const int col = n0 + wn0 + j * 16 + (lane >> 4) + 4 * rr;
         if (row < g.M && col < g.N && (!lower_out || row >= col)) {
           const double v = acc[i][j][rr];
           if (W) {
@@ -241,13 +243,29 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, int spli
 }
 
 template <int BM, int BN, bool TA, bool TB>
-int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws) {
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g, int split, double* ws) {
+  gemm_f64_body<BM, BN, TA, TB>(g, split, ws);
+}
+
+// The Schur Gram product S = Phat' Phat (lower tiles) under its own symbol,
+// so profiles attribute the dominant kernel of the Newton step unambiguously.
+__global__ __launch_bounds__(256) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
+  gemm_f64_body<128, 128, true, false>(g, split, ws);
+}
+
+template <int BM, int BN, bool TA, bool TB>
+int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEvent_t after_main = nullptr) {
   const int mt = ceil_div(g.M, BM), nt = ceil_div(g.N, BN);
   const long long tiles = (g.flags & kGemmLowerOut) ? (long long)mt * (mt + 1) / 2 : (long long)mt * nt;
   if (tiles <= 0) return kOk;
   dim3 grid((unsigned)tiles, (unsigned)(g.batch * split));
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB>), grid, dim3(256), 0, stream, g, split, ws);
+  if (g.flags & kGemmSchurGram) {
+    hipLaunchKernelGGL(schur_gram_f64_kernel, grid, dim3(256), 0, stream, g, split, ws);
+  } else {
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB>), grid, dim3(256), 0, stream, g, split, ws);
+  }
   SDPX_HIP(hipGetLastError());
+  if (after_main) SDPX_HIP(hipEventRecord(after_main, stream));
   if (split > 1) {
     const long long MN = (long long)g.M * g.N;
     dim3 rg((unsigned)std::min<long long>(ceil_div(MN, 256), 4096), (unsigned)g.batch);
@@ -267,8 +285,23 @@ int dispatch_trans(const GemmArgs& g, hipStream_t stream, int split, double* ws,
 
 }  // namespace
 
-int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* workspace) {
+int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* workspace, hipEvent_t after_main) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return kOk;
+  if (g0.flags & kGemmSchurGram) {
+    // S = Phat' Phat: 128 x 128 lower tiles, op(A) transposed
+    if (!(g0.flags & kGemmTransA) || (g0.flags & kGemmTransB) || !(g0.flags & kGemmLowerOut)) {
+      set_last_error("gemm_f64: the Schur Gram kernel is TN with lower output");
+      return kInvalidArgument;
+    }
+    GemmArgs g = g0;
+    g.flags &= ~(kGemmTransA | kGemmTransB);
+    if (split_k < 1) split_k = 1;
+    if (split_k > 1 && workspace == nullptr) {
+      set_last_error("gemm_f64: split_k > 1 needs a workspace");
+      return kInvalidArgument;
+    }
+    return launch<128, 128, true, false>(g, stream, split_k, workspace, after_main);
+  }
   GemmArgs g = g0;
   const bool ta = (g.flags & kGemmTransA) != 0;
   const bool tb = (g.flags & kGemmTransB) != 0;

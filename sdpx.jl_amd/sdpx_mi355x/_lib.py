@@ -52,6 +52,9 @@ SIGNATURES = {
     "sdpx_dense_scale_blocks": [_h, C.c_int, C.c_double],
     "sdpx_dense_factor_quality": [_h, _dp],
     "sdpx_dense_synchronize": [_h],
+    "sdpx_dense_set_profiling": [_h, C.c_int],
+    "sdpx_dense_kernel_times": [_h, C.c_int, _dp],
+    "sdpx_dense_schur_split": [_h, _ip],
 }
 _RESTYPES = {"sdpx_last_error": C.c_char_p}
 

@@ -235,5 +235,20 @@ class MI355XDenseBackend:
         check(self.lib.sdpx_dense_factor_quality(self.h, dptr(out)), "factor_quality")
         return float(out[0]), float(out[1]), float(out[2])
 
+    def set_profiling(self, enabled: bool):
+        check(self.lib.sdpx_dense_set_profiling(self.h, int(enabled)), "set_profiling")
+
+    def kernel_times(self, reset: bool = False) -> dict:
+        """Device-event milliseconds of the Schur Gram kernel, panels, reduction and factor."""
+        out = np.zeros(6)
+        check(self.lib.sdpx_dense_kernel_times(self.h, int(reset), dptr(out)), "kernel_times")
+        return dict(gram_ms=out[0], gram_launches=int(out[1]), panels_ms=out[2], reduce_ms=out[3],
+                    factor_ms=out[4], factor_calls=int(out[5]))
+
+    def schur_split(self) -> int:
+        s = C.c_int(0)
+        check(self.lib.sdpx_dense_schur_split(self.h, C.byref(s)), "schur_split")
+        return int(s.value)
+
     def synchronize(self):
         check(self.lib.sdpx_dense_synchronize(self.h), "synchronize")

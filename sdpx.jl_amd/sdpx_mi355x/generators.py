@@ -59,14 +59,25 @@ def dense_feasible(k: int, m: int, seed: int = 0) -> DenseSDP:
     X0 = I, b_i = tr(A_i), y0 ~ N(0,1), C = sum y0_i A_i + I.
     """
     rng = np.random.default_rng(seed)
-    A = _symmetric_gaussian(rng, m, k, 1.0 / np.sqrt(k))
+    # iid N(0,1)/sqrt(k) on and above the diagonal, mirrored below; built in
+    # one (m, k, k) buffer in chunks so multi-GB panels need no extra copies
+    A = np.empty((m, k, k))
+    scale = 1.0 / np.sqrt(k)
+    lower = np.tril_indices(k, -1)
+    chunk = max(1, (1 << 27) // (k * k))
+    for i0 in range(0, m, chunk):
+        blk = A[i0:i0 + chunk]
+        rng.standard_normal(out=blk)
+        blk *= scale
+        blk[:, lower[0], lower[1]] = blk[:, lower[1], lower[0]]
     y0 = rng.standard_normal(m)
     b = np.trace(A, axis1=1, axis2=2).copy()
-    Cstd = np.tensordot(y0, A, axes=1) + np.eye(k)
+    Cstd = (y0 @ A.reshape(m, k * k)).reshape(k, k) + np.eye(k)
+    np.negative(A, out=A)  # core sign: A_core = -A
     return DenseSDP(
         k=k,
         m=m,
-        Av_rows=np.ascontiguousarray((-A).reshape(m, k * k)),
+        Av_rows=A.reshape(m, k * k),
         C=-Cstd,
         c=-b,
         x0=y0.copy(),

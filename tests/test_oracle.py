@@ -1,0 +1,181 @@
+"""Pins of the CPU oracle against the reference's known answers, closed forms and brute force."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import sdpx_oracle as O
+from _helpers import golden, owner_problem, random_pd, regularization_matrix, rel_fro_lower
+from sdpx_mi355x import generators as G
+
+
+def test_t1_known_answer_and_iteration_pin():
+    """test/correctness.jl:71-94 and the T6 pin (19 iterations, fixed policy)."""
+    g = golden()
+    prob = O.ingest(g["t1_c"], [g["t1_A"]], [g["t1_C"]])
+    r = O.solve(prob, O.Options(parameter_strategy="fixed"))
+    assert r.status == "Optimal"
+    assert abs(r.pobj - float(g["t1_popt"])) < 1e-8
+    assert abs(r.pobj - r.dobj) < 1e-8
+    assert abs(r.iterations - int(g["t1_iterations_fixed"])) <= 2
+    assert all(np.all(np.linalg.eigvalsh(b + 1e-8 * np.eye(2)) > 0) for b in r.X)
+    ra = O.solve(prob, O.Options())
+    assert ra.status == "Optimal" and abs(ra.pobj - float(g["t1_popt"])) < 1e-8
+
+
+@pytest.mark.parametrize("n", range(4))
+def test_schur_brute_force_tiny(n):
+    g = golden()
+    A, X, Y, S = g[f"tiny{n}_A"], g[f"tiny{n}_X"], g[f"tiny{n}_Y"], g[f"tiny{n}_S"]
+    m = A.shape[0]
+    prob = O.ingest(np.zeros(m), [A], [np.zeros_like(X)])
+    Ss = O.schur_build(prob, [O.kchol(X)[0]], [O.kchol(Y)[0]])
+    assert rel_fro_lower(Ss, S) <= 1e-13
+
+
+def test_schur_two_block_fixture():
+    """kkt_regressions.jl:488-565 data, expected S by 50-digit brute force."""
+    g = golden()
+    prob = O.ingest(np.zeros(4), [g["kkt2_A1"], g["kkt2_A2"]], [np.zeros((2, 2))] * 2)
+    X = [g["kkt2_X1"], g["kkt2_X2"]]
+    Y = [g["kkt2_Y1"], g["kkt2_Y2"]]
+    S = O.schur_build(prob, [O.kchol(b)[0] for b in X], [O.kchol(b)[0] for b in Y])
+    assert rel_fro_lower(S, g["kkt2_S"]) <= 1e-14
+
+
+def test_schur_owner_problem_closed_form():
+    A, C, c, S_ref = owner_problem()
+    prob = O.ingest(c, A, C)
+    I6 = np.eye(6)
+    S = O.schur_build(prob, [I6] * 8, [I6] * 8)
+    assert np.max(np.abs(np.tril(S - S_ref))) / max(np.max(np.abs(S_ref)), 1.0) < 1e-12
+
+
+def test_planted_transpose_is_caught(monkeypatch):
+    """A plausible slip (M_Y transposed) must fail the brute-force pin."""
+    g = golden()
+    A, X, Y, S = g["tiny2_A"], g["tiny2_X"], g["tiny2_Y"], g["tiny2_S"]
+    prob = O.ingest(np.zeros(A.shape[0]), [A], [np.zeros_like(X)])
+    bad = O.schur_build(prob, [O.kchol(X)[0]], [O.kchol(Y)[0].T])
+    assert rel_fro_lower(bad, S) > 1e-3
+
+
+def test_kchol_matches_lapack_and_closed_forms():
+    rng = np.random.default_rng(0)
+    for k in (1, 2, 3, 17):
+        A = random_pd(rng, k)
+        Lk, ok = O.kchol(A)
+        assert ok
+        assert np.allclose(Lk, np.linalg.cholesky(A), rtol=1e-14, atol=1e-14)
+    assert not O.kchol(np.array([[1.0, 2.0], [2.0, 1.0]]))[1]
+    assert not O.kchol(np.array([[-1.0]]))[1]
+    assert not O.kchol(-np.eye(3))[1]
+
+
+def test_regularization_contract():
+    """kkt_regressions.jl:927-1000 on the oracle."""
+    m = 12
+    for smallest in (1.0, 1e-8, 1e-14):
+        _, ok, att = O.factor_dense_kkt(regularization_matrix(m, smallest))
+        assert ok and att == 0
+    for smallest in (0.0, -1e-14, -1e-8):
+        _, ok, att = O.factor_dense_kkt(regularization_matrix(m, smallest))
+        assert ok and 0 <= att <= 6
+    _, ok, att = O.factor_dense_kkt(regularization_matrix(m, -1e-6))
+    assert ok and 1 <= att <= 6
+    _, ok, att = O.factor_dense_kkt(regularization_matrix(m, -1.0))
+    assert not ok and att == 6
+
+
+def test_solve_and_refine_identities():
+    rng = np.random.default_rng(1)
+    m = 40
+    M = rng.standard_normal((m, m))
+    S = M @ M.T + np.eye(m)
+    Lf, ok, _ = O.factor_dense_kkt(S)
+    r = rng.standard_normal(m)
+    dx = O.solve_dense_kkt(Lf, r)
+    assert np.allclose(S @ dx, r, rtol=1e-11, atol=1e-11)
+    res, rho = O.direction_residual(S, r, dx)
+    assert np.allclose(rho, r - S @ dx)
+    dx2, steps, res2 = O.refine_direction(S, Lf, r, dx + 1e-6 * rng.standard_normal(m), O.Options())
+    assert steps >= 1 and res2 < 1e-9
+
+
+def test_fraction_to_boundary_matches_eigen_bound():
+    rng = np.random.default_rng(2)
+    for k in (3, 8, 20):
+        X = random_pd(rng, k)
+        dX = rng.standard_normal((k, k))
+        dX = 3 * (dX + dX.T)
+        Lx = np.linalg.cholesky(X)
+        W = sla.solve_triangular(Lx, sla.solve_triangular(Lx, dX, lower=True).T, lower=True)
+        lam = np.linalg.eigvalsh((W + W.T) / 2).min()
+        exact = min(1.0, -1.0 / lam) if lam < 0 else 1.0
+        t = O.fraction_to_boundary_bound(X, dX)
+        assert abs(t - exact) <= 1e-10 * max(exact, 1e-300)
+    # 2 x 2 closed form vs the determinant root
+    X = np.array([[1.0, 0.2], [0.2, 0.5]])
+    dX = np.array([[-2.0, 0.3], [0.3, 0.1]])
+    t = O.fraction_to_boundary_bound(X, dX)
+    assert abs(np.linalg.det(X + t * dX)) < 1e-12 or abs((X + t * dX)[0, 0]) < 1e-12
+
+
+def test_backtracking_is_first_accepted_power():
+    rng = np.random.default_rng(3)
+    k = 6
+    X = random_pd(rng, k)
+    dX = -2.5 * X + 0.1 * rng.standard_normal((k, k))
+    dX = (dX + dX.T) / 2
+    tX, _ = O.backtracking_line_search(O.Problem(np.zeros(1), [np.eye(k)], [np.zeros((k * k, 1))], [k]),
+                                       [X], [X], [dX], [dX], 0.9, 1e-10, 0.0)
+    assert O.trial_isposdef(X, tX, dX)
+    assert not O.trial_isposdef(X, tX / 0.9, dX)
+
+
+@pytest.mark.parametrize("k,m", [(20, 60), (50, 200)])
+def test_dense_generator_solves_with_certificate(k, m):
+    """cfg0 shape: strictly feasible start, optimal with a valid certificate, strong duality."""
+    g = G.dense_feasible(k, m, seed=0)
+    prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    tol = 1e-8
+    r = O.solve(prob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    assert r.status == "Optimal"
+    cert = O.certificate(prob, r, tol, tol, tol)
+    assert cert["valid"], cert
+    assert abs(r.pobj - r.dobj) <= 1e-7 * max(1.0, abs(r.pobj))
+    assert r.iterations < 40
+
+
+def test_generator_start_is_feasible():
+    g = G.dense_feasible(10, 25, seed=4)
+    prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    p, d = O.solution_residuals(prob, g.x0, [g.X0], [g.Y0])
+    assert p < 1e-12 and d < 1e-12
+
+
+def test_stagnation_detector():
+    s = O.Stagnation(3, 1e-3)
+    for merit in (10.0, 10.0, 10.0):
+        assert not s.observe(merit, False, 100)
+    assert s.observe(10.0, False, 100) and s.reason == "no_progress"
+    s = O.Stagnation(3, 1e-3)
+    for merit in (1e4, 1e3, 1e2, 10.0, 1.0):
+        assert not s.observe(merit, False, 100)
+
+
+def test_adaptive_policy_cold_start_uses_fallback_values():
+    o = O.Options()
+    p = O.AdaptivePolicy.from_options(o)
+    d = O.Diagnostics(primal_residual=2.0, dual_residual=0.1, relative_gap=0.5, mu=1.0, mu_aff=0.5,
+                      affine_primal_step=0.5, affine_dual_step=0.5)
+    par = O.select_adaptive_parameters(p, d, [])
+    assert par.sigma == o.beta and par.backtracking_factor == o.gamma and not par.fallback
+    d.primal_residual = 0.1
+    d.iteration = 3
+    par = O.select_adaptive_parameters(p, d, [])
+    # ratio 0.5, affine 0.5 -> sigma = clamp(0.8*max(0.25, 0.08) + 0.2*0.1)
+    assert math.isclose(par.sigma, 0.8 * 0.25 + 0.2 * 0.1)
+    assert par.primal_fraction_to_boundary == 19 / 20
