@@ -182,6 +182,13 @@ __global__ void symmetrize_kernel(double* A, int n) {
   A[c + (long long)r * n] = avg;
 }
 
+__global__ void zero_upper_kernel(double* A, int n, long long ld) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  if (r < c) A[r + (long long)c * ld] = 0.0;
+}
+
 // dst lower = src lower (+ shift * max(|src_ii|, 1) on the diagonal)
 __global__ void copy_lower_shift_kernel(const double* src, double* dst, int n, double shift) {
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -332,6 +339,12 @@ int add_diag_f64(double* A, int n, long long ld, double v, const double* vdev, d
 
 int symmetrize_f64(double* A, int n, hipStream_t stream) {
   hipLaunchKernelGGL(symmetrize_kernel, dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, stream, A, n);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+int zero_upper_f64(double* A, int n, long long ld, hipStream_t stream) {
+  hipLaunchKernelGGL(zero_upper_kernel, dim3(ceil_div((long long)n * n, 256)), dim3(256), 0, stream, A, n, ld);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }

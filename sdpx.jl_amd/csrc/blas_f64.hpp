@@ -24,6 +24,8 @@ int fill_f64(long long n, double v, double* z, hipStream_t stream);
 // A_ii += v   (or vdev_scale * *vdev when vdev is non-null)
 int add_diag_f64(double* A, int n, long long ld, double v, const double* vdev, double vdev_scale, hipStream_t stream);
 int symmetrize_f64(double* A, int n, hipStream_t stream);
+// strict upper triangle of the n x n matrix A := 0
+int zero_upper_f64(double* A, int n, long long ld, hipStream_t stream);
 // dst lower = src lower, diagonal += shift * max(|src_ii|, 1)
 int copy_lower_shift_f64(const double* src, double* dst, int n, double shift, hipStream_t stream);
 // y = alpha S x + beta y0 from the lower triangle of S; part needs ceil(n/64) * n doubles

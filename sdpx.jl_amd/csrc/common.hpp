@@ -52,6 +52,8 @@ inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b -
 // ---------------------------------------------------------------------------
 enum GemmFlags : unsigned {
   kGemmLowerOut = 1u << 0,  // only C(i,j) with i >= j is computed / written (M == N)
+  // Triangular hints only shorten the K range of a tile (to BK granularity);
+  // the operand must hold explicit zeros in its other triangle.
   kGemmALower = 1u << 1,    // op(A) is lower triangular: kk <= i
   kGemmBLower = 1u << 2,    // op(B) is lower triangular: kk >= j
   kGemmAUpper = 1u << 3,    // op(A) is upper triangular: kk >= i

@@ -693,9 +693,11 @@ int sdpx_dense_residuals(sdpx_dense_t* h, const double* mu, int affine, int fact
   bool ok = true;
   for (int i = 0; i < nint; ++i) ok = ok && h->hinfo[i] == 0;
   if (factor && ok) {
-    // L_X^-1 and X^-1 = L_X^-T L_X^-1 for the Schur panels and X^-1 M products
+    // L_X^-1 and X^-1 = L_X^-T L_X^-1 for the Schur panels and X^-1 M products;
+    // M_Y is used as a full matrix operand, so its strict upper triangle is cleared
     for (int l = 0; l < L; ++l) {
       Block& b = h->blk[l];
+      SDPX_CHECK(zero_upper_f64(b.MY, b.k, b.k, h->stream));
       SDPX_CHECK(trtri_lower_f64(b.LX, b.k, b.k, b.dinvX, b.LXinv, nullptr, h->stream));
       SDPX_CHECK(gemm_f64(gemm(b.k, b.k, b.k, 1.0, b.LXinv, b.k, b.LXinv, b.k, 0.0, b.Xinv, b.k,
                                kGemmTransA | kGemmAUpper | kGemmBLower),

@@ -8,28 +8,136 @@
 //
 // Work-group tile BM x BN, 256 threads = 4 waves in a 2 x 2 grid, each wave
 // owning a (BM/2) x (BN/2) sub-tile built from 16 x 16 MFMA blocks.  K is
-// staged through LDS in slices of BK = 16 with a register prefetch of the
-// next slice.  The MFMA is fed with (B-fragment, A-fragment) so the
-// accumulator holds C transposed: lane%16 indexes a row of C, which makes
-// the column-major epilogue stores 128-byte contiguous.
+// staged through double-buffered LDS in slices of BK = 16: the next slice is
+// fetched into registers (branch-free, 16-byte loads when the operand is
+// aligned) while the current one feeds the MFMAs, so each slice costs one
+// barrier.  The MFMA is fed with (B-fragment, A-fragment) so the accumulator
+// holds C transposed: lane%16 indexes a row of C, which makes the
+// column-major epilogue stores 128-byte contiguous.
 #include "common.hpp"
 
 namespace sdpx {
 namespace {
 
 constexpr int BK = 16;
-constexpr int PAD = 16;  // row stride = BM + 16 doubles: conflict-free 64-bit fragment reads
+constexpr int PAD = 16;  // row stride = R + 16 doubles: conflict-free 64-bit fragment reads
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef double double2_t __attribute__((ext_vector_type(2)));
 
-template <int BM, int BN, bool TA, bool TB>
+// Coordinates (r, kk) inside an R x BK operand tile of element (e, h) owned by
+// thread tid.  KC: the operand is contiguous along K in memory; VEC: pairs of
+// consecutive elements along the contiguous dimension are fetched together.
+template <int R, bool KC, bool VEC>
+__device__ __forceinline__ void tile_coords(int tid, int e, int h, int& r, int& kk) {
+  if (KC) {
+    if (VEC) {
+      kk = (tid % (BK / 2)) * 2 + h;
+      r = tid / (BK / 2) + (256 / (BK / 2)) * e;
+    } else {
+      kk = tid % BK;
+      r = tid / BK + (256 / BK) * e;
+    }
+  } else {
+    if (VEC) {
+      r = (tid % (R / 2)) * 2 + h;
+      kk = tid / (R / 2) + (512 / R) * e;
+    } else {
+      r = tid % R;
+      kk = tid / R + (256 / R) * e;
+    }
+  }
+}
+
+// Loads one R x BK operand slice into registers.  Rows outside the matrix are
+// clamped once onto its last row (their products only reach C entries that
+// the epilogue discards), so full slices need no per-element guard: the main
+// K loop is a single basic block of loads, MFMAs and one barrier.  Only the
+// final partial slice (K not a multiple of BK) masks along K.
+template <int R, bool KC, bool VEC>
+struct TileLoader {
+  static constexpr int NE = R * BK / 256;       // doubles per thread
+  static constexpr int NL = VEC ? NE / 2 : NE;  // loads per thread
+  long long rowoff[NL];                         // offset of (clamped row, kk = element's k)
+
+  __device__ __forceinline__ void init(long long ld, int r0, int rext, int tid) {
+    const int rmax = rext - ((VEC && !KC) ? 2 : 1);
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
+      int r, k;
+      tile_coords<R, KC, VEC>(tid, e, 0, r, k);
+      const int gr = min(r0 + r, rmax);
+      rowoff[e] = KC ? (long long)gr * ld + k : (long long)gr + (long long)k * ld;
+    }
+  }
+
+  __device__ __forceinline__ void load(double* v, const double* __restrict__ P, long long ld, int k0) const {
+    const double* Pk = P + (KC ? (long long)k0 : (long long)k0 * ld);
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
+      if (VEC) {
+        const double2_t p = *reinterpret_cast<const double2_t*>(Pk + rowoff[e]);
+        v[2 * e] = p[0];
+        v[2 * e + 1] = p[1];
+      } else {
+        v[e] = Pk[rowoff[e]];
+      }
+    }
+  }
+
+  // partial slice: elements with k >= ke read as zero (and are never fetched)
+  __device__ __forceinline__ void load_tail(double* v, const double* __restrict__ P, long long ld, int k0, int ke,
+                                            int tid) const {
+#pragma unroll
+    for (int e = 0; e < NL; ++e) {
+      int r, k;
+      tile_coords<R, KC, VEC>(tid, e, 0, r, k);
+      const int gk = k0 + k;
+      const long long rowbase = KC ? rowoff[e] - k : rowoff[e] - (long long)k * ld;
+      const bool in = gk < ke;
+      const long long off = in ? rowoff[e] + (KC ? (long long)k0 : (long long)k0 * ld) : rowbase;
+      if (VEC) {
+        const double2_t p = *reinterpret_cast<const double2_t*>(P + off);
+        v[2 * e] = in ? p[0] : 0.0;
+        v[2 * e + 1] = (KC ? gk + 1 < ke : in) ? p[1] : 0.0;
+      } else {
+        const double x = P[off];
+        v[e] = in ? x : 0.0;
+      }
+    }
+  }
+};
+
+template <int R, bool KC, bool VEC>
+__device__ __forceinline__ void tile_store(const double* v, double (*S)[R + PAD], int tid) {
+  constexpr int NE = R * BK / 256;
+  if (VEC) {
+#pragma unroll
+    for (int e = 0; e < NE / 2; ++e)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        int r, kk;
+        tile_coords<R, KC, VEC>(tid, e, h, r, kk);
+        S[kk][r] = v[2 * e + h];
+      }
+  } else {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      int r, kk;
+      tile_coords<R, KC, VEC>(tid, e, 0, r, kk);
+      S[kk][r] = v[e];
+    }
+  }
+}
+
+template <int BM, int BN, bool TA, bool TB, bool VEC>
 __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
   constexpr int B_ELEMS = BN * BK / 256;
-  __shared__ double As[BK][BM + PAD];
-  __shared__ double Bs[BK][BN + PAD];
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -48,8 +156,8 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   const int bid = blockIdx.x;
   const int xcd = bid % kNumXcd;
   const int slot = bid / kNumXcd;
-  const int q = total / kNumXcd, r = total % kNumXcd;
-  const int tile = xcd * q + (xcd < r ? xcd : r) + slot;
+  const int q = total / kNumXcd, rq = total % kNumXcd;
+  const int tile = xcd * q + (xcd < rq ? xcd : rq) + slot;
 
   int ti, tj;
   const bool lower_out = (g.flags & kGemmLowerOut) != 0;
@@ -84,7 +192,6 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
     kBegin = sb;
     kEnd = se;
   }
-
   double4_t acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -92,108 +199,66 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
     for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
   double ra[A_ELEMS], rb[B_ELEMS];
-
-  auto load_a = [&](int k0) {
-#pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      int i, kk;
-      if (!TA) {
-        i = tid % BM;
-        kk = tid / BM + e * (256 / BM);
-      } else {
-        kk = tid % BK;
-        i = tid / BK + e * (256 / BK);
-      }
-      const int gi = m0 + i, gk = k0 + kk;
-      bool ok = gi < g.M && gk < kEnd && gk >= kBegin;
-      if ((g.flags & kGemmALower) && gk > gi) ok = false;
-      if ((g.flags & kGemmAUpper) && gk < gi) ok = false;
-      double v = 0.0;
-      if (ok) v = TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
-      ra[e] = v;
+  // op(A)(i, kk): contiguous along K when TA;  op(B)(kk, j): contiguous along K when !TB
+  TileLoader<BM, TA, VEC> la;
+  TileLoader<BN, !TB, VEC> lb;
+  la.init(g.lda, m0, g.M, tid);
+  lb.init(g.ldb, n0, g.N, tid);
+  auto load = [&](int k0) {
+    if (k0 + BK <= kEnd) {
+      la.load(ra, A, g.lda, k0);
+      lb.load(rb, B, g.ldb, k0);
+    } else {
+      la.load_tail(ra, A, g.lda, k0, kEnd, tid);
+      lb.load_tail(rb, B, g.ldb, k0, kEnd, tid);
     }
   };
-  auto store_a = [&]() {
-#pragma unroll
-    for (int e = 0; e < A_ELEMS; ++e) {
-      int i, kk;
-      if (!TA) {
-        i = tid % BM;
-        kk = tid / BM + e * (256 / BM);
-      } else {
-        kk = tid % BK;
-        i = tid / BK + e * (256 / BK);
-      }
-      As[kk][i] = ra[e];
-    }
+  auto store = [&](int buf) {
+    tile_store<BM, TA, VEC>(ra, As[buf], tid);
+    tile_store<BN, !TB, VEC>(rb, Bs[buf], tid);
   };
-  auto load_b = [&](int k0) {
+  auto compute = [&](int cur) {
 #pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      int j, kk;
-      if (TB) {
-        j = tid % BN;
-        kk = tid / BN + e * (256 / BN);
-      } else {
-        kk = tid % BK;
-        j = tid / BK + e * (256 / BK);
-      }
-      const int gj = n0 + j, gk = k0 + kk;
-      bool ok = gj < g.N && gk < kEnd && gk >= kBegin;
-      if ((g.flags & kGemmBLower) && gk < gj) ok = false;
-      double v = 0.0;
-      if (ok) v = TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb];
-      rb[e] = v;
-    }
-  };
-  auto store_b = [&]() {
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double fa[FM], fb[FN];
 #pragma unroll
-    for (int e = 0; e < B_ELEMS; ++e) {
-      int j, kk;
-      if (TB) {
-        j = tid % BN;
-        kk = tid / BN + e * (256 / BN);
-      } else {
-        kk = tid % BK;
-        j = tid / BK + e * (256 / BK);
-      }
-      Bs[kk][j] = rb[e];
+      for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
     }
   };
 
   if (kBegin < kEnd) {
-    load_a(kBegin);
-    load_b(kBegin);
-    store_a();
-    store_b();
+    load(kBegin);
+    store(0);
     __syncthreads();
-    for (int k0 = kBegin; k0 < kEnd; k0 += BK) {
-      const bool more = k0 + BK < kEnd;
-      if (more) {
-        load_a(k0 + BK);
-        load_b(k0 + BK);
-      }
-#pragma unroll
-      for (int ks = 0; ks < BK / 4; ++ks) {
-        const int kr = ks * 4 + (lane >> 4);
-        double fa[FM], fb[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) fa[i] = As[kr][wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) fb[j] = Bs[kr][wn0 + j * 16 + (lane & 15)];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
-      }
+    int cur = 0;
+    // full slices: prefetch k0 + BK while computing k0 (one barrier per slice)
+    const int kFull = kBegin + ((kEnd - kBegin) / BK) * BK;  // end of the full slices
+    int k0 = kBegin;
+    for (; k0 + 2 * BK <= kFull; k0 += BK) {
+      la.load(ra, A, g.lda, k0 + BK);
+      lb.load(rb, B, g.ldb, k0 + BK);
+      compute(cur);
+      store(cur ^ 1);
       __syncthreads();
-      if (more) {
-        store_a();
-        store_b();
-        __syncthreads();
-      }
+      cur ^= 1;
     }
+    // remaining one or two slices (the second may be partial)
+    if (k0 + BK < kEnd) {
+      load(k0 + BK);
+      compute(cur);
+      store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+    compute(cur);
   }
 
   // Epilogue.  The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in
@@ -223,6 +288,18 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   }
 }
 
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g, int split, double* ws) {
+  gemm_f64_body<BM, BN, TA, TB, VEC>(g, split, ws);
+}
+
+// The Schur Gram product S = Phat' Phat (lower tiles) under its own symbol,
+// so profiles attribute the dominant kernel of the Newton step unambiguously.
+template <bool VEC>
+__global__ __launch_bounds__(256, 2) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
+  gemm_f64_body<128, 128, true, false, VEC>(g, split, ws);
+}
+
 // C = sum_s W[s] + beta*C over the (lower) entries, summed in split order.
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, int split, const double* ws) {
   const long long MN = (long long)g.M * g.N;
@@ -242,27 +319,16 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, int spli
   }
 }
 
-template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmArgs g, int split, double* ws) {
-  gemm_f64_body<BM, BN, TA, TB>(g, split, ws);
-}
-
-// The Schur Gram product S = Phat' Phat (lower tiles) under its own symbol,
-// so profiles attribute the dominant kernel of the Newton step unambiguously.
-__global__ __launch_bounds__(256) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
-  gemm_f64_body<128, 128, true, false>(g, split, ws);
-}
-
-template <int BM, int BN, bool TA, bool TB>
-int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEvent_t after_main = nullptr) {
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEvent_t after_main) {
   const int mt = ceil_div(g.M, BM), nt = ceil_div(g.N, BN);
   const long long tiles = (g.flags & kGemmLowerOut) ? (long long)mt * (mt + 1) / 2 : (long long)mt * nt;
   if (tiles <= 0) return kOk;
   dim3 grid((unsigned)tiles, (unsigned)(g.batch * split));
   if (g.flags & kGemmSchurGram) {
-    hipLaunchKernelGGL(schur_gram_f64_kernel, grid, dim3(256), 0, stream, g, split, ws);
+    hipLaunchKernelGGL((schur_gram_f64_kernel<VEC>), grid, dim3(256), 0, stream, g, split, ws);
   } else {
-    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB>), grid, dim3(256), 0, stream, g, split, ws);
+    hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB, VEC>), grid, dim3(256), 0, stream, g, split, ws);
   }
   SDPX_HIP(hipGetLastError());
   if (after_main) SDPX_HIP(hipEventRecord(after_main, stream));
@@ -275,50 +341,58 @@ int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEven
   return kOk;
 }
 
-template <int BM, int BN>
-int dispatch_trans(const GemmArgs& g, hipStream_t stream, int split, double* ws, bool ta, bool tb) {
-  if (!ta && !tb) return launch<BM, BN, false, false>(g, stream, split, ws);
-  if (ta && !tb) return launch<BM, BN, true, false>(g, stream, split, ws);
-  if (!ta && tb) return launch<BM, BN, false, true>(g, stream, split, ws);
-  return launch<BM, BN, true, true>(g, stream, split, ws);
+template <int BM, int BN, bool VEC>
+int dispatch_trans(const GemmArgs& g, hipStream_t stream, int split, double* ws, bool ta, bool tb, hipEvent_t ev) {
+  if (!ta && !tb) return launch<BM, BN, false, false, VEC>(g, stream, split, ws, ev);
+  if (ta && !tb) return launch<BM, BN, true, false, VEC>(g, stream, split, ws, ev);
+  if (!ta && tb) return launch<BM, BN, false, true, VEC>(g, stream, split, ws, ev);
+  return launch<BM, BN, true, true, VEC>(g, stream, split, ws, ev);
+}
+
+// 16-byte operand fetches need the contiguous dimension, leading dimension
+// and batch strides even and 16-byte aligned bases.
+bool vec_ok(const GemmArgs& g, bool ta, bool tb) {
+  auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+  const bool a_ok = aligned(g.A) && g.lda % 2 == 0 && (ta ? g.K % 2 == 0 : g.M % 2 == 0) &&
+                    (g.batch == 1 || g.strideA % 2 == 0);
+  const bool b_ok = aligned(g.B) && g.ldb % 2 == 0 && (!tb ? g.K % 2 == 0 : g.N % 2 == 0) &&
+                    (g.batch == 1 || g.strideB % 2 == 0);
+  return a_ok && b_ok;
 }
 
 }  // namespace
 
 int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* workspace, hipEvent_t after_main) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return kOk;
-  if (g0.flags & kGemmSchurGram) {
-    // S = Phat' Phat: 128 x 128 lower tiles, op(A) transposed
-    if (!(g0.flags & kGemmTransA) || (g0.flags & kGemmTransB) || !(g0.flags & kGemmLowerOut)) {
-      set_last_error("gemm_f64: the Schur Gram kernel is TN with lower output");
-      return kInvalidArgument;
-    }
-    GemmArgs g = g0;
-    g.flags &= ~(kGemmTransA | kGemmTransB);
-    if (split_k < 1) split_k = 1;
-    if (split_k > 1 && workspace == nullptr) {
-      set_last_error("gemm_f64: split_k > 1 needs a workspace");
-      return kInvalidArgument;
-    }
-    return launch<128, 128, true, false>(g, stream, split_k, workspace, after_main);
-  }
   GemmArgs g = g0;
   const bool ta = (g.flags & kGemmTransA) != 0;
   const bool tb = (g.flags & kGemmTransB) != 0;
   g.flags &= ~(kGemmTransA | kGemmTransB);
-  if (g.K <= 0) {
-    // C = beta*C: run the kernel with an empty K range (accumulators stay zero)
-    g.K = 0;
-  }
+  if (g.K < 0) g.K = 0;  // K = 0: C = beta*C (accumulators stay zero)
   if (split_k < 1) split_k = 1;
   if (split_k > 1 && workspace == nullptr) {
     set_last_error("gemm_f64: split_k > 1 needs a workspace");
     return kInvalidArgument;
   }
+  const bool vec = vec_ok(g, ta, tb);
+  if (g.flags & kGemmSchurGram) {
+    // S = Phat' Phat: 128 x 128 lower tiles, op(A) transposed
+    if (!ta || tb || !(g.flags & kGemmLowerOut)) {
+      set_last_error("gemm_f64: the Schur Gram kernel is TN with lower output");
+      return kInvalidArgument;
+    }
+    return vec ? launch<128, 128, true, false, true>(g, stream, split_k, workspace, after_main)
+               : launch<128, 128, true, false, false>(g, stream, split_k, workspace, after_main);
+  }
   // Small problems use 64 x 64 tiles so that k x k block products launch enough groups.
-  const bool small = (g.M <= 96 && g.N <= 96) || ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256);
-  if (small) return dispatch_trans<64, 64>(g, stream, split_k, workspace, ta, tb);
-  return dispatch_trans<128, 128>(g, stream, split_k, workspace, ta, tb);
+  const bool small = (g.M <= 96 && g.N <= 96) ||
+                     ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256);
+  if (small) {
+    return vec ? dispatch_trans<64, 64, true>(g, stream, split_k, workspace, ta, tb, after_main)
+               : dispatch_trans<64, 64, false>(g, stream, split_k, workspace, ta, tb, after_main);
+  }
+  return vec ? dispatch_trans<128, 128, true>(g, stream, split_k, workspace, ta, tb, after_main)
+             : dispatch_trans<128, 128, false>(g, stream, split_k, workspace, ta, tb, after_main);
 }
 
 }  // namespace sdpx
