@@ -19,60 +19,62 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long ld
   if (info[b] != 0) return;
   double* a = A + (long long)b * stride + j + (long long)j * lda;
   __shared__ double s[NB][NB + 1];
-  __shared__ double dgi[NB];
-  __shared__ int fail;
+  __shared__ double inv[NB][NB + 1];
+  __shared__ double red[4][NB];
   const int tid = threadIdx.x;
+  const int row = tid % NB;   // row owned by this thread in the updates
+  const int part = tid / NB;  // which quarter of the columns / sum terms
   for (int idx = tid; idx < jb * jb; idx += 256) {
     const int r = idx % jb, c = idx / jb;
     s[r][c] = (r >= c) ? a[r + (long long)c * lda] : 0.0;
   }
-  if (tid == 0) fail = 0;
   __syncthreads();
+  // Right-looking elimination on the unscaled columns: step c subtracts
+  // a_r a_c2 / p from the trailing lower triangle (one barrier per column);
+  // the columns are scaled by 1/sqrt(p) afterwards, all at once.
   for (int c = 0; c < jb; ++c) {
-    if (tid == 0) {
-      const double p = s[c][c];
-      if (!(p > 0.0)) {
-        fail = 1;
-        info[b] = j + c + 1;
-      } else {
-        s[c][c] = sqrt(p);
-      }
+    const double p = s[c][c];
+    if (!(p > 0.0)) {
+      if (tid == 0) info[b] = j + c + 1;
+      return;  // uniform: every thread read the same pivot
     }
-    __syncthreads();
-    if (fail) return;
-    const double d = s[c][c];
-    for (int r = c + 1 + tid; r < jb; r += 256) s[r][c] = s[r][c] / d;
-    __syncthreads();
-    const int nrem = jb - c - 1;
-    for (int idx = tid; idx < nrem * nrem; idx += 256) {
-      const int r = c + 1 + idx % nrem, c2 = c + 1 + idx / nrem;
-      if (r >= c2) s[r][c2] = s[r][c2] - s[r][c] * s[c2][c];
+    if (row > c && row < jb) {
+      const double f = s[row][c] / p;
+      for (int c2 = c + 1 + part; c2 <= row; c2 += 4) s[row][c2] -= f * s[c2][c];
     }
     __syncthreads();
   }
+  if (tid < jb) red[0][tid] = sqrt(s[tid][tid]);
+  __syncthreads();
+  for (int idx = tid; idx < jb * jb; idx += 256) {
+    const int r = idx % jb, c = idx / jb;
+    if (r > c) s[r][c] = s[r][c] / red[0][c];
+  }
+  __syncthreads();
+  if (tid < jb) s[tid][tid] = red[0][tid];
+  __syncthreads();
   for (int idx = tid; idx < jb * jb; idx += 256) {
     const int r = idx % jb, c = idx / jb;
     if (r >= c) a[r + (long long)c * lda] = s[r][c];
   }
-  // Inverse of the lower factor block, one column c per thread.  inv(r, c)
-  // for r > c is kept transposed in the (unused) upper triangle s[c][r].
-  if (tid < jb) dgi[tid] = 1.0 / s[tid][tid];
-  __syncthreads();
-  if (tid < jb) {
-    const int c = tid;
-    for (int r = c + 1; r < jb; ++r) {
-      double acc = s[r][c] * dgi[c];
-      for (int q = c + 1; q < r; ++q) acc += s[r][q] * s[c][q];
-      s[c][r] = -acc / s[r][r];
+  // Inverse of the lower factor block by forward substitution on the
+  // identity, one row r per step; the four threads of a column split the sum.
+  for (int r = 0; r < jb; ++r) {
+    const int c = row;
+    double partial = 0.0;
+    if (c < r)
+      for (int q = c + part; q < r; q += 4) partial += s[r][q] * inv[q][c];
+    red[part][c] = partial;
+    __syncthreads();
+    if (part == 0 && c <= r) {
+      inv[r][c] = (c == r) ? 1.0 / s[r][r] : -((red[0][c] + red[1][c]) + (red[2][c] + red[3][c])) / s[r][r];
     }
+    __syncthreads();
   }
-  __syncthreads();
   double* di = dinv + (long long)b * dinv_stride + (long long)(j / NB) * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) {
     const int r = idx % NB, c = idx / NB;
-    double v = 0.0;
-    if (r < jb && c < jb) v = (r == c) ? dgi[c] : (r > c ? s[c][r] : 0.0);
-    di[idx] = v;
+    di[idx] = (r < jb && c < jb && r >= c) ? inv[r][c] : 0.0;
   }
 }
 

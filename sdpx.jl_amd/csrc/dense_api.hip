@@ -22,7 +22,7 @@ const char* last_error() { return g_last_error.c_str(); }
 
 namespace {
 
-constexpr int kTrialCap = 64;  // batched line-search candidates per launch
+constexpr int kTrialCap = 256;  // batched line-search candidates per launch
 constexpr double kEps = 2.220446049250313e-16;
 
 struct Block {
@@ -131,14 +131,14 @@ __global__ void small_ftb_kernel(const double* X, const double* dX, int k, doubl
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = small_ftb(X, dX, k);
 }
 
-// trial[b] = X + ts[b] * dX
-__global__ void trial_build_kernel(const double* X, const double* dX, long long k2, const double* ts, int B,
-                                   double* trial) {
+// trial[b] = X + ts[b] dX for b < n0, Y + ts[b] dY otherwise
+__global__ void trial_build_kernel(const double* X, const double* dX, const double* Y, const double* dY, long long k2,
+                                   const double* ts, int n0, int B, double* trial) {
   const long long n = k2 * B;
   for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
     const int b = (int)(idx / k2);
     const long long i = idx % k2;
-    trial[idx] = X[i] + ts[b] * dX[i];
+    trial[idx] = b < n0 ? X[i] + ts[b] * dX[i] : Y[i] + ts[b] * dY[i];
   }
 }
 
@@ -304,20 +304,36 @@ int solve_into(sdpx_dense_t* h, const double* rhs, double* out) {
   return kOk;
 }
 
-// Batched trial test of M + t_j dM for candidates hts[0..B): dok[j] = pass (1) / fail (0).
-int trial_batch(sdpx_dense_t* h, Block& b, const double* M, const double* dM, int B, double ratio) {
+// Batched trial PSD tests for one block: candidates t0[0..n0) on X + t dX,
+// then t1[0..n1) on Y + t dY, all factored in one batched launch sequence.
+// pass[j] = 1 when the trial passes trial_has_cholesky_margin!
+// (kernels/generic.jl:318-340; ratio = 0 is plain positive definiteness).
+int trial_eval(sdpx_dense_t* h, Block& b, const std::vector<double>& t0, const std::vector<double>& t1, double ratio,
+               std::vector<int>& pass) {
+  const int n0 = (int)t0.size(), n1 = (int)t1.size(), B = n0 + n1;
+  pass.assign(B, 0);
+  if (B == 0) return kOk;
+  if (B > kTrialCap) {
+    set_last_error("trial_eval: too many line-search candidates");
+    return kInvalidArgument;
+  }
+  for (int j = 0; j < n0; ++j) h->hts[j] = t0[j];
+  for (int j = 0; j < n1; ++j) h->hts[n0 + j] = t1[j];
   SDPX_HIP(hipMemcpyAsync(h->dts, h->hts, sizeof(double) * B, hipMemcpyHostToDevice, h->stream));
   if (b.k <= 2) {
-    hipLaunchKernelGGL(small_trials_kernel, dim3(1), dim3(kTrialCap), 0, h->stream, M, dM, b.k, h->dts, B, ratio,
-                       h->dok);
+    if (n0) hipLaunchKernelGGL(small_trials_kernel, dim3(1), dim3(kTrialCap), 0, h->stream, b.X, b.dX, b.k, h->dts, n0,
+                               ratio, h->dok);
+    if (n1) hipLaunchKernelGGL(small_trials_kernel, dim3(1), dim3(kTrialCap), 0, h->stream, b.Y, b.dY, b.k,
+                               h->dts + n0, n1, ratio, h->dok + n0);
     SDPX_HIP(hipGetLastError());
     SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dok, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
     SDPX_HIP(hipStreamSynchronize(h->stream));
+    for (int j = 0; j < B; ++j) pass[j] = h->hinfo[j];
     return kOk;
   }
   const long long n = b.k2 * B;
-  hipLaunchKernelGGL(trial_build_kernel, dim3(std::min(4096, ceil_div(n, 256))), dim3(256), 0, h->stream, M, dM,
-                     b.k2, h->dts, B, b.trial);
+  hipLaunchKernelGGL(trial_build_kernel, dim3(std::min(8192, ceil_div(n, 256))), dim3(256), 0, h->stream, b.X, b.dX,
+                     b.Y, b.dY, b.k2, h->dts, n0, B, b.trial);
   SDPX_HIP(hipGetLastError());
   const long long dstride = (long long)b.nblk * kCholNb * kCholNb;
   SDPX_CHECK(potrf_f64(b.trial, b.k, b.k2, b.k, B, b.trial_dinv, dstride, h->dok, h->stream));
@@ -325,106 +341,159 @@ int trial_batch(sdpx_dense_t* h, Block& b, const double* M, const double* dM, in
   SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dok, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
   if (ratio > 0.0) SDPX_HIP(hipMemcpyAsync(h->hsc, h->dsc, sizeof(double) * 2 * B, hipMemcpyDeviceToHost, h->stream));
   SDPX_HIP(hipStreamSynchronize(h->stream));
-  // convert LAPACK info to pass flags (+ margin test, kernels/generic.jl:318-340)
   for (int j = 0; j < B; ++j) {
     bool ok = h->hinfo[j] == 0;
     if (ok && ratio > 0.0) {
       const double mn = h->hsc[2 * j], mx = h->hsc[2 * j + 1];
       ok = std::isfinite(mn) && std::isfinite(mx) && mx > 0.0 && mn > ratio * mx;
     }
-    h->hinfo[j] = ok ? 1 : 0;
+    pass[j] = ok ? 1 : 0;
   }
   return kOk;
 }
 
-// fraction_to_boundary_bound! (kernels/generic.jl:350-420) for one block, k > 2:
-// full-step test, then the 64-step bisection evaluated D levels at a time.  All
-// midpoints of the next D levels are generated with the same FP64 arithmetic
-// as the sequential bisection, tested in one batched launch, and the path is
-// then replayed from the outcomes, so the result equals the sequential one.
-int ftb_bound(sdpx_dense_t* h, Block& b, const double* M, const double* dM, double* bound) {
+// Heap-ordered tree of the midpoints the sequential bisection can visit in the
+// next `depth` levels from (lo, hi), computed with the same FP64 arithmetic.
+void bisection_tree(double lo, double hi, int depth, std::vector<double>& mids) {
+  const int nodes = (1 << depth) - 1;
+  std::vector<double> nlo(2 * nodes + 2), nhi(2 * nodes + 2);
+  mids.assign(nodes, 0.0);
+  nlo[1] = lo;
+  nhi[1] = hi;
+  for (int node = 1; node <= nodes; ++node) {
+    const double mid = (nlo[node] + nhi[node]) / 2.0;
+    mids[node - 1] = mid;
+    nlo[2 * node] = nlo[node];
+    nhi[2 * node] = mid;
+    nlo[2 * node + 1] = mid;
+    nhi[2 * node + 1] = nhi[node];
+  }
+}
+
+// Replays `depth` bisection levels from the tested tree (pass flags at offset).
+void bisection_replay(const std::vector<double>& mids, const std::vector<int>& pass, int offset, int depth, double& lo,
+                      double& hi) {
+  int node = 1;
+  for (int lvl = 0; lvl < depth; ++lvl) {
+    const double mid = mids[node - 1];
+    if (pass[offset + node - 1]) {
+      lo = mid;
+      node = 2 * node + 1;
+    } else {
+      hi = mid;
+      node = 2 * node;
+    }
+  }
+}
+
+// fraction_to_boundary_bound! (kernels/generic.jl:350-420) of X + t dX and
+// Y + t dY for one block: the full-step test, then the 64-step bisection.  The
+// midpoints of several levels of both sides are tested in one batched launch
+// and the sequential path is replayed from the outcomes, so the result equals
+// the sequential bisection's (the first round speculatively includes the
+// first six levels next to the full step).
+int ftb_block(sdpx_dense_t* h, Block& b, double* bx, double* by) {
   if (b.k <= 2) {
-    hipLaunchKernelGGL(small_ftb_kernel, dim3(1), dim3(64), 0, h->stream, M, dM, b.k, h->dsc);
+    hipLaunchKernelGGL(small_ftb_kernel, dim3(1), dim3(64), 0, h->stream, b.X, b.dX, b.k, h->dsc);
+    hipLaunchKernelGGL(small_ftb_kernel, dim3(1), dim3(64), 0, h->stream, b.Y, b.dY, b.k, h->dsc + 1);
     SDPX_HIP(hipGetLastError());
-    SDPX_CHECK(sync_scalars(h, 1));
-    *bound = h->hsc[0];
+    SDPX_CHECK(sync_scalars(h, 2));
+    *bx = h->hsc[0];
+    *by = h->hsc[1];
     return kOk;
   }
-  h->hts[0] = 1.0;
-  SDPX_CHECK(trial_batch(h, b, M, dM, 1, 0.0));
-  if (h->hinfo[0]) {
-    *bound = 1.0;
-    return kOk;
+  constexpr int D0 = 6, D = 7;  // (1 + 63) and 127 candidates per side and round
+  double lo[2] = {0.0, 0.0}, hi[2] = {1.0, 1.0}, res[2] = {1.0, 1.0};
+  int remaining[2] = {64, 64};
+  bool active[2] = {true, true};
+  std::vector<double> mids[2], cand[2];
+  std::vector<int> pass;
+  // round 0: full step + first D0 levels
+  for (int sd = 0; sd < 2; ++sd) {
+    bisection_tree(0.0, 1.0, D0, mids[sd]);
+    cand[sd].assign(1, 1.0);
+    cand[sd].insert(cand[sd].end(), mids[sd].begin(), mids[sd].end());
   }
-  double lo = 0.0, hi = 1.0;
-  int remaining = 64;
-  constexpr int D = 6;  // 63 candidates per launch
-  while (remaining > 0) {
-    const int depth = std::min(D, remaining);
-    // heap-ordered tree of midpoints: node 1 = (lo, hi)
-    double nlo[1 << (D + 1)], nhi[1 << (D + 1)];
-    nlo[1] = lo;
-    nhi[1] = hi;
-    const int nodes = (1 << depth) - 1;
-    for (int node = 1; node <= nodes; ++node) {
-      const double mid = (nlo[node] + nhi[node]) / 2.0;
-      h->hts[node - 1] = mid;
-      if (2 * node + 1 < (1 << (D + 1))) {
-        nlo[2 * node] = nlo[node];
-        nhi[2 * node] = mid;
-        nlo[2 * node + 1] = mid;
-        nhi[2 * node + 1] = nhi[node];
+  SDPX_CHECK(trial_eval(h, b, cand[0], cand[1], 0.0, pass));
+  for (int sd = 0; sd < 2; ++sd) {
+    const int off = sd == 0 ? 0 : (int)cand[0].size();
+    if (pass[off]) {
+      active[sd] = false;
+      res[sd] = 1.0;
+    } else {
+      bisection_replay(mids[sd], pass, off + 1, D0, lo[sd], hi[sd]);
+      remaining[sd] -= D0;
+    }
+  }
+  while (active[0] || active[1]) {
+    int depth[2] = {0, 0};
+    for (int sd = 0; sd < 2; ++sd) {
+      cand[sd].clear();
+      if (!active[sd]) continue;
+      depth[sd] = std::min(D, remaining[sd]);
+      bisection_tree(lo[sd], hi[sd], depth[sd], mids[sd]);
+      cand[sd] = mids[sd];
+    }
+    SDPX_CHECK(trial_eval(h, b, cand[0], cand[1], 0.0, pass));
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!active[sd]) continue;
+      const int off = sd == 0 ? 0 : (int)cand[0].size();
+      bisection_replay(mids[sd], pass, off, depth[sd], lo[sd], hi[sd]);
+      remaining[sd] -= depth[sd];
+      if (remaining[sd] <= 0) {
+        active[sd] = false;
+        res[sd] = lo[sd];
       }
     }
-    SDPX_CHECK(trial_batch(h, b, M, dM, nodes, 0.0));
-    int node = 1;
-    for (int lvl = 0; lvl < depth; ++lvl) {
-      const double mid = h->hts[node - 1];
-      if (h->hinfo[node - 1]) {
-        lo = mid;
-        node = 2 * node + 1;
-      } else {
-        hi = mid;
-        node = 2 * node;
-      }
-    }
-    remaining -= depth;
   }
-  *bound = lo;
+  *bx = res[0];
+  *by = res[1];
   return kOk;
 }
 
-int backtrack_side(sdpx_dense_t* h, bool primal, double gamma, double min_step, double ratio, double* t_out) {
-  double t = 1.0;
-  // candidates are generated exactly as the sequential loop does (t *= gamma)
-  while (true) {
-    int B = 0;
-    double tc = t;
-    std::vector<double> cand;
-    for (; B < kTrialCap; ++B) {
-      cand.push_back(tc);
-      if (tc < min_step) {
-        ++B;
-        break;
+// line_search! (step.jl:1025-1075) for both sides at once: candidates
+// t_{j+1} = t_j * gamma (sequential arithmetic) are tested in batches; a side
+// stops at its first passing candidate or its first candidate below min_step.
+int backtrack_both(sdpx_dense_t* h, double gamma, double min_step, double ratio, double* tX, double* tY) {
+  double next[2] = {1.0, 1.0}, res[2] = {1.0, 1.0};
+  bool active[2] = {true, true};
+  int batch = 8;  // the first candidates usually pass: start small, then widen
+  std::vector<double> cand[2];
+  std::vector<int> pass, all;
+  while (active[0] || active[1]) {
+    for (int sd = 0; sd < 2; ++sd) {
+      cand[sd].clear();
+      if (!active[sd]) continue;
+      double tc = next[sd];
+      for (int j = 0; j < batch; ++j) {
+        cand[sd].push_back(tc);
+        if (tc < min_step) break;
+        tc *= gamma;
       }
-      tc *= gamma;
     }
-    B = (int)cand.size();
-    std::vector<int> pass(B, 1);
+    const int n0 = (int)cand[0].size(), B = n0 + (int)cand[1].size();
+    all.assign(B, 1);
     for (int l = 0; l < h->L; ++l) {
-      Block& b = h->blk[l];
-      for (int j = 0; j < B; ++j) h->hts[j] = cand[j];
-      SDPX_CHECK(trial_batch(h, b, primal ? b.X : b.Y, primal ? b.dX : b.dY, B, ratio));
-      for (int j = 0; j < B; ++j) pass[j] = pass[j] && h->hinfo[j];
+      SDPX_CHECK(trial_eval(h, h->blk[l], cand[0], cand[1], ratio, pass));
+      for (int j = 0; j < B; ++j) all[j] = all[j] && pass[j];
     }
-    for (int j = 0; j < B; ++j) {
-      if (pass[j] || cand[j] < min_step) {
-        *t_out = cand[j];
-        return kOk;
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!active[sd]) continue;
+      const int off = sd == 0 ? 0 : n0;
+      for (size_t j = 0; j < cand[sd].size(); ++j) {
+        if (all[off + j] || cand[sd][j] < min_step) {
+          res[sd] = cand[sd][j];
+          active[sd] = false;
+          break;
+        }
       }
+      if (active[sd]) next[sd] = cand[sd].back() * gamma;
     }
-    t = cand[B - 1] * gamma;
+    batch = kTrialCap / 2;
   }
+  *tX = res[0];
+  *tY = res[1];
+  return kOk;
 }
 
 }  // namespace
@@ -513,15 +582,37 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
   if ((s = dalloc(h, &h->dinvS, (size_t)ceil_div(m, kCholNb) * kCholNb * kCholNb))) return fail(s);
   if ((s = dalloc(h, &h->T, (size_t)(maxk2 * m)))) return fail(s);
   if ((s = dalloc(h, &h->Ph, (size_t)(maxk2 * m)))) return fail(s);
-  // split-K for the Schur Gram product: aim for >= 1024 work-groups
+  // split-K for the Schur Gram product.  All lower tiles cost the same, so the
+  // launch runs ceil(tiles*split / slots) rounds of resident work-groups
+  // (two 128 x 128 groups per CU); pick the split that fills the rounds best,
+  // keeping >= 64 K-slices per group and the partial sums within 4 GiB.
   {
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+    const long long slots = 2LL * cus;
     const long long nt = ceil_div(m, 128);
     const long long tiles = nt * (nt + 1) / 2;
     const long long stages = ceil_div(maxk2, 16);
-    int split = (int)std::max<long long>(1, std::min<long long>(16, ceil_div(1024, tiles)));
-    split = (int)std::max<long long>(1, std::min<long long>(split, stages / 8));
-    h->split = split;
-    if (split > 1 && (s = dalloc(h, &h->splitws, (size_t)(split * mm2)))) return fail(s);
+    const long long mem_cap = (4LL << 30) / (8 * mm2);
+    int best = 1;
+    double best_cost = 1e300;
+    for (int sp = 1; sp <= 32; ++sp) {
+      if (sp > 1 && (stages / sp < 64 || sp > mem_cap)) break;
+      const long long groups = tiles * sp;
+      const long long rounds = (groups + slots - 1) / slots;
+      // cost in units of one group's K-slice (~6 us): per-group slices plus a
+      // prologue/epilogue allowance, and the split-K reduction pass that reads
+      // sp partial m x m matrices (~8 bytes * sp * m^2 at ~4 TB/s)
+      const double cost = (double)rounds * ((double)stages / sp + 24.0) + (sp > 1 ? 3.1e-7 * sp * mm2 : 0.0);
+      if (cost < best_cost * 0.995) {
+        best_cost = cost;
+        best = sp;
+      }
+    }
+    h->split = best;
+    if (best > 1 && (s = dalloc(h, &h->splitws, (size_t)(best * mm2)))) return fail(s);
   }
   h->gemv_part_cap = std::max<long long>(maxk2 * 64, 4096);
   if ((s = dalloc(h, &h->gemv_part, (size_t)h->gemv_part_cap))) return fail(s);
@@ -931,12 +1022,10 @@ int sdpx_dense_fraction_to_boundary(sdpx_dense_t* h, double pf, double df, doubl
   SDPX_CHECK(check_handle(h));
   double bx = 1.0, by = 1.0;
   for (int l = 0; l < h->L; ++l) {
-    Block& b = h->blk[l];
-    double v = 1.0;
-    SDPX_CHECK(ftb_bound(h, b, b.X, b.dX, &v));
-    bx = std::min(bx, v);
-    SDPX_CHECK(ftb_bound(h, b, b.Y, b.dY, &v));
-    by = std::min(by, v);
+    double vx = 1.0, vy = 1.0;
+    SDPX_CHECK(ftb_block(h, h->blk[l], &vx, &vy));
+    bx = std::min(bx, vx);
+    by = std::min(by, vy);
   }
   *tX = bx < 1.0 ? pf * bx : 1.0;
   *tY = by < 1.0 ? df * by : 1.0;
@@ -945,9 +1034,7 @@ int sdpx_dense_fraction_to_boundary(sdpx_dense_t* h, double pf, double df, doubl
 
 int sdpx_dense_backtrack(sdpx_dense_t* h, double gamma, double min_step, double ratio, double* tX, double* tY) {
   SDPX_CHECK(check_handle(h));
-  SDPX_CHECK(backtrack_side(h, true, gamma, min_step, ratio, tX));
-  SDPX_CHECK(backtrack_side(h, false, gamma, min_step, ratio, tY));
-  return kOk;
+  return backtrack_both(h, gamma, min_step, ratio, tX, tY);
 }
 
 int sdpx_dense_update(sdpx_dense_t* h, double tX, double tY, double* out) {
