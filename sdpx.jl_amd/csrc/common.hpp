@@ -75,7 +75,12 @@ struct GemmArgs {
   unsigned flags;
   const int* skip;        // optional per-batch flag: skip the batch entry when *skip != 0
   long long skip_stride;  // stride between per-batch skip flags
+  const int2* tile_order; // optional (ti, tj) visiting order of the lower tiles (kGemmLowerOut)
 };
+
+// Lower-tile visiting order in square super-blocks of `group` tile rows and
+// columns (row-major inside), so that co-resident work-groups share panels.
+void lower_tile_order(int nt, int group, int2* out);
 
 // Launches the GEMM on `stream`. `split_k` > 1 requires `workspace` of
 // split_k * M * N * batch doubles; partial sums are reduced in a fixed order

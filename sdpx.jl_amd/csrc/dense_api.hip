@@ -184,6 +184,7 @@ struct sdpx_dense {
   double *T = nullptr, *Ph = nullptr;  // Schur panels (max over blocks)
   double* splitws = nullptr;
   int split = 1;
+  int2* tile_order = nullptr;  // Schur Gram lower-tile visiting order
   double* gemv_part = nullptr;
   long long gemv_part_cap = 0;
   double* symv_part = nullptr;
@@ -613,6 +614,15 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
     }
     h->split = best;
     if (best > 1 && (s = dalloc(h, &h->splitws, (size_t)(best * mm2)))) return fail(s);
+    // Gram tiles visited in 8 x 8 super-blocks: the co-resident groups of one
+    // XCD then read few distinct Phat column panels, shared in its L2.
+    std::vector<int2> order((size_t)tiles);
+    lower_tile_order((int)nt, 8, order.data());
+    if ((s = dalloc(h, &h->tile_order, order.size()))) return fail(s);
+    if (hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+      set_last_error("hipMemcpy(tile order) failed");
+      return fail(kHipError);
+    }
   }
   h->gemv_part_cap = std::max<long long>(maxk2 * 64, 4096);
   if ((s = dalloc(h, &h->gemv_part, (size_t)h->gemv_part_cap))) return fail(s);
@@ -822,6 +832,7 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);
+    s.tile_order = h->tile_order;
     if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
     SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws, h->prof ? h->ev[2] : nullptr));
     if (h->prof) {

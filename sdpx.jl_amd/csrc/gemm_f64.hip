@@ -145,23 +145,29 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   const int wm0 = (wave & 1) * WM;
   const int wn0 = (wave >> 1) * WN;
 
-  const int bz = blockIdx.y;
-  const int b = bz / split;
-  const int s = bz % split;
+  const int b = blockIdx.y;
   if (g.skip && g.skip[(long long)b * g.skip_stride] != 0) return;
 
   // XCD-aware remap: blocks are dispatched round-robin over the 8 XCDs, so
-  // give each XCD a contiguous run of tiles (shared A/B panels stay in its L2).
+  // give each XCD a contiguous run of (split, tile) work items: with split-K
+  // an XCD owns whole K ranges, and within a range the tile order keeps the
+  // co-resident groups on few A/B panels (shared through that XCD's L2).
   const int total = gridDim.x;
   const int bid = blockIdx.x;
   const int xcd = bid % kNumXcd;
   const int slot = bid / kNumXcd;
   const int q = total / kNumXcd, rq = total % kNumXcd;
-  const int tile = xcd * q + (xcd < rq ? xcd : rq) + slot;
+  const int lin = xcd * q + (xcd < rq ? xcd : rq) + slot;
+  const int per_split = total / split;
+  const int s = lin / per_split;
+  const int tile = lin % per_split;
 
   int ti, tj;
   const bool lower_out = (g.flags & kGemmLowerOut) != 0;
-  if (lower_out) {
+  if (lower_out && g.tile_order) {
+    ti = g.tile_order[tile].x;
+    tj = g.tile_order[tile].y;
+  } else if (lower_out) {
     // linear index over lower tiles, row-major: tile = ti*(ti+1)/2 + tj
     int t = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
     while ((long long)(t + 1) * (t + 2) / 2 <= tile) ++t;
@@ -324,7 +330,7 @@ int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEven
   const int mt = ceil_div(g.M, BM), nt = ceil_div(g.N, BN);
   const long long tiles = (g.flags & kGemmLowerOut) ? (long long)mt * (mt + 1) / 2 : (long long)mt * nt;
   if (tiles <= 0) return kOk;
-  dim3 grid((unsigned)tiles, (unsigned)(g.batch * split));
+  dim3 grid((unsigned)(tiles * split), (unsigned)g.batch);
   if (g.flags & kGemmSchurGram) {
     hipLaunchKernelGGL((schur_gram_f64_kernel<VEC>), grid, dim3(256), 0, stream, g, split, ws);
   } else {
@@ -393,6 +399,18 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
   }
   return vec ? dispatch_trans<128, 128, true>(g, stream, split_k, workspace, ta, tb, after_main)
              : dispatch_trans<128, 128, false>(g, stream, split_k, workspace, ta, tb, after_main);
+}
+
+}  // namespace sdpx
+
+namespace sdpx {
+
+void lower_tile_order(int nt, int group, int2* out) {
+  int n = 0;
+  for (int si = 0; si < nt; si += group)
+    for (int sj = 0; sj <= si; sj += group)
+      for (int ti = si; ti < std::min(nt, si + group); ++ti)
+        for (int tj = sj; tj < std::min(sj + group, ti + 1); ++tj) out[n++] = make_int2(ti, tj);
 }
 
 }  // namespace sdpx
