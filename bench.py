@@ -66,6 +66,38 @@ def cpu_baseline(k: int, m_sample: int, seed: int = 1) -> dict:
                       f"assembly+Cholesky), {dt:.2f} s, OpenBLAS threads={os.environ.get('OMP_NUM_THREADS', 'default')}"}
 
 
+def soc_leg(ncones: int = 16_000_000, q: int = 4, reps: int = 10) -> dict:
+    """Lorentz-cone kernels on many cones (HBM-bound): achieved GB/s of the NT scaling and W apply launches."""
+    import numpy as np
+
+    from sdpx_mi355x.soc import APPLY_W, LorentzCones, random_interior
+
+    rng = np.random.default_rng(0)
+    dims = np.full(ncones, q, dtype=np.int32)
+    s = random_interior(rng, dims, 2.0)
+    z = random_interior(rng, dims, 0.5)
+    with LorentzCones(dims) as c:
+        c.set(0, s)
+        c.set(1, z)
+        del s, z
+        c.set_profiling(True)
+        out = {"workload": f"{ncones} Lorentz cones of dimension {q}"}
+        # algorithmic bytes per cone: nt_scaling reads s, z and writes w, lambda (8q each), eta, eta^2, ok;
+        # W apply reads w, x and eta and writes W x
+        for name, fn, per_cone in (("nt_scaling", lambda: c.nt_scaling(0, 1, 2, 3), 32 * q + 20),
+                                   ("apply_w", lambda: c.apply(APPLY_W, 2, 1, 4), 24 * q + 8)):
+            fn()
+            ms = []
+            for _ in range(reps):
+                fn()
+                ms.append(c.last_kernel_ms())
+            avg = sum(ms) / len(ms) / 1e3
+            gbs = per_cone * ncones / avg / 1e9
+            out[name] = {"avg_launch_ms": round(avg * 1e3, 4), "bytes_per_launch": per_cone * ncones,
+                         "achieved_GBps": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000.0, 4)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +109,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample-m", type=int, default=1500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-soc", action="store_true", help="skip the Lorentz-cone kernel leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is design-only",
@@ -158,6 +191,9 @@ def main():
                      "factor_ms_per_iter": round(kt["factor_ms"] / max(kt["factor_calls"], 1), 4)},
     }
     b.close()
+    if not args.no_soc:
+        out["soc"] = soc_leg()
+        log(f"soc: {out['soc']}")
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(k, args.cpu_sample_m)

@@ -165,6 +165,72 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 
+/* ==== a14: Lorentz (second-order) cone kernels ===============================
+ * Many cones Q_q = {(t, u): t >= ||u||} of arbitrary dimensions resident in
+ * HBM.  A handle owns SDPX_SOC_SLOTS coordinate vectors of sum_c q_c doubles
+ * each (cone c's coordinates follow cone c-1's, head first; the caller decides
+ * what a slot holds) plus the per-cone NT scalars eta, eta^2 and ok flags.
+ * Each call is one bandwidth-bound launch over every cone, in the reference's
+ * operation order (no floating-point contraction).  Replaces the per-cone
+ * loops over src/soc_lorentz_kernels.jl and src/soc.jl kernels in the native
+ * SOC workspace (soc_native.jl). */
+#define SDPX_SOC_SLOTS 8
+typedef struct sdpx_soc sdpx_soc_t;
+
+int sdpx_soc_create(int device, int ncones, const int* dims, sdpx_soc_t** out);
+int sdpx_soc_destroy(sdpx_soc_t* h);
+int sdpx_soc_memory(sdpx_soc_t* h, long long* bytes);
+/* Host <-> device copies of one slot (sum_c q_c doubles); device copy between slots. */
+int sdpx_soc_set_vector(sdpx_soc_t* h, int slot, const double* v);
+int sdpx_soc_get_vector(sdpx_soc_t* h, int slot, double* v);
+int sdpx_soc_copy_vector(sdpx_soc_t* h, int src, int dst);
+/* Per-cone eta, eta^2 and ok (1 when the last NT scaling succeeded) of the last
+ * sdpx_soc_nt_scaling call; null pointers skip. */
+int sdpx_soc_get_scaling(sdpx_soc_t* h, double* eta, double* eta2, int* ok);
+/* _soc_nt_scaling! (soc_lorentz_kernels.jl:72-184) for every cone: NT scaling
+ * point into slot w and lambda into slot lambda from slots s and z.  A cone
+ * whose checks fail keeps its previous w / lambda and gets ok = 0, eta = 0;
+ * *nfailed counts them.  Outputs must not alias inputs or each other. */
+int sdpx_soc_nt_scaling(sdpx_soc_t* h, int s, int z, int w, int lambda, int* nfailed);
+/* dst = op(src) per cone with the scaling point in slot w and the stored eta:
+ * op 0 W (_soc_nt_apply_w!, soc_lorentz_kernels.jl:187-205), 1 W^-1
+ * (_soc_nt_apply_winv!, :208-226), 2 (W'W)^-1 (_soc_nt_apply_hs_inverse!,
+ * :229-248).  dst may alias src, not w. */
+int sdpx_soc_apply(sdpx_soc_t* h, int op, int w, int src, int dst);
+/* dst = left o right (_soc_jordan!, soc.jl:94-112); dst may alias either. */
+int sdpx_soc_jordan(sdpx_soc_t* h, int left, int right, int dst);
+/* left o dst = right (_soc_jordan_solve!, soc_lorentz_kernels.jl:464-485).
+ * Returns 1 with *nbad cones when a divisor is not interior (those cones are
+ * left untouched; the reference throws ArgumentError). */
+int sdpx_soc_jordan_solve(sdpx_soc_t* h, int left, int right, int dst, int* nbad);
+/* Jordan inverse (_soc_inverse!, soc.jl:126-138); returns 1 like jordan_solve. */
+int sdpx_soc_inverse(sdpx_soc_t* h, int src, int dst, int* nbad);
+/* _soc_fraction_to_boundary (soc.jl:148-214) per cone for s + t ds; *min_bound
+ * = the minimum over cones (NaN propagates); per_cone (ncones) optional. */
+int sdpx_soc_fraction_to_boundary(sdpx_soc_t* h, int s, int ds, double* min_bound, double* per_cone);
+/* Per-cone margins v0 - ||v_tail|| (_soc_margin, soc_lorentz_kernels.jl:36)
+ * and the number of strictly interior cones (_soc_is_interior, soc.jl:122). */
+int sdpx_soc_margins(sdpx_soc_t* h, int slot, double* margins, int* ninterior);
+/* _native_soc_strict_step (soc_native.jl:2705-2745) over all cones: the full
+ * step when allowed and strictly interior, else the first of 0.99*bound
+ * halved up to 24 times whose trial point is interior in every cone, else 0. */
+int sdpx_soc_strict_step(sdpx_soc_t* h, int state, int direction, double bound, int allow_full_step,
+                         double* step);
+/* Fixed-trace Q3 kernels (every cone of dimension 3):
+ * _soc_fixed_trace_hkm_metric! (soc_lorentz_kernels.jl:353-370) into dst;
+ * cones outside the cone interior are counted and left untouched. */
+int sdpx_soc_hkm_metric(sdpx_soc_t* h, int primal, int dual, int dst, int* nfailed);
+/* mode 0: _soc_fixed_trace_hkm_rhs_coordinates (soc_lorentz_kernels.jl:372-418),
+ * v = primal residual; mode 1: _soc_fixed_trace_hkm_recovery! (:420-461),
+ * v = primal direction.  target = the centering target; include_affine != 0
+ * subtracts the affine product of slots affine_primal / affine_dual. */
+int sdpx_soc_hkm_contract(sdpx_soc_t* h, int mode, int primal, int dual, int v, int affine_primal,
+                          int affine_dual, double target, int include_affine, int dst);
+/* Device-event timing of each following call's main kernel (one event sync per call). */
+int sdpx_soc_set_profiling(sdpx_soc_t* h, int enabled);
+int sdpx_soc_last_kernel_ms(sdpx_soc_t* h, double* ms);
+int sdpx_soc_synchronize(sdpx_soc_t* h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,27 @@ SIGNATURES = {
     "sdpx_dense_set_profiling": [_h, C.c_int],
     "sdpx_dense_kernel_times": [_h, C.c_int, _dp],
     "sdpx_dense_schur_split": [_h, _ip],
+    "sdpx_soc_create": [C.c_int, C.c_int, _ip, C.POINTER(_h)],
+    "sdpx_soc_destroy": [_h],
+    "sdpx_soc_memory": [_h, C.POINTER(C.c_longlong)],
+    "sdpx_soc_set_vector": [_h, C.c_int, _dp],
+    "sdpx_soc_get_vector": [_h, C.c_int, _dp],
+    "sdpx_soc_copy_vector": [_h, C.c_int, C.c_int],
+    "sdpx_soc_get_scaling": [_h, _dp, _dp, _ip],
+    "sdpx_soc_nt_scaling": [_h, C.c_int, C.c_int, C.c_int, C.c_int, _ip],
+    "sdpx_soc_apply": [_h, C.c_int, C.c_int, C.c_int, C.c_int],
+    "sdpx_soc_jordan": [_h, C.c_int, C.c_int, C.c_int],
+    "sdpx_soc_jordan_solve": [_h, C.c_int, C.c_int, C.c_int, _ip],
+    "sdpx_soc_inverse": [_h, C.c_int, C.c_int, _ip],
+    "sdpx_soc_fraction_to_boundary": [_h, C.c_int, C.c_int, _dp, _dp],
+    "sdpx_soc_margins": [_h, C.c_int, _dp, _ip],
+    "sdpx_soc_strict_step": [_h, C.c_int, C.c_int, C.c_double, C.c_int, _dp],
+    "sdpx_soc_hkm_metric": [_h, C.c_int, C.c_int, C.c_int, _ip],
+    "sdpx_soc_hkm_contract": [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                              C.c_int],
+    "sdpx_soc_set_profiling": [_h, C.c_int],
+    "sdpx_soc_last_kernel_ms": [_h, _dp],
+    "sdpx_soc_synchronize": [_h],
 }
 _RESTYPES = {"sdpx_last_error": C.c_char_p}
 
