@@ -165,6 +165,60 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 
+/* ==== Batched route: many independent single-block SDPs, one launch per phase ==
+ * No counterpart in the reference: SDPX solves one problem per Workspace
+ * (sequential re-solves only, prepared.jl:391-447).  Each batch member is the
+ * dense-Schur Newton method above (one PSD block of side k <= 64, m <= 2048
+ * constraints, n_eq = 0).  Every phase is one launch over the members still
+ * iterating; the per-member interior-point control runs on the host over
+ * scalars read back once per phase, so members terminate independently. */
+typedef struct sdpx_batch sdpx_batch_t;
+
+typedef struct sdpx_batch_options {
+  /* SolverOptions fields (types/workspaces.jl:101-228) */
+  double beta, gamma, eps_gap, eps_primal, eps_dual, min_step, omega_step, stall_tolerance, refine_tol;
+  double adaptive_sigma_max;
+  int iter_max, max_restarts, max_centering, stall_iterations, restart;
+  int refine_policy;      /* 0 :auto / :adaptive, 1 :fixed */
+  int refine_steps, refine_max_steps;
+  int adaptive;           /* parameter_strategy: 1 :adaptive, 0 :fixed */
+  int step_rule;          /* 0 :auto, 1 :backtrack, 2 :fraction_to_boundary */
+  int precision;          /* 0: plain FP64 */
+  double cond_limit, refine_target;
+} sdpx_batch_options_t;
+
+/* Fill `o` with the reference defaults (SolverOptions{Float64}()) and tolerance
+ * `tol` for the gap, primal and dual criteria (public/settings.jl:308-338). */
+int sdpx_batch_default_options(sdpx_batch_options_t* o, double tol);
+/* B members; member p has one PSD block of side k[p] (1..64) and m[p] (1..2048) constraints. */
+int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_t** out);
+int sdpx_batch_destroy(sdpx_batch_t* h);
+int sdpx_batch_memory(sdpx_batch_t* h, long long* bytes);
+/* Dense data, members concatenated: Av (k_p^2 x m_p column-major each, the
+ * DenseCons panel, types/constraints.jl:26-28), C (k_p^2 each), c (m_p each). */
+int sdpx_batch_set_dense(sdpx_batch_t* h, const double* Av, const double* C, const double* c);
+/* Sparse data: constraint j of the batch (members concatenated, sum m_p
+ * constraints) has entries ptr[j]..ptr[j+1]-1 of (row, col, val), 0-based,
+ * row <= col; A_j is the symmetric matrix they define (duplicates add).  The
+ * panels are densified on the device (SparseBlockCOO, types/constraints.jl:59-109). */
+int sdpx_batch_set_coo(sdpx_batch_t* h, const long long* ptr, const int* row, const int* col, const double* val,
+                       const double* C, const double* c);
+/* Start point, members concatenated: x (m_p), X and Y (k_p^2 each, column-major). */
+int sdpx_batch_set_start(sdpx_batch_t* h, const double* x, const double* X, const double* Y);
+/* Runs the interior-point loop (interior_point.jl:1323-2420) of every member to
+ * termination; *iterations = number of batched Newton steps launched. */
+int sdpx_batch_solve(sdpx_batch_t* h, const sdpx_batch_options_t* o, int* iterations);
+/* Per member: status (0 Optimal, 1 IterLimit, 2 NumericalBreakdown, 3 Stalled,
+ * 4 InsufficientPrecision, 5 MaxRestartsExceeded), iterations, objectives, and
+ * the returned point (concatenated like set_start).  Null pointers skip. */
+int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double* pobj, double* dobj, double* x,
+                          double* X, double* Y);
+/* Schur matrix of one member (m_p x m_p, lower triangle authoritative). */
+int sdpx_batch_get_schur(sdpx_batch_t* h, int member, double* S);
+/* Schur assembly of every member at the uploaded start point (parity hook of
+ * schur_build!, schur.jl:511-547); returns 1 when a start block is not PD. */
+int sdpx_batch_schur_at_start(sdpx_batch_t* h);
+
 /* ==== a14: Lorentz (second-order) cone kernels ===============================
  * Many cones Q_q = {(t, u): t >= ||u||} of arbitrary dimensions resident in
  * HBM.  A handle owns SDPX_SOC_SLOTS coordinate vectors of sum_c q_c doubles

@@ -1,0 +1,1030 @@
+// Kernels of the batched route (see batch_kernels.hpp): one work-group of 256
+// threads per listed member; k x k blocks (k <= 64) are staged in LDS,
+// constraint panels are streamed from HBM with coalesced loads, the Schur
+// panel products run on FP64 MFMA.
+#include "batch_kernels.hpp"
+
+namespace sdpx {
+namespace batch {
+namespace {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// ---- block-level helpers ---------------------------------------------------------
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b != b) ? NAN : fmax(a, b); }
+
+// deterministic reductions over the 256 threads (red: >= 256 doubles of LDS)
+__device__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  red[t] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = red[t] + red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  red[t] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = nan_max(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_min(double v, double* red) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  red[t] = v;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmin(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// s(r, c) = s[r + c * LP]
+__device__ __forceinline__ void lds_load(double* s, const double* g, int k) {
+  for (int e = threadIdx.x; e < k * k; e += NT) s[(e % k) + (e / k) * LP] = g[e];
+}
+__device__ __forceinline__ void lds_store(double* g, const double* s, int k) {
+  for (int e = threadIdx.x; e < k * k; e += NT) g[e] = s[(e % k) + (e / k) * LP];
+}
+__device__ __forceinline__ void lds_store_lower(double* g, const double* s, int k) {
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    g[e] = r >= c ? s[r + c * LP] : 0.0;
+  }
+}
+
+// C = alpha op(A) op(B) + beta C   (C must not alias A or B)
+__device__ void lds_mm(double* C, const double* A, bool ta, const double* B, bool tb, int k, double alpha,
+                       double beta) {
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    double acc = 0.0;
+    for (int q = 0; q < k; ++q) {
+      const double av = ta ? A[q + r * LP] : A[r + q * LP];
+      const double bv = tb ? B[c + q * LP] : B[q + c * LP];
+      acc += av * bv;
+    }
+    double* cp = C + r + c * LP;
+    *cp = beta == 0.0 ? alpha * acc : alpha * acc + beta * (*cp);
+  }
+}
+
+// In-place lower Cholesky of s (k x k): the same unscaled right-looking
+// elimination and final scaling as the diagonal-block kernel of chol_f64.hip.
+// Returns 0 or the 1-based column of the first non-positive / NaN pivot.
+// The strict upper triangle of s is left as it was.
+__device__ int lds_chol(double* s, int k, double* red) {
+  const int tid = threadIdx.x;
+  const int row = tid % KMAX, part = tid / KMAX;
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    const double p = s[c + c * LP];
+    if (!(p > 0.0)) {
+      __syncthreads();
+      return c + 1;
+    }
+    if (row > c && row < k) {
+      const double f = s[row + c * LP] / p;
+      for (int c2 = c + 1 + part; c2 <= row; c2 += 4) s[row + c2 * LP] -= f * s[c2 + c * LP];
+    }
+    __syncthreads();
+  }
+  if (tid < k) red[tid] = sqrt(s[tid + tid * LP]);
+  __syncthreads();
+  for (int e = tid; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    if (r > c) s[r + c * LP] = s[r + c * LP] / red[c];
+  }
+  __syncthreads();
+  if (tid < k) s[tid + tid * LP] = red[tid];
+  __syncthreads();
+  return 0;
+}
+
+// Li = L^-1 (lower, strict upper zero) by forward substitution on the identity
+// (the same summation as the diagonal-block inverse of chol_f64.hip)
+__device__ void lds_trtri(const double* L, double* Li, int k, double* red4) {
+  const int tid = threadIdx.x;
+  const int c = tid % KMAX, part = tid / KMAX;
+  for (int e = tid; e < k * k; e += NT) Li[(e % k) + (e / k) * LP] = 0.0;
+  __syncthreads();
+  for (int r = 0; r < k; ++r) {
+    double partial = 0.0;
+    if (c < r)
+      for (int q = c + part; q < r; q += 4) partial += L[r + q * LP] * Li[q + c * LP];
+    red4[part * KMAX + c] = partial;
+    __syncthreads();
+    if (part == 0 && c <= r)
+      Li[r + c * LP] = (c == r) ? 1.0 / L[r + r * LP]
+                                : -((red4[c] + red4[KMAX + c]) + (red4[2 * KMAX + c] + red4[3 * KMAX + c])) /
+                                      L[r + r * LP];
+    __syncthreads();
+  }
+}
+
+// |diag| extrema of a factor in LDS (thread 0 result broadcast through red)
+__device__ void lds_diag_extrema(const double* s, int k, double* red, double& mn, double& mx) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = __builtin_inf(), b = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double v = fabs(s[i + i * LP]);
+      a = fmin(a, v);
+      b = nan_max(b, v);
+    }
+    red[0] = a;
+    red[1] = b;
+  }
+  __syncthreads();
+  mn = red[0];
+  mx = red[1];
+  __syncthreads();
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---- HBM-bound constraint-panel passes ----------------------------------------------
+__global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const long long k2 = (long long)M.k * M.k;
+  const double* Av = a.Av + M.oAv;
+  const double* vec = (mode == 1 ? a.dx : a.x) + M.oM;
+  double amax = 0.0;
+  for (long long e = threadIdx.x; e < k2; e += NT) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = 0;
+    for (; i + 4 <= M.m; i += 4) {
+      s0 += Av[(long long)i * k2 + e] * vec[i];
+      s1 += Av[(long long)(i + 1) * k2 + e] * vec[i + 1];
+      s2 += Av[(long long)(i + 2) * k2 + e] * vec[i + 2];
+      s3 += Av[(long long)(i + 3) * k2 + e] * vec[i + 3];
+    }
+    for (; i < M.m; ++i) s0 += Av[(long long)i * k2 + e] * vec[i];
+    const double acc = (s0 + s1) + (s2 + s3);
+    double out;
+    if (mode == 1) {
+      out = acc + a.P[M.oK + e];
+      a.dX[M.oK + e] = out;
+    } else {
+      out = acc - a.X[M.oK + e] - a.C[M.oK + e];
+      (mode == 0 ? a.P : a.W1)[M.oK + e] = out;
+    }
+    amax = nan_max(amax, fabs(out));
+  }
+  if (mode != 1) {
+    amax = block_max(amax, red);
+    if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_PRES : S_PSOL)] = amax;
+  }
+}
+
+__global__ __launch_bounds__(NT) void gemvt_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double W[KMAX * KMAX];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k2 = M.k * M.k;
+  const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
+  for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
+  __syncthreads();
+  const double* Av = a.Av + M.oAv;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double amax = 0.0;
+  for (int i = wave; i < M.m; i += NT / 64) {
+    const double* col = Av + (long long)i * k2;
+    double s = 0.0;
+    for (int e = lane; e < k2; e += 64) s += col[e] * W[e];
+    s = wave_sum(s);
+    if (lane == 0) {
+      const long long o = M.oM + i;
+      if (mode == 1) {
+        a.r[o] = -(a.d[o] + s);
+      } else {
+        const double v = a.c[o] - s;
+        if (mode == 0) a.d[o] = v;
+        amax = nan_max(amax, fabs(v));
+      }
+    }
+  }
+  if (mode != 1) {
+    amax = block_max(amax, red);
+    if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_DRES : S_DSOL)] = amax;
+  }
+}
+
+// ---- k x k block phases (LDS) ----------------------------------------------------------
+__global__ __launch_bounds__(NT) void block_factor_kernel(Arrays a, const int* list) {
+  __shared__ double sX[KMAX * LP], sY[KMAX * LP], sA[KMAX * LP], sB[KMAX * LP];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sX, a.X + o, k);
+  lds_load(sY, a.Y + o, k);
+  __syncthreads();
+  // R = -X Y (+ mu I)
+  lds_mm(sA, sX, false, sY, false, k, -1.0, 0.0);
+  __syncthreads();
+  const bool affine = a.par[(long long)p * NPARAM + P_AFF] != 0.0;
+  if (!affine && threadIdx.x < k) sA[threadIdx.x * (LP + 1)] += a.par[(long long)p * NPARAM + P_MU];
+  __syncthreads();
+  lds_store(a.R + o, sA, k);
+  __syncthreads();
+  // L_X, L_Y
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    sA[r + c * LP] = sX[r + c * LP];
+    sB[r + c * LP] = sY[r + c * LP];
+  }
+  const int ix = lds_chol(sA, k, red);
+  const int iy = lds_chol(sB, k, red);
+  double mn, mx;
+  if (ix == 0) {
+    lds_store_lower(a.LX + o, sA, k);
+    lds_diag_extrema(sA, k, red, mn, mx);
+    if (threadIdx.x == 0) {
+      a.sc[(long long)p * NSLOT + S_LXMIN] = mn;
+      a.sc[(long long)p * NSLOT + S_LXMAX] = mx;
+    }
+  }
+  if (iy == 0) {
+    lds_store_lower(a.LY + o, sB, k);
+    lds_diag_extrema(sB, k, red, mn, mx);
+    if (threadIdx.x == 0) {
+      a.sc[(long long)p * NSLOT + S_LYMIN] = mn;
+      a.sc[(long long)p * NSLOT + S_LYMAX] = mx;
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.si[(long long)p * NISLOT + I_INFOX] = ix;
+    a.si[(long long)p * NISLOT + I_INFOY] = iy;
+  }
+  if (ix != 0 || iy != 0) return;  // uniform
+  // L_X^-1 (into sX) and X^-1 = L_X^-T L_X^-1 (into sY)
+  __syncthreads();
+  lds_trtri(sA, sX, k, red);
+  lds_store(a.LXinv + o, sX, k);
+  lds_mm(sY, sX, true, sX, false, k, 1.0, 0.0);
+  __syncthreads();
+  lds_store(a.Xinv + o, sY, k);
+}
+
+// W2 = X^-1 (P Y - R)
+__global__ __launch_bounds__(NT) void rhs_block_kernel(Arrays a, const int* list) {
+  __shared__ double sP[KMAX * LP], sY[KMAX * LP], sA[KMAX * LP];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sP, a.P + o, k);
+  lds_load(sY, a.Y + o, k);
+  lds_load(sA, a.R + o, k);
+  __syncthreads();
+  // sA = P Y - R
+  lds_mm(sA, sP, false, sY, false, k, 1.0, -1.0);
+  __syncthreads();
+  lds_load(sP, a.Xinv + o, k);
+  __syncthreads();
+  lds_mm(sY, sP, false, sA, false, k, 1.0, 0.0);
+  __syncthreads();
+  lds_store(a.W2 + o, sY, k);
+}
+
+// dY = sym(X^-1 (R - dX Y))
+__global__ __launch_bounds__(NT) void dir_block_kernel(Arrays a, const int* list) {
+  __shared__ double sD[KMAX * LP], sY[KMAX * LP], sA[KMAX * LP];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sD, a.dX + o, k);
+  lds_load(sY, a.Y + o, k);
+  lds_load(sA, a.R + o, k);
+  __syncthreads();
+  lds_mm(sA, sD, false, sY, false, k, -1.0, 1.0);  // R - dX Y
+  __syncthreads();
+  lds_load(sD, a.Xinv + o, k);
+  __syncthreads();
+  lds_mm(sY, sD, false, sA, false, k, 1.0, 0.0);
+  __syncthreads();
+  double* dY = a.dY + o;
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    dY[e] = (sY[r + c * LP] + sY[c + r * LP]) / 2.0;
+  }
+}
+
+__global__ __launch_bounds__(NT) void comp_kernel(Arrays a, const int* list) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k2 = M.k * M.k;
+  const long long o = M.oK;
+  const double ta = a.par[(long long)p * NPARAM + P_A], tb = a.par[(long long)p * NPARAM + P_B];
+  double s0 = 0.0, s1 = 0.0;
+  for (int e = threadIdx.x; e < k2; e += NT) {
+    const double x = a.X[o + e], y = a.Y[o + e];
+    s0 += x * y;
+    s1 += (x + ta * a.dX[o + e]) * (y + tb * a.dY[o + e]);
+  }
+  s0 = block_sum(s0, red);
+  s1 = block_sum(s1, red);
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_COMP] = s0;
+    a.sc[(long long)p * NSLOT + S_COMPT] = s1;
+  }
+}
+
+// R = -X Y - dX dY + target I (threaded.jl:839-854, 943-1003)
+__global__ __launch_bounds__(NT) void corrector_block_kernel(Arrays a, const int* list) {
+  __shared__ double sX[KMAX * LP], sY[KMAX * LP], sA[KMAX * LP];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k, k2 = k * k;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  const int mode = (int)par[P_MODE];
+  double dotv = 0.0;
+  for (int e = threadIdx.x; e < k2; e += NT) {
+    const double x = a.X[o + e], y = a.Y[o + e];
+    dotv += mode == 0 ? (x + a.dX[o + e]) * (y + a.dY[o + e]) : x * y;
+  }
+  dotv = block_sum(dotv, red);
+  double target;
+  if (mode == 0) {
+    const double rl = dotv / par[P_MU] / (double)k;
+    target = fmax(rl < 1.0 ? rl * rl : rl, par[P_BETA]) * par[P_MU];
+  } else if (mode == 1) {
+    target = par[P_SIGMA] * dotv / (double)k;
+  } else {
+    target = par[P_SIGMA] * par[P_MUG];
+  }
+  lds_load(sX, a.X + o, k);
+  lds_load(sY, a.Y + o, k);
+  __syncthreads();
+  lds_mm(sA, sX, false, sY, false, k, -1.0, 0.0);
+  __syncthreads();
+  lds_load(sX, a.dX + o, k);
+  lds_load(sY, a.dY + o, k);
+  __syncthreads();
+  lds_mm(sA, sX, false, sY, false, k, -1.0, 1.0);
+  __syncthreads();
+  if (threadIdx.x < k) sA[threadIdx.x * (LP + 1)] += target;
+  __syncthreads();
+  lds_store(a.R + o, sA, k);
+}
+
+// ---- step length / PSD checks (kernels/generic.jl:292-420, step.jl:1025-1098) ---------
+// trial = base + t * dir (lower triangle) factored in sT; returns info
+__device__ int trial_chol(const double* base, const double* dir, double t, double* sT, int k, double* red) {
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    if (r >= c) sT[r + c * LP] = base[r + c * LP] + t * dir[r + c * LP];
+  }
+  return lds_chol(sT, k, red);
+}
+
+// trial_has_cholesky_margin! (generic.jl:318-340); ratio 0: plain positive definiteness
+__device__ bool trial_ok(const double* base, const double* dir, double t, double ratio, double* sT, int k,
+                         double* red) {
+  if (trial_chol(base, dir, t, sT, k, red) != 0) return false;
+  if (!(ratio > 0.0)) return true;
+  double mn, mx;
+  lds_diag_extrema(sT, k, red, mn, mx);
+  return isfinite(mn) && isfinite(mx) && mx > 0.0 && mn > ratio * mx;
+}
+
+// fraction_to_boundary_bound! for k >= 3: full step, else 64-step bisection
+__device__ double ftb_side(const double* base, const double* dir, double* sT, int k, double* red) {
+  if (trial_chol(base, dir, 1.0, sT, k, red) == 0) return 1.0;
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    const double mid = (lo + hi) / 2.0;
+    if (trial_chol(base, dir, mid, sT, k, red) == 0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+#pragma clang fp contract(off)
+// closed forms for k <= 2 (generic.jl:356-401), as in the dense route
+__device__ bool small_isposdef(const double* X, const double* dX, int k, double t) {
+  if (k == 1) return X[0] + t * dX[0] > 0.0;
+  const double a0 = X[0] + t * dX[0];
+  if (!(a0 > 0.0)) return false;
+  const double b0 = X[1] + t * dX[1];
+  const double d0 = X[LP + 1] + t * dX[LP + 1];
+  return a0 * d0 > b0 * b0;
+}
+
+__device__ double small_ftb(const double* X, const double* dX, int k) {
+  if (k == 1) {
+    const double d = dX[0];
+    return d < 0.0 ? fmin(1.0, -X[0] / d) : 1.0;
+  }
+  const double a0 = X[0], b0 = X[LP], d0 = X[LP + 1];
+  const double da = dX[0], db = (dX[LP] + dX[1]) / 2.0, dd = dX[LP + 1];
+  const double a1 = a0 + da, b1 = b0 + db, d1 = d0 + dd;
+  if (a1 > 0.0 && a1 * d1 > b1 * b1) return 1.0;
+  const double c0 = a0 * d0 - b0 * b0;
+  const double c1 = da * d0 + a0 * dd - 2.0 * b0 * db;
+  const double c2 = da * dd - db * db;
+  double root = 1.0;
+  bool found = false;
+  if (c2 == 0.0) {
+    if (c1 < 0.0) {
+      const double cand = -c0 / c1;
+      if (0.0 < cand && cand <= 1.0) {
+        root = cand;
+        found = true;
+      }
+    }
+  } else {
+    const double disc = c1 * c1 - 4.0 * c2 * c0;
+    if (disc >= 0.0) {
+      const double sd = sqrt(disc);
+      const double den = 2.0 * c2;
+      const double r1 = (-c1 - sd) / den;
+      const double r2 = (-c1 + sd) / den;
+      if (0.0 < r1 && r1 <= root) {
+        root = r1;
+        found = true;
+      }
+      if (0.0 < r2 && r2 <= root) {
+        root = r2;
+        found = true;
+      }
+    }
+  }
+  if (found) return root;
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    const double mid = (lo + hi) / 2.0;
+    if (small_isposdef(X, dX, k, mid))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ bool small_margin(const double* X, const double* dX, int k, double t, double ratio) {
+  if (!(ratio > 0.0)) return small_isposdef(X, dX, k, t);
+  double mn, mx;
+  const double d11 = X[0] + t * dX[0];
+  if (!(d11 > 0.0)) return false;
+  if (k == 1) {
+    mn = mx = fabs(sqrt(d11));
+  } else {
+    const double l11 = sqrt(d11);
+    const double l21 = (X[1] + t * dX[1]) / l11;
+    const double d22 = (X[LP + 1] + t * dX[LP + 1]) - l21 * l21;
+    if (!(d22 > 0.0)) return false;
+    const double l22 = sqrt(d22);
+    mn = fmin(fabs(l11), fabs(l22));
+    mx = fmax(fabs(l11), fabs(l22));
+  }
+  return isfinite(mn) && isfinite(mx) && mx > 0.0 && mn > ratio * mx;
+}
+#pragma clang fp contract(on)
+
+__global__ __launch_bounds__(NT) void ftb_kernel(Arrays a, const int* list) {
+  __shared__ double sB[KMAX * LP], sD[KMAX * LP], sT[KMAX * LP];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  double b[2];
+  for (int side = 0; side < 2; ++side) {
+    __syncthreads();
+    lds_load(sB, (side == 0 ? a.X : a.Y) + o, k);
+    lds_load(sD, (side == 0 ? a.dX : a.dY) + o, k);
+    __syncthreads();
+    b[side] = k <= 2 ? small_ftb(sB, sD, k) : ftb_side(sB, sD, sT, k, red);
+  }
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_TX] = b[0] < 1.0 ? par[P_PF] * b[0] : 1.0;
+    a.sc[(long long)p * NSLOT + S_TY] = b[1] < 1.0 ? par[P_DF] * b[1] : 1.0;
+  }
+}
+
+// line_search! (step.jl:1025-1075): t = 1, gamma, gamma^2, ... (repeated
+// multiplication); a side stops at its first passing trial or at the first
+// candidate below min_step
+__global__ __launch_bounds__(NT) void backtrack_kernel(Arrays a, const int* list) {
+  __shared__ double sB[KMAX * LP], sD[KMAX * LP], sT[KMAX * LP];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  const double gamma = par[P_GAMMA], min_step = par[P_MINSTEP], ratio = par[P_RATIO];
+  double res[2];
+  for (int side = 0; side < 2; ++side) {
+    __syncthreads();
+    lds_load(sB, (side == 0 ? a.X : a.Y) + o, k);
+    lds_load(sD, (side == 0 ? a.dX : a.dY) + o, k);
+    __syncthreads();
+    double t = 1.0;
+    while (true) {
+      const bool pass = k <= 2 ? small_margin(sB, sD, k, t, ratio) : trial_ok(sB, sD, t, ratio, sT, k, red);
+      if (pass || t < min_step || !(t > 0.0)) break;  // t reaches 0 after ~1100 steps: the loop always ends
+      t *= gamma;
+    }
+    res[side] = t;
+  }
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_TX] = res[0];
+    a.sc[(long long)p * NSLOT + S_TY] = res[1];
+  }
+}
+
+// ---- accepted step ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void update_kernel(Arrays a, const int* list) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k2 = M.k * M.k;
+  const long long o = M.oK;
+  const double tX = a.par[(long long)p * NPARAM + P_A], tY = a.par[(long long)p * NPARAM + P_B];
+  double comp = 0.0, xm = 0.0, ym = 0.0, vm = 0.0;
+  for (int e = threadIdx.x; e < k2; e += NT) {
+    const double x = a.X[o + e] + tX * a.dX[o + e];
+    const double y = a.Y[o + e] + tY * a.dY[o + e];
+    a.X[o + e] = x;
+    a.Y[o + e] = y;
+    comp += x * y;
+    xm = nan_max(xm, fabs(x));
+    ym = nan_max(ym, fabs(y));
+  }
+  for (int i = threadIdx.x; i < M.m; i += NT) {
+    const double v = a.x[M.oM + i] + tX * a.dx[M.oM + i];
+    a.x[M.oM + i] = v;
+    vm = nan_max(vm, fabs(v));
+  }
+  comp = block_sum(comp, red);
+  xm = block_max(xm, red);
+  ym = block_max(ym, red);
+  vm = block_max(vm, red);
+  if (threadIdx.x == 0) {
+    double* sc = a.sc + (long long)p * NSLOT;
+    sc[S_COMP] = comp;
+    sc[S_XBMAX] = xm;
+    sc[S_YBMAX] = ym;
+    sc[S_XMAX] = vm;
+  }
+}
+
+__global__ __launch_bounds__(NT) void accepted_kernel(Arrays a, const int* list) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const double tY = a.par[(long long)p * NPARAM + P_B];
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < M.m; i += NT) {
+    const long long o = M.oM + i;
+    const double v = (1.0 - tY) * a.d[o] - tY * a.rho[o];
+    a.d[o] = v;
+    mx = nan_max(mx, fabs(v));
+  }
+  mx = block_max(mx, red);
+  if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + S_DACC] = mx;
+}
+
+__global__ __launch_bounds__(NT) void objectives_kernel(Arrays a, const int* list) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  double po = 0.0, du = 0.0;
+  for (int i = threadIdx.x; i < M.m; i += NT) po += a.c[M.oM + i] * a.x[M.oM + i];
+  for (int e = threadIdx.x; e < M.k * M.k; e += NT) du += a.C[M.oK + e] * a.Y[M.oK + e];
+  po = block_sum(po, red);
+  du = block_sum(du, red);
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_POBJ] = po;
+    a.sc[(long long)p * NSLOT + S_DOBJ] = du;
+  }
+}
+
+__global__ __launch_bounds__(NT) void scale_kernel(Arrays a, const int* list, int which) {
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const double f = a.par[(long long)p * NPARAM + P_SCALE];
+  double* B = (which == 1 ? a.X : a.Y) + M.oK;
+  for (int e = threadIdx.x; e < M.k * M.k; e += NT) B[e] = f * B[e];
+}
+
+__global__ __launch_bounds__(NT) void best_kernel(Arrays a, const int* list, int dir) {
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  for (int e = threadIdx.x; e < M.k * M.k; e += NT) {
+    const long long o = M.oK + e;
+    if (dir == 0) {
+      a.bX[o] = a.X[o];
+      a.bY[o] = a.Y[o];
+    } else {
+      a.X[o] = a.bX[o];
+      a.Y[o] = a.bY[o];
+    }
+  }
+  for (int i = threadIdx.x; i < M.m; i += NT) {
+    const long long o = M.oM + i;
+    if (dir == 0)
+      a.bx[o] = a.x[o];
+    else
+      a.x[o] = a.bx[o];
+  }
+}
+
+__global__ void set_int_kernel(Arrays a, const int* list, int n, int slot, int value) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a.si[(long long)list[i] * NISLOT + slot] = value;
+}
+
+// ---- Schur panel Phat_i = L_X^-1 A_i L_Y on FP64 MFMA ------------------------------------
+// One work-group per (member, run of `chunk` constraints).  L_X^-1 and L_Y are
+// staged once; for each constraint A_i is staged, T = L_X^-1 A_i goes to LDS
+// and Phat_i = T L_Y to HBM.  Products run on 16 x 16 MFMA tiles of the block
+// padded to kp = 16 * ceil(k / 16); the operands are fed swapped so the
+// accumulator holds the tile transposed (lane % 16 indexes rows: contiguous
+// column-major stores).  Triangular factors shorten each tile's K range.
+__global__ __launch_bounds__(NT) void panel_kernel(Arrays a, const int2* work, int chunk) {
+  __shared__ double sLi[KMAX * LP], sLy[KMAX * LP], sA[KMAX * LP], sT[KMAX * LP];
+  const int2 w = work[blockIdx.x];
+  const int p = w.x;
+  const Member M = a.mem[p];
+  if (a.si[(long long)p * NISLOT + I_SKIP] != 0) return;
+  const int k = M.k, kp = M.kp, nt = kp / 16;
+  const int i0 = w.y, i1 = min(M.m, w.y + chunk);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int e = threadIdx.x; e < kp * kp; e += NT) {
+    const int r = e % kp, c = e / kp;
+    const bool in = r < k && c < k;
+    sLi[r + c * LP] = in ? a.LXinv[M.oK + r + c * k] : 0.0;
+    sLy[r + c * LP] = in ? a.LY[M.oK + r + c * k] : 0.0;
+  }
+  const int ntiles = nt * nt;
+  for (int i = i0; i < i1; ++i) {
+    const double* Ai = a.Av + M.oAv + (long long)i * k * k;
+    for (int e = threadIdx.x; e < kp * kp; e += NT) {
+      const int r = e % kp, c = e / kp;
+      sA[r + c * LP] = (r < k && c < k) ? Ai[r + c * k] : 0.0;
+    }
+    __syncthreads();
+    // T = L_X^-1 A_i: tile (ti, tj), rows r0 = 16 ti, K range q < r0 + 16
+    {
+      double4_t acc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
+      int qend = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = wave + 4 * u;
+        if (t < ntiles) qend = max(qend, (t % nt) * 16 + 16);
+      }
+      for (int q0 = 0; q0 < qend; q0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = wave + 4 * u;
+          if (t >= ntiles) continue;
+          const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+          if (q0 >= r0 + 16) continue;
+          const double fb = sA[(q0 + lq) + (c0 + lr) * LP];   // B'[i'][kq] = A[q0+kq][c0+i']
+          const double fl = sLi[(r0 + lr) + (q0 + lq) * LP];  // L'[kq][j'] = Li[r0+j'][q0+kq]
+          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = wave + 4 * u;
+        if (t >= ntiles) continue;
+        const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) sT[(r0 + lr) + (c0 + lq + 4 * rr) * LP] = acc[u][rr];
+      }
+    }
+    __syncthreads();
+    // Phat_i = T L_Y: tile (ti, tj), K range q >= c0
+    {
+      double4_t acc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
+      int qbeg = kp;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = wave + 4 * u;
+        if (t < ntiles) qbeg = min(qbeg, (t / nt) * 16);
+      }
+      for (int q0 = qbeg; q0 < kp; q0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int t = wave + 4 * u;
+          if (t >= ntiles) continue;
+          const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+          if (q0 < c0) continue;
+          const double fb = sLy[(q0 + lq) + (c0 + lr) * LP];  // Ly[q0+kq][c0+i']
+          const double fl = sT[(r0 + lr) + (q0 + lq) * LP];   // T[r0+j'][q0+kq]
+          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
+        }
+      }
+      double* Pi = a.Ph + M.oPh + (long long)i * M.ldP;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = wave + 4 * u;
+        if (t >= ntiles) continue;
+        const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+        const int row = r0 + lr;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int col = c0 + lq + 4 * rr;
+          if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
+        }
+      }
+    }
+  }
+}
+
+// ---- Schur factor helpers ----------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void shift_copy_kernel(Arrays a, const int* list) {
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const double shift = a.par[(long long)p * NPARAM + P_REG];
+  const double* S = a.S + M.oS;
+  double* Sf = a.Sf + M.oS;
+  const long long mm = (long long)M.m * M.m;
+  for (long long idx = threadIdx.x; idx < mm; idx += NT) {
+    const int r = (int)(idx % M.m), c = (int)(idx / M.m);
+    if (r < c) continue;
+    double v = S[idx];
+    if (r == c && shift != 0.0) v += shift * fmax(fabs(v), 1.0);
+    Sf[idx] = v;
+  }
+  if (threadIdx.x == 0) a.si[(long long)p * NISLOT + I_INFOS] = 0;
+}
+
+__global__ __launch_bounds__(NT) void factor_extrema_kernel(Arrays a, const int* list) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const double* L = a.Sf + M.oS;
+  double mn = __builtin_inf(), mx = 0.0;
+  for (int i = threadIdx.x; i < M.m; i += NT) {
+    const double v = fabs(L[i + (long long)i * M.m]);
+    mn = fmin(mn, v);
+    mx = nan_max(mx, v);
+  }
+  mn = block_min(mn, red);
+  mx = block_max(mx, red);
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_SFMIN] = mn;
+    a.sc[(long long)p * NSLOT + S_SFMAX] = mx;
+  }
+}
+
+// x = Sf^-T Sf^-1 b with the 64 x 64 diagonal-block inverses; x staged in LDS
+__global__ __launch_bounds__(NT) void solve_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double sx[MMAX];
+  __shared__ double part[4][64];
+  __shared__ double t[64];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int m = M.m;
+  const double* L = a.Sf + M.oS;
+  const double* dinv = a.dinv + M.oD;
+  const double* b = (mode == 0 ? a.r : a.rho) + M.oM;
+  double* out = (mode == 0 ? a.dx : a.delta) + M.oM;
+  const int tid = threadIdx.x, row = tid % 64, q = tid / 64;
+  for (int i = tid; i < m; i += NT) sx[i] = b[i];
+  __syncthreads();
+  // forward: x_J = Dinv_J (x_J - L_{J,<J} x_{<J})
+  for (int J = 0; J < M.nblk; ++J) {
+    const int r0 = J * 64, jb = min(64, m - r0);
+    double s = 0.0;
+    if (row < jb)
+      for (int c = q; c < r0; c += 4) s += L[(r0 + row) + (long long)c * m] * sx[c];
+    part[q][row] = s;
+    __syncthreads();
+    if (q == 0) t[row] = row < jb ? sx[r0 + row] - ((part[0][row] + part[1][row]) + (part[2][row] + part[3][row])) : 0.0;
+    __syncthreads();
+    const double* D = dinv + (long long)J * 64 * 64;
+    s = 0.0;
+    if (row < jb)
+      for (int c = q; c <= row; c += 4) s += D[row + c * 64] * t[c];
+    part[q][row] = s;
+    __syncthreads();
+    if (q == 0 && row < jb) sx[r0 + row] = (part[0][row] + part[1][row]) + (part[2][row] + part[3][row]);
+    __syncthreads();
+  }
+  // backward: x_J = Dinv_J' (x_J - L_{>J,J}' x_{>J})
+  for (int J = M.nblk - 1; J >= 0; --J) {
+    const int r0 = J * 64, jb = min(64, m - r0);
+    double s = 0.0;
+    if (row < jb)
+      for (int c = r0 + 64 + q; c < m; c += 4) s += L[c + (long long)(r0 + row) * m] * sx[c];
+    part[q][row] = s;
+    __syncthreads();
+    if (q == 0) t[row] = row < jb ? sx[r0 + row] - ((part[0][row] + part[1][row]) + (part[2][row] + part[3][row])) : 0.0;
+    __syncthreads();
+    const double* D = dinv + (long long)J * 64 * 64;
+    s = 0.0;
+    if (row < jb)
+      for (int c = row + q; c < jb; c += 4) s += D[c + row * 64] * t[c];
+    part[q][row] = s;
+    __syncthreads();
+    if (q == 0 && row < jb) sx[r0 + row] = (part[0][row] + part[1][row]) + (part[2][row] + part[3][row]);
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += NT) out[i] = sx[i];
+}
+
+// rho = r - S dx (S lower, unfactored); S_RHO = ||rho||_inf, S_RNORM = ||r||_inf.
+// Wave w walks columns c = w, w+4, ...: column entries below the diagonal feed
+// y_r (into the wave's own partial vector) and the column dot feeds y_c.
+__global__ __launch_bounds__(NT) void residual_kernel(Arrays a, const int* list) {
+  __shared__ double sdx[MMAX];
+  __shared__ double y[4][MMAX];
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int m = M.m;
+  const double* S = a.S + M.oS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < m; i += NT) {
+    sdx[i] = a.dx[M.oM + i];
+    y[0][i] = y[1][i] = y[2][i] = y[3][i] = 0.0;
+  }
+  __syncthreads();
+  for (int c = wave; c < m; c += 4) {
+    const double* col = S + (long long)c * m;
+    const double xc = sdx[c];
+    double dot = 0.0;
+    for (int r = c + 1 + lane; r < m; r += 64) {
+      const double s = col[r];
+      y[wave][r] += s * xc;
+      dot += s * sdx[r];
+    }
+    dot = wave_sum(dot);
+    if (lane == 0) y[wave][c] += col[c] * xc + dot;
+  }
+  __syncthreads();
+  double rm = 0.0, rr = 0.0;
+  for (int i = threadIdx.x; i < m; i += NT) {
+    const double sy = (y[0][i] + y[1][i]) + (y[2][i] + y[3][i]);
+    const double ri = a.r[M.oM + i];
+    const double v = ri - sy;
+    a.rho[M.oM + i] = v;
+    rm = nan_max(rm, fabs(v));
+    rr = nan_max(rr, fabs(ri));
+  }
+  rm = block_max(rm, red);
+  rr = block_max(rr, red);
+  if (threadIdx.x == 0) {
+    a.sc[(long long)p * NSLOT + S_RHO] = rm;
+    a.sc[(long long)p * NSLOT + S_RNORM] = rr;
+  }
+}
+
+__global__ __launch_bounds__(NT) void vec_op_kernel(Arrays a, const int* list, int mode) {
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  for (int i = threadIdx.x; i < M.m; i += NT) {
+    const long long o = M.oM + i;
+    if (mode == 0)
+      a.dxb[o] = a.dx[o];
+    else if (mode == 1)
+      a.dx[o] = a.dx[o] + a.delta[o];
+    else
+      a.dx[o] = a.dxb[o];
+  }
+}
+
+// one thread per constraint: A_j (upper COO) mirrored into its Av column
+__global__ void coo_densify_kernel(Arrays a, const long long* cptr, const int* crow, const int* ccol,
+                                   const double* cval, const long long* cons, int ncons) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncons) return;
+  const long long off = cons[2 * j];  // Av offset of the column
+  const int k = (int)cons[2 * j + 1];
+  double* col = a.Av + off;
+  for (long long e = cptr[j]; e < cptr[j + 1]; ++e) {
+    const int r = crow[e], c = ccol[e];
+    col[r + (long long)c * k] += cval[e];
+    if (r != c) col[c + (long long)r * k] += cval[e];
+  }
+}
+
+}  // namespace
+
+#define SDPX_LAUNCH(kernel, grid, ...)                                                   \
+  do {                                                                                   \
+    if ((grid) > 0) {                                                                    \
+      hipLaunchKernelGGL(kernel, dim3((unsigned)(grid)), dim3(NT), 0, st, __VA_ARGS__); \
+      SDPX_HIP(hipGetLastError());                                                       \
+    }                                                                                    \
+    return kOk;                                                                          \
+  } while (0)
+
+int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
+  SDPX_LAUNCH(gemv_kernel, n, a, list, mode);
+}
+int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
+  SDPX_LAUNCH(gemvt_kernel, n, a, list, mode);
+}
+int launch_block_factor(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(block_factor_kernel, n, a, list);
+}
+int launch_panel(const Arrays& a, const int2* work, int nwork, int chunk, hipStream_t st) {
+  SDPX_LAUNCH(panel_kernel, nwork, a, work, chunk);
+}
+int launch_shift_copy(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(shift_copy_kernel, n, a, list);
+}
+int launch_factor_extrema(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(factor_extrema_kernel, n, a, list);
+}
+int launch_rhs_block(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(rhs_block_kernel, n, a, list);
+}
+int launch_solve(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
+  SDPX_LAUNCH(solve_kernel, n, a, list, mode);
+}
+int launch_residual(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(residual_kernel, n, a, list);
+}
+int launch_vec_op(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
+  SDPX_LAUNCH(vec_op_kernel, n, a, list, mode);
+}
+int launch_dir_block(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(dir_block_kernel, n, a, list);
+}
+int launch_comp(const Arrays& a, const int* list, int n, hipStream_t st) { SDPX_LAUNCH(comp_kernel, n, a, list); }
+int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(corrector_block_kernel, n, a, list);
+}
+int launch_ftb(const Arrays& a, const int* list, int n, hipStream_t st) { SDPX_LAUNCH(ftb_kernel, n, a, list); }
+int launch_backtrack(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(backtrack_kernel, n, a, list);
+}
+int launch_update(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(update_kernel, n, a, list);
+}
+int launch_accepted(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(accepted_kernel, n, a, list);
+}
+int launch_objectives(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(objectives_kernel, n, a, list);
+}
+int launch_scale(const Arrays& a, const int* list, int n, int which, hipStream_t st) {
+  SDPX_LAUNCH(scale_kernel, n, a, list, which);
+}
+int launch_best(const Arrays& a, const int* list, int n, int dir, hipStream_t st) {
+  SDPX_LAUNCH(best_kernel, n, a, list, dir);
+}
+
+int launch_set_int(const Arrays& a, const int* list, int n, int slot, int value, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(set_int_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, a, list, n, slot, value);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_coo_densify(const Arrays& a, const long long* cptr, const int* crow, const int* ccol, const double* cval,
+                       const long long* cons, int ncons, hipStream_t st) {
+  if (ncons <= 0) return kOk;
+  hipLaunchKernelGGL(coo_densify_kernel, dim3((unsigned)ceil_div(ncons, 256)), dim3(256), 0, st, a, cptr, crow, ccol,
+                     cval, cons, ncons);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace batch
+}  // namespace sdpx

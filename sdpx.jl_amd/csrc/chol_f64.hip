@@ -13,11 +13,10 @@ namespace {
 
 constexpr int NB = kCholNb;
 
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long lda, long long stride, int j, int jb,
-                                                         double* dinv, long long dinv_stride, int* info) {
-  const int b = blockIdx.x;
-  if (info[b] != 0) return;
-  double* a = A + (long long)b * stride + j + (long long)j * lda;
+// Factors the jb x jb diagonal block at a (column j of its matrix), writes L
+// over its lower triangle and the block's inverse to di (NB x NB, zero
+// padded); a bad pivot sets *info = j + c + 1.
+__device__ void potrf_diag_body(double* a, long long lda, int j, int jb, double* di, int* info) {
   __shared__ double s[NB][NB + 1];
   __shared__ double inv[NB][NB + 1];
   __shared__ double red[4][NB];
@@ -35,7 +34,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long ld
   for (int c = 0; c < jb; ++c) {
     const double p = s[c][c];
     if (!(p > 0.0)) {
-      if (tid == 0) info[b] = j + c + 1;
+      if (tid == 0) *info = j + c + 1;
       return;  // uniform: every thread read the same pivot
     }
     if (row > c && row < jb) {
@@ -71,11 +70,27 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long ld
     }
     __syncthreads();
   }
-  double* di = dinv + (long long)b * dinv_stride + (long long)(j / NB) * NB * NB;
   for (int idx = tid; idx < NB * NB; idx += 256) {
     const int r = idx % NB, c = idx / NB;
     di[idx] = (r < jb && c < jb && r >= c) ? inv[r][c] : 0.0;
   }
+}
+
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long lda, long long stride, int j, int jb,
+                                                         double* dinv, long long dinv_stride, int* info) {
+  const int b = blockIdx.x;
+  if (info[b] != 0) return;
+  potrf_diag_body(A + (long long)b * stride + j + (long long)j * lda, lda, j, jb,
+                  dinv + (long long)b * dinv_stride + (long long)(j / NB) * NB * NB, info + b);
+}
+
+// one work-group per listed matrix: diagonal block j of matrix d[list[w]]
+__global__ __launch_bounds__(256) void potrf_diag_grouped_kernel(const CholDesc* __restrict__ d,
+                                                                 const int* __restrict__ list, int j) {
+  const CholDesc c = d[list[blockIdx.x]];
+  if (*c.info != 0 || j >= c.n) return;
+  const int jb = min(NB, c.n - j);
+  potrf_diag_body(c.A + j + (long long)j * c.lda, c.lda, j, jb, c.dinv + (long long)(j / NB) * NB * NB, c.info);
 }
 
 __global__ void zero_int_kernel(int* p, int n) {
@@ -84,6 +99,13 @@ __global__ void zero_int_kernel(int* p, int n) {
 }
 
 }  // namespace
+
+int potrf_diag_grouped(const CholDesc* d, const int* list, int count, int j, hipStream_t stream) {
+  if (count <= 0) return kOk;
+  hipLaunchKernelGGL(potrf_diag_grouped_kernel, dim3(count), dim3(256), 0, stream, d, list, j);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
 
 int potrf_f64(double* A, long long lda, long long stride, int n, int batch, double* dinv, long long dinv_stride,
               int* info, hipStream_t stream) {

@@ -78,6 +78,28 @@ struct GemmArgs {
   const int2* tile_order; // optional (ti, tj) visiting order of the lower tiles (kGemmLowerOut)
 };
 
+// Grouped launches: many independent GEMMs of different shapes in one
+// launch, one work-group per (GEMM, output tile) item.  Flags as above
+// (LowerOut and the triangular K-range hints; no split-K).
+struct GroupGemm {
+  const double* A;
+  const double* B;
+  double* C;
+  long long lda, ldb, ldc;
+  int M, N, K;
+  unsigned flags;
+  double alpha, beta;
+  const int* skip;  // optional: the GEMM is skipped when *skip != 0
+};
+struct GroupTile {
+  int g, ti, tj, pad;
+};
+// 64 x 64 tiles; ta / tb select op(A) = A' / op(B) = B'; vec: every GEMM meets
+// the 16-byte-load conditions (even leading dimensions, 16-byte aligned bases,
+// even M (ta=0) or K (ta=1) for A and even K (tb=0) or N (tb=1) for B).
+int gemm_f64_grouped(const GroupGemm* groups, const GroupTile* tiles, int ntiles, bool ta, bool tb, bool vec,
+                     hipStream_t stream);
+
 // Lower-tile visiting order in square super-blocks of `group` tile rows and
 // columns (row-major inside), so that co-resident work-groups share panels.
 void lower_tile_order(int nt, int group, int2* out);
@@ -100,6 +122,20 @@ constexpr int kCholNb = 64;
 // 1-based column of the first non-positive / non-finite pivot, 0 on success.
 int potrf_f64(double* A, long long lda, long long stride, int n, int batch, double* dinv,
               long long dinv_stride, int* info, hipStream_t stream);
+
+// One matrix of a grouped Cholesky: n x n lower at A (leading dim lda), its
+// diagonal-block inverses (ceil(n/nb) * nb*nb doubles) and its info flag.
+struct CholDesc {
+  double* A;
+  long long lda;
+  int n;
+  double* dinv;
+  int* info;
+};
+// Diagonal-block step j of the grouped Cholesky for matrices d[list[0..count)]
+// (device arrays); matrices with *info != 0 or n <= j are left alone.  The
+// panel and trailing updates of the step are grouped GEMMs built by the caller.
+int potrf_diag_grouped(const CholDesc* d, const int* list, int count, int j, hipStream_t stream);
 
 // Solves L x = b (trans=0) or L' x = b (trans=1) in place for one right-hand
 // side, using the diagonal-block inverses written by potrf_f64.

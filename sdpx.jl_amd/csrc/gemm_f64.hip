@@ -130,8 +130,13 @@ __device__ __forceinline__ void tile_store(const double* v, double (*S)[R + PAD]
   }
 }
 
+// One BM x BN tile of C(m0:, n0:) = alpha * op(A) op(B) (+ beta * C) over
+// K in [kBegin, kEnd); only entries with row < M, col < N (and row >= col
+// when lower_out) are written.
 template <int BM, int BN, bool TA, bool TB, bool VEC>
-__device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws) {
+__device__ __forceinline__ void gemm_core(const double* __restrict__ A, long long lda, const double* __restrict__ B,
+                                          long long ldb, int M, int N, int kBegin, int kEnd, int m0, int n0,
+                                          bool lower_out, double* C, long long ldc, double alpha, double beta) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
@@ -145,6 +150,98 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   const int wm0 = (wave & 1) * WM;
   const int wn0 = (wave >> 1) * WN;
 
+  double4_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  double ra[A_ELEMS], rb[B_ELEMS];
+  // op(A)(i, kk): contiguous along K when TA;  op(B)(kk, j): contiguous along K when !TB
+  TileLoader<BM, TA, VEC> la;
+  TileLoader<BN, !TB, VEC> lb;
+  la.init(lda, m0, M, tid);
+  lb.init(ldb, n0, N, tid);
+  auto load = [&](int k0) {
+    if (k0 + BK <= kEnd) {
+      la.load(ra, A, lda, k0);
+      lb.load(rb, B, ldb, k0);
+    } else {
+      la.load_tail(ra, A, lda, k0, kEnd, tid);
+      lb.load_tail(rb, B, ldb, k0, kEnd, tid);
+    }
+  };
+  auto store = [&](int buf) {
+    tile_store<BM, TA, VEC>(ra, As[buf], tid);
+    tile_store<BN, !TB, VEC>(rb, Bs[buf], tid);
+  };
+  auto compute = [&](int cur) {
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double fa[FM], fb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  if (kBegin < kEnd) {
+    load(kBegin);
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    // full slices: prefetch k0 + BK while computing k0 (one barrier per slice)
+    const int kFull = kBegin + ((kEnd - kBegin) / BK) * BK;  // end of the full slices
+    int k0 = kBegin;
+    for (; k0 + 2 * BK <= kFull; k0 += BK) {
+      la.load(ra, A, lda, k0 + BK);
+      lb.load(rb, B, ldb, k0 + BK);
+      compute(cur);
+      store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+    // remaining one or two slices (the second may be partial)
+    if (k0 + BK < kEnd) {
+      load(k0 + BK);
+      compute(cur);
+      store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+    compute(cur);
+  }
+
+  // Epilogue.  The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in
+  // register r of lane `lane`; with the operands swapped D = C', so
+  // acc[i][j][r] = C(row = m0+wm0+16i+lane%16, col = n0+wn0+16j+lane/16+4r).
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = m0 + wm0 + i * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int col = n0 + wn0 + j * 16 + (lane >> 4) + 4 * rr;
+        if (row < M && col < N && (!lower_out || row >= col)) {
+          const double v = acc[i][j][rr];
+          double* cp = C + row + (long long)col * ldc;
+          *cp = (beta == 0.0) ? alpha * v : alpha * v + beta * (*cp);
+        }
+      }
+    }
+  }
+}
+
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+__device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws) {
   const int b = blockIdx.y;
   if (g.skip && g.skip[(long long)b * g.skip_stride] != 0) return;
 
@@ -198,100 +295,28 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
     kBegin = sb;
     kEnd = se;
   }
-  double4_t acc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+  // split-K partials go to W (M x M per split, no beta); otherwise straight to C
+  const bool part = split > 1;
+  double* out = part ? ws + ((long long)b * split + s) * (long long)g.M * g.N : g.C + (long long)b * g.strideC;
+  gemm_core<BM, BN, TA, TB, VEC>(A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
+                                 part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta);
+}
 
-  double ra[A_ELEMS], rb[B_ELEMS];
-  // op(A)(i, kk): contiguous along K when TA;  op(B)(kk, j): contiguous along K when !TB
-  TileLoader<BM, TA, VEC> la;
-  TileLoader<BN, !TB, VEC> lb;
-  la.init(g.lda, m0, g.M, tid);
-  lb.init(g.ldb, n0, g.N, tid);
-  auto load = [&](int k0) {
-    if (k0 + BK <= kEnd) {
-      la.load(ra, A, g.lda, k0);
-      lb.load(rb, B, g.ldb, k0);
-    } else {
-      la.load_tail(ra, A, g.lda, k0, kEnd, tid);
-      lb.load_tail(rb, B, g.ldb, k0, kEnd, tid);
-    }
-  };
-  auto store = [&](int buf) {
-    tile_store<BM, TA, VEC>(ra, As[buf], tid);
-    tile_store<BN, !TB, VEC>(rb, Bs[buf], tid);
-  };
-  auto compute = [&](int cur) {
-#pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
-      const int kr = ks * 4 + (lane >> 4);
-      double fa[FM], fb[FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn0 + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[j], fa[i], acc[i][j], 0, 0, 0);
-    }
-  };
-
-  if (kBegin < kEnd) {
-    load(kBegin);
-    store(0);
-    __syncthreads();
-    int cur = 0;
-    // full slices: prefetch k0 + BK while computing k0 (one barrier per slice)
-    const int kFull = kBegin + ((kEnd - kBegin) / BK) * BK;  // end of the full slices
-    int k0 = kBegin;
-    for (; k0 + 2 * BK <= kFull; k0 += BK) {
-      la.load(ra, A, g.lda, k0 + BK);
-      lb.load(rb, B, g.ldb, k0 + BK);
-      compute(cur);
-      store(cur ^ 1);
-      __syncthreads();
-      cur ^= 1;
-    }
-    // remaining one or two slices (the second may be partial)
-    if (k0 + BK < kEnd) {
-      load(k0 + BK);
-      compute(cur);
-      store(cur ^ 1);
-      __syncthreads();
-      cur ^= 1;
-    }
-    compute(cur);
-  }
-
-  // Epilogue.  The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in
-  // register r of lane `lane`; with the operands swapped D = C', so
-  // acc[i][j][r] = C(row = m0+wm0+16i+lane%16, col = n0+wn0+16j+lane/16+4r).
-  double* C = g.C + (long long)b * g.strideC;
-  double* W = (split > 1) ? ws + ((long long)b * split + s) * (long long)g.M * g.N : nullptr;
-#pragma unroll
-  for (int i = 0; i < FM; ++i) {
-    const int row = m0 + wm0 + i * 16 + (lane & 15);
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int col = n0 + wn0 + j * 16 + (lane >> 4) + 4 * rr;
-        if (row < g.M && col < g.N && (!lower_out || row >= col)) {
-          const double v = acc[i][j][rr];
-          if (W) {
-            W[row + (long long)col * g.M] = g.alpha * v;
-          } else {
-            double* cp = C + row + (long long)col * g.ldc;
-            *cp = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * (*cp);
-          }
-        }
-      }
-    }
-  }
+// ---- grouped launches: one work-group per (GEMM, tile) item ------------------
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(256, 2) void gemm_grouped_kernel(const GroupGemm* __restrict__ G,
+                                                             const GroupTile* __restrict__ T) {
+  const GroupTile t = T[blockIdx.x];
+  const GroupGemm g = G[t.g];
+  if (g.skip && *g.skip != 0) return;
+  const int m0 = t.ti * BM, n0 = t.tj * BN;
+  int kBegin = 0, kEnd = g.K;
+  if (g.flags & kGemmALower) kEnd = min(kEnd, m0 + BM);
+  if (g.flags & kGemmBLower) kBegin = max(kBegin, n0);
+  if (g.flags & kGemmAUpper) kBegin = max(kBegin, m0);
+  kBegin = (kBegin / BK) * BK;
+  gemm_core<BM, BN, TA, TB, VEC>(g.A, g.lda, g.B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
+                                 (g.flags & kGemmLowerOut) != 0, g.C, g.ldc, g.alpha, g.beta);
 }
 
 template <int BM, int BN, bool TA, bool TB, bool VEC>
@@ -404,6 +429,33 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
 }  // namespace sdpx
 
 namespace sdpx {
+
+int gemm_f64_grouped(const GroupGemm* groups, const GroupTile* tiles, int ntiles, bool ta, bool tb, bool vec,
+                     hipStream_t stream) {
+  if (ntiles <= 0) return kOk;
+  const dim3 grid((unsigned)ntiles), block(256);
+  if (ta && !tb) {
+    if (vec)
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, true, false, true>), grid, block, 0, stream, groups, tiles);
+    else
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, true, false, false>), grid, block, 0, stream, groups, tiles);
+  } else if (!ta && tb) {
+    if (vec)
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, false, true, true>), grid, block, 0, stream, groups, tiles);
+    else
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, false, true, false>), grid, block, 0, stream, groups, tiles);
+  } else if (!ta && !tb) {
+    if (vec)
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, false, false, true>), grid, block, 0, stream, groups, tiles);
+    else
+      hipLaunchKernelGGL((gemm_grouped_kernel<64, 64, false, false, false>), grid, block, 0, stream, groups, tiles);
+  } else {
+    set_last_error("gemm_f64_grouped: TT products are not instantiated");
+    return kInvalidArgument;
+  }
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
 
 void lower_tile_order(int nt, int group, int2* out) {
   int n = 0;

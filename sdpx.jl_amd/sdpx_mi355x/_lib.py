@@ -55,6 +55,17 @@ SIGNATURES = {
     "sdpx_dense_set_profiling": [_h, C.c_int],
     "sdpx_dense_kernel_times": [_h, C.c_int, _dp],
     "sdpx_dense_schur_split": [_h, _ip],
+    "sdpx_batch_default_options": [C.c_void_p, C.c_double],
+    "sdpx_batch_create": [C.c_int, C.c_int, _ip, _ip, C.POINTER(_h)],
+    "sdpx_batch_destroy": [_h],
+    "sdpx_batch_memory": [_h, C.POINTER(C.c_longlong)],
+    "sdpx_batch_set_dense": [_h, _dp, _dp, _dp],
+    "sdpx_batch_set_coo": [_h, C.POINTER(C.c_longlong), _ip, _ip, _dp, _dp, _dp],
+    "sdpx_batch_set_start": [_h, _dp, _dp, _dp],
+    "sdpx_batch_solve": [_h, C.c_void_p, _ip],
+    "sdpx_batch_get_result": [_h, _ip, _ip, _dp, _dp, _dp, _dp, _dp],
+    "sdpx_batch_get_schur": [_h, C.c_int, _dp],
+    "sdpx_batch_schur_at_start": [_h],
     "sdpx_soc_create": [C.c_int, C.c_int, _ip, C.POINTER(_h)],
     "sdpx_soc_destroy": [_h],
     "sdpx_soc_memory": [_h, C.POINTER(C.c_longlong)],

@@ -1,0 +1,165 @@
+"""Batched solver: many independent single-block SDPs solved in lock step on one GPU.
+
+Host binding of the sdpx_batch_* entry points (include/sdpx_mi355x.h).  Each
+member is a core-form SDP  min c'x  s.t.  sum_i x_i A_i - C >= 0  with one PSD
+block of side k <= 64 and m <= 2048 constraints; every Newton-step phase is
+one launch over the members still iterating, the per-member interior-point
+control runs in the library.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+from .policy import SolverOptions
+
+STATUS = {0: "Optimal", 1: "IterLimit", 2: "NumericalBreakdown", 3: "Stalled", 4: "InsufficientPrecision",
+          5: "MaxRestartsExceeded"}
+
+
+class BatchOptions(C.Structure):
+    """sdpx_batch_options_t."""
+
+    _fields_ = [
+        ("beta", C.c_double), ("gamma", C.c_double), ("eps_gap", C.c_double), ("eps_primal", C.c_double),
+        ("eps_dual", C.c_double), ("min_step", C.c_double), ("omega_step", C.c_double),
+        ("stall_tolerance", C.c_double), ("refine_tol", C.c_double), ("adaptive_sigma_max", C.c_double),
+        ("iter_max", C.c_int), ("max_restarts", C.c_int), ("max_centering", C.c_int),
+        ("stall_iterations", C.c_int), ("restart", C.c_int), ("refine_policy", C.c_int),
+        ("refine_steps", C.c_int), ("refine_max_steps", C.c_int), ("adaptive", C.c_int), ("step_rule", C.c_int),
+        ("precision", C.c_int), ("cond_limit", C.c_double), ("refine_target", C.c_double),
+    ]
+
+    @classmethod
+    def from_solver_options(cls, o: SolverOptions, precision: int = 0) -> "BatchOptions":
+        b = cls()
+        check(_lib.load().sdpx_batch_default_options(C.addressof(b), o.eps_gap), "sdpx_batch_default_options")
+        for name in ("beta", "gamma", "eps_gap", "eps_primal", "eps_dual", "min_step", "omega_step",
+                     "stall_tolerance", "refine_tol", "adaptive_sigma_max", "iter_max", "max_restarts",
+                     "max_centering", "stall_iterations", "refine_steps", "refine_max_steps"):
+            setattr(b, name, getattr(o, name))
+        b.restart = int(o.restart)
+        b.refine_policy = 1 if o.refine_policy == "fixed" else 0
+        b.adaptive = 1 if o.parameter_strategy == "adaptive" else 0
+        b.step_rule = {"auto": 0, "backtrack": 1, "fraction_to_boundary": 2}[o.step_rule]
+        b.precision = precision
+        return b
+
+
+@dataclass
+class BatchResult:
+    status: list
+    iterations: np.ndarray
+    pobj: np.ndarray
+    dobj: np.ndarray
+    x: list
+    X: list
+    Y: list
+    launched: int
+
+    @property
+    def gap_rel(self) -> np.ndarray:
+        return np.abs(self.pobj - self.dobj) / np.maximum(1.0, (np.abs(self.pobj) + np.abs(self.dobj)) / 2)
+
+
+class BatchSolver:
+    """Device-resident batch of single-block SDPs (block sides ``k``, constraint counts ``m``)."""
+
+    def __init__(self, k, m, device: int = 0):
+        lib = _lib.load()
+        self.k = np.ascontiguousarray(k, dtype=np.int32)
+        self.m = np.ascontiguousarray(m, dtype=np.int32)
+        if self.k.shape != self.m.shape or self.k.ndim != 1:
+            raise ValueError("k and m must be 1-D of equal length")
+        self.B = int(self.k.size)
+        h = C.c_void_p()
+        check(lib.sdpx_batch_create(device, self.B, self.k.ctypes.data_as(_lib._ip),
+                                    self.m.ctypes.data_as(_lib._ip), C.byref(h)), "sdpx_batch_create")
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if self._h:
+            self._lib.sdpx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def memory_bytes(self) -> int:
+        v = C.c_longlong(0)
+        check(self._lib.sdpx_batch_memory(self._h, C.byref(v)), "sdpx_batch_memory")
+        return v.value
+
+    # ---- data -------------------------------------------------------------------------------
+    def set_dense(self, Av_list, C_list, c_list) -> None:
+        """Av_list[p]: (m_p, k_p^2) rows = vec(A_i) (column-major), C_list[p]: (k_p, k_p), c_list[p]: (m_p,)."""
+        Av = np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in Av_list])
+        Cc = np.concatenate([np.asarray(c_, dtype=np.float64).ravel(order="F") for c_ in C_list])
+        cc = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in c_list])
+        check(self._lib.sdpx_batch_set_dense(self._h, dptr(Av), dptr(Cc), dptr(cc)), "sdpx_batch_set_dense")
+
+    def set_coo(self, rows, cols, vals, C_list, c_list) -> None:
+        """rows/cols/vals: per constraint of every member (concatenated), upper-triangle COO."""
+        counts = np.array([len(r) for r in rows], dtype=np.int64)
+        ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        r = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
+        c = np.concatenate(cols).astype(np.int32) if cols else np.zeros(0, np.int32)
+        v = np.concatenate(vals).astype(np.float64) if vals else np.zeros(0)
+        Cc = np.concatenate([np.asarray(c_, dtype=np.float64).ravel(order="F") for c_ in C_list])
+        cc = np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in c_list])
+        check(self._lib.sdpx_batch_set_coo(self._h, ptr.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                           r.ctypes.data_as(_lib._ip), c.ctypes.data_as(_lib._ip), dptr(v),
+                                           dptr(Cc), dptr(cc)), "sdpx_batch_set_coo")
+
+    def set_start(self, x_list, X_list, Y_list) -> None:
+        x = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in x_list])
+        X = np.concatenate([np.asarray(v, dtype=np.float64).ravel(order="F") for v in X_list])
+        Y = np.concatenate([np.asarray(v, dtype=np.float64).ravel(order="F") for v in Y_list])
+        check(self._lib.sdpx_batch_set_start(self._h, dptr(x), dptr(X), dptr(Y)), "sdpx_batch_set_start")
+
+    # ---- solve ----------------------------------------------------------------------------------
+    def solve(self, o: SolverOptions | BatchOptions | None = None) -> BatchResult:
+        bo = o if isinstance(o, BatchOptions) else BatchOptions.from_solver_options(o or SolverOptions())
+        it = C.c_int(0)
+        check(self._lib.sdpx_batch_solve(self._h, C.addressof(bo), C.byref(it)), "sdpx_batch_solve")
+        return self.result(it.value)
+
+    def result(self, launched: int = 0) -> BatchResult:
+        B = self.B
+        status = np.empty(B, dtype=np.int32)
+        iters = np.empty(B, dtype=np.int32)
+        pobj, dobj = np.empty(B), np.empty(B)
+        x = np.empty(int(self.m.sum()))
+        k2 = (self.k.astype(np.int64) ** 2)
+        X, Y = np.empty(int(k2.sum())), np.empty(int(k2.sum()))
+        check(self._lib.sdpx_batch_get_result(self._h, status.ctypes.data_as(_lib._ip), iters.ctypes.data_as(_lib._ip),
+                                              dptr(pobj), dptr(dobj), dptr(x), dptr(X), dptr(Y)),
+              "sdpx_batch_get_result")
+        xs = np.split(x, np.cumsum(self.m)[:-1])
+        Xs = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(X, np.cumsum(k2)[:-1]), self.k)]
+        Ys = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(Y, np.cumsum(k2)[:-1]), self.k)]
+        return BatchResult([STATUS[int(s)] for s in status], iters, pobj, dobj, xs, Xs, Ys, launched)
+
+    def schur_at_start(self) -> int:
+        return check(self._lib.sdpx_batch_schur_at_start(self._h), "sdpx_batch_schur_at_start")
+
+    def schur(self, member: int) -> np.ndarray:
+        mm = int(self.m[member])
+        S = np.empty(mm * mm)
+        check(self._lib.sdpx_batch_get_schur(self._h, member, dptr(S)), "sdpx_batch_get_schur")
+        return S.reshape(mm, mm, order="F")

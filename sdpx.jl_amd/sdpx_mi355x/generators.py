@@ -118,21 +118,46 @@ def sparse_feasible(k: int, m: int, seed: int) -> SparseSDP:
     iu = np.triu_indices(k)
     npos = iu[0].size
     nnz = min(3 * k, npos)
-    rows, cols, vals = [], [], []
-    b = np.zeros(m)
     y0 = rng.standard_normal(m)
+    # nnz distinct positions per constraint: the first nnz of a random order
+    pick = np.argsort(rng.random((m, npos)), axis=1)[:, :nnz]
+    r = iu[0][pick].astype(np.int32)
+    cc = iu[1][pick].astype(np.int32)
+    v = rng.standard_normal((m, nnz))
+    diag = r == cc
+    b = np.where(diag, v, 0.0).sum(axis=1)
     Cstd = np.eye(k)
-    for i in range(m):
-        pick = rng.choice(npos, size=nnz, replace=False)
-        r = iu[0][pick].astype(np.int32)
-        cc = iu[1][pick].astype(np.int32)
-        v = rng.standard_normal(nnz)
-        rows.append(r)
-        cols.append(cc)
-        vals.append(-v)  # core sign
-        diag = r == cc
-        b[i] = float(v[diag].sum())
-        np.add.at(Cstd, (r, cc), y0[i] * v)
-        off = ~diag
-        np.add.at(Cstd, (cc[off], r[off]), y0[i] * v[off])
-    return SparseSDP(k=k, m=m, rows=rows, cols=cols, vals=vals, C=-Cstd, c=-b, x0=y0)
+    w = y0[:, None] * v
+    np.add.at(Cstd, (r.ravel(), cc.ravel()), w.ravel())
+    off = ~diag
+    np.add.at(Cstd, (cc[off], r[off]), w[off])
+    return SparseSDP(k=k, m=m, rows=list(r), cols=list(cc), vals=list(-v), C=-Cstd, c=-b, x0=y0)
+
+
+def batched_sparse(B: int, seed: int = 0, kmin: int = 8, kmax: int = 64) -> list:
+    """The uneven batched workload: B problems, k ~ U{kmin..kmax}, m ~ U{2k..floor(k(k+1)/4)},
+    3k nonzero positions per A_i (sparse_feasible), one seed per problem."""
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(kmin, kmax + 1, size=B)
+    out = []
+    for p, k in enumerate(ks):
+        k = int(k)
+        mlo, mhi = 2 * k, max(2 * k, (k * (k + 1)) // 4)
+        m = int(rng.integers(mlo, mhi + 1))
+        out.append(sparse_feasible(k, m, seed=seed * 1_000_003 + p + 1))
+    return out
+
+
+def batched_dense_layer(B: int, seed: int = 0, k: int = 32, m: int = 128, noise: float = 0.1) -> list:
+    """The differentiable-layer workload: B dense Gaussian problems (dense_feasible), with C and b
+    perturbed by seeded N(0, noise^2) noise (C's noise symmetric)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for p in range(B):
+        g = dense_feasible(k, m, seed=seed * 1_000_003 + p + 1)
+        E = rng.standard_normal((k, k)) * noise
+        E = np.triu(E) + np.triu(E, 1).T
+        db = rng.standard_normal(m) * noise
+        # core signs: C_core = -C, c = -b
+        out.append(DenseSDP(k=k, m=m, Av_rows=g.Av_rows, C=g.C - E, c=g.c - db, x0=g.x0, X0=g.X0, Y0=g.Y0))
+    return out

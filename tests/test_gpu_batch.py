@@ -1,0 +1,164 @@
+"""Batched route: every member against the single-problem oracle, through the C-ABI.
+
+There is no batched solver in the reference, so each member is pinned
+individually: its Schur matrix against the oracle's schur_build at the same
+point, its solve against the oracle's solve of that member alone (status,
+objective 1e-8 relative, iterations +-2, certificate).
+"""
+
+import numpy as np
+import pytest
+
+import sdpx_oracle as O
+
+from sdpx_mi355x import generators as G
+from sdpx_mi355x.batch import BatchSolver
+from sdpx_mi355x.policy import SolverOptions
+
+from _helpers import rel_fro_lower
+
+
+def _oracle_problem(g):
+    A = g.A_blocks() if isinstance(g, G.DenseSDP) else g.dense_rows().reshape(g.m, g.k, g.k)
+    return O.ingest(g.c, [A], [g.C])
+
+
+def _load(bs, probs):
+    dense = [g for g in probs if isinstance(g, G.DenseSDP)]
+    if len(dense) == len(probs):
+        bs.set_dense([g.Av_rows for g in probs], [g.C for g in probs], [g.c for g in probs])
+    else:
+        rows, cols, vals = [], [], []
+        for g in probs:
+            rows += list(g.rows)
+            cols += list(g.cols)
+            vals += list(g.vals)
+        bs.set_coo(rows, cols, vals, [g.C for g in probs], [g.c for g in probs])
+
+
+def _random_pd(rng, k):
+    Q = rng.standard_normal((k, k))
+    return Q @ Q.T / k + 0.5 * np.eye(k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["dense", "coo"])
+def test_batch_schur_matches_oracle(gpu_lib, kind):
+    rng = np.random.default_rng(3)
+    ks = [3, 8, 17, 33, 64, 1, 2, 40]
+    ms = [7, 20, 50, 70, 130, 3, 5, 300]
+    probs = [G.dense_feasible(k, m, seed=10 + i) if kind == "dense" else G.sparse_feasible(max(k, 1), m, seed=10 + i)
+             for i, (k, m) in enumerate(zip(ks, ms))]
+    Xs = [_random_pd(rng, k) for k in ks]
+    Ys = [_random_pd(rng, k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], Xs, Ys)
+        assert bs.schur_at_start() == 0
+        for p, g in enumerate(probs):
+            prob = _oracle_problem(g)
+            LX, _ = O.kchol(Xs[p])
+            MY, _ = O.kchol(Ys[p])
+            ref = O.schur_build(prob, [LX], [MY])
+            got = bs.schur(p)
+            print(f"member {p}: k={ks[p]} m={ms[p]} rel={rel_fro_lower(got, ref):.2e}", flush=True)
+            assert rel_fro_lower(got, ref) <= 1e-12
+
+
+def _check_members(res, probs, o, oo, max_iter_diff=2):
+    for p, g in enumerate(probs):
+        prob = _oracle_problem(g)
+        X0 = np.eye(g.k)
+        ref = O.solve(prob, oo, x0=g.x0, X0=[X0], Y0=[X0])
+        print(f"member {p}: k={g.k} m={g.m} gpu {res.status[p]} it={res.iterations[p]} pobj={res.pobj[p]:.10e} | "
+              f"oracle {ref.status} it={ref.iterations} pobj={ref.pobj:.10e}", flush=True)
+        assert res.status[p] == ref.status
+        if ref.status != "Optimal":
+            continue
+        assert abs(res.pobj[p] - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+        assert abs(int(res.iterations[p]) - ref.iterations) <= max_iter_diff
+        r = O.Result(res.status[p], res.x[p], [res.X[p]], [res.Y[p]], res.pobj[p], res.dobj[p],
+                     int(res.iterations[p]), 0.0, 0.0, float(res.gap_rel[p]))
+        cert = O.certificate(prob, r, o.eps_primal, o.eps_dual, o.eps_gap)
+        assert cert["valid"], (p, cert)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strategy", ["adaptive", "fixed"])
+def test_batch_dense_members_match_oracle(gpu_lib, strategy):
+    ks = [5, 12, 20, 32, 9, 3, 2, 1]
+    ms = [12, 40, 60, 128, 30, 6, 3, 2]
+    probs = [G.dense_feasible(k, m, seed=100 + i) for i, (k, m) in enumerate(zip(ks, ms))]
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, parameter_strategy=strategy)
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol, parameter_strategy=strategy)
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], [np.eye(k) for k in ks], [np.eye(k) for k in ks])
+        res = bs.solve(o)
+    _check_members(res, probs, o, oo)
+
+
+@pytest.mark.gpu
+def test_batch_sparse_uneven_members_match_oracle(gpu_lib):
+    """A small draw of the uneven sparse workload (k in [8, 24] here), COO input densified on the device."""
+    probs = G.batched_sparse(6, seed=5, kmin=8, kmax=24)
+    ks = [g.k for g in probs]
+    ms = [g.m for g in probs]
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], [np.eye(k) for k in ks], [np.eye(k) for k in ks])
+        res = bs.solve(o)
+    _check_members(res, probs, o, oo)
+
+
+@pytest.mark.gpu
+def test_batch_layer_members_match_oracle(gpu_lib):
+    """The differentiable-layer shape (k = 32, m = 128, perturbed C and b) at gap 1e-9."""
+    probs = G.batched_dense_layer(4, seed=2)
+    tol = 1e-9
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    with BatchSolver([32] * 4, [128] * 4) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], [np.eye(32)] * 4, [np.eye(32)] * 4)
+        res = bs.solve(o)
+    _check_members(res, probs, o, oo)
+
+
+@pytest.mark.gpu
+def test_batch_member_failure_is_isolated(gpu_lib):
+    """A member whose start is not positive definite breaks down alone; the others solve."""
+    ks, ms = [6, 6, 10], [15, 15, 30]
+    probs = [G.dense_feasible(k, m, seed=200 + i) for i, (k, m) in enumerate(zip(ks, ms))]
+    X0 = [np.eye(k) for k in ks]
+    X0[1] = -np.eye(6)
+    o = SolverOptions(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8)
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], X0, [np.eye(k) for k in ks])
+        res = bs.solve(o)
+    assert res.status[1] == "NumericalBreakdown"
+    assert res.status[0] == res.status[2] == "Optimal"
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_problem_gpu_path(gpu_lib):
+    """Same member through the batched route and the single-problem dense route."""
+    from sdpx_mi355x import ingest, solve
+
+    g = G.dense_feasible(24, 90, seed=7)
+    o = SolverOptions(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8)
+    single = solve(ingest(g.c, [g.A_blocks()], [g.C]), o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    with BatchSolver([24, 24], [90, 90]) as bs:
+        _load(bs, [g, g])
+        bs.set_start([g.x0, g.x0], [g.X0, g.X0], [g.Y0, g.Y0])
+        res = bs.solve(o)
+    assert res.status == ["Optimal", "Optimal"]
+    assert res.iterations[0] == res.iterations[1]
+    assert res.pobj[0] == res.pobj[1]  # members are independent and deterministic
+    assert abs(int(res.iterations[0]) - single.iterations) <= 1
+    assert abs(res.pobj[0] - single.pobj) <= 1e-8 * max(1.0, abs(single.pobj))
