@@ -7,6 +7,7 @@
 // restarts, best iterate) runs on the host in C++ over per-member scalars
 // read back once per phase.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <deque>
 #include <string>
@@ -32,6 +33,7 @@ constexpr double kProjectionSlack = 10.0;     // stagnation.jl:26
 constexpr int kPanelChunk = 16;               // constraints per Schur panel work-group
 constexpr int kAlign = 32;                    // doubles: every member offset is 256-byte aligned
 constexpr int kListRing = 64;                 // member-list staging slots between stream drains
+constexpr int kTraceCols = SDPX_BATCH_TRACE_COLS;
 
 long long align_up(long long v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -355,6 +357,14 @@ struct MemberState {
   bool refine_fixed = false, refine_active = false;
   double refine_abstol = 0, refine_residual = 0;
   double tX = 1, tY = 1;
+  // one row per Newton step: iteration, pobj, dobj, p_res, d_res (of the step), tX, tY, sigma,
+  // affine steps, regularization attempts, refinement steps, outcome (0 update, 1 centering,
+  // 2 restart, 3 termination)
+  std::vector<std::array<double, kTraceCols>> trace;
+  void log(int outcome) {
+    trace.push_back({(double)iteration, pobj, dobj, p_it, d_it, tX, tY, prm.sigma, ap, ad, (double)reg_attempts,
+                     (double)refine_steps, (double)outcome});
+  }
 };
 
 }  // namespace
@@ -369,7 +379,7 @@ struct sdpx_batch {
   std::vector<Member> mem;
   Arrays arr{};
   Member* dmem = nullptr;
-  long long nK = 0, nM = 0, nAv = 0, nPh = 0, nS = 0, nD = 0;
+  long long nK = 0, nM = 0, nAv = 0, nPh = 0, nS = 0, nD = 0, nG = 0;
   // grouped Schur Gram product
   GroupGemm* dgram = nullptr;
   GroupTile* dgram_tiles = nullptr;
@@ -924,17 +934,20 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
         if (s.gap_rel < near * o.eps_gap && s.p_res / s.scale_p < near * o.eps_primal &&
             s.d_res / s.scale_d < near * o.eps_dual) {
           s.status = kStalled;
+          s.log(3);
           continue;
         }
         if (o.stall_iterations > 0 && std::isfinite(s.initial_merit) && s.best_merit < s.initial_merit * 1e-3 &&
             s.best_merit < kNearSolutionMerit) {
           s.status = kInsufficient;
+          s.log(3);
           continue;
         }
         if (s.centering < o.max_centering && s.ctrl.beta < kCenteringBetaMax) {
           s.centering += 1;
           s.ctrl.beta = std::min(s.ctrl.beta * kCenteringBetaStep, kCenteringBetaMax);
           center.push_back(p);
+          s.log(1);
           continue;
         }
         const bool rescale_X = x_stuck && !pf_ok;
@@ -945,12 +958,14 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
           par(h, p, P_A) = rescale_X ? 1.0 : 0.0;  // which blocks to rescale (host bookkeeping only)
           par(h, p, P_B) = rescale_Y ? 1.0 : 0.0;
           restart.push_back(p);
+          s.log(2);
           continue;
         }
         if (o.restart && s.restarts < o.max_restarts)
           s.status = kStalled;
         else
           s.status = s.restarts >= o.max_restarts ? kMaxRestarts : kBreakdown;
+        s.log(3);
         continue;
       }
       par(h, p, P_A) = tX;
@@ -1010,6 +1025,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       s.iteration += 1;
       s.p_res = p_after;
       s.d_res = d_after;
+      s.log(0);
     }
   }
   // ---- results: the best iterate replaces a non-optimal final one ---------------------------------
@@ -1114,6 +1130,10 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
     h->nS += align_up((long long)m[p] * m[p]);
     M.oD = h->nD;
     h->nD += (long long)M.nblk * kCholNb * kCholNb;
+    const int nch = ceil_div(m[p], kColChunk);
+    M.oG = h->nG;
+    h->nG += align_up((long long)nch * k2);
+    h->arr.max_chunks = std::max(h->arr.max_chunks, nch);
   }
   int s;
   Arrays& a = h->arr;
@@ -1131,6 +1151,7 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
   if ((s = balloc_n(h, &a.S, h->nS))) return fail(s);
   if ((s = balloc_n(h, &a.Sf, h->nS))) return fail(s);
   if ((s = balloc_n(h, &a.dinv, h->nD))) return fail(s);
+  if ((s = balloc_n(h, &a.gpart, h->nG))) return fail(s);
   if ((s = balloc_n(h, &a.sc, (long long)B * NSLOT))) return fail(s);
   if ((s = balloc_n(h, &a.si, (long long)B * NISLOT))) return fail(s);
   if ((s = balloc_n(h, &a.par, (long long)B * NPARAM))) return fail(s);
@@ -1493,6 +1514,21 @@ int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double*
   if (X) SDPX_CHECK(download_blocks(h, X, h->arr.X, true));
   if (Y) SDPX_CHECK(download_blocks(h, Y, h->arr.Y, true));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_batch_get_trace(sdpx_batch_t* h, int member, double* rows, int cap, int* n) {
+  SDPX_CHECK(check_batch(h));
+  if (member < 0 || member >= h->B) {
+    set_last_error("sdpx_batch_get_trace: invalid member");
+    return kInvalidArgument;
+  }
+  const auto& tr = h->st[member].trace;
+  const int cnt = (int)tr.size();
+  if (n) *n = cnt;
+  if (rows)
+    for (int i = 0; i < std::min(cnt, cap); ++i)
+      for (int j = 0; j < kTraceCols; ++j) rows[(size_t)i * kTraceCols + j] = tr[i][j];
   return kOk;
 }
 

@@ -164,25 +164,45 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ---- HBM-bound constraint-panel passes ----------------------------------------------
-__global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int mode) {
-  __shared__ double red[NT];
+// The panel passes split every member's m columns into chunks of kColChunk
+// (grid.y = chunk; work-groups past a member's last chunk exit at once), so a
+// large member streams through many CUs.  Av x accumulates per chunk into
+// gpart (fixed chunk order in the combining kernel: deterministic).
+__global__ __launch_bounds__(NT) void gemv_part_kernel(Arrays a, const int* list, int mode) {
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
+  const int i0 = blockIdx.y * kColChunk;
+  if (i0 >= M.m) return;
+  const int i1 = min(M.m, i0 + kColChunk);
   const long long k2 = (long long)M.k * M.k;
   const double* Av = a.Av + M.oAv;
   const double* vec = (mode == 1 ? a.dx : a.x) + M.oM;
-  double amax = 0.0;
+  double* part = a.gpart + M.oG + (long long)blockIdx.y * k2;
   for (long long e = threadIdx.x; e < k2; e += NT) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = 0;
-    for (; i + 4 <= M.m; i += 4) {
+    int i = i0;
+    for (; i + 4 <= i1; i += 4) {
       s0 += Av[(long long)i * k2 + e] * vec[i];
       s1 += Av[(long long)(i + 1) * k2 + e] * vec[i + 1];
       s2 += Av[(long long)(i + 2) * k2 + e] * vec[i + 2];
       s3 += Av[(long long)(i + 3) * k2 + e] * vec[i + 3];
     }
-    for (; i < M.m; ++i) s0 += Av[(long long)i * k2 + e] * vec[i];
-    const double acc = (s0 + s1) + (s2 + s3);
+    for (; i < i1; ++i) s0 += Av[(long long)i * k2 + e] * vec[i];
+    part[e] = (s0 + s1) + (s2 + s3);
+  }
+}
+
+__global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const long long k2 = (long long)M.k * M.k;
+  const int nch = (M.m + kColChunk - 1) / kColChunk;
+  const double* part = a.gpart + M.oG;
+  double amax = 0.0;
+  for (long long e = threadIdx.x; e < k2; e += NT) {
+    double acc = 0.0;
+    for (int c = 0; c < nch; ++c) acc += part[(long long)c * k2 + e];
     double out;
     if (mode == 1) {
       out = acc + a.P[M.oK + e];
@@ -201,36 +221,53 @@ __global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int
 
 __global__ __launch_bounds__(NT) void gemvt_kernel(Arrays a, const int* list, int mode) {
   __shared__ double W[KMAX * KMAX];
-  __shared__ double red[NT];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
+  const int i0 = blockIdx.y * kColChunk;
+  if (i0 >= M.m) return;
+  const int i1 = min(M.m, i0 + kColChunk);
   const int k2 = M.k * M.k;
   const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
   for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
   __syncthreads();
   const double* Av = a.Av + M.oAv;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double amax = 0.0;
-  for (int i = wave; i < M.m; i += NT / 64) {
-    const double* col = Av + (long long)i * k2;
-    double s = 0.0;
-    for (int e = lane; e < k2; e += 64) s += col[e] * W[e];
-    s = wave_sum(s);
+  // two columns per pass for more loads in flight
+  for (int i = i0 + 2 * wave; i < i1; i += 2 * (NT / 64)) {
+    const bool two = i + 1 < i1;
+    const double* c0 = Av + (long long)i * k2;
+    const double* c1 = c0 + k2;
+    double s0 = 0.0, s1 = 0.0;
+    for (int e = lane; e < k2; e += 64) {
+      const double we = W[e];
+      s0 += c0[e] * we;
+      if (two) s1 += c1[e] * we;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
     if (lane == 0) {
-      const long long o = M.oM + i;
-      if (mode == 1) {
-        a.r[o] = -(a.d[o] + s);
-      } else {
-        const double v = a.c[o] - s;
-        if (mode == 0) a.d[o] = v;
-        amax = nan_max(amax, fabs(v));
+      for (int h = 0; h < (two ? 2 : 1); ++h) {
+        const long long o = M.oM + i + h;
+        const double s = h == 0 ? s0 : s1;
+        if (mode == 1)
+          a.r[o] = -(a.d[o] + s);
+        else
+          (mode == 0 ? a.d : a.v)[o] = a.c[o] - s;
       }
     }
   }
-  if (mode != 1) {
-    amax = block_max(amax, red);
-    if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_DRES : S_DSOL)] = amax;
-  }
+}
+
+// ||d||_inf (mode 0, S_DRES) or ||v||_inf (mode 2, S_DSOL) after gemvt
+__global__ __launch_bounds__(NT) void vec_norm_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double red[NT];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const double* v = (mode == 0 ? a.d : a.v) + M.oM;
+  double amax = 0.0;
+  for (int i = threadIdx.x; i < M.m; i += NT) amax = nan_max(amax, fabs(v[i]));
+  amax = block_max(amax, red);
+  if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_DRES : S_DSOL)] = amax;
 }
 
 // ---- k x k block phases (LDS) ----------------------------------------------------------
@@ -399,10 +436,11 @@ __global__ __launch_bounds__(NT) void corrector_block_kernel(Arrays a, const int
 
 // ---- step length / PSD checks (kernels/generic.jl:292-420, step.jl:1025-1098) ---------
 // trial = base + t * dir (lower triangle) factored in sT; returns info
+// (base and dir are read from global memory, leading dimension k: they stay in L2)
 __device__ int trial_chol(const double* base, const double* dir, double t, double* sT, int k, double* red) {
   for (int e = threadIdx.x; e < k * k; e += NT) {
     const int r = e % k, c = e / k;
-    if (r >= c) sT[r + c * LP] = base[r + c * LP] + t * dir[r + c * LP];
+    if (r >= c) sT[r + c * LP] = base[e] + t * dir[e];
   }
   return lds_chol(sT, k, red);
 }
@@ -514,7 +552,7 @@ __device__ bool small_margin(const double* X, const double* dX, int k, double t,
 #pragma clang fp contract(on)
 
 __global__ __launch_bounds__(NT) void ftb_kernel(Arrays a, const int* list) {
-  __shared__ double sB[KMAX * LP], sD[KMAX * LP], sT[KMAX * LP];
+  __shared__ double sB[2 * LP], sD[2 * LP], sT[KMAX * LP];  // sB, sD: blocks of side <= 2
   __shared__ double red[NT];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
@@ -523,11 +561,17 @@ __global__ __launch_bounds__(NT) void ftb_kernel(Arrays a, const int* list) {
   const double* par = a.par + (long long)p * NPARAM;
   double b[2];
   for (int side = 0; side < 2; ++side) {
-    __syncthreads();
-    lds_load(sB, (side == 0 ? a.X : a.Y) + o, k);
-    lds_load(sD, (side == 0 ? a.dX : a.dY) + o, k);
-    __syncthreads();
-    b[side] = k <= 2 ? small_ftb(sB, sD, k) : ftb_side(sB, sD, sT, k, red);
+    const double* base = (side == 0 ? a.X : a.Y) + o;
+    const double* dir = (side == 0 ? a.dX : a.dY) + o;
+    if (k <= 2) {
+      __syncthreads();
+      lds_load(sB, base, k);
+      lds_load(sD, dir, k);
+      __syncthreads();
+      b[side] = small_ftb(sB, sD, k);
+    } else {
+      b[side] = ftb_side(base, dir, sT, k, red);
+    }
   }
   if (threadIdx.x == 0) {
     a.sc[(long long)p * NSLOT + S_TX] = b[0] < 1.0 ? par[P_PF] * b[0] : 1.0;
@@ -539,7 +583,7 @@ __global__ __launch_bounds__(NT) void ftb_kernel(Arrays a, const int* list) {
 // multiplication); a side stops at its first passing trial or at the first
 // candidate below min_step
 __global__ __launch_bounds__(NT) void backtrack_kernel(Arrays a, const int* list) {
-  __shared__ double sB[KMAX * LP], sD[KMAX * LP], sT[KMAX * LP];
+  __shared__ double sB[2 * LP], sD[2 * LP], sT[KMAX * LP];  // sB, sD: blocks of side <= 2
   __shared__ double red[NT];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
@@ -549,13 +593,17 @@ __global__ __launch_bounds__(NT) void backtrack_kernel(Arrays a, const int* list
   const double gamma = par[P_GAMMA], min_step = par[P_MINSTEP], ratio = par[P_RATIO];
   double res[2];
   for (int side = 0; side < 2; ++side) {
-    __syncthreads();
-    lds_load(sB, (side == 0 ? a.X : a.Y) + o, k);
-    lds_load(sD, (side == 0 ? a.dX : a.dY) + o, k);
-    __syncthreads();
+    const double* base = (side == 0 ? a.X : a.Y) + o;
+    const double* dir = (side == 0 ? a.dX : a.dY) + o;
+    if (k <= 2) {
+      __syncthreads();
+      lds_load(sB, base, k);
+      lds_load(sD, dir, k);
+      __syncthreads();
+    }
     double t = 1.0;
     while (true) {
-      const bool pass = k <= 2 ? small_margin(sB, sD, k, t, ratio) : trial_ok(sB, sD, t, ratio, sT, k, red);
+      const bool pass = k <= 2 ? small_margin(sB, sD, k, t, ratio) : trial_ok(base, dir, t, ratio, sT, k, red);
       if (pass || t < min_step || !(t > 0.0)) break;  // t reaches 0 after ~1100 steps: the loop always ends
       t *= gamma;
     }
@@ -671,12 +719,17 @@ __global__ void set_int_kernel(Arrays a, const int* list, int n, int slot, int v
 
 // ---- Schur panel Phat_i = L_X^-1 A_i L_Y on FP64 MFMA ------------------------------------
 // One work-group per (member, run of `chunk` constraints).  L_X^-1 and L_Y are
-// staged once; for each constraint A_i is staged, T = L_X^-1 A_i goes to LDS
-// and Phat_i = T L_Y to HBM.  Products run on 16 x 16 MFMA tiles of the block
-// padded to kp = 16 * ceil(k / 16); the operands are fed swapped so the
-// accumulator holds the tile transposed (lane % 16 indexes rows: contiguous
-// column-major stores).  Triangular factors shorten each tile's K range.
-__global__ __launch_bounds__(NT) void panel_kernel(Arrays a, const int2* work, int chunk) {
+// staged once; each A_i is prefetched into registers one constraint ahead and
+// staged in LDS, T = L_X^-1 A_i goes to LDS and Phat_i = T L_Y to HBM.
+// Products run on 16 x 16 MFMA tiles of the block padded to kp = 16 ceil(k/16);
+// tile (ti, tj) belongs to wave (ti + tj) % 4, so every wave holds tiles of
+// every row block and the triangular K ranges balance across waves.  The
+// operands are fed swapped so the accumulator holds the tile transposed
+// (lane % 16 indexes rows: contiguous column-major stores).
+constexpr int PT = 512;                           // 8 waves: two per SIMD hide MFMA / LDS latency
+constexpr int kPanelPrefetch = KMAX * KMAX / PT;  // doubles of A_i per thread
+
+__global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, int chunk) {
   __shared__ double sLi[KMAX * LP], sLy[KMAX * LP], sA[KMAX * LP], sT[KMAX * LP];
   const int2 w = work[blockIdx.x];
   const int p = w.x;
@@ -686,86 +739,108 @@ __global__ __launch_bounds__(NT) void panel_kernel(Arrays a, const int2* work, i
   const int i0 = w.y, i1 = min(M.m, w.y + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lq = lane >> 4;
-  for (int e = threadIdx.x; e < kp * kp; e += NT) {
-    const int r = e % kp, c = e / kp;
-    const bool in = r < k && c < k;
-    sLi[r + c * LP] = in ? a.LXinv[M.oK + r + c * k] : 0.0;
-    sLy[r + c * LP] = in ? a.LY[M.oK + r + c * k] : 0.0;
-  }
-  const int ntiles = nt * nt;
-  for (int i = i0; i < i1; ++i) {
-    const double* Ai = a.Av + M.oAv + (long long)i * k * k;
-    for (int e = threadIdx.x; e < kp * kp; e += NT) {
-      const int r = e % kp, c = e / kp;
-      sA[r + c * LP] = (r < k && c < k) ? Ai[r + c * k] : 0.0;
+  // this wave's tiles: group g = (ti + tj) % 4 holds one tile per row block
+  // (nt = 4); its tiles of rank 0 and 3 (by ti) go to wave g, ranks 1 and 2 to
+  // wave g + 4, which balances the triangular K ranges of both products
+  int r0s[4], c0s[4], nmine = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) r0s[u] = c0s[u] = 0;
+  for (int t = 0; t < nt * nt; ++t) {
+    const int ti = t % nt, tj = t / nt;
+    const int g = (ti + tj) % 4;
+    int rank = 0, size = 0;
+    for (int t2 = 0; t2 < nt * nt; ++t2) {
+      const int ti2 = t2 % nt, tj2 = t2 / nt;
+      if ((ti2 + tj2) % 4 != g) continue;
+      ++size;
+      if (ti2 < ti) ++rank;
     }
+    const int half = size == 4 ? (rank == 1 || rank == 2) : (rank & 1);
+    if (g + 4 * half == wave && nmine < 4) {
+      r0s[nmine] = ti * 16;
+      c0s[nmine] = tj * 16;
+      ++nmine;
+    }
+  }
+  int qend = 0, qbeg = kp;
+  for (int u = 0; u < nmine; ++u) {
+    qend = max(qend, r0s[u] + 16);
+    qbeg = min(qbeg, c0s[u]);
+  }
+  // this thread's share of a staged block: global offset (or -1 for padding) and LDS offset
+  int goff[kPanelPrefetch], loff[kPanelPrefetch];
+#pragma unroll
+  for (int j = 0; j < kPanelPrefetch; ++j) {
+    const int e = threadIdx.x + PT * j;
+    const int r = e % kp, c = e / kp;
+    const bool live = e < kp * kp;
+    loff[j] = live ? r + c * LP : -1;
+    goff[j] = (live && r < k && c < k) ? r + c * k : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kPanelPrefetch; ++j) {
+    if (loff[j] < 0) continue;
+    sLi[loff[j]] = goff[j] >= 0 ? a.LXinv[M.oK + goff[j]] : 0.0;
+    sLy[loff[j]] = goff[j] >= 0 ? a.LY[M.oK + goff[j]] : 0.0;
+  }
+  const long long k2 = (long long)k * k;
+  double pf[kPanelPrefetch];
+  auto prefetch = [&](int i) {
+    const double* Ai = a.Av + M.oAv + (long long)i * k2;
+#pragma unroll
+    for (int j = 0; j < kPanelPrefetch; ++j) pf[j] = goff[j] >= 0 ? Ai[goff[j]] : 0.0;
+  };
+  prefetch(i0);
+  for (int i = i0; i < i1; ++i) {
+#pragma unroll
+    for (int j = 0; j < kPanelPrefetch; ++j)
+      if (loff[j] >= 0) sA[loff[j]] = pf[j];
+    if (i + 1 < i1) prefetch(i + 1);
     __syncthreads();
-    // T = L_X^-1 A_i: tile (ti, tj), rows r0 = 16 ti, K range q < r0 + 16
+    // T = L_X^-1 A_i: tile rows r0, K range q < r0 + 16 (L_X^-1 lower)
     {
       double4_t acc[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-      int qend = 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = wave + 4 * u;
-        if (t < ntiles) qend = max(qend, (t % nt) * 16 + 16);
-      }
       for (int q0 = 0; q0 < qend; q0 += 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int t = wave + 4 * u;
-          if (t >= ntiles) continue;
-          const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-          if (q0 >= r0 + 16) continue;
-          const double fb = sA[(q0 + lq) + (c0 + lr) * LP];   // B'[i'][kq] = A[q0+kq][c0+i']
-          const double fl = sLi[(r0 + lr) + (q0 + lq) * LP];  // L'[kq][j'] = Li[r0+j'][q0+kq]
+          if (u >= nmine || q0 >= r0s[u] + 16) continue;
+          const double fb = sA[(q0 + lq) + (c0s[u] + lr) * LP];   // B'[i'][kq] = A[q0+kq][c0+i']
+          const double fl = sLi[(r0s[u] + lr) + (q0 + lq) * LP];  // L'[kq][j'] = Li[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int t = wave + 4 * u;
-        if (t >= ntiles) continue;
-        const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
+        if (u >= nmine) continue;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) sT[(r0 + lr) + (c0 + lq + 4 * rr) * LP] = acc[u][rr];
+        for (int rr = 0; rr < 4; ++rr) sT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LP] = acc[u][rr];
       }
     }
     __syncthreads();
-    // Phat_i = T L_Y: tile (ti, tj), K range q >= c0
+    // Phat_i = T L_Y: tile columns c0, K range q >= c0 (L_Y lower)
     {
       double4_t acc[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-      int qbeg = kp;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = wave + 4 * u;
-        if (t < ntiles) qbeg = min(qbeg, (t / nt) * 16);
-      }
       for (int q0 = qbeg; q0 < kp; q0 += 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int t = wave + 4 * u;
-          if (t >= ntiles) continue;
-          const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-          if (q0 < c0) continue;
-          const double fb = sLy[(q0 + lq) + (c0 + lr) * LP];  // Ly[q0+kq][c0+i']
-          const double fl = sT[(r0 + lr) + (q0 + lq) * LP];   // T[r0+j'][q0+kq]
+          if (u >= nmine || q0 < c0s[u]) continue;
+          const double fb = sLy[(q0 + lq) + (c0s[u] + lr) * LP];  // Ly[q0+kq][c0+i']
+          const double fl = sT[(r0s[u] + lr) + (q0 + lq) * LP];   // T[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
       double* Pi = a.Ph + M.oPh + (long long)i * M.ldP;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int t = wave + 4 * u;
-        if (t >= ntiles) continue;
-        const int r0 = (t % nt) * 16, c0 = (t / nt) * 16;
-        const int row = r0 + lr;
+        if (u >= nmine) continue;
+        const int row = r0s[u] + lr;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int col = c0 + lq + 4 * rr;
+          const int col = c0s[u] + lq + 4 * rr;
           if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
         }
       }
@@ -954,16 +1029,26 @@ __global__ void coo_densify_kernel(Arrays a, const long long* cptr, const int* c
   } while (0)
 
 int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(gemv_part_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
+  SDPX_HIP(hipGetLastError());
   SDPX_LAUNCH(gemv_kernel, n, a, list, mode);
 }
 int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
-  SDPX_LAUNCH(gemvt_kernel, n, a, list, mode);
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(gemvt_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
+  SDPX_HIP(hipGetLastError());
+  if (mode == 1) return kOk;
+  SDPX_LAUNCH(vec_norm_kernel, n, a, list, mode);
 }
 int launch_block_factor(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(block_factor_kernel, n, a, list);
 }
 int launch_panel(const Arrays& a, const int2* work, int nwork, int chunk, hipStream_t st) {
-  SDPX_LAUNCH(panel_kernel, nwork, a, work, chunk);
+  if (nwork <= 0) return kOk;
+  hipLaunchKernelGGL(panel_kernel, dim3((unsigned)nwork), dim3(PT), 0, st, a, work, chunk);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
 }
 int launch_shift_copy(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(shift_copy_kernel, n, a, list);

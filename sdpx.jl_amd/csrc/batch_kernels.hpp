@@ -48,7 +48,10 @@ struct Member {
   long long oM;            // offset of its m-vectors
   long long oS;            // offset of its m x m matrices
   long long oD;            // offset of its diagonal-block inverses (nblk * 64 * 64)
+  long long oG;            // offset of its per-chunk Av x partial sums (ceil(m / kColChunk) * k^2)
 };
+
+constexpr int kColChunk = 64;  // constraints per work-group in the panel passes
 
 // Device buffers of a batch (all offsets per Member).
 struct Arrays {
@@ -57,6 +60,8 @@ struct Arrays {
   double *C, *X, *Y, *LX, *LY, *LXinv, *Xinv, *P, *R, *dX, *dY, *W1, *W2, *bX, *bY;  // k x k blocks
   double *c, *x, *dx, *dxb, *d, *v, *r, *rho, *delta, *bx;                             // m-vectors
   double *S, *Sf, *dinv;
+  double* gpart;   // Av x partial sums
+  int max_chunks;  // most column chunks of any member
   double* sc;   // B x NSLOT
   int* si;      // B x NISLOT
   double* par;  // B x NPARAM

@@ -155,6 +155,26 @@ class BatchSolver:
         Ys = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(Y, np.cumsum(k2)[:-1]), self.k)]
         return BatchResult([STATUS[int(s)] for s in status], iters, pobj, dobj, xs, Xs, Ys, launched)
 
+    TRACE_COLUMNS = ("iteration", "pobj", "dobj", "p_res", "d_res", "tX", "tY", "sigma", "affine_primal",
+                     "affine_dual", "reg_attempts", "refine_steps", "outcome")
+
+    def trace(self, member: int) -> list:
+        """Per-step rows of the member's last solve as dicts (outcome: update / centering / restart / end)."""
+        n = C.c_int(0)
+        check(self._lib.sdpx_batch_get_trace(self._h, member, None, 0, C.byref(n)), "sdpx_batch_get_trace")
+        cols = len(self.TRACE_COLUMNS)
+        buf = np.empty(max(n.value, 1) * cols)
+        check(self._lib.sdpx_batch_get_trace(self._h, member, dptr(buf), n.value, C.byref(n)),
+              "sdpx_batch_get_trace")
+        rows = buf[: n.value * cols].reshape(n.value, cols)
+        names = {0: "update", 1: "centering", 2: "restart", 3: "end"}
+        out = []
+        for r in rows:
+            d = dict(zip(self.TRACE_COLUMNS, r.tolist()))
+            d["outcome"] = names[int(d["outcome"])]
+            out.append(d)
+        return out
+
     def schur_at_start(self) -> int:
         return check(self._lib.sdpx_batch_schur_at_start(self._h), "sdpx_batch_schur_at_start")
 
