@@ -98,6 +98,88 @@ def soc_leg(ncones: int = 16_000_000, q: int = 4, reps: int = 10) -> dict:
     return out
 
 
+def sparse_reference_flops(k: int, m: int) -> float:
+    """The reference's sparse-A Schur + factor count per Newton step: 2k^3 + 12k^3 m + 6k m(m+1) + m^3/3."""
+    return 2.0 * k ** 3 + 12.0 * k ** 3 * m + 6.0 * k * m * (m + 1) + m ** 3 / 3.0
+
+
+def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
+    """Many independent SDPs in one batch (sdpx_batch_*), against the plain FP64 single-problem GPU route
+    run member by member on a sample of the same problems."""
+    import numpy as np
+
+    from sdpx_mi355x import MI355XDenseBackend, SolverOptions, generators, ingest, solve
+    from sdpx_mi355x.batch import BatchSolver
+
+    t0 = time.perf_counter()
+    if which == "sparse":
+        probs = generators.batched_sparse(B, seed=0)
+        tol, desc = 1e-8, f"{B} uneven sparse SDPs, n ~ U[8,64], m ~ U[2n, n(n+1)/4], 3n nonzeros per A_i"
+    else:
+        probs = generators.batched_dense_layer(B, seed=0)
+        tol, desc = 1e-9, f"{B} dense SDPs n=32, m=128 with C, b perturbed by N(0, 0.1^2)"
+    ks = [g.k for g in probs]
+    ms = [g.m for g in probs]
+    log(f"{which}: generated {B} problems in {time.perf_counter() - t0:.1f}s")
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    eyes = [np.eye(k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        if which == "sparse":
+            rows, cols, vals = [], [], []
+            for g in probs:
+                rows += list(g.rows)
+                cols += list(g.cols)
+                vals += list(g.vals)
+            bs.set_coo(rows, cols, vals, [g.C for g in probs], [g.c for g in probs])
+        else:
+            bs.set_dense([g.Av_rows for g in probs], [g.C for g in probs], [g.c for g in probs])
+        dev_gb = bs.memory_bytes() / 1e9
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        bs.solve(o)  # warm-up
+        wall, res = 0.0, None
+        for _ in range(steps):
+            bs.set_start([g.x0 for g in probs], eyes, eyes)
+            t0 = time.perf_counter()
+            res = bs.solve(o)
+            wall += time.perf_counter() - t0
+    it = np.asarray(res.iterations, dtype=np.float64)
+    opt = np.array([s == "Optimal" for s in res.status])
+    dense_F = sum(it[p] * newton_flops(ks[p], ms[p]) for p in range(B))
+    sparse_F = sum(it[p] * sparse_reference_flops(ks[p], ms[p]) for p in range(B))
+    per = wall / steps
+    out = {"workload": desc, "problems": B, "rel_gap": tol, "device_GB": round(dev_gb, 2),
+           "solves_per_s": round(B / per, 1), "newton_iterations_per_s": round(float(it.sum()) / per, 1),
+           "seconds_per_batch": round(per, 4), "batched_steps_launched": res.launched,
+           "optimal": int(opt.sum()), "statuses": {s: res.status.count(s) for s in sorted(set(res.status))},
+           "max_gap_of_optimal": float(np.max(res.gap_rel[opt])) if opt.any() else None,
+           "schur_tflops_dense_count": round(dense_F / per / 1e12, 3)}
+    if which == "sparse":
+        out["schur_tflops_reference_sparse_count"] = round(sparse_F / per / 1e12, 3)
+    # the plain FP64 single-problem GPU route, member by member, on a sample
+    rng = np.random.default_rng(1)
+    idx = rng.choice(B, size=min(seq_sample, B), replace=False)
+    tt, n = 0.0, 0
+    for p in idx:
+        g = probs[p]
+        A = g.A_blocks() if which != "sparse" else g.dense_rows().reshape(g.m, g.k, g.k)
+        prob = ingest(g.c, [A], [g.C])
+        b = MI355XDenseBackend(prob)
+        try:
+            solve(prob, o, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
+            b.synchronize()
+            t0 = time.perf_counter()
+            solve(prob, o, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
+            b.synchronize()
+            tt += time.perf_counter() - t0
+            n += 1
+        finally:
+            b.close()
+    seq = n / tt if tt > 0 else None
+    out["single_problem_fp64_gpu_route"] = {"solves_per_s": round(seq, 2) if seq else None, "sample": n}
+    out["speedup_vs_single_problem_route"] = round(B / per / seq, 1) if seq else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +192,9 @@ def main():
     ap.add_argument("--cpu-sample-m", type=int, default=1500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-soc", action="store_true", help="skip the Lorentz-cone kernel leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched legs")
+    ap.add_argument("--batch-sparse", type=int, default=4096, help="problems in the uneven sparse batch")
+    ap.add_argument("--batch-layer", type=int, default=1024, help="problems in the dense layer batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is design-only",
@@ -191,6 +276,11 @@ def main():
                      "factor_ms_per_iter": round(kt["factor_ms"] / max(kt["factor_calls"], 1), 4)},
     }
     b.close()
+    if not args.no_batched:
+        out["batched_sparse"] = batched_leg("sparse", args.batch_sparse, 2, 8)
+        log(f"batched_sparse: {out['batched_sparse']}")
+        out["batched_layer"] = batched_leg("layer", args.batch_layer, 3, 8)
+        log(f"batched_layer: {out['batched_layer']}")
     if not args.no_soc:
         out["soc"] = soc_leg()
         log(f"soc: {out['soc']}")

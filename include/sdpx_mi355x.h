@@ -165,6 +165,22 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 
+/* ---- a13: mixed precision ----------------------------------------------------
+ * The reference's mixed-precision template (kernels/mixed_precision_kkt.jl:
+ * low-precision factor, target-precision residuals and refinement,
+ * _try_factor_mixed_kkt! :972-1111, _refine_mixed_direction! :1634-1713) applied
+ * to the FP64 target: precision 1 assembles the Schur Gram product from an
+ * FP32 copy of Phat on FP32 MFMA (S stays FP64, lower); 0 restores FP64. */
+int sdpx_dense_set_precision(sdpx_dense_t* h, int precision);
+/* rho = r - S dx with S applied matrix-free in FP64,
+ * (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y> (the exact operator the FP32
+ * assembly approximates); out[0] = ||rho||_inf, out[1] = ||r||_inf. */
+int sdpx_dense_exact_residual(sdpx_dense_t* h, double* out);
+/* Iterative refinement of dx against the matrix-free FP64 operator with the
+ * current factor as preconditioner: stop at ||rho||_inf <= abstol, after
+ * max_steps, or when a correction does not decrease the residual (rolled back). */
+int sdpx_dense_refine_exact(sdpx_dense_t* h, double abstol, int max_steps, int* steps, double* residual);
+
 /* ==== Batched route: many independent single-block SDPs, one launch per phase ==
  * No counterpart in the reference: SDPX solves one problem per Workspace
  * (sequential re-solves only, prepared.jl:391-447).  Each batch member is the
@@ -213,6 +229,13 @@ int sdpx_batch_solve(sdpx_batch_t* h, const sdpx_batch_options_t* o, int* iterat
  * the returned point (concatenated like set_start).  Null pointers skip. */
 int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double* pobj, double* dobj, double* x,
                           double* X, double* Y);
+/* Iteration trace of one member's last solve: *n rows of SDPX_BATCH_TRACE_COLS
+ * doubles (iteration, pobj, dobj, p_res, d_res of the step, tX, tY, sigma,
+ * affine primal / dual steps, regularization attempts, refinement steps,
+ * outcome 0 update / 1 centering / 2 restart / 3 termination); at most cap rows
+ * are copied. */
+#define SDPX_BATCH_TRACE_COLS 13
+int sdpx_batch_get_trace(sdpx_batch_t* h, int member, double* rows, int cap, int* n);
 /* Schur matrix of one member (m_p x m_p, lower triangle authoritative). */
 int sdpx_batch_get_schur(sdpx_batch_t* h, int member, double* S);
 /* Schur assembly of every member at the uploaded start point (parity hook of

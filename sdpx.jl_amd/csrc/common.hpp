@@ -111,6 +111,16 @@ int gemm_f64(const GemmArgs& g, hipStream_t stream, int split_k = 1, double* wor
              hipEvent_t after_main = nullptr);
 
 // ---------------------------------------------------------------------------
+// FP32 Schur Gram product (gemm_f32.hip): S (lower, FP64) = P' P + beta S for
+// an FP32 panel P (m columns of K floats, leading dimension ldp % 4 == 0,
+// 16-byte aligned); split-K partials in ws (split * m * m doubles).
+int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long lds, double beta, int split,
+             double* ws, const int2* tile_order, hipStream_t stream, hipEvent_t after_main = nullptr);
+// dst (ld_dst >= rows, padding rows zeroed) = (float) src (ld_src), rows x cols
+int convert_panel_f32(const double* src, long long ld_src, float* dst, long long ld_dst, long long rows, int cols,
+                      hipStream_t stream);
+
+// ---------------------------------------------------------------------------
 // Dense Cholesky and triangular solves (chol_f64.hip)
 // ---------------------------------------------------------------------------
 constexpr int kCholNb = 64;

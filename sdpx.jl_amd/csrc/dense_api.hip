@@ -185,6 +185,10 @@ struct sdpx_dense {
   double* splitws = nullptr;
   int split = 1;
   int2* tile_order = nullptr;  // Schur Gram lower-tile visiting order
+  // mixed precision (sdpx_dense_set_precision): FP32 copy of the Schur panel
+  int precision = 0;
+  float* Pf = nullptr;
+  long long ldPf = 0, maxk2 = 0;
   double* gemv_part = nullptr;
   long long gemv_part_cap = 0;
   double* symv_part = nullptr;
@@ -560,6 +564,7 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
     b.k2 = (long long)b.k * b.k;
     b.nblk = ceil_div(b.k, kCholNb);
     maxk2 = std::max(maxk2, b.k2);
+    h->maxk2 = maxk2;
     const long long dn = (long long)b.nblk * kCholNb * kCholNb;
     int s = kOk;
     if ((s = dalloc(h, &b.Av, (size_t)(b.k2 * m)))) return fail(s);
@@ -833,8 +838,16 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);
     s.tile_order = h->tile_order;
-    if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
-    SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws, h->prof ? h->ev[2] : nullptr));
+    if (h->precision == 1) {
+      // mixed precision: Phat rounded to FP32, Gram product on FP32 MFMA, S kept in FP64
+      SDPX_CHECK(convert_panel_f32(h->Ph, b.k2, h->Pf, h->ldPf, b.k2, m, h->stream));
+      if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
+      SDPX_CHECK(gram_f32(h->Pf, h->ldPf, m, (int)b.k2, h->S, m, l == 0 ? 0.0 : 1.0, h->split, h->splitws,
+                          h->tile_order, h->stream, h->prof ? h->ev[2] : nullptr));
+    } else {
+      if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
+      SDPX_CHECK(gemm_f64(s, h->stream, h->split, h->splitws, h->prof ? h->ev[2] : nullptr));
+    }
     if (h->prof) {
       SDPX_HIP(hipEventRecord(h->ev[3], h->stream));
       SDPX_HIP(hipEventSynchronize(h->ev[3]));
@@ -994,6 +1007,76 @@ int sdpx_dense_refine(sdpx_dense_t* h, double abstol, int max_steps, int fixed_p
     const bool small = corrected > residual * 0.5;
     residual = corrected;
     if (!fixed_policy && small) break;
+  }
+  if (steps_out) *steps_out = steps;
+  if (residual_out) *residual_out = residual;
+  return kOk;
+}
+
+int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
+  SDPX_CHECK(check_handle(h));
+  if (precision != 0 && precision != 1) {
+    set_last_error("sdpx_dense_set_precision: precision must be 0 (FP64) or 1 (FP32 Schur Gram)");
+    return kInvalidArgument;
+  }
+  if (precision == 1 && !h->Pf) {
+    h->ldPf = (h->maxk2 + 3) / 4 * 4;
+    int s = dalloc(h, &h->Pf, (size_t)(h->ldPf * h->m));
+    if (s) return s;
+  }
+  h->precision = precision;
+  return kOk;
+}
+
+// rho = r - S dx with S applied matrix-free in FP64 (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y>;
+// dsc[slot] = ||rho||_inf, dsc[slot + 1] = ||r||_inf.  Uses the W1 / W2 blocks as scratch.
+static int exact_residual_dev(sdpx_dense_t* h, int slot) {
+  for (int l = 0; l < h->L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avx_f64(b.Av, b.k2, h->m, h->dx, 1.0, b.W1, 0.0, nullptr, 0.0, nullptr, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+    SDPX_CHECK(mm(h, b.k, 1.0, b.Xinv, false, b.W1, false, 0.0, b.W2));
+    SDPX_CHECK(mm(h, b.k, 1.0, b.W2, false, b.Y, false, 0.0, b.W1));
+    SDPX_CHECK(avt_f64(b.Av, b.k2, h->m, b.W1, 1.0, l == 0 ? 0.0 : 1.0, h->v2, h->stream));
+  }
+  SDPX_CHECK(axpbyz_f64(h->m, 1.0, h->r, -1.0, h->v2, h->rho, h->stream));
+  SDPX_CHECK(absmax_f64(h->rho, h->m, h->red_part, h->dsc + slot, h->stream));
+  SDPX_CHECK(absmax_f64(h->r, h->m, h->red_part, h->dsc + slot + 1, h->stream));
+  return kOk;
+}
+
+int sdpx_dense_exact_residual(sdpx_dense_t* h, double* out) {
+  SDPX_CHECK(check_handle(h));
+  SDPX_CHECK(exact_residual_dev(h, 0));
+  SDPX_CHECK(sync_scalars(h, 2));
+  if (out) {
+    out[0] = h->hsc[0];
+    out[1] = h->hsc[1];
+  }
+  return kOk;
+}
+
+int sdpx_dense_refine_exact(sdpx_dense_t* h, double abstol, int max_steps, int* steps_out, double* residual_out) {
+  SDPX_CHECK(check_handle(h));
+  int steps = 0;
+  SDPX_CHECK(exact_residual_dev(h, 0));
+  SDPX_CHECK(sync_scalars(h, 1));
+  double residual = h->hsc[0];
+  for (int it = 0; it < max_steps && !(residual <= abstol); ++it) {
+    SDPX_HIP(hipMemcpyAsync(h->dx_best, h->dx, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+    SDPX_CHECK(solve_into(h, h->rho, h->delta));
+    SDPX_CHECK(axpbyz_f64(h->m, 1.0, h->dx, 1.0, h->delta, h->dx, h->stream));
+    SDPX_CHECK(exact_residual_dev(h, 0));
+    SDPX_CHECK(sync_scalars(h, 1));
+    const double corrected = h->hsc[0];
+    if (!std::isfinite(corrected) || corrected > residual) {
+      SDPX_HIP(hipMemcpyAsync(h->dx, h->dx_best, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+      SDPX_CHECK(exact_residual_dev(h, 0));
+      SDPX_HIP(hipStreamSynchronize(h->stream));
+      break;
+    }
+    steps += 1;
+    residual = corrected;
   }
   if (steps_out) *steps_out = steps;
   if (residual_out) *residual_out = residual;

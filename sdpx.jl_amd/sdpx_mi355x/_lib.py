@@ -55,6 +55,9 @@ SIGNATURES = {
     "sdpx_dense_set_profiling": [_h, C.c_int],
     "sdpx_dense_kernel_times": [_h, C.c_int, _dp],
     "sdpx_dense_schur_split": [_h, _ip],
+    "sdpx_dense_set_precision": [_h, C.c_int],
+    "sdpx_dense_exact_residual": [_h, _dp],
+    "sdpx_dense_refine_exact": [_h, C.c_double, C.c_int, _ip, _dp],
     "sdpx_batch_default_options": [C.c_void_p, C.c_double],
     "sdpx_batch_create": [C.c_int, C.c_int, _ip, _ip, C.POINTER(_h)],
     "sdpx_batch_destroy": [_h],

@@ -250,5 +250,24 @@ class MI355XDenseBackend:
         check(self.lib.sdpx_dense_schur_split(self.h, C.byref(s)), "schur_split")
         return int(s.value)
 
+    # ---- mixed precision -------------------------------------------------------------
+    def set_precision(self, precision: int):
+        """0: FP64 Schur assembly; 1: FP32 Schur Gram product (S kept in FP64)."""
+        check(self.lib.sdpx_dense_set_precision(self.h, int(precision)), "set_precision")
+
+    def exact_residual(self):
+        """(||r - S dx||_inf with S applied matrix-free in FP64, ||r||_inf)."""
+        out = np.zeros(2)
+        check(self.lib.sdpx_dense_exact_residual(self.h, dptr(out)), "exact_residual")
+        return float(out[0]), float(out[1])
+
+    def refine_exact(self, abstol: float, max_steps: int):
+        """Refinement against the matrix-free FP64 operator -> (steps, residual)."""
+        steps = C.c_int(0)
+        res = C.c_double(0.0)
+        check(self.lib.sdpx_dense_refine_exact(self.h, abstol, int(max_steps), C.byref(steps), C.byref(res)),
+              "refine_exact")
+        return int(steps.value), float(res.value)
+
     def synchronize(self):
         check(self.lib.sdpx_dense_synchronize(self.h), "synchronize")

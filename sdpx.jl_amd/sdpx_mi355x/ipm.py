@@ -39,6 +39,8 @@ class SDPResult:
     phase_times: dict = field(default_factory=dict)
     history: list = field(default_factory=list)
     schur_trace: list = field(default_factory=list)
+    fp32_iterations: int = 0   # Newton steps whose Schur matrix was assembled in FP32 (mixed precision)
+    fp64_iterations: int = 0
 
 
 def resolved_step_rule(k, rule: str) -> str:
@@ -69,8 +71,14 @@ class _Clock:
 
 
 def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, mu, ctrl: AdaptiveIPMController,
-                iteration: int, gap_rel: float, scale_p: float, scale_d: float, clock: _Clock, trace_schur=None):
-    """newton_step! (step.jl:643-1006) on the device backend."""
+                iteration: int, gap_rel: float, scale_p: float, scale_d: float, clock: _Clock, trace_schur=None,
+                use32: bool = False):
+    """newton_step! (step.jl:643-1006) on the device backend.
+
+    use32 (mixed precision): the Schur Gram product is assembled in FP32 and both directions are
+    refined against the matrix-free FP64 Schur operator.  Returns status "retry_fp64" when the
+    factor's condition estimate exceeds o.mixed_cond_limit or a direction misses its refinement
+    target (the caller redoes the step in FP64)."""
     k = b.k
     clock.start()
     p_res, d_res, ok = b.compute_residuals(mu, affine=o_iter.parameter_strategy == "adaptive", factor=True)
@@ -78,6 +86,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     if not ok:
         return dict(status="breakdown", reason="Cholesky factorization of X or Y failed (iterate left the interior)",
                     p_res=p_res, d_res=d_res)
+    if o.precision == "mixed":
+        b.set_precision(1 if use32 else 0)
     b.schur_build()
     clock.lap("schur_assembly")
     if trace_schur is not None:
@@ -85,12 +95,26 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     fok, reg_attempts = b.factorize()
     clock.lap("kkt_factorization")
     if not fok:
+        if use32:
+            return dict(status="retry_fp64", reason="FP32-assembled Schur matrix not positive definite")
         return dict(status="breakdown",
                     reason=f"Schur complement not positive definite after {reg_attempts} regularization attempt(s)",
                     p_res=p_res, d_res=d_res, reg_attempts=reg_attempts)
+    if use32:
+        # _triangular_condition_estimate (mixed_precision_kkt.jl:371-392): squared diagonal ratio of the factor
+        _, _, q = b.factor_quality()
+        cond_est = 1.0 / (q * q) if q > 0 else math.inf
+        if not cond_est <= o.mixed_cond_limit:
+            return dict(status="retry_fp64", reason=f"Schur condition estimate {cond_est:.2e} above the FP32 limit")
     # predictor (step.jl:699-726)
     b.predictor_rhs()
     b.solve_direction()
+    if use32:
+        # predictor guard (mixed_precision_kkt.jl:1475-1586): residual <= 1e-8 max(||r||, 1)
+        _, rn = b.exact_residual()
+        _, pres = b.refine_exact(1e-8 * max(rn, 1.0), o.mixed_max_refine)
+        if not pres <= 1e-8 * max(rn, 1.0):
+            return dict(status="retry_fp64", reason=f"predictor refinement stalled at {pres:.2e}")
     b.direction_blocks()
     clock.lap("predictor")
     adaptive = ctrl.strategy == "adaptive" and not ctrl.fallback
@@ -148,15 +172,24 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     else:
         refine_tol, refine_cap = o_iter.refine_tol, o_iter.refine_max_steps
     fixed_policy = o_iter.refine_policy == "fixed"
-    if fixed_policy:
-        refine_cap = o_iter.refine_steps
-        abstol = 0.0
+    if use32:
+        # _refine_mixed_direction! (mixed_precision_kkt.jl:1634-1713) to the FP64 target
+        _, r_norm = b.exact_residual()
+        target = o.mixed_refine_target * max(r_norm, 1.0)
+        steps, res = b.refine_exact(target, o.mixed_max_refine)
+        if not res <= target:
+            return dict(status="retry_fp64", reason=f"corrector refinement stalled at {res:.2e}")
+        refine_steps = min(steps, 1)  # the controller sees a native-equivalent count
     else:
-        _, r_norm = b.direction_residual()
-        rel = refine_tol if refine_tol > 0 else REFINE_DEFAULT_TOL_ULPS * EPS
-        abstol = rel * max(r_norm, 1.0)
-    refine_steps, _ = b.refine(abstol, refine_cap, fixed_policy)
-    res, r_norm = b.direction_residual()
+        if fixed_policy:
+            refine_cap = o_iter.refine_steps
+            abstol = 0.0
+        else:
+            _, r_norm = b.direction_residual()
+            rel = refine_tol if refine_tol > 0 else REFINE_DEFAULT_TOL_ULPS * EPS
+            abstol = rel * max(r_norm, 1.0)
+        refine_steps, _ = b.refine(abstol, refine_cap, fixed_policy)
+        res, r_norm = b.direction_residual()
     tol = max(SQRT_EPS, o.eps_primal, o.eps_dual) * max(r_norm, 1.0)
     if not math.isfinite(res) or res > tol:
         return dict(status="breakdown",
@@ -208,6 +241,8 @@ def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
     comp_cur = float(sum(np.sum(X[l] * Y[l]) for l in range(L)))
     iteration, restarts, centering = 0, 0, 0
     status, message = "NotStarted", ""
+    mixed_active = o.precision == "mixed"
+    fp32_iters = fp64_iters = 0
     gap_rel = math.inf
     history = []
     while True:
@@ -238,7 +273,16 @@ def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
             break
         o_iter = ctrl.iteration_options(o)
         tr = trace if (trace is not None and len(trace) < trace_schur) else None
-        res = newton_step(b, o_iter, o, mu, ctrl, iteration + 1, gap_rel, scale_p, scale_d, clock, tr)
+        res = newton_step(b, o_iter, o, mu, ctrl, iteration + 1, gap_rel, scale_p, scale_d, clock, tr,
+                          use32=mixed_active)
+        if res["status"] == "retry_fp64":
+            # FP32 assembly no longer good enough: redo this step in FP64 and stay there
+            mixed_active = False
+            res = newton_step(b, o_iter, o, mu, ctrl, iteration + 1, gap_rel, scale_p, scale_d, clock, tr)
+        if mixed_active:
+            fp32_iters += 1
+        else:
+            fp64_iters += 1
         if res["status"] == "breakdown":
             status, message = "NumericalBreakdown", res["reason"]
             break
@@ -325,4 +369,4 @@ def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
     else:
         x, X, Y = b.get_iterate()
     return SDPResult(status, x, X, Y, pobj, dobj, iteration, p_res, d_res, gap_rel, message, clock.t, history,
-                     trace or [])
+                     trace or [], fp32_iters, fp64_iters)

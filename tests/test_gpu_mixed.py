@@ -1,0 +1,96 @@
+"""Mixed precision (FP32 Schur Gram + FP64 matrix-free refinement) on the dense route.
+
+Pins: the matrix-free FP64 residual equals the assembled-S residual when S is
+assembled in FP64; the FP32-assembled S is within FP32 accuracy of the oracle's
+S; a mixed solve reaches the same optimum as the oracle (1e-8) with a valid
+certificate and at least one FP32 iteration.
+"""
+
+import numpy as np
+import pytest
+
+import sdpx_oracle as O
+
+from sdpx_mi355x import MI355XDenseBackend, SolverOptions, generators, ingest, solve
+
+from _helpers import rel_fro_lower
+
+
+def _state(k, m, seed):
+    g = generators.dense_feasible(k, m, seed=seed)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    rng = np.random.default_rng(seed)
+    Q = rng.standard_normal((k, k))
+    X = Q @ Q.T / k + 0.5 * np.eye(k)
+    Q = rng.standard_normal((k, k))
+    Y = Q @ Q.T / k + 0.5 * np.eye(k)
+    return g, prob, oprob, X, Y
+
+
+@pytest.mark.gpu
+def test_exact_residual_matches_assembled_operator(gpu_lib):
+    g, prob, oprob, X, Y = _state(40, 150, 1)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(g.x0, [X], [Y])
+        mu = np.array([0.1 * float(np.sum(X * Y)) / 40])
+        b.compute_residuals(mu, affine=False, factor=True)
+        b.schur_build()
+        assert b.factorize()[0]
+        b.predictor_rhs()
+        b.solve_direction()
+        # perturb dx so the residual is not at rounding level
+        dx, _, _ = b.get_direction()
+        b.set_dx(dx * (1.0 + 1e-3))
+        res_s, rn = b.direction_residual()
+        res_e, rn2 = b.exact_residual()
+        assert rn == rn2
+        assert abs(res_s - res_e) <= 1e-10 * max(res_s, 1e-300) + 1e-13 * max(rn, 1.0)
+    finally:
+        b.close()
+
+
+@pytest.mark.gpu
+def test_fp32_schur_within_fp32_accuracy(gpu_lib):
+    g, prob, oprob, X, Y = _state(64, 300, 2)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(g.x0, [X], [Y])
+        b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        b.set_precision(1)
+        b.schur_build()
+        S32 = b.get_schur()
+        b.set_precision(0)
+        b.schur_build()
+        S64 = b.get_schur()
+    finally:
+        b.close()
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    ref = O.schur_build(oprob, [LX], [MY])
+    assert rel_fro_lower(S64, ref) <= 1e-12
+    err = rel_fro_lower(S32, ref)
+    print(f"FP32-assembled Schur rel Frobenius error {err:.2e}", flush=True)
+    assert 1e-10 < err <= 2e-5  # genuinely FP32, and within FP32 accuracy
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,m", [(50, 200), (64, 500)])
+def test_mixed_solve_matches_oracle(gpu_lib, k, m):
+    g = generators.dense_feasible(k, m, seed=3)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed")
+    res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    print(f"mixed: {res.status} it={res.iterations} fp32={res.fp32_iterations} fp64={res.fp64_iterations} "
+          f"pobj={res.pobj:.12e} | oracle {ref.status} it={ref.iterations} pobj={ref.pobj:.12e}", flush=True)
+    assert res.status == ref.status == "Optimal"
+    assert res.fp32_iterations >= 1
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    assert abs(res.iterations - ref.iterations) <= 3
+    cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
+                                         res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
+    assert cert["valid"], cert
