@@ -201,6 +201,9 @@ typedef struct sdpx_batch_options {
   int step_rule;          /* 0 :auto, 1 :backtrack, 2 :fraction_to_boundary */
   int precision;          /* 0: plain FP64 */
   double cond_limit, refine_target;
+  /* differentiation checkpoint: the first iterate with <X, Y>/k <= adjoint_mu
+   * is kept (0 disables; see sdpx_batch_restore_checkpoint) */
+  double adjoint_mu;
 } sdpx_batch_options_t;
 
 /* Fill `o` with the reference defaults (SolverOptions{Float64}()) and tolerance
@@ -236,6 +239,30 @@ int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double*
  * are copied. */
 #define SDPX_BATCH_TRACE_COLS 13
 int sdpx_batch_get_trace(sdpx_batch_t* h, int member, double* rows, int cap, int* n);
+/* ---- differentiation (no counterpart in the reference) -----------------------
+ * For the user's dual-form model  max b'y  s.t.  C - sum_i y_i A_i >= 0  with
+ * primal matrix X* (the core Y) and multipliers y* (the core x), and the loss
+ * L = <G, X*> + h'y*, sdpx_batch_adjoint writes dL/dC (k_p^2 per member,
+ * symmetric) and dL/db (m_p per member) at the current iterate of every
+ * member, by implicit differentiation of the central-path conditions
+ * (sum x_i A_i - X - C = 0, <A_i, Y> = c_i, XY = mu I; HKM linearisation):
+ *   W = sym(X^-1 G Y),  lambda = S^-1 (h - [<A_i, W>]),  A_l = sum lambda_i A_i,
+ *   dL/dC = -(sym(X^-1 A_l Y) + W),  dL/db = lambda,
+ * with S the Schur matrix refactored at the iterate.  G, h, dC, db are
+ * concatenated over members like set_start.  Returns 1 when a member could not
+ * be factored (its outputs are NaN). */
+int sdpx_batch_adjoint(sdpx_batch_t* h, const double* G, const double* hv, double* dC, double* db);
+/* `iterations` Newton steps on the central-path equations XY = mu I (mu fixed
+ * at <X, Y>/k on entry) with primal and dual feasibility: moves a solved
+ * iterate onto the central path, where the adjoint above is the exact
+ * derivative of the central-path point. */
+int sdpx_batch_center(sdpx_batch_t* h, int iterations);
+/* Replace each member's iterate by its differentiation checkpoint (members
+ * without one keep their iterate); *restored = members restored, mu[p] = the
+ * checkpoint's <X, Y>/k (NaN without one).  The adjoint is taken at the
+ * checkpoint re-centred by sdpx_batch_center: the final iterate of a solve is
+ * too far from the central path, relative to its mu, for the linearisation. */
+int sdpx_batch_restore_checkpoint(sdpx_batch_t* h, int* restored, double* mu);
 /* Schur matrix of one member (m_p x m_p, lower triangle authoritative). */
 int sdpx_batch_get_schur(sdpx_batch_t* h, int member, double* S);
 /* Schur assembly of every member at the uploaded start point (parity hook of

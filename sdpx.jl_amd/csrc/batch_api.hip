@@ -350,6 +350,8 @@ struct MemberState {
   int reg_attempts = 0;
   double reg = 0;
   bool factor_ok = false;
+  bool has_ck = false;  // differentiation checkpoint taken (first iterate with <X,Y>/k <= adjoint_mu)
+  double ck_mu = 0;
   Params prm;
   double comp = 0, mu_g = 0, mu_aff = 0;
   double ap = 1, ad = 1;
@@ -1005,9 +1007,15 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       if (std::find(restart.begin(), restart.end(), p) != restart.end()) s.comp_cur = comp;
       s.mu = s.ctrl.beta * comp / h->mem[p].k;
     }
+    std::vector<int> ck;
     for (int p : upd) {
       MemberState& s = S[p];
       const double comp_after = sc(h, p, S_COMP);
+      if (o.adjoint_mu > 0 && !s.has_ck && comp_after / h->mem[p].k <= o.adjoint_mu) {
+        s.has_ck = true;
+        s.ck_mu = comp_after / h->mem[p].k;
+        ck.push_back(p);
+      }
       const bool finite = std::isfinite(sc(h, p, S_XMAX)) && std::isfinite(sc(h, p, S_XBMAX)) &&
                           std::isfinite(sc(h, p, S_YBMAX));
       if (!(finite && std::isfinite(s.mu))) {
@@ -1027,6 +1035,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       s.d_res = d_after;
       s.log(0);
     }
+    if (!ck.empty()) SDPX_CHECK(launch_best(h->arr, put_list(h, ck), (int)ck.size(), 2, h->stream));
   }
   // ---- results: the best iterate replaces a non-optimal final one ---------------------------------
   std::vector<int> restore;
@@ -1077,6 +1086,7 @@ int sdpx_batch_default_options(sdpx_batch_options_t* o, double tol) {
   o->adaptive = 1;
   o->step_rule = 0;
   o->precision = 0;
+  o->adjoint_mu = 1e-6;
   return kOk;
 }
 
@@ -1142,10 +1152,10 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
   if ((s = balloc_n(h, &a.Av, h->nAv))) return fail(s);
   if ((s = balloc_n(h, &a.Ph, h->nPh))) return fail(s);
   double** kb[] = {&a.C, &a.X, &a.Y, &a.LX, &a.LY, &a.LXinv, &a.Xinv, &a.P, &a.R, &a.dX, &a.dY, &a.W1, &a.W2,
-                   &a.bX, &a.bY};
+                   &a.bX, &a.bY, &a.G, &a.ckX, &a.ckY};
   for (double** q : kb)
     if ((s = balloc_n(h, q, h->nK))) return fail(s);
-  double** mb[] = {&a.c, &a.x, &a.dx, &a.dxb, &a.d, &a.v, &a.r, &a.rho, &a.delta, &a.bx};
+  double** mb[] = {&a.c, &a.x, &a.dx, &a.dxb, &a.d, &a.v, &a.r, &a.rho, &a.delta, &a.bx, &a.ckx};
   for (double** q : mb)
     if ((s = balloc_n(h, q, h->nM))) return fail(s);
   if ((s = balloc_n(h, &a.S, h->nS))) return fail(s);
@@ -1515,6 +1525,136 @@ int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double*
   if (Y) SDPX_CHECK(download_blocks(h, Y, h->arr.Y, true));
   SDPX_HIP(hipStreamSynchronize(h->stream));
   return kOk;
+}
+
+namespace {
+// L_X, L_Y, X^-1 and the factored Schur matrix at the current iterate for every
+// member; members whose blocks or Schur matrix do not factor are dropped from `ok`.
+int factor_at_iterate(sdpx_batch_t* h, double mu_scale, std::vector<int>& ok) {
+  std::vector<int> all(h->B);
+  for (int p = 0; p < h->B; ++p) all[p] = p;
+  SDPX_CHECK(launch_comp(h->arr, put_list(h, all), h->B, h->stream));
+  SDPX_CHECK(pull_scalars(h));
+  for (int p = 0; p < h->B; ++p) {
+    par(h, p, P_MU) = mu_scale * sc(h, p, S_COMP) / h->mem[p].k;
+    par(h, p, P_AFF) = 0.0;
+  }
+  SDPX_CHECK(push_params(h));
+  const int* l = put_list(h, all);
+  SDPX_CHECK(launch_gemv(h->arr, l, h->B, 0, h->stream));
+  SDPX_CHECK(launch_gemvt(h->arr, l, h->B, 0, h->stream));
+  SDPX_CHECK(launch_block_factor(h->arr, l, h->B, h->stream));
+  SDPX_CHECK(pull_scalars(h, true));
+  ok.clear();
+  for (int p = 0; p < h->B; ++p)
+    if (si(h, p, I_INFOX) == 0 && si(h, p, I_INFOY) == 0) ok.push_back(p);
+  SDPX_CHECK(schur_build(h, ok));
+  SDPX_CHECK(schur_factor(h, ok));
+  std::vector<int> keep;
+  for (int p : ok)
+    if (h->st[p].factor_ok) keep.push_back(p);
+  ok.swap(keep);
+  return kOk;
+}
+}  // namespace
+
+int sdpx_batch_restore_checkpoint(sdpx_batch_t* h, int* restored, double* mu) {
+  SDPX_CHECK(check_batch(h));
+  std::vector<int> l;
+  for (int p = 0; p < h->B; ++p) {
+    if (h->st[p].has_ck) l.push_back(p);
+    if (mu) mu[p] = h->st[p].has_ck ? h->st[p].ck_mu : NAN;
+  }
+  if (!l.empty()) SDPX_CHECK(launch_best(h->arr, put_list(h, l), (int)l.size(), 3, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  if (restored) *restored = (int)l.size();
+  return kOk;
+}
+
+int sdpx_batch_center(sdpx_batch_t* h, int iterations) {
+  SDPX_CHECK(check_batch(h));
+  if (!h->have_data || !h->have_start) {
+    set_last_error("sdpx_batch_center: data and an iterate are required");
+    return kInvalidArgument;
+  }
+  // mu fixed at its value on entry
+  std::vector<int> all(h->B);
+  for (int p = 0; p < h->B; ++p) {
+    all[p] = p;
+    par(h, p, P_A) = par(h, p, P_B) = 0.0;
+  }
+  SDPX_CHECK(push_params(h));
+  SDPX_CHECK(launch_comp(h->arr, put_list(h, all), h->B, h->stream));
+  SDPX_CHECK(pull_scalars(h));
+  std::vector<double> mu(h->B);
+  for (int p = 0; p < h->B; ++p) mu[p] = sc(h, p, S_COMP) / h->mem[p].k;
+  for (int it = 0; it < iterations; ++it) {
+    std::vector<int> ok;
+    SDPX_CHECK(factor_at_iterate(h, 1.0, ok));
+    for (int p = 0; p < h->B; ++p) par(h, p, P_MU) = mu[p];
+    SDPX_CHECK(push_params(h));
+    // R = mu I - X Y with the fixed mu, then one Newton step on the central-path equations
+    const int n = (int)ok.size();
+    const int* l = put_list(h, ok);
+    SDPX_CHECK(launch_block_factor(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_rhs_block(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_gemvt(h->arr, l, n, 1, h->stream));
+    SDPX_CHECK(launch_solve(h->arr, l, n, 0, h->stream));
+    SDPX_CHECK(launch_gemv(h->arr, l, n, 1, h->stream));
+    SDPX_CHECK(launch_dir_block(h->arr, l, n, h->stream));
+    for (int p : ok) {
+      par(h, p, P_GAMMA) = 0.9;
+      par(h, p, P_MINSTEP) = 1e-10;
+      par(h, p, P_RATIO) = 0.0;
+    }
+    SDPX_CHECK(push_params(h));
+    SDPX_CHECK(launch_backtrack(h->arr, put_list(h, ok), n, h->stream));
+    SDPX_CHECK(pull_scalars(h));
+    for (int p : ok) {
+      par(h, p, P_A) = sc(h, p, S_TX);
+      par(h, p, P_B) = sc(h, p, S_TY);
+    }
+    SDPX_CHECK(push_params(h));
+    SDPX_CHECK(launch_update(h->arr, put_list(h, ok), n, h->stream));
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_batch_adjoint(sdpx_batch_t* h, const double* G, const double* hv, double* dC, double* db) {
+  SDPX_CHECK(check_batch(h));
+  if (!h->have_data || !h->have_start || !G || !hv) {
+    set_last_error("sdpx_batch_adjoint: data, an iterate, G and h are required");
+    return kInvalidArgument;
+  }
+  std::vector<int> ok;
+  SDPX_CHECK(factor_at_iterate(h, 0.0, ok));
+  SDPX_CHECK(upload_blocks(h, h->arr.G, G, true));
+  SDPX_CHECK(upload_blocks(h, h->arr.v, hv, false));
+  const int n = (int)ok.size();
+  const int* l = put_list(h, ok);
+  SDPX_CHECK(launch_adj_w(h->arr, l, n, h->stream));
+  SDPX_CHECK(launch_gemvt(h->arr, l, n, 1, h->stream));    // r = h - Av' vec(W)
+  SDPX_CHECK(launch_solve(h->arr, l, n, 0, h->stream));    // dx = lambda = S^-1 r
+  SDPX_CHECK(launch_gemv(h->arr, l, n, 3, h->stream));     // dX = A_lambda
+  SDPX_CHECK(launch_adj_final(h->arr, l, n, h->stream));   // W2 = dL/dC_std
+  if (dC) SDPX_CHECK(download_blocks(h, dC, h->arr.W2, true));
+  if (db) SDPX_CHECK(download_blocks(h, db, h->arr.dx, false));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  // members that could not be factored get NaN
+  std::vector<char> good(h->B, 0);
+  for (int p : ok) good[p] = 1;
+  long long oc = 0, om = 0;
+  for (int p = 0; p < h->B; ++p) {
+    const long long k2 = (long long)h->mem[p].k * h->mem[p].k;
+    if (!good[p]) {
+      if (dC) std::fill(dC + oc, dC + oc + k2, NAN);
+      if (db) std::fill(db + om, db + om + h->mem[p].m, NAN);
+    }
+    oc += k2;
+    om += h->mem[p].m;
+  }
+  return n == h->B ? kOk : kNotPositiveDefinite;
 }
 
 int sdpx_batch_get_trace(sdpx_batch_t* h, int member, double* rows, int cap, int* n) {

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(NT) void gemv_part_kernel(Arrays a, const int* list
   const int i1 = min(M.m, i0 + kColChunk);
   const long long k2 = (long long)M.k * M.k;
   const double* Av = a.Av + M.oAv;
-  const double* vec = (mode == 1 ? a.dx : a.x) + M.oM;
+  const double* vec = (mode == 1 || mode == 3 ? a.dx : a.x) + M.oM;
   double* part = a.gpart + M.oG + (long long)blockIdx.y * k2;
   for (long long e = threadIdx.x; e < k2; e += NT) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -204,8 +204,8 @@ __global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int
     double acc = 0.0;
     for (int c = 0; c < nch; ++c) acc += part[(long long)c * k2 + e];
     double out;
-    if (mode == 1) {
-      out = acc + a.P[M.oK + e];
+    if (mode == 1 || mode == 3) {
+      out = mode == 1 ? acc + a.P[M.oK + e] : acc;
       a.dX[M.oK + e] = out;
     } else {
       out = acc - a.X[M.oK + e] - a.C[M.oK + e];
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int
     }
     amax = nan_max(amax, fabs(out));
   }
-  if (mode != 1) {
+  if (mode == 0 || mode == 2) {
     amax = block_max(amax, red);
     if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_PRES : S_PSOL)] = amax;
   }
@@ -693,22 +693,75 @@ __global__ __launch_bounds__(NT) void scale_kernel(Arrays a, const int* list, in
 __global__ __launch_bounds__(NT) void best_kernel(Arrays a, const int* list, int dir) {
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
+  const bool ck = dir >= 2;
+  double* sX = ck ? a.ckX : a.bX;
+  double* sY = ck ? a.ckY : a.bY;
+  double* sx = ck ? a.ckx : a.bx;
+  const bool save = (dir & 1) == 0;
   for (int e = threadIdx.x; e < M.k * M.k; e += NT) {
     const long long o = M.oK + e;
-    if (dir == 0) {
-      a.bX[o] = a.X[o];
-      a.bY[o] = a.Y[o];
+    if (save) {
+      sX[o] = a.X[o];
+      sY[o] = a.Y[o];
     } else {
-      a.X[o] = a.bX[o];
-      a.Y[o] = a.bY[o];
+      a.X[o] = sX[o];
+      a.Y[o] = sY[o];
     }
   }
   for (int i = threadIdx.x; i < M.m; i += NT) {
     const long long o = M.oM + i;
-    if (dir == 0)
-      a.bx[o] = a.x[o];
+    if (save)
+      sx[o] = a.x[o];
     else
-      a.x[o] = a.bx[o];
+      a.x[o] = sx[o];
+  }
+}
+
+// ---- adjoint of L = <G, Y> + h'x (the core form of tr(G X*) + h'y*) ------------------
+// W = sym(X^-1 G Y) into W1 and W2; d = -h (so the gemvt pass forms r = h - Av' vec(W))
+__global__ __launch_bounds__(NT) void adj_w_kernel(Arrays a, const int* list) {
+  __shared__ double sA[KMAX * LP], sB[KMAX * LP], sC[KMAX * LP];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sA, a.G + o, k);
+  lds_load(sB, a.Y + o, k);
+  __syncthreads();
+  lds_mm(sC, sA, false, sB, false, k, 1.0, 0.0);  // G Y
+  __syncthreads();
+  lds_load(sA, a.Xinv + o, k);
+  __syncthreads();
+  lds_mm(sB, sA, false, sC, false, k, 1.0, 0.0);  // X^-1 G Y
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    const double w = (sB[r + c * LP] + sB[c + r * LP]) / 2.0;
+    a.W1[o + e] = w;
+    a.W2[o + e] = w;
+  }
+  for (int i = threadIdx.x; i < M.m; i += NT) a.d[M.oM + i] = -a.v[M.oM + i];
+}
+
+// dL/dC_std = -(sym(X^-1 A_lambda Y) + W) into W2 (A_lambda = sum lambda_i A_i is in dX)
+__global__ __launch_bounds__(NT) void adj_final_kernel(Arrays a, const int* list) {
+  __shared__ double sA[KMAX * LP], sB[KMAX * LP], sC[KMAX * LP];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sA, a.dX + o, k);
+  lds_load(sB, a.Y + o, k);
+  __syncthreads();
+  lds_mm(sC, sA, false, sB, false, k, 1.0, 0.0);  // A_lambda Y
+  __syncthreads();
+  lds_load(sA, a.Xinv + o, k);
+  __syncthreads();
+  lds_mm(sB, sA, false, sC, false, k, 1.0, 0.0);  // X^-1 A_lambda Y
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int r = e % k, c = e / k;
+    a.W2[o + e] = -((sB[r + c * LP] + sB[c + r * LP]) / 2.0 + a.W1[o + e]);
   }
 }
 
@@ -1093,6 +1146,11 @@ int launch_scale(const Arrays& a, const int* list, int n, int which, hipStream_t
 }
 int launch_best(const Arrays& a, const int* list, int n, int dir, hipStream_t st) {
   SDPX_LAUNCH(best_kernel, n, a, list, dir);
+}
+
+int launch_adj_w(const Arrays& a, const int* list, int n, hipStream_t st) { SDPX_LAUNCH(adj_w_kernel, n, a, list); }
+int launch_adj_final(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(adj_final_kernel, n, a, list);
 }
 
 int launch_set_int(const Arrays& a, const int* list, int n, int slot, int value, hipStream_t st) {

@@ -57,8 +57,9 @@ constexpr int kColChunk = 64;  // constraints per work-group in the panel passes
 struct Arrays {
   const Member* mem;
   double *Av, *Ph;
-  double *C, *X, *Y, *LX, *LY, *LXinv, *Xinv, *P, *R, *dX, *dY, *W1, *W2, *bX, *bY;  // k x k blocks
-  double *c, *x, *dx, *dxb, *d, *v, *r, *rho, *delta, *bx;                             // m-vectors
+  double *C, *X, *Y, *LX, *LY, *LXinv, *Xinv, *P, *R, *dX, *dY, *W1, *W2, *bX, *bY, *G;  // k x k blocks
+  double *c, *x, *dx, *dxb, *d, *v, *r, *rho, *delta, *bx, *ckx;                       // m-vectors
+  double *ckX, *ckY;  // differentiation checkpoint (k x k blocks)
   double *S, *Sf, *dinv;
   double* gpart;   // Av x partial sums
   int max_chunks;  // most column chunks of any member
@@ -68,7 +69,7 @@ struct Arrays {
 };
 
 // ---- launchers (batch_kernels.hip) ------------------------------------------
-// mode 0: P = Av x - X - C, S_PRES;  1: dX = Av dx + P;  2: W1 = Av x - X - C, S_PSOL
+// mode 0: P = Av x - X - C, S_PRES;  1: dX = Av dx + P;  2: W1 = Av x - X - C, S_PSOL;  3: dX = Av dx
 int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st);
 // mode 0: d = c - Av' vec(Y), S_DRES;  1: r = -(d + Av' vec(W2));  2: v = c - Av' vec(Y), S_DSOL
 int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st);
@@ -106,8 +107,12 @@ int launch_accepted(const Arrays& a, const int* list, int n, hipStream_t st);
 int launch_objectives(const Arrays& a, const int* list, int n, hipStream_t st);
 // which 1: X *= P_SCALE, 2: Y *= P_SCALE
 int launch_scale(const Arrays& a, const int* list, int n, int which, hipStream_t st);
-// dir 0: (bx, bX, bY) = (x, X, Y);  1: the reverse
+// dir 0: (bx, bX, bY) = (x, X, Y);  1: the reverse;  2: (ckx, ckX, ckY) = (x, X, Y);  3: the reverse
 int launch_best(const Arrays& a, const int* list, int n, int dir, hipStream_t st);
+// adjoint: W = sym(X^-1 G Y) into W1 / W2 and d = -v (v holds h)
+int launch_adj_w(const Arrays& a, const int* list, int n, hipStream_t st);
+// adjoint: W2 = -(sym(X^-1 A_lambda Y) + W1), A_lambda in dX
+int launch_adj_final(const Arrays& a, const int* list, int n, hipStream_t st);
 // si[p * NISLOT + slot] = value for the listed members
 int launch_set_int(const Arrays& a, const int* list, int n, int slot, int value, hipStream_t st);
 // symmetric A_j densified from upper-triangle COO into Av (members' panels zeroed first)

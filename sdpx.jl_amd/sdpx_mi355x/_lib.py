@@ -69,6 +69,9 @@ SIGNATURES = {
     "sdpx_batch_get_result": [_h, _ip, _ip, _dp, _dp, _dp, _dp, _dp],
     "sdpx_batch_get_schur": [_h, C.c_int, _dp],
     "sdpx_batch_get_trace": [_h, C.c_int, _dp, C.c_int, _ip],
+    "sdpx_batch_adjoint": [_h, _dp, _dp, _dp, _dp],
+    "sdpx_batch_center": [_h, C.c_int],
+    "sdpx_batch_restore_checkpoint": [_h, _ip, _dp],
     "sdpx_batch_schur_at_start": [_h],
     "sdpx_soc_create": [C.c_int, C.c_int, _ip, C.POINTER(_h)],
     "sdpx_soc_destroy": [_h],

@@ -33,6 +33,7 @@ class BatchOptions(C.Structure):
         ("stall_iterations", C.c_int), ("restart", C.c_int), ("refine_policy", C.c_int),
         ("refine_steps", C.c_int), ("refine_max_steps", C.c_int), ("adaptive", C.c_int), ("step_rule", C.c_int),
         ("precision", C.c_int), ("cond_limit", C.c_double), ("refine_target", C.c_double),
+        ("adjoint_mu", C.c_double),
     ]
 
     @classmethod
@@ -154,6 +155,32 @@ class BatchSolver:
         Xs = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(X, np.cumsum(k2)[:-1]), self.k)]
         Ys = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(Y, np.cumsum(k2)[:-1]), self.k)]
         return BatchResult([STATUS[int(s)] for s in status], iters, pobj, dobj, xs, Xs, Ys, launched)
+
+    # ---- differentiation ---------------------------------------------------------------------
+    def restore_checkpoint(self):
+        """Replace each member's iterate by its differentiation checkpoint -> (restored count, mu per member)."""
+        n = C.c_int(0)
+        mu = np.empty(self.B)
+        check(self._lib.sdpx_batch_restore_checkpoint(self._h, C.byref(n), dptr(mu)), "sdpx_batch_restore_checkpoint")
+        return n.value, mu
+
+    def center(self, iterations: int = 4) -> None:
+        """Newton steps on XY = mu I (mu fixed at entry) from the current iterate."""
+        check(self._lib.sdpx_batch_center(self._h, int(iterations)), "sdpx_batch_center")
+
+    def adjoint(self, G_list, h_list):
+        """dL/dC and dL/db of L = sum_p <G_p, X*_p> + h_p'y*_p (user's dual-form model) at the current iterate.
+
+        Returns (dC list of symmetric k_p x k_p, db list of m_p vectors); members that could not be
+        factored get NaN."""
+        G = np.concatenate([np.asarray(g, dtype=np.float64).ravel(order="F") for g in G_list])
+        hv = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in h_list])
+        k2 = self.k.astype(np.int64) ** 2
+        dC = np.empty(int(k2.sum()))
+        db = np.empty(int(self.m.sum()))
+        check(self._lib.sdpx_batch_adjoint(self._h, dptr(G), dptr(hv), dptr(dC), dptr(db)), "sdpx_batch_adjoint")
+        dCs = [b.reshape(int(kk), int(kk), order="F") for b, kk in zip(np.split(dC, np.cumsum(k2)[:-1]), self.k)]
+        return dCs, np.split(db, np.cumsum(self.m)[:-1])
 
     TRACE_COLUMNS = ("iteration", "pobj", "dobj", "p_res", "d_res", "tX", "tY", "sigma", "affine_primal",
                      "affine_dual", "reg_attempts", "refine_steps", "outcome")
