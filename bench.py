@@ -142,6 +142,25 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
             t0 = time.perf_counter()
             res = bs.solve(o)
             wall += time.perf_counter() - t0
+        adjoint = None
+        if which == "layer":
+            # backward pass of L = sum tr(G X*) + h'y*: checkpoint, 4 centring steps, adjoint
+            rng = np.random.default_rng(7)
+            Gs = []
+            for k in ks:
+                E = rng.standard_normal((k, k))
+                E = (E + E.T) / 2
+                Gs.append(E / np.linalg.norm(E))
+            hs = [v / np.linalg.norm(v) for v in (rng.standard_normal(m) for m in ms)]
+            t0 = time.perf_counter()
+            restored, _ = bs.restore_checkpoint()
+            bs.center(4)
+            dC, db = bs.adjoint(Gs, hs)
+            ta = time.perf_counter() - t0
+            finite = sum(bool(np.all(np.isfinite(a)) and np.all(np.isfinite(b_))) for a, b_ in zip(dC, db))
+            adjoint = {"seconds": round(ta, 4), "gradients_per_s": round(B / ta, 1), "checkpoints": restored,
+                       "finite_gradients": finite,
+                       "note": "dL/dC, dL/db at the central-path point of the first iterate with mu <= 1e-6"}
     it = np.asarray(res.iterations, dtype=np.float64)
     opt = np.array([s == "Optimal" for s in res.status])
     dense_F = sum(it[p] * newton_flops(ks[p], ms[p]) for p in range(B))
@@ -155,6 +174,8 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
            "schur_tflops_dense_count": round(dense_F / per / 1e12, 3)}
     if which == "sparse":
         out["schur_tflops_reference_sparse_count"] = round(sparse_F / per / 1e12, 3)
+    if adjoint is not None:
+        out["adjoint"] = adjoint
     # the plain FP64 single-problem GPU route, member by member, on a sample
     rng = np.random.default_rng(1)
     idx = rng.choice(B, size=min(seq_sample, B), replace=False)
@@ -180,6 +201,39 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
     return out
 
 
+def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
+    """The mixed-precision configuration: one dense SDP n=500, m=6000 solved in FP64 and in mixed precision
+    (FP32 Schur Gram product, FP64 matrix-free refinement); objective difference and timings."""
+    from sdpx_mi355x import MI355XDenseBackend, SolverOptions, generators, ingest, solve
+
+    t0 = time.perf_counter()
+    g = generators.dense_feasible(k, m, seed=0)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    log(f"mixed: generated n={k} m={m} in {time.perf_counter() - t0:.1f}s")
+    out = {"workload": f"dense SDP n={k}, m={m}, rel gap {tol:g}"}
+    b = MI355XDenseBackend(prob)
+    try:
+        res = {}
+        for prec in ("fp64", "mixed", "fp64", "mixed"):  # first pair warms up
+            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec)
+            b.synchronize()
+            t0 = time.perf_counter()
+            r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
+            b.synchronize()
+            res[prec] = (r, time.perf_counter() - t0)
+        F = newton_flops(k, m)
+        for prec, (r, dt) in res.items():
+            out[prec] = {"status": r.status, "iterations": r.iterations, "fp32_iterations": r.fp32_iterations,
+                         "seconds": round(dt, 3), "schur_tflops": round(r.iterations * F / dt / 1e12, 2),
+                         "pobj": r.pobj, "gap": r.gap_rel}
+        p64, pm = res["fp64"][0].pobj, res["mixed"][0].pobj
+        out["objective_rel_diff"] = abs(pm - p64) / max(1.0, abs(p64))
+        out["speedup_mixed_vs_fp64"] = round(res["fp64"][1] / res["mixed"][1], 3)
+    finally:
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +247,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-soc", action="store_true", help="skip the Lorentz-cone kernel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched legs")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg (n=500, m=6000)")
     ap.add_argument("--batch-sparse", type=int, default=4096, help="problems in the uneven sparse batch")
     ap.add_argument("--batch-layer", type=int, default=1024, help="problems in the dense layer batch")
     args = ap.parse_args()
@@ -281,6 +336,9 @@ def main():
         log(f"batched_sparse: {out['batched_sparse']}")
         out["batched_layer"] = batched_leg("layer", args.batch_layer, 3, 8)
         log(f"batched_layer: {out['batched_layer']}")
+    if not args.no_mixed:
+        out["mixed_precision"] = mixed_leg()
+        log(f"mixed_precision: {out['mixed_precision']}")
     if not args.no_soc:
         out["soc"] = soc_leg()
         log(f"soc: {out['soc']}")
