@@ -234,6 +234,34 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
     return out
 
 
+COPY_BYTES = 1 << 31
+
+
+def copy_calibration() -> None:
+    """A plain device copy of known size: the bytes the HBM counters must report for it."""
+    import torch
+
+    x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "launches": 3}), flush=True)
+
+
+GRAM_PROFILE = ROOT / "profiles" / "schur_gram_f64_dense_n400_m4000.md"
+
+
+def measured_traffic(k: int, m: int):
+    """HBM bytes per Schur Gram launch at n=400, m=4000 from the committed counter profile (None elsewhere)."""
+    if (k, m) != (400, 4000) or not GRAM_PROFILE.exists():
+        return None
+    for line in GRAM_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,11 +278,24 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg (n=500, m=6000)")
     ap.add_argument("--batch-sparse", type=int, default=4096, help="problems in the uneven sparse batch")
     ap.add_argument("--batch-layer", type=int, default=1024, help="problems in the dense layer batch")
+    ap.add_argument("--only", choices=("batched_sparse", "batched_layer", "mixed", "soc"),
+                    help="run one extra leg alone and print its JSON (for per-kernel profiles)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 2 GiB device copy three times (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU layout is design-only",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.only:
+        legs = {"batched_sparse": lambda: batched_leg("sparse", args.batch_sparse, 1, 0),
+                "batched_layer": lambda: batched_leg("layer", args.batch_layer, 1, 0),
+                "mixed": mixed_leg, "soc": soc_leg}
+        print(json.dumps({args.only: legs[args.only]()}, default=float), flush=True)
+        return
 
     import numpy as np
 
@@ -324,7 +365,8 @@ def main():
         "schur_phase_tflops": round(rp.iterations * F / asm_fac_s / 1e12, 3) if asm_fac_s > 0 else None,
         "phase_seconds_per_solve": {kk: round(v, 5) for kk, v in phase.items()},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": measured_traffic(k, m),
+                     "traffic_source": f"profiles/{GRAM_PROFILE.name} (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "kernel": "schur_gram_f64_kernel", "flops_per_launch": gram_flops,
                      "avg_launch_ms": round(gram_avg_s * 1e3, 4), "split_k": b.schur_split(),
                      "panels_ms_per_iter": round(kt["panels_ms"] / max(kt["gram_launches"], 1), 4),
