@@ -1,7 +1,7 @@
 // Dense FP64 Cholesky (batched, right-looking, blocked) and triangular solves.
 //
-// The diagonal nb x nb block of every panel is factored inside one work-group
-// in LDS; the kernel also inverts that block (kept in `dinv`), so that the
+// The diagonal nb x nb block of every panel is factored by one wavefront in
+// registers; the kernel also inverts that block (kept in `dinv`), so that the
 // panel solve, the trailing update and every later triangular solve are MFMA
 // GEMMs (gemm_f64.hip).  A non-positive or non-finite pivot sets the 1-based
 // LAPACK-style `info` and every later kernel of that batch entry returns
@@ -16,67 +16,72 @@ constexpr int NB = kCholNb;
 // Factors the jb x jb diagonal block at a (column j of its matrix), writes L
 // over its lower triangle and the block's inverse to di (NB x NB, zero
 // padded); a bad pivot sets *info = j + c + 1.
-__device__ void potrf_diag_body(double* a, long long lda, int j, int jb, double* di, int* info) {
-  __shared__ double s[NB][NB + 1];
-  __shared__ double inv[NB][NB + 1];
-  __shared__ double red[4][NB];
-  const int tid = threadIdx.x;
-  const int row = tid % NB;   // row owned by this thread in the updates
-  const int part = tid / NB;  // which quarter of the columns / sum terms
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    const int r = idx % jb, c = idx / jb;
-    s[r][c] = (r >= c) ? a[r + (long long)c * lda] : 0.0;
-  }
-  __syncthreads();
-  // Right-looking elimination on the unscaled columns: step c subtracts
-  // a_r a_c2 / p from the trailing lower triangle (one barrier per column);
-  // the columns are scaled by 1/sqrt(p) afterwards, all at once.
-  for (int c = 0; c < jb; ++c) {
-    const double p = s[c][c];
+//
+// Two wavefronts in lock step, state in registers (every loop is unrolled
+// over compile-time indices, the block is padded to the identity beyond jb):
+//  * wave 0, lane r = row r of the block: right-looking elimination on the
+//    unscaled columns; step c publishes column c through LDS (double
+//    buffered, one barrier per step) and lane r subtracts (a_rc / p) a_c2c
+//    from its row (entries right of its diagonal are scratch, never read);
+//  * wave 1, lane q = column q of M = L_u^-1 (A = L_u D L_u'): it applies the
+//    same row operations to the identity, M_r -= (a_rc / p) M_c for r > c,
+//    with the multipliers read from the published column.
+// Then L = L_u D^1/2 (column c scaled by 1/sqrt(p_c)) and
+// L^-1 = D^-1/2 M (row r scaled by 1/sqrt(p_r)).
+constexpr int kDiagThreads = 2 * NB;
+static_assert(NB == 64, "one lane per row / column of the diagonal block");
+
+__device__ __forceinline__ void potrf_diag_body(double* a, long long lda, int j, int jb, double* di, int* info) {
+  __shared__ double col[2][NB];
+  __shared__ double piv[NB];
+  __shared__ __align__(16) double s[NB][NB + 2];
+  const int lane = threadIdx.x & (NB - 1);
+  const bool elim = threadIdx.x < NB;
+  const bool live = lane < jb;
+  double v[NB];  // wave 0: row `lane` of the block; wave 1: column `lane` of M
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    v[c] = (elim && live && c <= lane) ? a[lane + (long long)c * lda] : (c == lane ? 1.0 : 0.0);
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    double* cc = col[c & 1];
+    if (elim) cc[lane] = v[c];
+    __syncthreads();
+    const double p = cc[c];
     if (!(p > 0.0)) {
-      if (tid == 0) *info = j + c + 1;
-      return;  // uniform: every thread read the same pivot
+      if (threadIdx.x == 0) *info = j + c + 1;
+      return;  // uniform: both waves read the same pivot
     }
-    if (row > c && row < jb) {
-      const double f = s[row][c] / p;
-      for (int c2 = c + 1 + part; c2 <= row; c2 += 4) s[row][c2] -= f * s[c2][c];
+    if (elim) {
+      if (lane == 0) piv[c] = p;
+      const double f = v[c] / p;
+#pragma unroll
+      for (int c2 = c + 1; c2 < NB; ++c2) v[c2] -= f * cc[c2];
+    } else {
+      const double g = v[c] / p;  // M_cq / p
+#pragma unroll
+      for (int r = c + 1; r < NB; ++r) v[r] -= cc[r] * g;
     }
-    __syncthreads();
-  }
-  if (tid < jb) red[0][tid] = sqrt(s[tid][tid]);
-  __syncthreads();
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    const int r = idx % jb, c = idx / jb;
-    if (r > c) s[r][c] = s[r][c] / red[0][c];
   }
   __syncthreads();
-  if (tid < jb) s[tid][tid] = red[0][tid];
-  __syncthreads();
-  for (int idx = tid; idx < jb * jb; idx += 256) {
-    const int r = idx % jb, c = idx / jb;
-    if (r >= c) a[r + (long long)c * lda] = s[r][c];
-  }
-  // Inverse of the lower factor block by forward substitution on the
-  // identity, one row r per step; the four threads of a column split the sum.
-  for (int r = 0; r < jb; ++r) {
-    const int c = row;
-    double partial = 0.0;
-    if (c < r)
-      for (int q = c + part; q < r; q += 4) partial += s[r][q] * inv[q][c];
-    red[part][c] = partial;
-    __syncthreads();
-    if (part == 0 && c <= r) {
-      inv[r][c] = (c == r) ? 1.0 / s[r][r] : -((red[0][c] + red[1][c]) + (red[2][c] + red[3][c])) / s[r][r];
+  if (elim) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const double d = sqrt(piv[c]);
+      const double l = c < lane ? v[c] / d : (c == lane ? d : 0.0);
+      if (live && c <= lane && c < jb) a[lane + (long long)c * lda] = l;
     }
-    __syncthreads();
+  } else {
+    // s[q][r] = (L^-1)_rq, transposed through LDS for coalesced column stores
+#pragma unroll
+    for (int r = 0; r < NB; ++r) s[lane][r] = v[r] / sqrt(piv[r]);
   }
-  for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int r = idx % NB, c = idx / NB;
-    di[idx] = (r < jb && c < jb && r >= c) ? inv[r][c] : 0.0;
-  }
+  __syncthreads();
+  for (int c = threadIdx.x / NB; c < NB; c += kDiagThreads / NB)
+    di[lane + (long long)c * NB] = (live && c < jb) ? s[c][lane] : 0.0;
 }
 
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long lda, long long stride, int j, int jb,
+__global__ __launch_bounds__(kDiagThreads) void potrf_diag_kernel(double* A, long long lda, long long stride, int j, int jb,
                                                          double* dinv, long long dinv_stride, int* info) {
   const int b = blockIdx.x;
   if (info[b] != 0) return;
@@ -85,7 +90,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, long long ld
 }
 
 // one work-group per listed matrix: diagonal block j of matrix d[list[w]]
-__global__ __launch_bounds__(256) void potrf_diag_grouped_kernel(const CholDesc* __restrict__ d,
+__global__ __launch_bounds__(kDiagThreads) void potrf_diag_grouped_kernel(const CholDesc* __restrict__ d,
                                                                  const int* __restrict__ list, int j) {
   const CholDesc c = d[list[blockIdx.x]];
   if (*c.info != 0 || j >= c.n) return;
@@ -102,7 +107,7 @@ __global__ void zero_int_kernel(int* p, int n) {
 
 int potrf_diag_grouped(const CholDesc* d, const int* list, int count, int j, hipStream_t stream) {
   if (count <= 0) return kOk;
-  hipLaunchKernelGGL(potrf_diag_grouped_kernel, dim3(count), dim3(256), 0, stream, d, list, j);
+  hipLaunchKernelGGL(potrf_diag_grouped_kernel, dim3(count), dim3(kDiagThreads), 0, stream, d, list, j);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }
@@ -114,7 +119,7 @@ int potrf_f64(double* A, long long lda, long long stride, int n, int batch, doub
   SDPX_HIP(hipGetLastError());
   for (int j = 0; j < n; j += NB) {
     const int jb = std::min(NB, n - j);
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(batch), dim3(256), 0, stream, A, lda, stride, j, jb, dinv,
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(batch), dim3(kDiagThreads), 0, stream, A, lda, stride, j, jb, dinv,
                        dinv_stride, info);
     SDPX_HIP(hipGetLastError());
     const int rest = n - j - jb;

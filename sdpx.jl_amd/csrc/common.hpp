@@ -57,6 +57,9 @@ enum GemmFlags : unsigned {
   kGemmALower = 1u << 1,    // op(A) is lower triangular: kk <= i
   kGemmBLower = 1u << 2,    // op(B) is lower triangular: kk >= j
   kGemmAUpper = 1u << 3,    // op(A) is upper triangular: kk >= i
+  // Row tiles start at -(mt*BM - M) so that the partial tile is the first one
+  // (with kGemmALower it then has the shortest K range); no lower output.
+  kGemmRaggedFirst = 1u << 4,
   kGemmTransA = 1u << 8,    // op(A) = A'
   kGemmTransB = 1u << 9,    // op(B) = B'
   kGemmSchurGram = 1u << 10,  // route to the dedicated Schur Gram kernel (TN, 128 x 128, lower output)
@@ -76,6 +79,11 @@ struct GemmArgs {
   const int* skip;        // optional per-batch flag: skip the batch entry when *skip != 0
   long long skip_stride;  // stride between per-batch skip flags
   const int2* tile_order; // optional (ti, tj) visiting order of the lower tiles (kGemmLowerOut)
+  // Optional blocked output: when block_rows > 0, C(row, col) is stored at
+  // C + row % block_rows + (row / block_rows) * block_stride + col * ldc
+  // (the rows of a vertically stacked product go back to separate blocks).
+  long long block_rows, block_stride;
+  int row_off;            // set by the launcher for kGemmRaggedFirst
 };
 
 // Grouped launches: many independent GEMMs of different shapes in one

@@ -3,7 +3,7 @@
 //
 // Device layout (FP64, column-major, per PSD block l of side k):
 //   Av_l   k^2 x m   constraint panel (column i = vec(A_i))       [problem]
-//   T, Ph  k^2 x m   Schur panels L_X^-1 A_i and L_X^-1 A_i M_Y   [max over blocks]
+//   T, Ph  k^2 x m   Schur panels A_i M_Y and L_X^-1 A_i M_Y        [max over blocks]
 //   S, Sf  m x m     Schur matrix (lower authoritative) and its Cholesky factor
 //   X, Y, LX, MY, LXinv, Xinv, P, R, dX, dY, W1, W2   k x k blocks
 //   trial  cap x k^2 batched line-search trial matrices
@@ -825,15 +825,17 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     Block& b = h->blk[l];
     const int k = b.k;
     if (h->prof) SDPX_HIP(hipEventRecord(h->ev[0], h->stream));
-    // T = L_X^-1 [A_1 ... A_m]   (k x km, op(A) lower triangular)
-    SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, b.Av, k, 0.0, h->T, k, kGemmALower), h->stream));
-    // Phat_i = T_i M_Y           (batched k x k, op(B) lower triangular)
-    GemmArgs g = gemm(k, k, k, 1.0, h->T, k, b.MY, k, 0.0, h->Ph, k, kGemmBLower);
-    g.strideA = b.k2;
-    g.strideB = 0;
-    g.strideC = b.k2;
-    g.batch = m;
-    SDPX_CHECK(gemm_f64(g, h->stream));
+    // T_i = A_i M_Y for every i as one product over the stacked panel
+    // [A_1; ...; A_m] = Av' (each A_i is symmetric), M = km rows, op(B) lower;
+    // rows (i, r) are written back to the k^2 x m panel layout (block remap)
+    GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower);
+    w.block_rows = k;
+    w.block_stride = b.k2;
+    SDPX_CHECK(gemm_f64(w, h->stream));
+    // Phat = L_X^-1 [T_1 ... T_m]   (k x km, op(A) lower; the partial row tile
+    // comes first, where the triangle leaves it the shortest K range)
+    SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k, kGemmALower | kGemmRaggedFirst),
+                        h->stream));
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);

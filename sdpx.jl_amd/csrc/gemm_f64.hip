@@ -50,7 +50,7 @@ __device__ __forceinline__ void tile_coords(int tid, int e, int h, int& r, int& 
 }
 
 // Loads one R x BK operand slice into registers.  Rows outside the matrix are
-// clamped once onto its last row (their products only reach C entries that
+// clamped once onto its first or last row (their products only reach C entries that
 // the epilogue discards), so full slices need no per-element guard: the main
 // K loop is a single basic block of loads, MFMAs and one barrier.  Only the
 // final partial slice (K not a multiple of BK) masks along K.
@@ -66,7 +66,7 @@ struct TileLoader {
     for (int e = 0; e < NL; ++e) {
       int r, k;
       tile_coords<R, KC, VEC>(tid, e, 0, r, k);
-      const int gr = min(r0 + r, rmax);
+      const int gr = max(min(r0 + r, rmax), 0);
       rowoff[e] = KC ? (long long)gr * ld + k : (long long)gr + (long long)k * ld;
     }
   }
@@ -131,12 +131,14 @@ __device__ __forceinline__ void tile_store(const double* v, double (*S)[R + PAD]
 }
 
 // One BM x BN tile of C(m0:, n0:) = alpha * op(A) op(B) (+ beta * C) over
-// K in [kBegin, kEnd); only entries with row < M, col < N (and row >= col
-// when lower_out) are written.
+// K in [kBegin, kEnd); only entries with 0 <= row < M, col < N (and
+// row >= col when lower_out) are written, at C + row + col * ldc or, with
+// brows > 0, at C + row % brows + (row / brows) * bstride + col * ldc.
 template <int BM, int BN, bool TA, bool TB, bool VEC>
 __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long long lda, const double* __restrict__ B,
                                           long long ldb, int M, int N, int kBegin, int kEnd, int m0, int n0,
-                                          bool lower_out, double* C, long long ldc, double alpha, double beta) {
+                                          bool lower_out, double* C, long long ldc, double alpha, double beta,
+                                          long long brows = 0, long long bstride = 0) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
@@ -225,14 +227,15 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
     const int row = m0 + wm0 + i * 16 + (lane & 15);
+    const long long roff = brows > 0 ? row % (int)brows + (long long)(row / (int)brows) * bstride : (long long)row;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int col = n0 + wn0 + j * 16 + (lane >> 4) + 4 * rr;
-        if (row < M && col < N && (!lower_out || row >= col)) {
+        if (row >= 0 && row < M && col < N && (!lower_out || row >= col)) {
           const double v = acc[i][j][rr];
-          double* cp = C + row + (long long)col * ldc;
+          double* cp = C + roff + (long long)col * ldc;
           *cp = (beta == 0.0) ? alpha * v : alpha * v + beta * (*cp);
         }
       }
@@ -272,11 +275,19 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
     ti = t;
     tj = tile - t * (t + 1) / 2;
   } else {
-    const int mt = (g.M + BM - 1) / BM;
-    ti = tile % mt;
-    tj = tile / mt;
+    // the shorter tile dimension varies fastest: neighbouring groups share
+    // the panel of the longer one through L2
+    const int mt = (g.M + g.row_off + BM - 1) / BM;
+    const int nt = (g.N + BN - 1) / BN;
+    if (mt > nt) {
+      tj = tile % nt;
+      ti = tile / nt;
+    } else {
+      ti = tile % mt;
+      tj = tile / mt;
+    }
   }
-  const int m0 = ti * BM;
+  const int m0 = ti * BM - g.row_off;
   const int n0 = tj * BN;
 
   const double* A = g.A + (long long)b * g.strideA;
@@ -299,7 +310,8 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   const bool part = split > 1;
   double* out = part ? ws + ((long long)b * split + s) * (long long)g.M * g.N : g.C + (long long)b * g.strideC;
   gemm_core<BM, BN, TA, TB, VEC>(A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
-                                 part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta);
+                                 part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta,
+                                 part ? 0 : g.block_rows, g.block_stride);
 }
 
 // ---- grouped launches: one work-group per (GEMM, tile) item ------------------
@@ -351,8 +363,10 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(GemmArgs g, int spli
 }
 
 template <int BM, int BN, bool TA, bool TB, bool VEC>
-int launch(const GemmArgs& g, hipStream_t stream, int split, double* ws, hipEvent_t after_main) {
+int launch(const GemmArgs& g0, hipStream_t stream, int split, double* ws, hipEvent_t after_main) {
+  GemmArgs g = g0;
   const int mt = ceil_div(g.M, BM), nt = ceil_div(g.N, BN);
+  g.row_off = (g.flags & kGemmRaggedFirst) ? mt * BM - g.M : 0;
   const long long tiles = (g.flags & kGemmLowerOut) ? (long long)mt * (mt + 1) / 2 : (long long)mt * nt;
   if (tiles <= 0) return kOk;
   dim3 grid((unsigned)(tiles * split), (unsigned)g.batch);
@@ -405,6 +419,10 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
     set_last_error("gemm_f64: split_k > 1 needs a workspace");
     return kInvalidArgument;
   }
+  if ((g.block_rows > 0 || (g.flags & kGemmRaggedFirst)) && (split_k > 1 || (g.flags & kGemmLowerOut))) {
+    set_last_error("gemm_f64: blocked output and ragged-first tiles need split_k = 1 and a full output");
+    return kInvalidArgument;
+  }
   const bool vec = vec_ok(g, ta, tb);
   if (g.flags & kGemmSchurGram) {
     // S = Phat' Phat: 128 x 128 lower tiles, op(A) transposed
@@ -416,7 +434,8 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
                : launch<128, 128, true, false, false>(g, stream, split_k, workspace, after_main);
   }
   // Small problems use 64 x 64 tiles so that k x k block products launch enough groups.
-  const bool small = (g.M <= 96 && g.N <= 96) ||
+  // (and so does a product with a side of at most 64, which would leave half of a 128-wide tile idle)
+  const bool small = (g.M <= 96 && g.N <= 96) || g.M <= 64 || g.N <= 64 ||
                      ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256);
   if (small) {
     return vec ? dispatch_trans<64, 64, true>(g, stream, split_k, workspace, ta, tb, after_main)
