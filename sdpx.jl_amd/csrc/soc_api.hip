@@ -13,6 +13,7 @@
 // compiled without floating-point contraction, so results are reproducible
 // operation for operation.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sdpx_mi355x.h"
@@ -24,7 +25,8 @@ namespace sdpx {
 namespace {
 
 constexpr int kSocSlots = SDPX_SOC_SLOTS;
-constexpr int kStrictCandidates = 25;  // one full step + 24 halvings (soc_native.jl:2705-2745)
+constexpr int kStrictCandidates = 25;
+constexpr int kFixedQMax = 8;  // uniform cone dimensions with register-resident kernels: 2-6, 8  // one full step + 24 halvings (soc_native.jl:2705-2745)
 
 struct Cone {
   const long long* ptr;
@@ -37,7 +39,7 @@ __device__ __forceinline__ void cone_range(const long long* ptr, int c, long lon
 }
 
 // _soc_tail_norm (soc_lorentz_kernels.jl:22-34)
-__device__ double tail_norm(const double* v, int q) {
+__device__ __forceinline__ double tail_norm(const double* v, int q) {
   double scale = 0.0;
   for (int i = 1; i < q; ++i) scale = fmax(scale, fabs(v[i]));
   if (scale == 0.0) return scale;
@@ -80,7 +82,7 @@ __device__ double determinant(const double* v, int q) {
 }
 
 // _soc_sqrt_determinant (soc_lorentz_kernels.jl:42-62)
-__device__ bool sqrt_determinant(const double* v, int q, double& r) {
+__device__ __forceinline__ bool sqrt_determinant(const double* v, int q, double& r) {
   r = 0.0;
   double scale = 0.0;
   for (int i = 0; i < q; ++i) {
@@ -105,8 +107,8 @@ __device__ __forceinline__ double jmax(double a, double b) { return a > b ? a : 
 // _soc_nt_scaling! (soc_lorentz_kernels.jl:72-184).  The a, b, raw vectors
 // of the reference are recomputed from s and z where needed (same values).
 // Returns false without writing w / lambda when any check fails.
-__device__ bool nt_scaling(const double* s, const double* z, int q, double* w, double* lam, double& eta_out,
-                           double& eta2_out) {
+__device__ __forceinline__ bool nt_scaling(const double* s, const double* z, int q, double* w, double* lam,
+                                           double& eta_out, double& eta2_out) {
   eta_out = 0.0;
   eta2_out = 0.0;
   double ss, zs;
@@ -190,26 +192,16 @@ __global__ void nt_scaling_kernel(Cone cn, const double* __restrict__ S, const d
 // op 0: W (soc_lorentz_kernels.jl:187-205), 1: W^-1 (:208-226), 2: (W'W)^-1 (:229-248).
 // dst may alias src (the source cone is read into the output cone in order:
 // every output entry depends only on src[i] and the two reductions).
-__global__ void apply_kernel(Cone cn, int op, const double* __restrict__ W, const double* eta,
-                             const double* eta2, const double* src, double* dst) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cn.ncones) return;
-  long long b;
-  int q;
-  cone_range(cn.ptr, c, b, q);
-  const double* w = W + b;
-  const double* x = src + b;
-  double* y = dst + b;
+__device__ __forceinline__ void apply_cone(int op, const double* w, double e, double e2, const double* x, int q,
+                                           double* y) {
   const double head = x[0];
   if (op == 2) {
-    const double e2 = eta2[c];
     double ld = w[0] * head;
     for (int i = 1; i < q; ++i) ld -= w[i] * x[i];
     y[0] = (2.0 * w[0] * ld - head) / e2;
     for (int i = 1; i < q; ++i) y[i] = (-2.0 * w[i] * ld + x[i]) / e2;
     return;
   }
-  const double e = eta[c];
   double dt = 0.0;
   for (int i = 1; i < q; ++i) dt += w[i] * x[i];
   if (op == 0) {
@@ -221,6 +213,80 @@ __global__ void apply_kernel(Cone cn, int op, const double* __restrict__ W, cons
     y[0] = (w[0] * head - dt) / e;
     for (int i = 1; i < q; ++i) y[i] = (x[i] + corr * w[i]) / e;
   }
+}
+
+__global__ void apply_kernel(Cone cn, int op, const double* __restrict__ W, const double* eta,
+                             const double* eta2, const double* src, double* dst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cn.ncones) return;
+  long long b;
+  int q;
+  cone_range(cn.ptr, c, b, q);
+  // the source cone is read in order before each output entry is written, so dst may alias src
+  apply_cone(op, W + b, op == 2 ? 0.0 : eta[c], op == 2 ? eta2[c] : 0.0, src + b, q, dst + b);
+}
+
+// ---- every cone of the same dimension Q (the common case: SOCP constraints of
+// one shape): no offset table, the cone lives in registers, 16-byte accesses
+template <int Q>
+__device__ __forceinline__ void load_cone(const double* __restrict__ p, double* v) {
+  if constexpr (Q % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < Q; i += 2) {
+      const double2 t = *reinterpret_cast<const double2*>(p + i);
+      v[i] = t.x;
+      v[i + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) v[i] = p[i];
+  }
+}
+
+template <int Q>
+__device__ __forceinline__ void store_cone(double* p, const double* v) {
+  if constexpr (Q % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < Q; i += 2) *reinterpret_cast<double2*>(p + i) = make_double2(v[i], v[i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) p[i] = v[i];
+  }
+}
+
+template <int Q>
+__global__ void nt_scaling_fixed_kernel(int ncones, const double* __restrict__ S, const double* __restrict__ Z,
+                                        double* W, double* L, double* eta, double* eta2, int* ok, int* nfail) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncones) return;
+  const long long b = (long long)c * Q;
+  double s[Q], z[Q], w[Q], lam[Q];
+  load_cone<Q>(S + b, s);
+  load_cone<Q>(Z + b, z);
+  double e = 0.0, e2 = 0.0;
+  const bool good = nt_scaling(s, z, Q, w, lam, e, e2);
+  if (good) {
+    store_cone<Q>(W + b, w);
+    store_cone<Q>(L + b, lam);
+  } else {
+    atomicAdd(nfail, 1);
+  }
+  eta[c] = e;
+  eta2[c] = e2;
+  ok[c] = good ? 1 : 0;
+}
+
+template <int Q>
+__global__ void apply_fixed_kernel(int ncones, int op, const double* __restrict__ W, const double* eta,
+                                   const double* eta2, const double* src, double* dst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncones) return;
+  const long long b = (long long)c * Q;
+  double w[Q], x[Q], y[Q];
+  load_cone<Q>(W + b, w);
+  load_cone<Q>(src + b, x);
+  apply_cone(op, w, op == 2 ? 0.0 : eta[c], op == 2 ? eta2[c] : 0.0, x, Q, y);
+  store_cone<Q>(dst + b, y);
 }
 
 // _soc_jordan! (soc.jl:94-112); dst may alias either operand
@@ -486,6 +552,7 @@ struct sdpx_soc {
   long long total = 0;
   int maxq = 0;
   bool all_q3 = false;
+  int uniform_q = 0;  // every cone has this dimension (2..kFixedQMax), else 0
   hipStream_t stream = nullptr;
   long long* ptr = nullptr;
   double* vec[kSocSlots] = {};
@@ -529,6 +596,21 @@ int soc_check(sdpx_soc_t* h, int slot = 0) {
 }
 
 inline dim3 soc_grid(int n) { return dim3((unsigned)std::max(1, ceil_div(n, 256))); }
+
+// Calls f(std::integral_constant<int, q>) for a supported uniform cone
+// dimension q; returns false (nothing launched) otherwise.
+template <class F>
+bool launch_fixed_q(int q, F&& f) {
+  switch (q) {
+    case 2: f(std::integral_constant<int, 2>{}); return true;
+    case 3: f(std::integral_constant<int, 3>{}); return true;
+    case 4: f(std::integral_constant<int, 4>{}); return true;
+    case 5: f(std::integral_constant<int, 5>{}); return true;
+    case 6: f(std::integral_constant<int, 6>{}); return true;
+    case 8: f(std::integral_constant<int, 8>{}); return true;
+    default: return false;
+  }
+}
 
 int begin(sdpx_soc_t* h) {
   if (h->prof) SDPX_HIP(hipEventRecord(h->ev[0], h->stream));
@@ -590,6 +672,11 @@ int sdpx_soc_create(int device, int ncones, const int* dims, sdpx_soc_t** out) {
   h->total = ptr[ncones];
   h->maxq = maxq;
   h->all_q3 = all3;
+  {
+    bool uniform = true;
+    for (int c = 1; c < ncones && uniform; ++c) uniform = dims[c] == dims[0];
+    h->uniform_q = (uniform && dims[0] >= 2 && dims[0] <= kFixedQMax) ? dims[0] : 0;
+  }
   auto fail = [&](int s) {
     sdpx_soc_destroy(h);
     return s;
@@ -692,8 +779,13 @@ int sdpx_soc_nt_scaling(sdpx_soc_t* h, int s, int z, int w, int lambda, int* nfa
   SDPX_CHECK(reset_counters(h, 1, 0));
   SDPX_CHECK(begin(h));
   Cone cn{h->ptr, h->ncones};
-  hipLaunchKernelGGL(nt_scaling_kernel, soc_grid(h->ncones), dim3(256), 0, h->stream, cn, h->vec[s], h->vec[z],
-                     h->vec[w], h->vec[lambda], h->eta, h->eta2, h->ok, h->counters);
+  if (!launch_fixed_q(h->uniform_q, [&](auto qc) {
+        constexpr int Q = decltype(qc)::value;
+        hipLaunchKernelGGL(nt_scaling_fixed_kernel<Q>, soc_grid(h->ncones), dim3(256), 0, h->stream, h->ncones,
+                           h->vec[s], h->vec[z], h->vec[w], h->vec[lambda], h->eta, h->eta2, h->ok, h->counters);
+      }))
+    hipLaunchKernelGGL(nt_scaling_kernel, soc_grid(h->ncones), dim3(256), 0, h->stream, cn, h->vec[s], h->vec[z],
+                       h->vec[w], h->vec[lambda], h->eta, h->eta2, h->ok, h->counters);
   SDPX_CHECK(end(h));
   int nf = 0;
   SDPX_CHECK(read_counter(h, &nf));
@@ -711,8 +803,13 @@ int sdpx_soc_apply(sdpx_soc_t* h, int op, int w, int src, int dst) {
   }
   SDPX_CHECK(begin(h));
   Cone cn{h->ptr, h->ncones};
-  hipLaunchKernelGGL(apply_kernel, soc_grid(h->ncones), dim3(256), 0, h->stream, cn, op, h->vec[w], h->eta,
-                     h->eta2, h->vec[src], h->vec[dst]);
+  if (!launch_fixed_q(h->uniform_q, [&](auto qc) {
+        constexpr int Q = decltype(qc)::value;
+        hipLaunchKernelGGL(apply_fixed_kernel<Q>, soc_grid(h->ncones), dim3(256), 0, h->stream, h->ncones, op,
+                           h->vec[w], h->eta, h->eta2, h->vec[src], h->vec[dst]);
+      }))
+    hipLaunchKernelGGL(apply_kernel, soc_grid(h->ncones), dim3(256), 0, h->stream, cn, op, h->vec[w], h->eta,
+                       h->eta2, h->vec[src], h->vec[dst]);
   return end(h);
 }
 

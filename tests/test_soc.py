@@ -199,10 +199,16 @@ def _split(v, dims):
 
 
 @pytest.mark.gpu
-def test_gpu_nt_scaling_and_apply_match_oracle(cones):
-    rng = np.random.default_rng(1)
-    dims = _mixed_dims(rng, 1500)
-    dims[dims == 1] = 2
+@pytest.mark.parametrize("q", [None, 2, 3, 4, 5, 8])
+def test_gpu_nt_scaling_and_apply_match_oracle(cones, q):
+    # q None: mixed dimensions (offset-table kernels); q fixed: every cone of
+    # dimension q (register-resident kernels)
+    rng = np.random.default_rng(1 if q is None else 10 + q)
+    if q is None:
+        dims = _mixed_dims(rng, 1500)
+        dims[dims == 1] = 2
+    else:
+        dims = np.full(1500, q)
     s = random_interior(rng, dims, 2.0)
     z = random_interior(rng, dims, 0.3)
     # a few cones outside the interior: scaling must fail there and leave w / lambda untouched
