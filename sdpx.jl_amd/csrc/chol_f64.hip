@@ -17,68 +17,93 @@ constexpr int NB = kCholNb;
 // over its lower triangle and the block's inverse to di (NB x NB, zero
 // padded); a bad pivot sets *info = j + c + 1.
 //
-// Two wavefronts in lock step, state in registers (every loop is unrolled
-// over compile-time indices, the block is padded to the identity beyond jb):
-//  * wave 0, lane r = row r of the block: right-looking elimination on the
-//    unscaled columns; step c publishes column c through LDS (double
-//    buffered, one barrier per step) and lane r subtracts (a_rc / p) a_c2c
-//    from its row (entries right of its diagonal are scratch, never read);
-//  * wave 1, lane q = column q of M = L_u^-1 (A = L_u D L_u'): it applies the
-//    same row operations to the identity, M_r -= (a_rc / p) M_c for r > c,
-//    with the multipliers read from the published column.
-// Then L = L_u D^1/2 (column c scaled by 1/sqrt(p_c)) and
-// L^-1 = D^-1/2 M (row r scaled by 1/sqrt(p_r)).
-constexpr int kDiagThreads = 2 * NB;
+// Four wavefronts in lock step, state in registers (loops unrolled over
+// compile-time indices; the block is padded to the identity beyond jb):
+//  * waves 0, 1 (lane r = row r of the block, 32 columns each): right-looking
+//    elimination on the unscaled columns, a_rg -= (a_rc / p) a_gc for g > c;
+//  * waves 2, 3 (lane q = column q of M = L_u^-1 where A = L_u D L_u', 32 rows
+//    each): the same row operations applied to the identity,
+//    M_gq -= (M_cq / p) a_gc for g > c.
+// Step c: the wave holding column c publishes it, the wave holding row c of M
+// publishes that row (LDS, double buffered: one barrier per step); every wave
+// reads its 32 entries of the column as broadcasts in one batch and updates.
+// Column c of L = L_u D^1/2 and row c of L^-1 = D^-1/2 M are final at step c
+// and leave the registers there.
+constexpr int kDiagThreads = 4 * NB;
+constexpr int kDiagHalf = NB / 2;
 static_assert(NB == 64, "one lane per row / column of the diagonal block");
 
-__device__ __forceinline__ void potrf_diag_body(double* a, long long lda, int j, int jb, double* di, int* info) {
-  __shared__ double col[2][NB];
-  __shared__ double piv[NB];
-  __shared__ __align__(16) double s[NB][NB + 2];
-  const int lane = threadIdx.x & (NB - 1);
-  const bool elim = threadIdx.x < NB;
+// The wave holding entries [H0, H0 + kDiagHalf) (columns of A for the
+// elimination waves, rows of M for the inverse waves); `elim` is wave-uniform.
+// Every loop is fully unrolled, so the conditions on c fold at compile time.
+// Returns the failing step (first non-positive pivot), or -1.
+template <int H0>
+__device__ __forceinline__ int diag_wave(double* a, long long lda, int jb, bool elim, int lane, double (*col)[NB],
+                                         double (*mrow)[NB], double (*s)[NB + 2]) {
   const bool live = lane < jb;
-  double v[NB];  // wave 0: row `lane` of the block; wave 1: column `lane` of M
+  double v[kDiagHalf];
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
-    v[c] = (elim && live && c <= lane) ? a[lane + (long long)c * lda] : (c == lane ? 1.0 : 0.0);
+  for (int i = 0; i < kDiagHalf; ++i) {
+    const int g = H0 + i;
+    v[i] = (elim && live && g <= lane) ? a[lane + (long long)g * lda] : (g == lane ? 1.0 : 0.0);
+  }
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
+    const bool mine = c >= H0 && c < H0 + kDiagHalf;
     double* cc = col[c & 1];
-    if (elim) cc[lane] = v[c];
+    double* mr = mrow[c & 1];
+    if (mine) {
+      if (elim) cc[lane] = v[c - H0];
+      else mr[lane] = v[c - H0];
+    }
     __syncthreads();
     const double p = cc[c];
-    if (!(p > 0.0)) {
-      if (threadIdx.x == 0) *info = j + c + 1;
-      return;  // uniform: both waves read the same pivot
+    if (!(p > 0.0)) return c;  // uniform over the work-group
+    const double own = elim ? cc[lane] : mr[lane];  // a_rc (elimination) or M_cq (inverse)
+    const double f = own / p;
+    if (H0 + kDiagHalf - 1 > c) {
+      const int lo = c + 1 > H0 ? c + 1 : H0;
+      double cv[kDiagHalf];
+#pragma unroll
+      for (int g = lo & ~1; g < H0 + kDiagHalf; g += 2) {
+        const double2 t = reinterpret_cast<const double2*>(cc)[g / 2];
+        cv[g - H0] = t.x;
+        cv[g + 1 - H0] = t.y;
+      }
+      __builtin_amdgcn_sched_group_barrier(0x100, kDiagHalf / 2, 0);
+#pragma unroll
+      for (int g = lo; g < H0 + kDiagHalf; ++g) v[g - H0] -= f * cv[g - H0];
     }
-    if (elim) {
-      if (lane == 0) piv[c] = p;
-      const double f = v[c] / p;
-#pragma unroll
-      for (int c2 = c + 1; c2 < NB; ++c2) v[c2] -= f * cc[c2];
-    } else {
-      const double g = v[c] / p;  // M_cq / p
-#pragma unroll
-      for (int r = c + 1; r < NB; ++r) v[r] -= cc[r] * g;
+    // column c of A and row c of M are final: column c of L = L_u D^1/2 and
+    // row c of L^-1 = D^-1/2 M leave the registers here
+    if (mine) {
+      const double d = sqrt(p);
+      if (elim) {
+        if (live && lane >= c && c < jb) a[lane + (long long)c * lda] = lane == c ? d : own / d;
+      } else {
+        s[lane][c] = own / d;  // (L^-1)_cq, transposed for the coalesced store of di
+      }
     }
   }
-  __syncthreads();
-  if (elim) {
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const double d = sqrt(piv[c]);
-      const double l = c < lane ? v[c] / d : (c == lane ? d : 0.0);
-      if (live && c <= lane && c < jb) a[lane + (long long)c * lda] = l;
-    }
-  } else {
-    // s[q][r] = (L^-1)_rq, transposed through LDS for coalesced column stores
-#pragma unroll
-    for (int r = 0; r < NB; ++r) s[lane][r] = v[r] / sqrt(piv[r]);
+  return -1;
+}
+
+__device__ __forceinline__ void potrf_diag_body(double* a, long long lda, int j, int jb, double* di, int* info) {
+  __shared__ __align__(16) double col[2][NB];
+  __shared__ __align__(16) double mrow[2][NB];
+  __shared__ __align__(16) double s[NB][NB + 2];  // s[q][c] = (L^-1)_cq
+  const int wave = threadIdx.x / NB;
+  const int lane = threadIdx.x & (NB - 1);
+  const bool elim = wave < 2;
+  const int bad = (wave & 1) ? diag_wave<kDiagHalf>(a, lda, jb, elim, lane, col, mrow, s)
+                             : diag_wave<0>(a, lda, jb, elim, lane, col, mrow, s);
+  if (bad >= 0) {
+    if (threadIdx.x == 0) *info = j + bad + 1;
+    return;
   }
   __syncthreads();
-  for (int c = threadIdx.x / NB; c < NB; c += kDiagThreads / NB)
-    di[lane + (long long)c * NB] = (live && c < jb) ? s[c][lane] : 0.0;
+  for (int c = wave; c < NB; c += kDiagThreads / NB)
+    di[lane + (long long)c * NB] = (lane < jb && c < jb) ? s[c][lane] : 0.0;
 }
 
 __global__ __launch_bounds__(kDiagThreads) void potrf_diag_kernel(double* A, long long lda, long long stride, int j, int jb,
