@@ -213,8 +213,28 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
     out = {"workload": f"dense SDP n={k}, m={m}, rel gap {tol:g}"}
     b = MI355XDenseBackend(prob)
     try:
+        # Schur Gram launch time per assembly precision at the start point (HIP events)
+        import numpy as np
+
+        b.set_iterate(g.x0, [g.X0], [g.Y0])
+        b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        gram = {}
+        for code, name in ((0, "fp64"), (1, "fp32"), (2, "bf16x3")):
+            b.set_precision(code)
+            b.schur_build()
+            b.set_profiling(True)
+            b.kernel_times(reset=True)
+            for _ in range(2):
+                b.schur_build()
+            kt = b.kernel_times(reset=True)
+            b.set_profiling(False)
+            ms = kt["gram_ms"] / max(kt["gram_launches"], 1)
+            gram[name] = {"avg_launch_ms": round(ms, 3),
+                          "tflops_fp64_equivalent": round(float(k) ** 2 * m * (m + 1) / ms / 1e9, 1)}
+        b.set_precision(0)
+        out["schur_gram"] = gram
         res = {}
-        for prec in ("fp64", "mixed", "fp64", "mixed"):  # first pair warms up
+        for prec in ("fp64", "mixed", "mixed_bf16x3", "fp64", "mixed", "mixed_bf16x3"):  # first round warms up
             o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec)
             b.synchronize()
             t0 = time.perf_counter()
@@ -226,9 +246,11 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
             out[prec] = {"status": r.status, "iterations": r.iterations, "fp32_iterations": r.fp32_iterations,
                          "seconds": round(dt, 3), "schur_tflops": round(r.iterations * F / dt / 1e12, 2),
                          "pobj": r.pobj, "gap": r.gap_rel}
-        p64, pm = res["fp64"][0].pobj, res["mixed"][0].pobj
+        p64, pm, pb = res["fp64"][0].pobj, res["mixed"][0].pobj, res["mixed_bf16x3"][0].pobj
         out["objective_rel_diff"] = abs(pm - p64) / max(1.0, abs(p64))
+        out["objective_rel_diff_bf16x3"] = abs(pb - p64) / max(1.0, abs(p64))
         out["speedup_mixed_vs_fp64"] = round(res["fp64"][1] / res["mixed"][1], 3)
+        out["speedup_mixed_bf16x3_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3"][1], 3)
     finally:
         b.close()
     return out

@@ -170,7 +170,9 @@ int sdpx_dense_synchronize(sdpx_dense_t* h);
  * low-precision factor, target-precision residuals and refinement,
  * _try_factor_mixed_kkt! :972-1111, _refine_mixed_direction! :1634-1713) applied
  * to the FP64 target: precision 1 assembles the Schur Gram product from an
- * FP32 copy of Phat on FP32 MFMA (S stays FP64, lower); 0 restores FP64. */
+ * FP32 copy of Phat on FP32 MFMA, precision 2 from a bf16 hi / lo split of
+ * Phat as three bf16 MFMA products accumulated in FP32 (BF16x3); S stays FP64
+ * (lower) in both; 0 restores FP64. */
 int sdpx_dense_set_precision(sdpx_dense_t* h, int precision);
 /* rho = r - S dx with S applied matrix-free in FP64,
  * (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y> (the exact operator the FP32

@@ -124,6 +124,14 @@ int gemm_f64(const GemmArgs& g, hipStream_t stream, int split_k = 1, double* wor
 // 16-byte aligned); split-K partials in ws (split * m * m doubles).
 int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long lds, double beta, int split,
              double* ws, const int2* tile_order, hipStream_t stream, hipEvent_t after_main = nullptr);
+// BF16x3 Schur Gram product (gemm_f32.hip): S (lower, FP64) = P' P + beta S
+// with P ~= H + L given as two bf16 planes (ld ldp, Kp rows, both multiples of
+// 32, padding rows zero), three bf16 MFMA products H'H + H'L + L'H per step.
+int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, double* S, long long lds, double beta,
+                int split, double* ws, const int2* tile_order, hipStream_t stream, hipEvent_t after_main = nullptr);
+// H = bf16(src), L = bf16(src - H) (ld ld_dst, rows >= `rows` zeroed), rows x cols
+int split_panel_bf16(const double* src, long long ld_src, __bf16* H, __bf16* L, long long ld_dst, long long rows,
+                     int cols, hipStream_t stream);
 // dst (ld_dst >= rows, padding rows zeroed) = (float) src (ld_src), rows x cols
 int convert_panel_f32(const double* src, long long ld_src, float* dst, long long ld_dst, long long rows, int cols,
                       hipStream_t stream);

@@ -186,9 +186,12 @@ struct sdpx_dense {
   int split = 1;
   int2* tile_order = nullptr;  // Schur Gram lower-tile visiting order
   // mixed precision (sdpx_dense_set_precision): FP32 copy of the Schur panel
+  // (precision 1) or its bf16 hi / lo planes (precision 2)
   int precision = 0;
   float* Pf = nullptr;
   long long ldPf = 0, maxk2 = 0;
+  __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
+  long long ldPb = 0;
   double* gemv_part = nullptr;
   long long gemv_part_cap = 0;
   double* symv_part = nullptr;
@@ -840,7 +843,15 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);
     s.tile_order = h->tile_order;
-    if (h->precision == 1) {
+    if (h->precision == 2) {
+      // mixed precision: Phat split into bf16 hi + lo planes, Gram product as three
+      // bf16 MFMA products accumulated in FP32, S kept in FP64
+      const long long kp = (b.k2 + 31) / 32 * 32;
+      SDPX_CHECK(split_panel_bf16(h->Ph, b.k2, h->Ph_hi, h->Ph_lo, h->ldPb, b.k2, m, h->stream));
+      if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
+      SDPX_CHECK(gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, l == 0 ? 0.0 : 1.0, h->split,
+                             h->splitws, h->tile_order, h->stream, h->prof ? h->ev[2] : nullptr));
+    } else if (h->precision == 1) {
       // mixed precision: Phat rounded to FP32, Gram product on FP32 MFMA, S kept in FP64
       SDPX_CHECK(convert_panel_f32(h->Ph, b.k2, h->Pf, h->ldPf, b.k2, m, h->stream));
       if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
@@ -1017,13 +1028,19 @@ int sdpx_dense_refine(sdpx_dense_t* h, double abstol, int max_steps, int fixed_p
 
 int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
   SDPX_CHECK(check_handle(h));
-  if (precision != 0 && precision != 1) {
-    set_last_error("sdpx_dense_set_precision: precision must be 0 (FP64) or 1 (FP32 Schur Gram)");
+  if (precision < 0 || precision > 2) {
+    set_last_error("sdpx_dense_set_precision: precision must be 0 (FP64), 1 (FP32 Schur Gram) or 2 (BF16x3 Schur Gram)");
     return kInvalidArgument;
   }
   if (precision == 1 && !h->Pf) {
     h->ldPf = (h->maxk2 + 3) / 4 * 4;
     int s = dalloc(h, &h->Pf, (size_t)(h->ldPf * h->m));
+    if (s) return s;
+  }
+  if (precision == 2 && !h->Ph_hi) {
+    h->ldPb = (h->maxk2 + 31) / 32 * 32;
+    int s = dalloc(h, &h->Ph_hi, (size_t)(h->ldPb * h->m));
+    if (!s) s = dalloc(h, &h->Ph_lo, (size_t)(h->ldPb * h->m));
     if (s) return s;
   }
   h->precision = precision;

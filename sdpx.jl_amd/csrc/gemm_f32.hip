@@ -1,7 +1,14 @@
-// FP32 Schur Gram product on CDNA4 matrix cores (v_mfma_f32_16x16x4_f32) for
-// the mixed-precision route: S = Phat' Phat over lower 128 x 128 tiles, Phat
-// in FP32 (columns K-contiguous), accumulated in FP32 per K range and written
-// to FP64 S (or to FP64 split-K partials reduced in a fixed order).
+// Reduced-precision Schur Gram products on CDNA4 matrix cores for the
+// mixed-precision route: S = Phat' Phat over lower 128 x 128 tiles, Phat
+// columns K-contiguous, accumulated in FP32 per K range and written to FP64 S
+// (or to FP64 split-K partials reduced in a fixed order).
+//
+//  * FP32: Phat rounded to FP32, v_mfma_f32_16x16x4_f32.
+//  * BF16x3: Phat split into two bf16 planes, Phat ~= H + L with
+//    H = bf16(Phat), L = bf16(Phat - H) (about 16 significant bits), and
+//    S_ij ~= sum_k (H_ki H_kj + H_ki L_kj + L_ki H_kj) as three
+//    v_mfma_f32_16x16x32_bf16 products per K step (the dropped L L term and
+//    the splitting leave a relative error of order 2^-16 per product).
 //
 // Same organisation as the FP64 kernel (gemm_f64.hip): 256 threads = 2 x 2
 // waves of 64 x 64, BK = 16 slices double-buffered through LDS, the next slice
@@ -19,6 +26,7 @@ constexpr int BK = 16;
 constexpr int PADF = 4;  // LDS row pitch BM + 4 floats
 
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 __global__ __launch_bounds__(256, 2) void schur_gram_f32_kernel(const float* __restrict__ P, long long ldp, int m,
                                                                 int K, double* S, long long lds, double beta,
@@ -174,6 +182,157 @@ __global__ void panel_f32_kernel(const double* __restrict__ src, long long ld_sr
   }
 }
 
+// ---- BF16x3 ------------------------------------------------------------------
+constexpr int BKB = 32;       // K per slice: one bf16 MFMA step
+constexpr int LDB = BKB + 8;  // LDS row pitch in bf16 (80 bytes: 16-byte aligned, 2-way banks)
+
+// H = bf16(p), L = bf16(p - H) for every entry; rows >= rows of each column are zero
+__global__ void split_bf16_kernel(const double* __restrict__ src, long long ld_src, __bf16* __restrict__ H,
+                                  __bf16* __restrict__ L, long long ld_dst, long long rows, int cols) {
+  const long long n = ld_dst * cols;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const long long r = i % ld_dst, c = i / ld_dst;
+    const double p = r < rows ? src[r + c * ld_src] : 0.0;
+    const __bf16 h = (__bf16)(float)p;
+    H[i] = h;
+    L[i] = (__bf16)(float)(p - (double)(float)h);
+  }
+}
+
+// One 128 x 128 lower tile of S (or of a split-K partial).  256 threads =
+// 2 x 2 waves of 64 x 64 (4 x 4 blocks of 16 x 16); the next 32-deep slice of
+// both planes of both operand tiles is prefetched into registers while the
+// current one, staged in LDS, feeds 48 MFMAs per wave.  Operand layout of
+// v_mfma_f32_16x16x32_bf16: lane l holds rows (l % 16) with K entries
+// 8 (l / 16) .. +7 of both operands (one 16-byte LDS read each); the
+// accumulator holds D[4 (l / 16) + r][l % 16].  Operands are fed (B, A) so the
+// accumulator holds C' and stores are contiguous.  The planes are zero padded
+// to a multiple of 32 rows, so no slice needs masking.
+__global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16* __restrict__ H,
+                                                                   const __bf16* __restrict__ L, long long ldp,
+                                                                   int m, int Kp, double* S, long long lds,
+                                                                   double beta, int split, double* ws,
+                                                                   const int2* tile_order) {
+  __shared__ __align__(16) __bf16 As[2][BM][LDB];
+  __shared__ __align__(16) __bf16 Bs[2][BM][LDB];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
+
+  const int total = gridDim.x;
+  const int bid = blockIdx.x;
+  const int xcd = bid % kNumXcd, slot = bid / kNumXcd;
+  const int q = total / kNumXcd, rq = total % kNumXcd;
+  const int lin = xcd * q + (xcd < rq ? xcd : rq) + slot;
+  const int per_split = total / split;
+  const int s = lin / per_split;
+  const int tile = lin % per_split;
+  int ti, tj;
+  if (tile_order) {
+    ti = tile_order[tile].x;
+    tj = tile_order[tile].y;
+  } else {
+    int t = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+    while ((long long)(t + 1) * (t + 2) / 2 <= tile) ++t;
+    while ((long long)t * (t + 1) / 2 > tile) --t;
+    ti = t;
+    tj = tile - t * (t + 1) / 2;
+  }
+  const int m0 = ti * BM, n0 = tj * BM;
+
+  int kBegin = 0, kEnd = Kp;
+  if (split > 1) {
+    const int stages = Kp / BKB;
+    const int per = (stages + split - 1) / split;
+    kBegin = min(Kp, s * per * BKB);
+    kEnd = min(Kp, kBegin + per * BKB);
+  }
+
+  // thread loads: rows lr and lr + 64 of each operand tile, K entries kc .. kc + 7, both planes
+  const int lr = tid >> 2, kc = (tid & 3) * 8;
+  const long long ra0 = (long long)min(m0 + lr, m - 1) * ldp, ra1 = (long long)min(m0 + lr + 64, m - 1) * ldp;
+  const long long rb0 = (long long)min(n0 + lr, m - 1) * ldp, rb1 = (long long)min(n0 + lr + 64, m - 1) * ldp;
+
+  float4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+  bf16x8_t v[8];
+  auto load = [&](int k0) {
+    const long long o = k0 + kc;
+    v[0] = *reinterpret_cast<const bf16x8_t*>(H + ra0 + o);
+    v[1] = *reinterpret_cast<const bf16x8_t*>(H + ra1 + o);
+    v[2] = *reinterpret_cast<const bf16x8_t*>(L + ra0 + o);
+    v[3] = *reinterpret_cast<const bf16x8_t*>(L + ra1 + o);
+    v[4] = *reinterpret_cast<const bf16x8_t*>(H + rb0 + o);
+    v[5] = *reinterpret_cast<const bf16x8_t*>(H + rb1 + o);
+    v[6] = *reinterpret_cast<const bf16x8_t*>(L + rb0 + o);
+    v[7] = *reinterpret_cast<const bf16x8_t*>(L + rb1 + o);
+  };
+  auto store = [&]() {
+    *reinterpret_cast<bf16x8_t*>(&As[0][lr][kc]) = v[0];
+    *reinterpret_cast<bf16x8_t*>(&As[0][lr + 64][kc]) = v[1];
+    *reinterpret_cast<bf16x8_t*>(&As[1][lr][kc]) = v[2];
+    *reinterpret_cast<bf16x8_t*>(&As[1][lr + 64][kc]) = v[3];
+    *reinterpret_cast<bf16x8_t*>(&Bs[0][lr][kc]) = v[4];
+    *reinterpret_cast<bf16x8_t*>(&Bs[0][lr + 64][kc]) = v[5];
+    *reinterpret_cast<bf16x8_t*>(&Bs[1][lr][kc]) = v[6];
+    *reinterpret_cast<bf16x8_t*>(&Bs[1][lr + 64][kc]) = v[7];
+  };
+  auto compute = [&]() {
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    bf16x8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ah[i] = *reinterpret_cast<const bf16x8_t*>(&As[0][wm0 + 16 * i + fr][fk]);
+      al[i] = *reinterpret_cast<const bf16x8_t*>(&As[1][wm0 + 16 * i + fr][fk]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][wn0 + 16 * j + fr][fk]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][wn0 + 16 * j + fr][fk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
+      }
+  };
+  if (kBegin < kEnd) {
+    load(kBegin);
+    for (int k0 = kBegin; k0 < kEnd; k0 += BKB) {
+      store();
+      __syncthreads();
+      if (k0 + BKB < kEnd) load(k0 + BKB);
+      compute();
+      __syncthreads();
+    }
+  }
+  const bool part = split > 1;
+  double* out = part ? ws + (long long)s * m * m : S;
+  const long long ldo = part ? (long long)m : lds;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = m0 + wm0 + i * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + 4 * (lane >> 4) + r;
+        if (row < m && col < m && row >= col) {
+          double* cp = out + row + (long long)col * ldo;
+          const double v2 = (double)acc[i][j][r];
+          *cp = (part || beta == 0.0) ? v2 : v2 + beta * (*cp);
+        }
+      }
+  }
+}
+
 }  // namespace
 
 int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long lds, double beta, int split,
@@ -188,6 +347,41 @@ int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long l
   const long long tiles = (long long)nt * (nt + 1) / 2;
   hipLaunchKernelGGL(schur_gram_f32_kernel, dim3((unsigned)(tiles * split)), dim3(256), 0, stream, P, ldp, m, K, S,
                      lds, beta, split, ws, tile_order);
+  SDPX_HIP(hipGetLastError());
+  if (after_main) SDPX_HIP(hipEventRecord(after_main, stream));
+  if (split > 1) {
+    const long long mm = (long long)m * m;
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)std::min<long long>(ceil_div(mm, 256), 4096)), dim3(256), 0,
+                       stream, ws, split, m, S, lds, beta);
+    SDPX_HIP(hipGetLastError());
+  }
+  return kOk;
+}
+
+int split_panel_bf16(const double* src, long long ld_src, __bf16* H, __bf16* L, long long ld_dst, long long rows,
+                     int cols, hipStream_t stream) {
+  if (rows <= 0 || cols <= 0) return kOk;
+  const long long n = ld_dst * cols;
+  hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)std::min<long long>(ceil_div(n, 256), 65536)), dim3(256), 0,
+                     stream, src, ld_src, H, L, ld_dst, rows, cols);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, double* S, long long lds,
+                double beta, int split, double* ws, const int2* tile_order, hipStream_t stream,
+                hipEvent_t after_main) {
+  if (m <= 0) return kOk;
+  if (ldp % BKB != 0 || Kp % BKB != 0 || Kp > ldp || (reinterpret_cast<uintptr_t>(H) & 15) != 0 ||
+      (reinterpret_cast<uintptr_t>(L) & 15) != 0) {
+    set_last_error("gram_bf16x3: the bf16 planes need 16-byte aligned bases and K, ld multiples of 32");
+    return kInvalidArgument;
+  }
+  if (split < 1) split = 1;
+  const int nt = ceil_div(m, BM);
+  const long long tiles = (long long)nt * (nt + 1) / 2;
+  hipLaunchKernelGGL(schur_gram_bf16x3_kernel, dim3((unsigned)(tiles * split)), dim3(256), 0, stream, H, L, ldp, m,
+                     Kp, S, lds, beta, split, ws, tile_order);
   SDPX_HIP(hipGetLastError());
   if (after_main) SDPX_HIP(hipEventRecord(after_main, stream));
   if (split > 1) {

@@ -252,7 +252,7 @@ class MI355XDenseBackend:
 
     # ---- mixed precision -------------------------------------------------------------
     def set_precision(self, precision: int):
-        """0: FP64 Schur assembly; 1: FP32 Schur Gram product (S kept in FP64)."""
+        """0: FP64 Schur assembly; 1: FP32 Schur Gram product; 2: BF16x3 Schur Gram product (S kept in FP64)."""
         check(self.lib.sdpx_dense_set_precision(self.h, int(precision)), "set_precision")
 
     def exact_residual(self):

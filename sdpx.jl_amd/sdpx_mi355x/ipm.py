@@ -39,7 +39,7 @@ class SDPResult:
     phase_times: dict = field(default_factory=dict)
     history: list = field(default_factory=list)
     schur_trace: list = field(default_factory=list)
-    fp32_iterations: int = 0   # Newton steps whose Schur matrix was assembled in FP32 (mixed precision)
+    fp32_iterations: int = 0   # Newton steps whose Schur matrix was assembled in FP32 or BF16x3 (mixed precision)
     fp64_iterations: int = 0
 
 
@@ -86,8 +86,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     if not ok:
         return dict(status="breakdown", reason="Cholesky factorization of X or Y failed (iterate left the interior)",
                     p_res=p_res, d_res=d_res)
-    if o.precision == "mixed":
-        b.set_precision(1 if use32 else 0)
+    if o.precision != "fp64":
+        b.set_precision((1 if o.precision == "mixed" else 2) if use32 else 0)
     b.schur_build()
     clock.lap("schur_assembly")
     if trace_schur is not None:
@@ -96,7 +96,7 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     clock.lap("kkt_factorization")
     if not fok:
         if use32:
-            return dict(status="retry_fp64", reason="FP32-assembled Schur matrix not positive definite")
+            return dict(status="retry_fp64", reason="reduced-precision Schur matrix not positive definite")
         return dict(status="breakdown",
                     reason=f"Schur complement not positive definite after {reg_attempts} regularization attempt(s)",
                     p_res=p_res, d_res=d_res, reg_attempts=reg_attempts)
@@ -104,8 +104,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
         # _triangular_condition_estimate (mixed_precision_kkt.jl:371-392): squared diagonal ratio of the factor
         _, _, q = b.factor_quality()
         cond_est = 1.0 / (q * q) if q > 0 else math.inf
-        if not cond_est <= o.mixed_cond_limit:
-            return dict(status="retry_fp64", reason=f"Schur condition estimate {cond_est:.2e} above the FP32 limit")
+        if not cond_est <= (o.mixed_cond_limit if o.precision == "mixed" else o.bf16x3_cond_limit):
+            return dict(status="retry_fp64", reason=f"Schur condition estimate {cond_est:.2e} above the reduced-precision limit")
     # predictor (step.jl:699-726)
     b.predictor_rhs()
     b.solve_direction()
@@ -241,7 +241,7 @@ def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
     comp_cur = float(sum(np.sum(X[l] * Y[l]) for l in range(L)))
     iteration, restarts, centering = 0, 0, 0
     status, message = "NotStarted", ""
-    mixed_active = o.precision == "mixed"
+    mixed_active = o.precision != "fp64"
     fp32_iters = fp64_iters = 0
     gap_rel = math.inf
     history = []

@@ -53,13 +53,16 @@ class SolverOptions:
     parameter_strategy: str = "adaptive"
     adaptive_sigma_max: float = 0.0
     verbosity: int = 0
-    # mixed precision (the FP32 -> FP64 instance of kernels/mixed_precision_kkt.jl): "fp64" or
-    # "mixed".  Mixed assembles the Schur Gram product in FP32 while the factor's condition
-    # estimate stays below mixed_cond_limit, and refines every direction against the matrix-free
-    # FP64 Schur operator to mixed_refine_target * max(||r||_inf, 1) in at most mixed_max_refine
-    # steps; otherwise the iteration is redone in FP64 and FP64 is kept from then on.
+    # mixed precision (the FP32 -> FP64 instance of kernels/mixed_precision_kkt.jl): "fp64",
+    # "mixed" or "mixed_bf16x3".  Mixed assembles the Schur Gram product in FP32 ("mixed") or as
+    # three bf16 MFMA products of a hi / lo split of Phat ("mixed_bf16x3", about 16 significant
+    # bits) while the factor's condition estimate stays below mixed_cond_limit (FP32) or
+    # bf16x3_cond_limit, and refines every direction against the matrix-free FP64 Schur operator
+    # to mixed_refine_target * max(||r||_inf, 1) in at most mixed_max_refine steps; otherwise the
+    # iteration is redone in FP64 and FP64 is kept from then on.
     precision: str = "fp64"
     mixed_cond_limit: float = 1e6
+    bf16x3_cond_limit: float = 1e4
     mixed_refine_target: float = 1e-10
     mixed_max_refine: int = 32
 
@@ -74,8 +77,8 @@ class SolverOptions:
             raise ValueError("step_rule must be 'backtrack', 'fraction_to_boundary', or 'auto'")
         if self.refine_policy not in ("fixed", "adaptive", "auto"):
             raise ValueError("refine_policy must be 'fixed', 'adaptive', or 'auto'")
-        if self.precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if self.precision not in ("fp64", "mixed", "mixed_bf16x3"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_bf16x3'")
 
 
 def settings(tolerance: float = 1e-8, **kw) -> SolverOptions:

@@ -1,9 +1,11 @@
-"""Mixed precision (FP32 Schur Gram + FP64 matrix-free refinement) on the dense route.
+"""Mixed precision (FP32 or BF16x3 Schur Gram + FP64 matrix-free refinement) on the dense route.
 
 Pins: the matrix-free FP64 residual equals the assembled-S residual when S is
 assembled in FP64; the FP32-assembled S is within FP32 accuracy of the oracle's
-S; a mixed solve reaches the same optimum as the oracle (1e-8) with a valid
-certificate and at least one FP32 iteration.
+S and the BF16x3-assembled S within the 2^-16 accuracy of its hi / lo split
+(a dropped cross term or plane would leave 2^-8); mixed solves reach the same
+optimum as the oracle (1e-8) with a valid certificate and at least one
+reduced-precision iteration.
 """
 
 import numpy as np
@@ -86,6 +88,51 @@ def test_mixed_solve_matches_oracle(gpu_lib, k, m):
     oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
     ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
     print(f"mixed: {res.status} it={res.iterations} fp32={res.fp32_iterations} fp64={res.fp64_iterations} "
+          f"pobj={res.pobj:.12e} | oracle {ref.status} it={ref.iterations} pobj={ref.pobj:.12e}", flush=True)
+    assert res.status == ref.status == "Optimal"
+    assert res.fp32_iterations >= 1
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    assert abs(res.iterations - ref.iterations) <= 3
+    cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
+                                         res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
+    assert cert["valid"], cert
+
+
+@pytest.mark.gpu
+def test_bf16x3_schur_within_split_accuracy(gpu_lib):
+    g, prob, oprob, X, Y = _state(64, 300, 2)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(g.x0, [X], [Y])
+        b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        b.set_precision(2)
+        b.schur_build()
+        Sb = b.get_schur()
+        b.set_precision(1)
+        b.schur_build()
+        S32 = b.get_schur()
+    finally:
+        b.close()
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    ref = O.schur_build(oprob, [LX], [MY])
+    err = rel_fro_lower(Sb, ref)
+    err32 = rel_fro_lower(S32, ref)
+    print(f"BF16x3-assembled Schur rel Frobenius error {err:.2e} (FP32 {err32:.2e})", flush=True)
+    assert err32 < err <= 1e-4
+
+
+@pytest.mark.gpu
+def test_mixed_bf16x3_solve_matches_oracle(gpu_lib):
+    k, m = 50, 200
+    g = generators.dense_feasible(k, m, seed=3)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3")
+    res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    print(f"mixed_bf16x3: {res.status} it={res.iterations} low={res.fp32_iterations} fp64={res.fp64_iterations} "
           f"pobj={res.pobj:.12e} | oracle {ref.status} it={ref.iterations} pobj={ref.pobj:.12e}", flush=True)
     assert res.status == ref.status == "Optimal"
     assert res.fp32_iterations >= 1
