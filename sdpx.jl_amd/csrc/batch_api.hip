@@ -382,6 +382,13 @@ struct sdpx_batch {
   Arrays arr{};
   Member* dmem = nullptr;
   long long nK = 0, nM = 0, nAv = 0, nPh = 0, nS = 0, nD = 0, nG = 0;
+  // sparse copy of COO constraints for the gather kernels (freed by set_dense / destroy)
+  long long* sp_ptr = nullptr;
+  int* sp_idx = nullptr;
+  double* sp_val = nullptr;
+  long long* sp_pptr = nullptr;
+  int* sp_pcon = nullptr;
+  double* sp_pval = nullptr;
   // grouped Schur Gram product
   GroupGemm* dgram = nullptr;
   GroupTile* dgram_tiles = nullptr;
@@ -440,6 +447,126 @@ int check_batch(sdpx_batch_t* h) {
     return kInvalidArgument;
   }
   SDPX_HIP(hipSetDevice(h->device));
+  return kOk;
+}
+
+void clear_sparse(sdpx_batch_t* h) {
+  void* bufs[] = {h->sp_ptr, h->sp_idx, h->sp_val, h->sp_pptr, h->sp_pcon, h->sp_pval};
+  bool any = false;
+  for (void* b : bufs) any = any || b;
+  if (any) hipStreamSynchronize(h->stream);
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  h->sp_ptr = h->sp_pptr = nullptr;
+  h->sp_idx = h->sp_pcon = nullptr;
+  h->sp_val = h->sp_pval = nullptr;
+  h->arr.sp_ptr = h->arr.sp_pptr = nullptr;
+  h->arr.sp_idx = h->arr.sp_pcon = nullptr;
+  h->arr.sp_val = h->arr.sp_pval = nullptr;
+}
+
+// Sparse copy of COO constraints (upper-triangle entries per constraint, in
+// member order) for the gather forms of Av x and Av' w: per constraint the
+// duplicates are summed in entry order (as the densified panel sums them) and
+// every off-diagonal entry is stored at both mirror positions, so the
+// positions of one constraint are unique.  Offsets follow the m-vector layout.
+int build_sparse(sdpx_batch_t* h, const long long* ptr, const int* row, const int* col, const double* val) {
+  clear_sparse(h);
+  std::vector<long long> sptr((size_t)h->nM + 1, 0);
+  std::vector<int> sidx;
+  std::vector<double> sval;
+  long long ncons = 0;
+  for (int p = 0; p < h->B; ++p) ncons += h->mem[p].m;
+  sidx.reserve((size_t)(2 * ptr[ncons] + 16));
+  sval.reserve((size_t)(2 * ptr[ncons] + 16));
+  std::vector<int> stamp(KMAX * KMAX, -1), slot(KMAX * KMAX, 0);
+  std::vector<int> upos;
+  std::vector<double> uval;
+  long long j = 0, t = 0;
+  int gen = 0;
+  for (int p = 0; p < h->B; ++p) {
+    const Member& M = h->mem[p];
+    for (; t < M.oM; ++t) sptr[(size_t)t] = (long long)sidx.size();
+    for (int i = 0; i < M.m; ++i, ++j, ++gen) {
+      sptr[(size_t)(t++)] = (long long)sidx.size();
+      upos.clear();
+      uval.clear();
+      for (long long e = ptr[j]; e < ptr[j + 1]; ++e) {
+        const int pos = row[e] + col[e] * M.k;
+        if (stamp[pos] == gen) {
+          uval[slot[pos]] += val[e];
+        } else {
+          stamp[pos] = gen;
+          slot[pos] = (int)upos.size();
+          upos.push_back(pos);
+          uval.push_back(val[e]);
+        }
+      }
+      for (size_t u = 0; u < upos.size(); ++u) {
+        const int r = upos[u] % M.k, c = upos[u] / M.k;
+        sidx.push_back(upos[u]);
+        sval.push_back(uval[u]);
+        if (r != c) {
+          sidx.push_back(c + r * M.k);
+          sval.push_back(uval[u]);
+        }
+      }
+    }
+  }
+  for (; t <= h->nM; ++t) sptr[(size_t)t] = (long long)sidx.size();
+  // position-ordered copy: per member a counting sort of its entries by position,
+  // stable in constraint order
+  const size_t ne = std::max<size_t>(sidx.size(), 1);
+  std::vector<long long> pptr((size_t)h->nK + 1, 0);
+  std::vector<int> pcon(ne, 0);
+  std::vector<double> pval(ne, 0.0);
+  {
+    long long u = 0;  // running position offset in the nK layout
+    for (int p = 0; p < h->B; ++p) {
+      const Member& M = h->mem[p];
+      const int k2 = M.k * M.k;
+      for (; u < M.oK; ++u) pptr[(size_t)u] = sptr[(size_t)M.oM];
+      const long long e0 = sptr[(size_t)M.oM], e1 = sptr[(size_t)(M.oM + M.m)];
+      std::vector<long long> cnt((size_t)k2 + 1, 0);
+      for (long long e = e0; e < e1; ++e) ++cnt[(size_t)sidx[(size_t)e] + 1];
+      for (int q = 0; q < k2; ++q) cnt[(size_t)q + 1] += cnt[(size_t)q];
+      for (int q = 0; q < k2; ++q) pptr[(size_t)(M.oK + q)] = e0 + cnt[(size_t)q];
+      u = M.oK + k2;
+      for (int i = 0; i < M.m; ++i)
+        for (long long e = sptr[(size_t)(M.oM + i)]; e < sptr[(size_t)(M.oM + i + 1)]; ++e) {
+          const long long slotq = e0 + cnt[(size_t)sidx[(size_t)e]]++;
+          pcon[(size_t)slotq] = i;
+          pval[(size_t)slotq] = sval[(size_t)e];
+        }
+    }
+    for (; u <= h->nK; ++u) pptr[(size_t)u] = (long long)sidx.size();
+  }
+  if (hipMalloc(&h->sp_ptr, sizeof(long long) * sptr.size()) != hipSuccess ||
+      hipMalloc(&h->sp_idx, sizeof(int) * ne) != hipSuccess || hipMalloc(&h->sp_val, sizeof(double) * ne) != hipSuccess ||
+      hipMalloc(&h->sp_pptr, sizeof(long long) * pptr.size()) != hipSuccess ||
+      hipMalloc(&h->sp_pcon, sizeof(int) * ne) != hipSuccess ||
+      hipMalloc(&h->sp_pval, sizeof(double) * ne) != hipSuccess) {
+    clear_sparse(h);
+    set_last_error("sdpx_batch_set_coo: hipMalloc of the sparse copy failed");
+    return kOutOfMemory;
+  }
+  const size_t n = sidx.size();
+  if (hipMemcpy(h->sp_ptr, sptr.data(), sizeof(long long) * sptr.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->sp_pptr, pptr.data(), sizeof(long long) * pptr.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (n > 0 && (hipMemcpy(h->sp_idx, sidx.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(h->sp_val, sval.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(h->sp_pcon, pcon.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                 hipMemcpy(h->sp_pval, pval.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess))) {
+    clear_sparse(h);
+    set_last_error("sdpx_batch_set_coo: upload of the sparse copy failed");
+    return kHipError;
+  }
+  h->arr.sp_ptr = h->sp_ptr;
+  h->arr.sp_idx = h->sp_idx;
+  h->arr.sp_val = h->sp_val;
+  h->arr.sp_pptr = h->sp_pptr;
+  h->arr.sp_pcon = h->sp_pcon;
+  h->arr.sp_pval = h->sp_pval;
   return kOk;
 }
 
@@ -1326,6 +1453,7 @@ int sdpx_batch_destroy(sdpx_batch_t* h) {
   if (!h) return kOk;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
+  clear_sparse(h);
   for (void* p : h->allocs) hipFree(p);
   if (h->hlist) hipHostFree(h->hlist);
   if (h->hsc) hipHostFree(h->hsc);
@@ -1391,6 +1519,7 @@ int sdpx_batch_set_dense(sdpx_batch_t* h, const double* Av, const double* C, con
     set_last_error("sdpx_batch_set_dense: null data");
     return kInvalidArgument;
   }
+  clear_sparse(h);  // dense constraints: the gather kernels no longer apply
   long long o = 0;
   for (int p = 0; p < h->B; ++p) {
     const Member& M = h->mem[p];
@@ -1463,6 +1592,7 @@ int sdpx_batch_set_coo(sdpx_batch_t* h, const long long* ptr, const int* row, co
     st = kHipError;
   }
   if (st == kOk) st = launch_coo_densify(h->arr, dptr, drow, dcol, dval, dcons, (int)ncons, h->stream);
+  if (st == kOk) st = build_sparse(h, ptr, row, col, val);
   if (st == kOk) st = upload_blocks(h, h->arr.C, C, true);
   if (st == kOk) st = upload_blocks(h, h->arr.c, c, false);
   if (st == kOk && hipStreamSynchronize(h->stream) != hipSuccess) st = kHipError;

@@ -192,12 +192,64 @@ __global__ __launch_bounds__(NT) void gemv_part_kernel(Arrays a, const int* list
   }
 }
 
+// Sparse form of gemv_part: (sum_i vec_i A_i)_e gathered per position e from
+// the position-ordered copy (vec in LDS).  A group of 16 lanes owns one
+// position at a time: its entries are read 16 at a time (coalesced), each lane
+// keeps a running sum and the group reduces them in a fixed tree, so the
+// result is deterministic; it is written as the member's single partial.
+__global__ __launch_bounds__(NT) void sp_gemv_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double xs[MMAX];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k2 = M.k * M.k;
+  const double* vec = (mode == 1 || mode == 3 ? a.dx : a.x) + M.oM;
+  for (int i = threadIdx.x; i < M.m; i += NT) xs[i] = vec[i];
+  __syncthreads();
+  const long long* pp = a.sp_pptr + M.oK;
+  double* part = a.gpart + M.oG;
+  const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  for (int e = g; e < k2; e += NT / 16) {
+    double acc = 0.0;
+    const long long q1 = pp[e + 1];
+    for (long long q = pp[e] + gl; q < q1; q += 16) acc += a.sp_pval[q] * xs[a.sp_pcon[q]];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+    if (gl == 0) part[e] = acc;
+  }
+}
+
+// Sparse form of gemvt: <A_i, W> gathered from an LDS copy of W, one
+// wavefront per constraint (four per member), reduced across the wave.
+__global__ __launch_bounds__(NT) void sp_gemvt_kernel(Arrays a, const int* list, int mode) {
+  __shared__ double W[KMAX * KMAX];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k2 = M.k * M.k;
+  const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
+  for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long* ptr = a.sp_ptr + M.oM;
+  for (int i = wave; i < M.m; i += NT / 64) {
+    double s = 0.0;
+    for (long long e = ptr[i] + lane; e < ptr[i + 1]; e += 64) s += a.sp_val[e] * W[a.sp_idx[e]];
+    s = wave_sum(s);
+    if (lane == 0) {
+      const long long o = M.oM + i;
+      if (mode == 1)
+        a.r[o] = -(a.d[o] + s);
+      else
+        (mode == 0 ? a.d : a.v)[o] = a.c[o] - s;
+    }
+  }
+}
+
 __global__ __launch_bounds__(NT) void gemv_kernel(Arrays a, const int* list, int mode) {
   __shared__ double red[NT];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
   const long long k2 = (long long)M.k * M.k;
-  const int nch = (M.m + kColChunk - 1) / kColChunk;
+  const int nch = a.sp_ptr ? 1 : (M.m + kColChunk - 1) / kColChunk;  // the sparse form leaves one partial
   const double* part = a.gpart + M.oG;
   double amax = 0.0;
   for (long long e = threadIdx.x; e < k2; e += NT) {
@@ -1083,13 +1135,19 @@ __global__ void coo_densify_kernel(Arrays a, const long long* cptr, const int* c
 
 int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
   if (n <= 0) return kOk;
-  hipLaunchKernelGGL(gemv_part_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
+  if (a.sp_ptr)
+    hipLaunchKernelGGL(sp_gemv_kernel, dim3((unsigned)n), dim3(NT), 0, st, a, list, mode);
+  else
+    hipLaunchKernelGGL(gemv_part_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
   SDPX_HIP(hipGetLastError());
   SDPX_LAUNCH(gemv_kernel, n, a, list, mode);
 }
 int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
   if (n <= 0) return kOk;
-  hipLaunchKernelGGL(gemvt_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
+  if (a.sp_ptr)
+    hipLaunchKernelGGL(sp_gemvt_kernel, dim3((unsigned)n), dim3(NT), 0, st, a, list, mode);
+  else
+    hipLaunchKernelGGL(gemvt_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
   SDPX_HIP(hipGetLastError());
   if (mode == 1) return kOk;
   SDPX_LAUNCH(vec_norm_kernel, n, a, list, mode);

@@ -63,6 +63,19 @@ struct Arrays {
   double *S, *Sf, *dinv;
   double* gpart;   // Av x partial sums
   int max_chunks;  // most column chunks of any member
+  // Sparse copy of the constraints when they were given as COO (null otherwise):
+  // constraint i of member p owns entries [sp_ptr[oM + i], sp_ptr[oM + i + 1]), each a
+  // position r + c k of the k x k block (both mirror positions of an off-diagonal
+  // entry, duplicates summed, positions unique within a constraint) and its value.
+  const long long* sp_ptr;
+  const int* sp_idx;
+  const double* sp_val;
+  // the same entries ordered by position: position e of member p owns entries
+  // [sp_pptr[oK + e], sp_pptr[oK + e + 1]), each a constraint index and its value,
+  // in increasing constraint order
+  const long long* sp_pptr;
+  const int* sp_pcon;
+  const double* sp_pval;
   double* sc;   // B x NSLOT
   int* si;      // B x NISLOT
   double* par;  // B x NPARAM

@@ -195,6 +195,103 @@ int potrf_f64(double* A, long long lda, long long stride, int n, int batch, doub
   return kOk;
 }
 
+namespace {
+
+constexpr int kTrsvRows = 64;  // rows (forward) or columns of L (backward) per work-group of one step
+
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One block step of a triangular solve with L (lower, diagonal-block inverses
+// in dinv), as a single launch: every work-group applies the block's inverse
+// (or its transpose) to x_b itself, group 0 writes the solved block to y
+// (x_b is only read during the launch), and each group updates its 64 rows of
+// x: forward x_i -= L(i, b) y_b for the rows below the block (lanes along
+// rows, coalesced); backward x_i -= L(b, i)' y_b for the rows above it (one
+// wave per column of L, 64 contiguous entries reduced across the wave).  All
+// global loads of a thread are issued before the first barrier.
+__global__ __launch_bounds__(256) void trsv_step_kernel(const double* __restrict__ L, long long lda, int n,
+                                                        const double* __restrict__ D, int j0, int jb,
+                                                        double* x, double* y, int trans) {
+  constexpr int kPer = NB / 4;  // entries per thread in each 64 x 64 product
+  __shared__ double xb[NB];
+  __shared__ double yb[NB];
+  __shared__ double red[4][NB];
+  const int tid = threadIdx.x, r = tid & 63, part = tid >> 6;
+  double dv[kPer], lv[kPer];
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const int c = part + 4 * t;
+    dv[t] = c < jb && r < jb ? (trans ? D[c + r * NB] : D[r + c * NB]) : 0.0;
+  }
+  const int i_fwd = j0 + jb + blockIdx.x * kTrsvRows + r;
+  const int i_bwd0 = blockIdx.x * kTrsvRows + part;
+  const int i_end = min(j0, (blockIdx.x + 1) * kTrsvRows);
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    if (!trans) {
+      const int c = part + 4 * t;
+      lv[t] = (i_fwd < n && c < jb) ? L[i_fwd + (long long)(j0 + c) * lda] : 0.0;
+    } else {
+      const int i = i_bwd0 + 4 * t;
+      lv[t] = (i < i_end && r < jb) ? L[j0 + r + (long long)i * lda] : 0.0;
+    }
+  }
+  if (tid < NB) xb[tid] = tid < jb ? x[j0 + tid] : 0.0;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) s += dv[t] * xb[part + 4 * t];
+  red[part][r] = s;
+  __syncthreads();
+  if (tid < NB) {
+    const double v = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    yb[tid] = tid < jb ? v : 0.0;
+    if (blockIdx.x == 0 && tid < jb) y[j0 + tid] = v;
+  }
+  __syncthreads();
+  if (!trans) {
+    double a = 0.0;
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) a += lv[t] * yb[part + 4 * t];
+    red[part][r] = a;
+    __syncthreads();
+    if (part == 0 && i_fwd < n) x[i_fwd] -= (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+  } else {
+    const double yl = yb[r];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      const int i = i_bwd0 + 4 * t;
+      const double tsum = wave_sum64(lv[t] * yl);
+      if (r == 0 && i < i_end) x[i] -= tsum;
+    }
+  }
+}
+
+}  // namespace
+
+int trsv_f64_steps(const double* L, long long lda, int n, const double* dinv, double* x, double* y, int trans,
+                   hipStream_t stream) {
+  if (n <= 0) return kOk;
+  const int nblk = ceil_div(n, NB);
+  for (int step = 0; step < nblk; ++step) {
+    const int bi = trans ? (nblk - 1 - step) : step;
+    const int j0 = bi * NB;
+    const int jb = std::min(NB, n - j0);
+    const int rows = trans ? j0 : n - j0 - jb;
+    const int groups = std::max(1, ceil_div(rows, kTrsvRows));
+    hipLaunchKernelGGL(trsv_step_kernel, dim3(groups), dim3(256), 0, stream, L, lda, n,
+                       dinv + (long long)bi * NB * NB, j0, jb, x, y, trans);
+    SDPX_HIP(hipGetLastError());
+  }
+  // y now holds the solution (every block was written once by its step)
+  SDPX_HIP(hipMemcpyAsync(x, y, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+  return kOk;
+}
+
 int trsv_f64(const double* L, long long lda, int n, const double* dinv, double* x, int trans, hipStream_t stream) {
   const int nblk = ceil_div(n, NB);
   for (int step = 0; step < nblk; ++step) {

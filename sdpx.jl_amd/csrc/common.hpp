@@ -167,6 +167,10 @@ int potrf_diag_grouped(const CholDesc* d, const int* list, int count, int j, hip
 // side, using the diagonal-block inverses written by potrf_f64.
 int trsv_f64(const double* L, long long lda, int n, const double* dinv, double* x, int trans,
              hipStream_t stream);
+// The same solve with one launch per diagonal block (block inverse and the
+// update of the other rows fused); y is an n-vector of scratch.
+int trsv_f64_steps(const double* L, long long lda, int n, const double* dinv, double* x, double* y, int trans,
+                   hipStream_t stream);
 
 // Inverse of a lower-triangular n x n factor (Linv written to out, ld n),
 // blocked on the diagonal-block inverses written by potrf_f64.

@@ -180,7 +180,7 @@ struct sdpx_dense {
   hipStream_t stream = nullptr;
   double *c = nullptr, *x = nullptr, *dx = nullptr, *dx_best = nullptr, *d = nullptr, *v = nullptr, *r = nullptr;
   double *rho = nullptr, *delta = nullptr, *v2 = nullptr;
-  double *S = nullptr, *Sf = nullptr, *dinvS = nullptr;
+  double *S = nullptr, *Sf = nullptr, *dinvS = nullptr, *trsv_y = nullptr;
   double *T = nullptr, *Ph = nullptr;  // Schur panels (max over blocks)
   double* splitws = nullptr;
   int split = 1;
@@ -307,8 +307,8 @@ int direction_residual_dev(sdpx_dense_t* h, int slot) {
 
 int solve_into(sdpx_dense_t* h, const double* rhs, double* out) {
   if (rhs != out) SDPX_HIP(hipMemcpyAsync(out, rhs, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
-  SDPX_CHECK(trsv_f64(h->Sf, h->m, h->m, h->dinvS, out, 0, h->stream));
-  SDPX_CHECK(trsv_f64(h->Sf, h->m, h->m, h->dinvS, out, 1, h->stream));
+  SDPX_CHECK(trsv_f64_steps(h->Sf, h->m, h->m, h->dinvS, out, h->trsv_y, 0, h->stream));
+  SDPX_CHECK(trsv_f64_steps(h->Sf, h->m, h->m, h->dinvS, out, h->trsv_y, 1, h->stream));
   return kOk;
 }
 
@@ -589,6 +589,7 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
   if ((s = dalloc(h, &h->S, (size_t)mm2))) return fail(s);
   if ((s = dalloc(h, &h->Sf, (size_t)mm2))) return fail(s);
   if ((s = dalloc(h, &h->dinvS, (size_t)ceil_div(m, kCholNb) * kCholNb * kCholNb))) return fail(s);
+  if ((s = dalloc(h, &h->trsv_y, (size_t)m))) return fail(s);
   if ((s = dalloc(h, &h->T, (size_t)(maxk2 * m)))) return fail(s);
   if ((s = dalloc(h, &h->Ph, (size_t)(maxk2 * m)))) return fail(s);
   // split-K for the Schur Gram product.  All lower tiles cost the same, so the

@@ -165,3 +165,38 @@ def test_batch_matches_single_problem_gpu_path(gpu_lib):
     # the iteration tolerance is the one each route meets against the oracle (+-2)
     assert abs(int(res.iterations[0]) - single.iterations) <= 2
     assert abs(res.pobj[0] - single.pobj) <= 1e-8 * max(1.0, abs(single.pobj))
+
+
+@pytest.mark.gpu
+def test_batch_coo_gather_matches_dense_panel(gpu_lib):
+    """COO input runs Av x and Av' w as sparse gathers (duplicates summed in entry order);
+    the same problems given dense run the dense panel passes.  With the fixed
+    strategy both trajectories agree to rounding."""
+    probs = G.batched_sparse(5, seed=9, kmin=6, kmax=20)
+    ks = [g.k for g in probs]
+    ms = [g.m for g in probs]
+    rows, cols, vals = [], [], []
+    for g in probs:
+        for i in range(g.m):
+            # split the first entry of every constraint into two exact halves (a duplicate position)
+            r, c, v = list(g.rows[i]), list(g.cols[i]), list(g.vals[i])
+            rows.append(np.array(r + [r[0]], dtype=np.int32))
+            cols.append(np.array(c + [c[0]], dtype=np.int32))
+            vals.append(np.array([v[0] / 2] + v[1:] + [v[0] / 2]))
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, parameter_strategy="fixed")
+    eyes = [np.eye(k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        bs.set_coo(rows, cols, vals, [g.C for g in probs], [g.c for g in probs])
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        res_s = bs.solve(o)
+    with BatchSolver(ks, ms) as bs:
+        bs.set_dense([g.dense_rows() for g in probs], [g.C for g in probs], [g.c for g in probs])
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        res_d = bs.solve(o)
+    for p in range(len(probs)):
+        print(f"member {p}: coo {res_s.status[p]} it={res_s.iterations[p]} pobj={res_s.pobj[p]:.12e} | "
+              f"dense {res_d.status[p]} it={res_d.iterations[p]} pobj={res_d.pobj[p]:.12e}", flush=True)
+        assert res_s.status[p] == res_d.status[p] == "Optimal"
+        assert abs(int(res_s.iterations[p]) - int(res_d.iterations[p])) <= 1
+        assert abs(res_s.pobj[p] - res_d.pobj[p]) <= 1e-9 * max(1.0, abs(res_d.pobj[p]))
