@@ -667,6 +667,155 @@ __global__ __launch_bounds__(NT) void backtrack_kernel(Arrays a, const int* list
   }
 }
 
+// ---- wavefront forms of the step-length searches -------------------------
+// One wavefront per member (four members per work-group, no block barriers):
+// lane r holds row r of the trial matrix base + t dir in registers (KB >= k
+// columns, padded to the identity), eliminates on the unscaled columns as
+// lds_chol does (same operations and order, so every pass / fail decision is
+// the same) with the pivot column published through a wave-private LDS row.
+// |diag L| extrema (sqrt of the pivots) serve the margin test.
+template <int KB>
+__device__ __forceinline__ int wave_trial_chol(const double* base, const double* dir, double t, int k, double* col, int lane,
+                               double& dmin, double& dmax) {
+  // opaque per trial: the row addresses are recomputed here instead of being
+  // hoisted out of the bisection loop, where they would hold 2 KB registers
+  const double* bp = base;
+  const double* dp = dir;
+  int row = lane;
+  asm volatile("" : "+v"(bp), "+v"(dp), "+v"(row));
+  double v[KB];
+#pragma unroll
+  for (int c = 0; c < KB; ++c)
+    v[c] = (row < k && c <= row) ? bp[row + c * k] + t * dp[row + c * k] : (c == row ? 1.0 : 0.0);
+  dmin = __builtin_inf();
+  dmax = 0.0;
+  int info = 0;
+  // no early exit inside the unrolled loop (it keeps v[] in registers): steps
+  // past k or past a failed pivot are skipped by a wave-uniform test
+#pragma unroll
+  for (int c = 0; c < KB; ++c) {
+    if (c < k && info == 0) {
+      col[lane] = v[c];
+      __builtin_amdgcn_wave_barrier();
+      const double p = col[c];
+      if (!(p > 0.0)) {
+        info = c + 1;
+      } else {
+        const double d = fabs(sqrt(p));
+        dmin = fmin(dmin, d);
+        dmax = nan_max(dmax, d);
+        const double f = v[c] / p;
+#pragma unroll
+        for (int c2 = c + 1; c2 < KB; ++c2) v[c2] -= f * col[c2];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  return info;
+}
+
+template <int KB>
+__device__ __forceinline__ bool wave_trial_ok(const double* base, const double* dir, double t, double ratio, int k, double* col,
+                              int lane) {
+  double mn, mx;
+  if (wave_trial_chol<KB>(base, dir, t, k, col, lane, mn, mx) != 0) return false;
+  if (!(ratio > 0.0)) return true;
+  return isfinite(mn) && isfinite(mx) && mx > 0.0 && mn > ratio * mx;
+}
+
+// fraction_to_boundary_bound! for k >= 3: full step, else bisection.  Once lo
+// and hi are adjacent doubles every remaining level of the 64 re-tests one of
+// them with a known outcome, so the search stops there (same result).
+template <int KB>
+__device__ __forceinline__ double wave_ftb_side(const double* base, const double* dir, int k, double* col, int lane) {
+  if (wave_trial_ok<KB>(base, dir, 1.0, 0.0, k, col, lane)) return 1.0;
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    const double mid = (lo + hi) / 2.0;
+    if (mid == lo || mid == hi) break;
+    if (wave_trial_ok<KB>(base, dir, mid, 0.0, k, col, lane))
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <int KB>
+__device__ __forceinline__ double wave_backtrack_side(const double* base, const double* dir, int k, double gamma, double min_step,
+                                      double ratio, double* col, int lane) {
+  double t = 1.0;
+  while (true) {
+    const bool pass = wave_trial_ok<KB>(base, dir, t, ratio, k, col, lane);
+    if (pass || t < min_step || !(t > 0.0)) break;  // t reaches 0 after ~1100 steps: the loop always ends
+    t *= gamma;
+  }
+  return t;
+}
+
+constexpr int kStepWaves = NT / 64;  // members per work-group in the wavefront kernels
+
+// op 0: fraction to the boundary (S_TX, S_TY scaled by P_PF / P_DF); op 1: backtracking.
+// One instance per register width KB handles the listed members with
+// KB - 16 < k <= KB (the KB = 16 instance also the closed forms of k <= 2).
+template <int KB>
+__global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list, int n, int op) {
+  __shared__ double cols[kStepWaves][KMAX];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * kStepWaves + w;
+  if (slot >= n) return;  // whole wavefront; no block barrier in this kernel
+  const int p = list[slot];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  if (k > KB || k <= KB - 16) return;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  double* col = cols[w];
+  double res[2];
+  for (int side = 0; side < 2; ++side) {
+    const double* base = (side == 0 ? a.X : a.Y) + o;
+    const double* dir = (side == 0 ? a.dX : a.dY) + o;
+    if (k <= 2) {
+      // closed forms on the 2 x 2 (or 1 x 1) block, LP-strided as in the LDS kernels
+      double sb[LP + 2] = {}, sd[LP + 2] = {};
+      sb[0] = base[0];
+      sd[0] = dir[0];
+      if (k == 2) {
+        sb[1] = base[1];
+        sb[LP] = base[2];
+        sb[LP + 1] = base[3];
+        sd[1] = dir[1];
+        sd[LP] = dir[2];
+        sd[LP + 1] = dir[3];
+      }
+      if (op == 0) {
+        res[side] = small_ftb(sb, sd, k);
+      } else {
+        double t = 1.0;
+        while (true) {
+          const bool pass = small_margin(sb, sd, k, t, par[P_RATIO]);
+          if (pass || t < par[P_MINSTEP] || !(t > 0.0)) break;
+          t *= par[P_GAMMA];
+        }
+        res[side] = t;
+      }
+      continue;
+    }
+    res[side] = op == 0 ? wave_ftb_side<KB>(base, dir, k, col, lane)
+                        : wave_backtrack_side<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], col,
+                                                  lane);
+  }
+  if (lane == 0) {
+    if (op == 0) {
+      a.sc[(long long)p * NSLOT + S_TX] = res[0] < 1.0 ? par[P_PF] * res[0] : 1.0;
+      a.sc[(long long)p * NSLOT + S_TY] = res[1] < 1.0 ? par[P_DF] * res[1] : 1.0;
+    } else {
+      a.sc[(long long)p * NSLOT + S_TX] = res[0];
+      a.sc[(long long)p * NSLOT + S_TY] = res[1];
+    }
+  }
+}
+
 // ---- accepted step ---------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void update_kernel(Arrays a, const int* list) {
   __shared__ double red[NT];
@@ -1186,9 +1335,20 @@ int launch_comp(const Arrays& a, const int* list, int n, hipStream_t st) { SDPX_
 int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(corrector_block_kernel, n, a, list);
 }
-int launch_ftb(const Arrays& a, const int* list, int n, hipStream_t st) { SDPX_LAUNCH(ftb_kernel, n, a, list); }
+// one launch per register width; members outside a width's range exit at once
+int launch_step_waves(const Arrays& a, const int* list, int n, int op, hipStream_t st) {
+  if (n <= 0) return kOk;
+  const dim3 grid((unsigned)ceil_div(n, kStepWaves)), block(NT);
+  hipLaunchKernelGGL(step_wave_kernel<16>, grid, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<32>, grid, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<48>, grid, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<64>, grid, block, 0, st, a, list, n, op);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+int launch_ftb(const Arrays& a, const int* list, int n, hipStream_t st) { return launch_step_waves(a, list, n, 0, st); }
 int launch_backtrack(const Arrays& a, const int* list, int n, hipStream_t st) {
-  SDPX_LAUNCH(backtrack_kernel, n, a, list);
+  return launch_step_waves(a, list, n, 1, st);
 }
 int launch_update(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(update_kernel, n, a, list);
