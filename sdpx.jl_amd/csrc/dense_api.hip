@@ -378,6 +378,14 @@ void bisection_tree(double lo, double hi, int depth, std::vector<double>& mids) 
   }
 }
 
+// Once lo and hi are adjacent doubles, every further bisection level tests
+// (lo + hi) / 2, which is lo (known to pass) or hi (known to fail): the
+// remaining levels change nothing and the search can stop with lo.
+bool bounds_adjacent(double lo, double hi) {
+  const double mid = (lo + hi) / 2.0;
+  return mid == lo || mid == hi;
+}
+
 // Replays `depth` bisection levels from the tested tree (pass flags at offset).
 void bisection_replay(const std::vector<double>& mids, const std::vector<int>& pass, int offset, int depth, double& lo,
                       double& hi) {
@@ -431,6 +439,10 @@ int ftb_block(sdpx_dense_t* h, Block& b, double* bx, double* by) {
     } else {
       bisection_replay(mids[sd], pass, off + 1, D0, lo[sd], hi[sd]);
       remaining[sd] -= D0;
+      if (bounds_adjacent(lo[sd], hi[sd])) {
+        active[sd] = false;
+        res[sd] = lo[sd];
+      }
     }
   }
   while (active[0] || active[1]) {
@@ -448,7 +460,7 @@ int ftb_block(sdpx_dense_t* h, Block& b, double* bx, double* by) {
       const int off = sd == 0 ? 0 : (int)cand[0].size();
       bisection_replay(mids[sd], pass, off, depth[sd], lo[sd], hi[sd]);
       remaining[sd] -= depth[sd];
-      if (remaining[sd] <= 0) {
+      if (remaining[sd] <= 0 || bounds_adjacent(lo[sd], hi[sd])) {
         active[sd] = false;
         res[sd] = lo[sd];
       }
