@@ -20,7 +20,21 @@ namespace sdpx {
 namespace {
 
 constexpr int BK = 16;
-constexpr int PAD = 16;  // row stride = R + 16 doubles: conflict-free 64-bit fragment reads
+constexpr int PAD = 16;  // [BK][R + 16] layout of operands contiguous along the tile rows
+// Operands contiguous along K are staged row-major, [R][BK + 1]: the 17-double
+// pitch keeps both the transposing 8-byte stores of the K-contiguous loads and
+// the 64-bit fragment reads free of LDS bank conflicts (a [BK][R + 16] layout
+// would put the eight lanes that load one row's 16 K values on one bank pair).
+constexpr int KPITCH = BK + 1;
+
+template <int R, bool KC>
+__device__ __forceinline__ int lds_at(int kk, int r) {
+  return KC ? r * KPITCH + kk : kk * (R + PAD) + r;
+}
+template <int R, bool KC>
+constexpr int lds_size() {
+  return KC ? R * KPITCH : BK * (R + PAD);
+}
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef double double2_t __attribute__((ext_vector_type(2)));
@@ -109,7 +123,7 @@ struct TileLoader {
 };
 
 template <int R, bool KC, bool VEC>
-__device__ __forceinline__ void tile_store(const double* v, double (*S)[R + PAD], int tid) {
+__device__ __forceinline__ void tile_store(const double* v, double* S, int tid) {
   constexpr int NE = R * BK / 256;
   if (VEC) {
 #pragma unroll
@@ -118,14 +132,14 @@ __device__ __forceinline__ void tile_store(const double* v, double (*S)[R + PAD]
       for (int h = 0; h < 2; ++h) {
         int r, kk;
         tile_coords<R, KC, VEC>(tid, e, h, r, kk);
-        S[kk][r] = v[2 * e + h];
+        S[lds_at<R, KC>(kk, r)] = v[2 * e + h];
       }
   } else {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       int r, kk;
       tile_coords<R, KC, VEC>(tid, e, 0, r, kk);
-      S[kk][r] = v[e];
+      S[lds_at<R, KC>(kk, r)] = v[e];
     }
   }
 }
@@ -143,8 +157,8 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
   constexpr int B_ELEMS = BN * BK / 256;
-  __shared__ double As[2][BK][BM + PAD];
-  __shared__ double Bs[2][BK][BN + PAD];
+  __shared__ double As[2][lds_size<BM, TA>()];
+  __shared__ double Bs[2][lds_size<BN, !TB>()];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -183,9 +197,9 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
       const int kr = ks * 4 + (lane >> 4);
       double fa[FM], fb[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) fa[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
+      for (int i = 0; i < FM; ++i) fa[i] = As[cur][lds_at<BM, TA>(kr, wm0 + i * 16 + (lane & 15))];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][kr][wn0 + j * 16 + (lane & 15)];
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][lds_at<BN, !TB>(kr, wn0 + j * 16 + (lane & 15))];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
