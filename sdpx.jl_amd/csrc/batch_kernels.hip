@@ -984,7 +984,12 @@ constexpr int PT = 512;                           // 8 waves: two per SIMD hide 
 constexpr int kPanelPrefetch = KMAX * KMAX / PT;  // doubles of A_i per thread
 
 __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, int chunk) {
-  __shared__ double sLi[KMAX * LP], sLy[KMAX * LP], sA[KMAX * LP], sT[KMAX * LP];
+  // Two LDS pitches: operands read as (16 consecutive columns x 4 rows) fragments
+  // use LPA (66: 8-byte reads of a half-wave land on 32 distinct bank pairs),
+  // operands read as (16 consecutive rows x 4 columns) use LPB (80 = 16 mod 32
+  // doubles: the two row groups of a half-wave fall in opposite bank halves).
+  constexpr int LPA = 66, LPB = 80;
+  __shared__ double sLi[KMAX * LPB], sT[KMAX * LPB], sA[KMAX * LPA], sLy[KMAX * LPA];
   const int2 w = work[blockIdx.x];
   const int p = w.x;
   const Member M = a.mem[p];
@@ -1028,13 +1033,14 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
     const int e = threadIdx.x + PT * j;
     const int r = e % kp, c = e / kp;
     const bool live = e < kp * kp;
-    loff[j] = live ? r + c * LP : -1;
+    loff[j] = live ? r + c * LPA : -1;
     goff[j] = (live && r < k && c < k) ? r + c * k : -1;
   }
 #pragma unroll
   for (int j = 0; j < kPanelPrefetch; ++j) {
+    const int e = threadIdx.x + PT * j;
     if (loff[j] < 0) continue;
-    sLi[loff[j]] = goff[j] >= 0 ? a.LXinv[M.oK + goff[j]] : 0.0;
+    sLi[e % kp + (e / kp) * LPB] = goff[j] >= 0 ? a.LXinv[M.oK + goff[j]] : 0.0;
     sLy[loff[j]] = goff[j] >= 0 ? a.LY[M.oK + goff[j]] : 0.0;
   }
   const long long k2 = (long long)k * k;
@@ -1060,8 +1066,8 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (u >= nmine || q0 >= r0s[u] + 16) continue;
-          const double fb = sA[(q0 + lq) + (c0s[u] + lr) * LP];   // B'[i'][kq] = A[q0+kq][c0+i']
-          const double fl = sLi[(r0s[u] + lr) + (q0 + lq) * LP];  // L'[kq][j'] = Li[r0+j'][q0+kq]
+          const double fb = sA[(q0 + lq) + (c0s[u] + lr) * LPA];   // B'[i'][kq] = A[q0+kq][c0+i']
+          const double fl = sLi[(r0s[u] + lr) + (q0 + lq) * LPB];  // L'[kq][j'] = Li[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
@@ -1069,7 +1075,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
       for (int u = 0; u < 4; ++u) {
         if (u >= nmine) continue;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) sT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LP] = acc[u][rr];
+        for (int rr = 0; rr < 4; ++rr) sT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LPB] = acc[u][rr];
       }
     }
     __syncthreads();
@@ -1082,8 +1088,8 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (u >= nmine || q0 < c0s[u]) continue;
-          const double fb = sLy[(q0 + lq) + (c0s[u] + lr) * LP];  // Ly[q0+kq][c0+i']
-          const double fl = sT[(r0s[u] + lr) + (q0 + lq) * LP];   // T[r0+j'][q0+kq]
+          const double fb = sLy[(q0 + lq) + (c0s[u] + lr) * LPA];  // Ly[q0+kq][c0+i']
+          const double fl = sT[(r0s[u] + lr) + (q0 + lq) * LPB];   // T[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
