@@ -162,6 +162,15 @@ int sdpx_dense_set_profiling(sdpx_dense_t* h, int enabled);
 int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out);
 /* split-K factor used by the Schur Gram kernel of this handle. */
 int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
+/* Plan-time autotuner for the Schur build of this handle's shape (no
+ * counterpart in the reference, whose BLAS calls take no tiling choices): runs
+ * and times on the device the P-hat panel products on 128x128 or 64x64 tiles
+ * and the Schur Gram product at three split-K factors and three tile-order
+ * super-block sides, and keeps the fastest.  Overwrites the panel and Schur
+ * buffers (call before solving or between solves).  out (optional, 6 doubles):
+ * panel tile side, split, super-block side, panel ms, Gram ms, create-time
+ * split. */
+int sdpx_dense_autotune(sdpx_dense_t* h, double* out);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 

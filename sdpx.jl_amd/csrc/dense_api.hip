@@ -188,6 +188,10 @@ struct sdpx_dense {
   // mixed precision (sdpx_dense_set_precision): FP32 copy of the Schur panel
   // (precision 1) or its bf16 hi / lo planes (precision 2)
   int precision = 0;
+  unsigned panel_tiles = 0;  // kGemmSmallTiles when the autotuner found 64 x 64 panel tiles faster
+  int tile_group = 8;        // super-block side of the Gram tile order
+  int split_created = 1;     // split-K factor chosen at create (the workspace holds that many partials)
+  float tune_panel_ms = 0.f, tune_gram_ms = 0.f;
   float* Pf = nullptr;
   long long ldPf = 0, maxk2 = 0;
   __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
@@ -634,6 +638,7 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
       }
     }
     h->split = best;
+    h->split_created = best;
     if (best > 1 && (s = dalloc(h, &h->splitws, (size_t)(best * mm2)))) return fail(s);
     // Gram tiles visited in 8 x 8 super-blocks: the co-resident groups of one
     // XCD then read few distinct Phat column panels, shared in its L2.
@@ -844,13 +849,14 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     // T_i = A_i M_Y for every i as one product over the stacked panel
     // [A_1; ...; A_m] = Av' (each A_i is symmetric), M = km rows, op(B) lower;
     // rows (i, r) are written back to the k^2 x m panel layout (block remap)
-    GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower);
+    GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower | h->panel_tiles);
     w.block_rows = k;
     w.block_stride = b.k2;
     SDPX_CHECK(gemm_f64(w, h->stream));
     // Phat = L_X^-1 [T_1 ... T_m]   (k x km, op(A) lower; the partial row tile
     // comes first, where the triangle leaves it the shortest K range)
-    SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k, kGemmALower | kGemmRaggedFirst),
+    SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k,
+                             kGemmALower | kGemmRaggedFirst | h->panel_tiles),
                         h->stream));
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
@@ -948,6 +954,118 @@ int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out) {
 int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
   SDPX_CHECK(check_handle(h));
   if (split) *split = h->split;
+  return kOk;
+}
+
+// Plan-time tuning of the Schur build for this handle's shape: every candidate
+// is run on the device and timed with events (kernel timings do not depend on
+// the operand values, so it can run before any iterate is set; it overwrites
+// the panel and Schur buffers).  Candidates: P-hat panel products on 128 x 128
+// or 64 x 64 tiles; Gram split-K factor s, 3s/4, s/2 of the create-time choice
+// s (the partial-sum workspace is sized for s); Gram tile order in 4 x 4,
+// 8 x 8 or 16 x 16 super-blocks.  out (optional, 6 doubles): chosen panel tile
+// side, split, super-block side, and the panel and Gram milliseconds.
+int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const Block* bp = &h->blk[0];
+  for (int l = 1; l < h->L; ++l)
+    if (h->blk[l].k > bp->k) bp = &h->blk[l];
+  const Block& b = *bp;
+  const int m = h->m, k = b.k;
+  hipEvent_t ev[2] = {};
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto& f : ev)
+        if (f) (void)hipEventDestroy(f);
+      set_last_error("sdpx_dense_autotune: hipEventCreate failed");
+      return kHipError;
+    }
+  auto time_ms = [&](auto&& launch, float* ms) -> int {
+    SDPX_CHECK(launch());  // warm
+    SDPX_HIP(hipEventRecord(ev[0], h->stream));
+    SDPX_CHECK(launch());
+    SDPX_HIP(hipEventRecord(ev[1], h->stream));
+    SDPX_HIP(hipEventSynchronize(ev[1]));
+    SDPX_HIP(hipEventElapsedTime(ms, ev[0], ev[1]));
+    return kOk;
+  };
+  auto body = [&]() -> int {
+    // panel products, both tile sides
+    for (unsigned tiles : {0u, (unsigned)kGemmSmallTiles}) {
+      float ms = 0.f;
+      SDPX_CHECK(time_ms(
+          [&]() -> int {
+            GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower | tiles);
+            w.block_rows = k;
+            w.block_stride = b.k2;
+            SDPX_CHECK(gemm_f64(w, h->stream));
+            return gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k,
+                                 kGemmALower | kGemmRaggedFirst | tiles),
+                            h->stream);
+          },
+          &ms));
+      if (tiles == 0 || ms < h->tune_panel_ms) {
+        h->tune_panel_ms = ms;
+        h->panel_tiles = tiles;
+      }
+    }
+    // Gram product in the handle's precision: split-K factor and tile order
+    const long long kp = (b.k2 + 31) / 32 * 32;
+    if (h->precision == 2)
+      SDPX_CHECK(split_panel_bf16(h->Ph, b.k2, h->Ph_hi, h->Ph_lo, h->ldPb, b.k2, m, h->stream));
+    else if (h->precision == 1)
+      SDPX_CHECK(convert_panel_f32(h->Ph, b.k2, h->Pf, h->ldPf, b.k2, m, h->stream));
+    auto gram = [&](int sp) -> int {
+      if (h->precision == 2)
+        return gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, 0.0, sp, h->splitws, h->tile_order,
+                           h->stream, nullptr);
+      if (h->precision == 1)
+        return gram_f32(h->Pf, h->ldPf, m, (int)b.k2, h->S, m, 0.0, sp, h->splitws, h->tile_order, h->stream,
+                        nullptr);
+      GemmArgs g = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, 0.0, h->S, m,
+                        kGemmTransA | kGemmLowerOut | kGemmSchurGram);
+      g.tile_order = h->tile_order;
+      return gemm_f64(g, h->stream, sp, h->splitws);
+    };
+    const int nt = ceil_div(m, 128);
+    std::vector<int2> order((size_t)nt * (nt + 1) / 2);
+    const int s0 = h->split_created;
+    int best_split = s0, best_group = 8;
+    bool first = true;
+    for (int group : {4, 8, 16}) {
+      lower_tile_order(nt, group, order.data());
+      SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+      int last = 0;
+      for (int sp : {s0, (3 * s0) / 4, s0 / 2}) {
+        if (sp < 1 || sp == last) continue;
+        last = sp;
+        float ms = 0.f;
+        SDPX_CHECK(time_ms([&]() -> int { return gram(sp); }, &ms));
+        if (first || ms < h->tune_gram_ms) {
+          first = false;
+          h->tune_gram_ms = ms;
+          best_split = sp;
+          best_group = group;
+        }
+      }
+    }
+    h->split = best_split;
+    h->tile_group = best_group;
+    lower_tile_order(nt, best_group, order.data());
+    SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+    return kOk;
+  };
+  const int rc = body();
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc != kOk) return rc;
+  if (out) {
+    out[0] = h->panel_tiles ? 64.0 : 128.0;
+    out[1] = h->split;
+    out[2] = h->tile_group;
+    out[3] = h->tune_panel_ms;
+    out[4] = h->tune_gram_ms;
+    out[5] = h->split_created;
+  }
   return kOk;
 }
 

@@ -449,7 +449,7 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
   }
   // Small problems use 64 x 64 tiles so that k x k block products launch enough groups.
   // (and so does a product with a side of at most 64, which would leave half of a 128-wide tile idle)
-  const bool small = (g.M <= 96 && g.N <= 96) || g.M <= 64 || g.N <= 64 ||
+  const bool small = (g.flags & kGemmSmallTiles) || (g.M <= 96 && g.N <= 96) || g.M <= 64 || g.N <= 64 ||
                      ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256);
   if (small) {
     return vec ? dispatch_trans<64, 64, true>(g, stream, split_k, workspace, ta, tb, after_main)

@@ -245,6 +245,13 @@ class MI355XDenseBackend:
         return dict(gram_ms=out[0], gram_launches=int(out[1]), panels_ms=out[2], reduce_ms=out[3],
                     factor_ms=out[4], factor_calls=int(out[5]))
 
+    def autotune(self) -> dict:
+        """Time the Schur build's tiling candidates on the device and keep the fastest (plan time)."""
+        out = np.zeros(6)
+        check(self.lib.sdpx_dense_autotune(self.h, dptr(out)), "autotune")
+        return dict(panel_tile=int(out[0]), split=int(out[1]), tile_group=int(out[2]), panel_ms=float(out[3]),
+                    gram_ms=float(out[4]), default_split=int(out[5]))
+
     def schur_split(self) -> int:
         s = C.c_int(0)
         check(self.lib.sdpx_dense_schur_split(self.h, C.byref(s)), "schur_split")

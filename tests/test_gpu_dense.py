@@ -48,6 +48,31 @@ def test_schur_build_matches_oracle(gpu_lib, k, m):
     b.close()
 
 
+@pytest.mark.parametrize("k,m", [(65, 130), (130, 700)])
+def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
+    """The autotuner's tile and split choices leave the Schur matrix within the FP64 bar."""
+    rng = np.random.default_rng(k + m)
+    g, prob, oprob = _dense_case(k, m, seed=4)
+    X, Y = _random_pd(rng, k), _random_pd(rng, k)
+    b = MI355XDenseBackend(prob)
+    default_split = b.schur_split()
+    t = b.autotune()  # before the iterate: timings do not depend on values
+    print(t, flush=True)
+    assert t["panel_tile"] in (64, 128) and t["tile_group"] in (4, 8, 16)
+    assert t["default_split"] == default_split
+    assert t["split"] in {default_split, max(1, 3 * default_split // 4), max(1, default_split // 2)}
+    assert b.schur_split() == t["split"] and t["panel_ms"] > 0 and t["gram_ms"] > 0
+    b.set_iterate(g.x0, [X], [Y])
+    _, _, ok = b.compute_residuals([0.1], affine=True, factor=True)
+    assert ok
+    b.schur_build()
+    S = b.get_schur()
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    assert _rel_fro_lower(S, O.schur_build(oprob, [LX], [MY])) <= 1e-12
+    b.close()
+
+
 def test_schur_identity_is_gram_and_upper_untouched(gpu_lib):
     k, m = 40, 150
     g, prob, _ = _dense_case(k, m, seed=5)
