@@ -753,6 +753,96 @@ __device__ __forceinline__ double wave_backtrack_side(const double* base, const 
   return t;
 }
 
+// ---- both sides in one wavefront (k <= 32) --------------------------------------------------
+// Lanes 0-31 run the X-side search and lanes 32-63 the Y-side search in lock
+// step, one trial per side at a time: each half factors its own trial exactly
+// as wave_trial_chol does (same operations and order, so every decision is
+// the same), publishing its pivot column through its own LDS row; the two rows
+// are 33 doubles apart, so the halves' reads fall in different banks.  A half
+// whose search has ended still steps through the trials of the other half
+// (on t = 0, the base block itself) and ignores the outcome.
+constexpr int kPairPitch = 33;
+
+template <int KB>
+__device__ __forceinline__ bool pair_trial_ok(const double* base, const double* dir, double t, double ratio, int k,
+                                              double* col, int row) {
+  const double* bp = base;
+  const double* dp = dir;
+  int r = row;
+  asm volatile("" : "+v"(bp), "+v"(dp), "+v"(r));
+  double v[KB];
+#pragma unroll
+  for (int c = 0; c < KB; ++c)
+    v[c] = (r < k && c <= r) ? bp[r + c * k] + t * dp[r + c * k] : (c == r ? 1.0 : 0.0);
+  double dmin = __builtin_inf(), dmax = 0.0;
+  int info = 0;
+#pragma unroll
+  for (int c = 0; c < KB; ++c) {
+    if (c < k && info == 0) {  // uniform within a half
+      col[r] = v[c];
+      __builtin_amdgcn_wave_barrier();
+      const double p = col[c];
+      if (!(p > 0.0)) {
+        info = c + 1;
+      } else {
+        if (ratio > 0.0) {
+          const double d = fabs(sqrt(p));
+          dmin = fmin(dmin, d);
+          dmax = nan_max(dmax, d);
+        }
+        const double f = v[c] / p;
+#pragma unroll
+        for (int c2 = c + 1; c2 < KB; ++c2) v[c2] -= f * col[c2];
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (info != 0) return false;
+  if (!(ratio > 0.0)) return true;
+  return isfinite(dmin) && isfinite(dmax) && dmax > 0.0 && dmin > ratio * dmax;
+}
+
+// fraction_to_boundary_bound! of this half's side (full step, then the
+// bisection that stops at adjacent bounds), as wave_ftb_side.
+template <int KB>
+__device__ __forceinline__ double pair_ftb(const double* base, const double* dir, int k, double* col, int row) {
+  const bool full = pair_trial_ok<KB>(base, dir, 1.0, 0.0, k, col, row);
+  bool active = !full;
+  double lo = 0.0, hi = 1.0;
+  for (int it = 0; it < 64; ++it) {
+    const double mid = (lo + hi) / 2.0;
+    if (mid == lo || mid == hi) active = false;
+    if (!__any(active)) break;
+    const bool pass = pair_trial_ok<KB>(base, dir, active ? mid : 0.0, 0.0, k, col, row);
+    if (active) {
+      if (pass)
+        lo = mid;
+      else
+        hi = mid;
+    }
+  }
+  return full ? 1.0 : lo;
+}
+
+// line_search! backtracking of this half's side, as wave_backtrack_side.
+template <int KB>
+__device__ __forceinline__ double pair_backtrack(const double* base, const double* dir, int k, double gamma,
+                                                 double min_step, double ratio, double* col, int row) {
+  double t = 1.0;
+  bool active = true;
+  while (true) {  // each half's t reaches 0 after ~1100 steps: the loop always ends
+    const bool pass = pair_trial_ok<KB>(base, dir, active ? t : 0.0, ratio, k, col, row);
+    if (active) {
+      if (pass || t < min_step || !(t > 0.0))
+        active = false;
+      else
+        t *= gamma;
+    }
+    if (!__any(active)) break;
+  }
+  return t;
+}
+
 constexpr int kStepWaves = NT / 64;  // members per work-group in the wavefront kernels
 
 // op 0: fraction to the boundary (S_TX, S_TY scaled by P_PF / P_DF); op 1: backtracking.
@@ -760,7 +850,7 @@ constexpr int kStepWaves = NT / 64;  // members per work-group in the wavefront 
 // KB - 16 < k <= KB (the KB = 16 instance also the closed forms of k <= 2).
 template <int KB>
 __global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list, int n, int op) {
-  __shared__ double cols[kStepWaves][KMAX];
+  __shared__ double cols[kStepWaves][2 * kPairPitch > KMAX ? 2 * kPairPitch : KMAX];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int slot = blockIdx.x * kStepWaves + w;
   if (slot >= n) return;  // whole wavefront; no block barrier in this kernel
@@ -771,6 +861,20 @@ __global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list
   const long long o = M.oK;
   const double* par = a.par + (long long)p * NPARAM;
   double* col = cols[w];
+  if (KB <= 32 && k > 2) {
+    // both sides at once, one per half wavefront
+    const int half = lane >> 5, row = lane & 31;
+    const double* base = (half == 0 ? a.X : a.Y) + o;
+    const double* dir = (half == 0 ? a.dX : a.dY) + o;
+    double* hcol = col + half * kPairPitch;
+    const double r = op == 0 ? pair_ftb<KB>(base, dir, k, hcol, row)
+                             : pair_backtrack<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], hcol, row);
+    if (row == 0) {
+      const double frac = half == 0 ? par[P_PF] : par[P_DF];
+      a.sc[(long long)p * NSLOT + (half == 0 ? S_TX : S_TY)] = op == 0 ? (r < 1.0 ? frac * r : 1.0) : r;
+    }
+    return;
+  }
   double res[2];
   for (int side = 0; side < 2; ++side) {
     const double* base = (side == 0 ? a.X : a.Y) + o;
