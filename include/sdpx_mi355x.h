@@ -164,13 +164,19 @@ int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out);
 int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Plan-time autotuner for the Schur build of this handle's shape (no
  * counterpart in the reference, whose BLAS calls take no tiling choices): runs
- * and times on the device the P-hat panel products on 128x128 or 64x64 tiles
- * and the Schur Gram product at three split-K factors and three tile-order
- * super-block sides, and keeps the fastest.  Overwrites the panel and Schur
- * buffers (call before solving or between solves).  out (optional, 6 doubles):
- * panel tile side, split, super-block side, panel ms, Gram ms, create-time
- * split. */
+ * and times on the device each of the two P-hat panel products on 128x128,
+ * 64x64, 128x64 or 64x128 tiles and the Schur Gram product at three split-K
+ * factors and three tile-order super-block sides, and keeps the fastest.
+ * Overwrites the panel and Schur buffers (call before solving or between
+ * solves).  out (optional, 15 doubles): tile shape code of each panel product
+ * (0..3 in the order above), split, super-block side, panel ms, Gram ms,
+ * create-time split, then the 2 x 4 panel timings (product-major). */
 int sdpx_dense_autotune(sdpx_dense_t* h, double* out);
+/* Applies tiling choices without timing them (no counterpart in the
+ * reference): panel shape codes as reported by sdpx_dense_autotune, Gram split
+ * in [1, create-time split], tile-order super-block side >= 1.  Lets a caller
+ * re-apply a saved autotune result. */
+int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 

@@ -61,6 +61,8 @@ enum GemmFlags : unsigned {
   // (with kGemmALower it then has the shortest K range); no lower output.
   kGemmRaggedFirst = 1u << 4,
   kGemmSmallTiles = 1u << 5,  // force 64 x 64 tiles (more resident groups for short K ranges)
+  kGemmTallTiles = 1u << 6,   // 128 x 64 tiles (NN and TN products only)
+  kGemmWideTiles = 1u << 7,   // 64 x 128 tiles (NN and TN products only)
   kGemmTransA = 1u << 8,    // op(A) = A'
   kGemmTransB = 1u << 9,    // op(B) = B'
   kGemmSchurGram = 1u << 10,  // route to the dedicated Schur Gram kernel (TN, 128 x 128, lower output)

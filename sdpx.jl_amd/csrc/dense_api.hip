@@ -188,10 +188,11 @@ struct sdpx_dense {
   // mixed precision (sdpx_dense_set_precision): FP32 copy of the Schur panel
   // (precision 1) or its bf16 hi / lo planes (precision 2)
   int precision = 0;
-  unsigned panel_tiles = 0;  // kGemmSmallTiles when the autotuner found 64 x 64 panel tiles faster
+  unsigned panel_tiles[2] = {0, 0};  // tile-shape flags of the two P-hat panel products (autotuned)
   int tile_group = 8;        // super-block side of the Gram tile order
   int split_created = 1;     // split-K factor chosen at create (the workspace holds that many partials)
   float tune_panel_ms = 0.f, tune_gram_ms = 0.f;
+  float tune_ms[2][4] = {};  // panel products x tile shapes (128x128, 64x64, 128x64, 64x128)
   float* Pf = nullptr;
   long long ldPf = 0, maxk2 = 0;
   __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
@@ -849,14 +850,14 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     // T_i = A_i M_Y for every i as one product over the stacked panel
     // [A_1; ...; A_m] = Av' (each A_i is symmetric), M = km rows, op(B) lower;
     // rows (i, r) are written back to the k^2 x m panel layout (block remap)
-    GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower | h->panel_tiles);
+    GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower | h->panel_tiles[0]);
     w.block_rows = k;
     w.block_stride = b.k2;
     SDPX_CHECK(gemm_f64(w, h->stream));
     // Phat = L_X^-1 [T_1 ... T_m]   (k x km, op(A) lower; the partial row tile
     // comes first, where the triangle leaves it the shortest K range)
     SDPX_CHECK(gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k,
-                             kGemmALower | kGemmRaggedFirst | h->panel_tiles),
+                             kGemmALower | kGemmRaggedFirst | h->panel_tiles[1]),
                         h->stream));
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
@@ -960,11 +961,13 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
 // Plan-time tuning of the Schur build for this handle's shape: every candidate
 // is run on the device and timed with events (kernel timings do not depend on
 // the operand values, so it can run before any iterate is set; it overwrites
-// the panel and Schur buffers).  Candidates: P-hat panel products on 128 x 128
-// or 64 x 64 tiles; Gram split-K factor s, 3s/4, s/2 of the create-time choice
-// s (the partial-sum workspace is sized for s); Gram tile order in 4 x 4,
-// 8 x 8 or 16 x 16 super-blocks.  out (optional, 6 doubles): chosen panel tile
-// side, split, super-block side, and the panel and Gram milliseconds.
+// the panel and Schur buffers).  Candidates: each of the two P-hat panel
+// products on 128 x 128, 64 x 64, 128 x 64 or 64 x 128 tiles; Gram split-K
+// factor s, 3s/4, s/2 of the create-time choice s (the partial-sum workspace
+// is sized for s); Gram tile order in 4 x 4, 8 x 8 or 16 x 16 super-blocks.
+// out (optional, 15 doubles): tile shape code of each panel product (0..3 in
+// the order above), split, super-block side, panel ms, Gram ms, create-time
+// split, then the 2 x 4 panel timings.
 int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
   SDPX_CHECK(check_handle(h));
   const Block* bp = &h->blk[0];
@@ -990,24 +993,35 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     return kOk;
   };
   auto body = [&]() -> int {
-    // panel products, both tile sides
-    for (unsigned tiles : {0u, (unsigned)kGemmSmallTiles}) {
-      float ms = 0.f;
-      SDPX_CHECK(time_ms(
-          [&]() -> int {
-            GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k, kGemmTransA | kGemmBLower | tiles);
-            w.block_rows = k;
-            w.block_stride = b.k2;
-            SDPX_CHECK(gemm_f64(w, h->stream));
-            return gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k,
-                                 kGemmALower | kGemmRaggedFirst | tiles),
-                            h->stream);
-          },
-          &ms));
-      if (tiles == 0 || ms < h->tune_panel_ms) {
-        h->tune_panel_ms = ms;
-        h->panel_tiles = tiles;
+    h->tune_panel_ms = 0.f;
+    // the two panel products, each on four tile shapes
+    const unsigned shapes[4] = {0u, (unsigned)kGemmSmallTiles, (unsigned)kGemmTallTiles, (unsigned)kGemmWideTiles};
+    for (int prod = 0; prod < 2; ++prod) {
+      float best = 0.f;
+      for (int c = 0; c < 4; ++c) {
+        const unsigned tiles = shapes[c];
+        float ms = 0.f;
+        SDPX_CHECK(time_ms(
+            [&]() -> int {
+              if (prod == 0) {
+                GemmArgs w = gemm(k * m, k, k, 1.0, b.Av, k, b.MY, k, 0.0, h->T, k,
+                                  kGemmTransA | kGemmBLower | tiles);
+                w.block_rows = k;
+                w.block_stride = b.k2;
+                return gemm_f64(w, h->stream);
+              }
+              return gemm_f64(gemm(k, k * m, k, 1.0, b.LXinv, k, h->T, k, 0.0, h->Ph, k,
+                                   kGemmALower | kGemmRaggedFirst | tiles),
+                              h->stream);
+            },
+            &ms));
+        h->tune_ms[prod][c] = ms;
+        if (c == 0 || ms < best) {
+          best = ms;
+          h->panel_tiles[prod] = tiles;
+        }
       }
+      h->tune_panel_ms += best;
     }
     // Gram product in the handle's precision: split-K factor and tile order
     const long long kp = (b.k2 + 31) / 32 * 32;
@@ -1059,13 +1073,37 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc != kOk) return rc;
   if (out) {
-    out[0] = h->panel_tiles ? 64.0 : 128.0;
-    out[1] = h->split;
-    out[2] = h->tile_group;
-    out[3] = h->tune_panel_ms;
-    out[4] = h->tune_gram_ms;
-    out[5] = h->split_created;
+    for (int prod = 0; prod < 2; ++prod) {
+      const unsigned t = h->panel_tiles[prod];
+      out[prod] = t == kGemmSmallTiles ? 1.0 : t == kGemmTallTiles ? 2.0 : t == kGemmWideTiles ? 3.0 : 0.0;
+      for (int c = 0; c < 4; ++c) out[7 + 4 * prod + c] = h->tune_ms[prod][c];
+    }
+    out[2] = h->split;
+    out[3] = h->tile_group;
+    out[4] = h->tune_panel_ms;
+    out[5] = h->tune_gram_ms;
+    out[6] = h->split_created;
   }
+  return kOk;
+}
+
+// Applies tiling choices (as reported by sdpx_dense_autotune) without timing:
+// panel shape codes 0..3, split in [1, create-time split], super-block side >= 1.
+int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group) {
+  SDPX_CHECK(check_handle(h));
+  const unsigned shapes[4] = {0u, (unsigned)kGemmSmallTiles, (unsigned)kGemmTallTiles, (unsigned)kGemmWideTiles};
+  if (panel0 < 0 || panel0 > 3 || panel1 < 0 || panel1 > 3 || split < 1 || split > h->split_created || group < 1) {
+    set_last_error("sdpx_dense_set_tiling: shape codes 0..3, split 1..create-time split, group >= 1");
+    return kInvalidArgument;
+  }
+  h->panel_tiles[0] = shapes[panel0];
+  h->panel_tiles[1] = shapes[panel1];
+  h->split = split;
+  h->tile_group = group;
+  const int nt = ceil_div(h->m, 128);
+  std::vector<int2> order((size_t)nt * (nt + 1) / 2);
+  lower_tile_order(nt, group, order.data());
+  SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
   return kOk;
 }
 

@@ -447,6 +447,23 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
     return vec ? launch<128, 128, true, false, true>(g, stream, split_k, workspace, after_main)
                : launch<128, 128, true, false, false>(g, stream, split_k, workspace, after_main);
   }
+  // Rectangular tiles on request (chosen by the plan-time autotuner for the
+  // P-hat panel products; instantiated for the NN and TN forms those use).
+  if ((g.flags & (kGemmTallTiles | kGemmWideTiles)) && !tb) {
+    const bool tall = (g.flags & kGemmTallTiles) != 0;
+    if (ta) {
+      if (tall)
+        return vec ? launch<128, 64, true, false, true>(g, stream, split_k, workspace, after_main)
+                   : launch<128, 64, true, false, false>(g, stream, split_k, workspace, after_main);
+      return vec ? launch<64, 128, true, false, true>(g, stream, split_k, workspace, after_main)
+                 : launch<64, 128, true, false, false>(g, stream, split_k, workspace, after_main);
+    }
+    if (tall)
+      return vec ? launch<128, 64, false, false, true>(g, stream, split_k, workspace, after_main)
+                 : launch<128, 64, false, false, false>(g, stream, split_k, workspace, after_main);
+    return vec ? launch<64, 128, false, false, true>(g, stream, split_k, workspace, after_main)
+               : launch<64, 128, false, false, false>(g, stream, split_k, workspace, after_main);
+  }
   // Small problems use 64 x 64 tiles so that k x k block products launch enough groups.
   // (and so does a product with a side of at most 64, which would leave half of a 128-wide tile idle)
   const bool small = (g.flags & kGemmSmallTiles) || (g.M <= 96 && g.N <= 96) || g.M <= 64 || g.N <= 64 ||

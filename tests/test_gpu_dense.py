@@ -58,7 +58,10 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     default_split = b.schur_split()
     t = b.autotune()  # before the iterate: timings do not depend on values
     print(t, flush=True)
-    assert t["panel_tile"] in (64, 128) and t["tile_group"] in (4, 8, 16)
+    assert all(sh in ("128x128", "64x64", "128x64", "64x128") for sh in t["panel_tiles"])
+    assert t["tile_group"] in (4, 8, 16)
+    for p, cand in enumerate(t["panel_candidates_ms"]):
+        assert cand[t["panel_tiles"][p]] == min(cand.values())
     assert t["default_split"] == default_split
     assert t["split"] in {default_split, max(1, 3 * default_split // 4), max(1, default_split // 2)}
     assert b.schur_split() == t["split"] and t["panel_ms"] > 0 and t["gram_ms"] > 0
@@ -70,6 +73,31 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     LX, _ = O.kchol(X)
     MY, _ = O.kchol(Y)
     assert _rel_fro_lower(S, O.schur_build(oprob, [LX], [MY])) <= 1e-12
+    b.close()
+
+
+@pytest.mark.parametrize("shapes", [("64x64", "64x64"), ("128x64", "64x128"), ("64x128", "128x64"),
+                                    ("128x128", "128x64")])
+def test_panel_tile_shapes_match_oracle(gpu_lib, shapes):
+    """Every panel tile shape the autotuner may pick gives the Schur matrix within the FP64 bar
+    (k = 130: partial tiles on both sides and triangular K ranges)."""
+    k, m = 130, 300
+    rng = np.random.default_rng(11)
+    g, prob, oprob = _dense_case(k, m, seed=6)
+    X, Y = _random_pd(rng, k), _random_pd(rng, k)
+    b = MI355XDenseBackend(prob)
+    b.set_tiling(shapes, split=1, tile_group=4)
+    assert b.schur_split() == 1
+    b.set_iterate(g.x0, [X], [Y])
+    _, _, ok = b.compute_residuals([0.1], affine=True, factor=True)
+    assert ok
+    b.schur_build()
+    S = b.get_schur()
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    assert _rel_fro_lower(S, O.schur_build(oprob, [LX], [MY])) <= 1e-12
+    with pytest.raises(Exception):
+        b.set_tiling(shapes, split=10 ** 6)
     b.close()
 
 
