@@ -167,16 +167,20 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
  * and times on the device each of the two P-hat panel products on 128x128,
  * 64x64, 128x64 or 64x128 tiles and the Schur Gram product at three split-K
  * factors and three tile-order super-block sides, and keeps the fastest.
- * Overwrites the panel and Schur buffers (call before solving or between
- * solves).  out (optional, 15 doubles): tile shape code of each panel product
- * (0..3 in the order above), split, super-block side, panel ms, Gram ms,
- * create-time split, then the 2 x 4 panel timings (product-major). */
+ * It also picks the bisection levels per batched trial round of
+ * sdpx_dense_fraction_to_boundary (4..7; the result is the same for any
+ * value).  Overwrites the panel, Schur and trial buffers (call before solving
+ * or between solves).  out (optional, 20 doubles): tile shape code of each
+ * panel product (0..3 in the order above), split, super-block side, panel ms,
+ * Gram ms, create-time split, the 2 x 4 panel timings (product-major), the
+ * levels per round, and the modelled search ms for 4..7 levels. */
 int sdpx_dense_autotune(sdpx_dense_t* h, double* out);
 /* Applies tiling choices without timing them (no counterpart in the
  * reference): panel shape codes as reported by sdpx_dense_autotune, Gram split
- * in [1, create-time split], tile-order super-block side >= 1.  Lets a caller
- * re-apply a saved autotune result. */
-int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group);
+ * in [1, create-time split], tile-order super-block side >= 1, bisection
+ * levels per trial round 1..7.  Lets a caller re-apply a saved autotune
+ * result. */
+int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group, int ftb_depth);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 

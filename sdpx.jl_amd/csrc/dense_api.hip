@@ -8,6 +8,7 @@
 //   X, Y, LX, MY, LXinv, Xinv, P, R, dX, dY, W1, W2   k x k blocks
 //   trial  cap x k^2 batched line-search trial matrices
 // Host code only sequences kernels and reads back the scalars of each phase.
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -142,6 +143,15 @@ __global__ void trial_build_kernel(const double* X, const double* dX, const doub
   }
 }
 
+// B identity matrices (k x k each): stand-in trials for timing a line-search round
+__global__ void identity_trials_kernel(long long k2, int k, int B, double* trial) {
+  const long long n = k2 * B;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    const long long i = idx % k2;
+    trial[idx] = i % (k + 1) == 0 ? 1.0 : 0.0;
+  }
+}
+
 // corrector target (threaded.jl:839-854 / 943-1003) from the block dot in *dotv:
 //   mode 0 (fixed):     max(rl < 1 ? rl^2 : rl, beta) * mu,  rl = dot / mu / k
 //   mode 1 (local):     sigma * dot / k
@@ -190,9 +200,11 @@ struct sdpx_dense {
   int precision = 0;
   unsigned panel_tiles[2] = {0, 0};  // tile-shape flags of the two P-hat panel products (autotuned)
   int tile_group = 8;        // super-block side of the Gram tile order
+  int ftb_depth = 7;         // bisection levels per batched trial round (fraction to the boundary)
   int split_created = 1;     // split-K factor chosen at create (the workspace holds that many partials)
   float tune_panel_ms = 0.f, tune_gram_ms = 0.f;
   float tune_ms[2][4] = {};  // panel products x tile shapes (128x128, 64x64, 128x64, 64x128)
+  float tune_ftb_ms[4] = {};  // modelled fraction-to-the-boundary search, D = 4..7 levels per round
   float* Pf = nullptr;
   long long ldPf = 0, maxk2 = 0;
   __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
@@ -423,7 +435,10 @@ int ftb_block(sdpx_dense_t* h, Block& b, double* bx, double* by) {
     *by = h->hsc[1];
     return kOk;
   }
-  constexpr int D0 = 6, D = 7;  // (1 + 63) and 127 candidates per side and round
+  // D levels (2^D - 1 candidates per side) per round; round 0 takes the full
+  // step and D - 1 levels.  D is a pure throughput choice (the replay gives
+  // the sequential search's result for any D): 7 unless the autotuner set it.
+  const int D = h->ftb_depth, D0 = h->ftb_depth - 1;
   double lo[2] = {0.0, 0.0}, hi[2] = {1.0, 1.0}, res[2] = {1.0, 1.0};
   int remaining[2] = {64, 64};
   bool active[2] = {true, true};
@@ -965,9 +980,11 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
 // products on 128 x 128, 64 x 64, 128 x 64 or 64 x 128 tiles; Gram split-K
 // factor s, 3s/4, s/2 of the create-time choice s (the partial-sum workspace
 // is sized for s); Gram tile order in 4 x 4, 8 x 8 or 16 x 16 super-blocks.
-// out (optional, 15 doubles): tile shape code of each panel product (0..3 in
-// the order above), split, super-block side, panel ms, Gram ms, create-time
-// split, then the 2 x 4 panel timings.
+// Also the number of bisection levels per batched trial round of the fraction
+// to the boundary (4..7; any value gives the same result).  out (optional, 20
+// doubles): tile shape code of each panel product (0..3 in the order above),
+// split, super-block side, panel ms, Gram ms, create-time split, the 2 x 4
+// panel timings, the levels per round, and the modelled search ms for 4..7.
 int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
   SDPX_CHECK(check_handle(h));
   const Block* bp = &h->blk[0];
@@ -1067,6 +1084,38 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     h->tile_group = best_group;
     lower_tile_order(nt, best_group, order.data());
     SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+    // fraction-to-the-boundary rounds: wall time of one batched trial round
+    // (trial fill, batched Cholesky, read-back, host wait) at the batch sizes
+    // of D = 4..7 levels per round; a search of ~53 levels costs round 0
+    // (2 * 2^(D-1) trials) plus ceil((53 - (D-1)) / D) rounds of 2 (2^D - 1)
+    if (b.k > 2) {
+      const long long dstride = (long long)b.nblk * kCholNb * kCholNb;
+      auto round_s = [&](int B, double* sec) -> int {
+        for (int rep = 0; rep < 3; ++rep) {  // first run warms
+          const auto t0 = std::chrono::steady_clock::now();
+          hipLaunchKernelGGL(identity_trials_kernel, dim3(std::min(8192, ceil_div(b.k2 * B, 256))), dim3(256), 0,
+                             h->stream, b.k2, b.k, B, b.trial);
+          SDPX_HIP(hipGetLastError());
+          SDPX_CHECK(potrf_f64(b.trial, b.k, b.k2, b.k, B, b.trial_dinv, dstride, h->dok, h->stream));
+          SDPX_HIP(hipMemcpyAsync(h->hinfo, h->dok, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+          SDPX_HIP(hipStreamSynchronize(h->stream));
+          if (rep > 0) *sec += 0.5 * std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return kOk;
+      };
+      double best = 0.0;
+      for (int D = 4; D <= 7; ++D) {
+        double r0 = 0.0, r1 = 0.0;
+        SDPX_CHECK(round_s(2 << (D - 1), &r0));
+        SDPX_CHECK(round_s(2 * ((1 << D) - 1), &r1));
+        const double cost = 1e3 * (r0 + std::ceil((53.0 - (D - 1)) / D) * r1);
+        h->tune_ftb_ms[D - 4] = (float)cost;
+        if (D == 4 || cost < best) {
+          best = cost;
+          h->ftb_depth = D;
+        }
+      }
+    }
     return kOk;
   };
   const int rc = body();
@@ -1083,23 +1132,27 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     out[4] = h->tune_panel_ms;
     out[5] = h->tune_gram_ms;
     out[6] = h->split_created;
+    out[15] = h->ftb_depth;
+    for (int d = 0; d < 4; ++d) out[16 + d] = h->tune_ftb_ms[d];
   }
   return kOk;
 }
 
 // Applies tiling choices (as reported by sdpx_dense_autotune) without timing:
 // panel shape codes 0..3, split in [1, create-time split], super-block side >= 1.
-int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group) {
+int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group, int ftb_depth) {
   SDPX_CHECK(check_handle(h));
   const unsigned shapes[4] = {0u, (unsigned)kGemmSmallTiles, (unsigned)kGemmTallTiles, (unsigned)kGemmWideTiles};
-  if (panel0 < 0 || panel0 > 3 || panel1 < 0 || panel1 > 3 || split < 1 || split > h->split_created || group < 1) {
-    set_last_error("sdpx_dense_set_tiling: shape codes 0..3, split 1..create-time split, group >= 1");
+  if (panel0 < 0 || panel0 > 3 || panel1 < 0 || panel1 > 3 || split < 1 || split > h->split_created || group < 1 ||
+      ftb_depth < 1 || ftb_depth > 7) {
+    set_last_error("sdpx_dense_set_tiling: shape codes 0..3, split 1..create-time split, group >= 1, levels 1..7");
     return kInvalidArgument;
   }
   h->panel_tiles[0] = shapes[panel0];
   h->panel_tiles[1] = shapes[panel1];
   h->split = split;
   h->tile_group = group;
+  h->ftb_depth = ftb_depth;
   const int nt = ceil_div(h->m, 128);
   std::vector<int2> order((size_t)nt * (nt + 1) / 2);
   lower_tile_order(nt, group, order.data());

@@ -247,20 +247,23 @@ class MI355XDenseBackend:
 
     def autotune(self) -> dict:
         """Time the Schur build's tiling candidates on the device and keep the fastest (plan time)."""
-        out = np.zeros(15)
+        out = np.zeros(20)
         check(self.lib.sdpx_dense_autotune(self.h, dptr(out)), "autotune")
         shapes = ("128x128", "64x64", "128x64", "64x128")
         return dict(panel_tiles=[shapes[int(out[0])], shapes[int(out[1])]], split=int(out[2]),
                     tile_group=int(out[3]), panel_ms=round(float(out[4]), 4), gram_ms=round(float(out[5]), 4),
                     default_split=int(out[6]),
                     panel_candidates_ms=[{sh: round(float(out[7 + 4 * p + c]), 4) for c, sh in enumerate(shapes)}
-                                         for p in range(2)])
+                                         for p in range(2)],
+                    ftb_levels_per_round=int(out[15]),
+                    ftb_model_ms={d: round(float(out[16 + d - 4]), 4) for d in range(4, 8)})
 
-    def set_tiling(self, panel_tiles=("128x128", "128x128"), split: int = 1, tile_group: int = 8) -> None:
+    def set_tiling(self, panel_tiles=("128x128", "128x128"), split: int = 1, tile_group: int = 8,
+                   ftb_levels_per_round: int = 7) -> None:
         """Apply tiling choices (e.g. a saved autotune result) without timing them."""
         shapes = ("128x128", "64x64", "128x64", "64x128")
         check(self.lib.sdpx_dense_set_tiling(self.h, shapes.index(panel_tiles[0]), shapes.index(panel_tiles[1]),
-                                             int(split), int(tile_group)), "set_tiling")
+                                             int(split), int(tile_group), int(ftb_levels_per_round)), "set_tiling")
 
     def schur_split(self) -> int:
         s = C.c_int(0)

@@ -218,6 +218,10 @@ def test_step_phases_match_oracle(gpu_lib):
     fX, fY = b.fraction_to_boundary(1.0, 1.0)
     fX_o, fY_o = O.fraction_line_search(oprob, [X], [Y], st.dX, st.dY, 1.0, 1.0)
     assert abs(fX - fX_o) <= 1e-12 and abs(fY - fY_o) <= 1e-12
+    # the levels per batched trial round only regroup the same sequential bisection
+    for levels in (1, 3, 4, 6):
+        b.set_tiling(("128x128", "128x128"), split=1, tile_group=8, ftb_levels_per_round=levels)
+        assert b.fraction_to_boundary(1.0, 1.0) == (fX, fY), levels
     b.close()
 
 
