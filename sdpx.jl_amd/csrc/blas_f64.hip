@@ -127,6 +127,50 @@ __global__ __launch_bounds__(256) void avx_partial_kernel(const double* __restri
   if (r < K2) part[(long long)blockIdx.y * K2 + r] = (a0 + a1) + (a2 + a3);
 }
 
+// Same partial sums with two rows per thread (16-byte fetches): needs an even
+// K2 and a 16-byte aligned Av.
+__global__ __launch_bounds__(256) void avx_partial2_kernel(const double* __restrict__ Av, long long K2, int m,
+                                                           const double* __restrict__ x, int cols, double* part) {
+  __shared__ double xs[256];
+  const long long K2h = K2 / 2;
+  const long long r2 = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int c0 = blockIdx.y * cols;
+  const int c1 = min(m, c0 + cols);
+  double s0[2] = {0.0, 0.0}, s1[2] = {0.0, 0.0}, s2[2] = {0.0, 0.0}, s3[2] = {0.0, 0.0};
+  for (int cb = c0; cb < c1; cb += 256) {
+    const int nc = min(256, c1 - cb);
+    __syncthreads();
+    if ((int)threadIdx.x < nc) xs[threadIdx.x] = x[cb + threadIdx.x];
+    __syncthreads();
+    if (r2 < K2h) {
+      const double2* p = reinterpret_cast<const double2*>(Av + (long long)cb * K2) + r2;
+      int c = 0;
+      for (; c + 4 <= nc; c += 4) {
+        const double2 v0 = p[(long long)c * K2h], v1 = p[(long long)(c + 1) * K2h];
+        const double2 v2 = p[(long long)(c + 2) * K2h], v3 = p[(long long)(c + 3) * K2h];
+        s0[0] += v0.x * xs[c];
+        s0[1] += v0.y * xs[c];
+        s1[0] += v1.x * xs[c + 1];
+        s1[1] += v1.y * xs[c + 1];
+        s2[0] += v2.x * xs[c + 2];
+        s2[1] += v2.y * xs[c + 2];
+        s3[0] += v3.x * xs[c + 3];
+        s3[1] += v3.y * xs[c + 3];
+      }
+      for (; c < nc; ++c) {
+        const double2 v0 = p[(long long)c * K2h];
+        s0[0] += v0.x * xs[c];
+        s0[1] += v0.y * xs[c];
+      }
+    }
+  }
+  if (r2 < K2h) {
+    double* o = part + (long long)blockIdx.y * K2 + 2 * r2;
+    o[0] = (s0[0] + s1[0]) + (s2[0] + s3[0]);
+    o[1] = (s0[1] + s1[1]) + (s2[1] + s3[1]);
+  }
+}
+
 __global__ __launch_bounds__(256) void avx_combine_kernel(const double* part, int splits, long long K2, double alpha,
                                                           double* out, double b1, const double* y1, double b2,
                                                           const double* y2) {
@@ -155,6 +199,38 @@ __global__ __launch_bounds__(256) void avt_kernel(const double* __restrict__ Av,
   if (r < K2) a0 += col[r] * M[r];
   double acc = block_reduce(a0 + a1, SumOp{}, sh);
   if (threadIdx.x == 0) v[i] = (beta == 0.0) ? alpha * acc : beta * v[i] + alpha * acc;
+}
+
+// Four constraints per work-group: each 16-byte fetch of M serves four 16-byte
+// fetches of the panel (M is re-read m/4 times from L2 instead of m times).
+// Needs an even K2 and 16-byte aligned Av and M.
+constexpr int kAvtCols = 4;
+__global__ __launch_bounds__(256) void avt4_kernel(const double* __restrict__ Av, long long K2, int m,
+                                                   const double* __restrict__ M, double alpha, double beta, double* v) {
+  __shared__ double sh[4];
+  const int i0 = blockIdx.x * kAvtCols;
+  const int nc = min(kAvtCols, m - i0);
+  const long long K2h = K2 / 2;
+  const double2* Mv = reinterpret_cast<const double2*>(M);
+  const double2* cv[kAvtCols];
+#pragma unroll
+  for (int c = 0; c < kAvtCols; ++c)
+    cv[c] = reinterpret_cast<const double2*>(Av + (long long)(i0 + (c < nc ? c : 0)) * K2);
+  double acc[kAvtCols] = {};
+#pragma unroll 2
+  for (long long r = threadIdx.x; r < K2h; r += 256) {
+    const double2 w = Mv[r];
+#pragma unroll
+    for (int c = 0; c < kAvtCols; ++c) {
+      const double2 a = cv[c][r];
+      acc[c] += a.x * w.x + a.y * w.y;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kAvtCols; ++c) {
+    const double t = block_reduce(acc[c], SumOp{}, sh);
+    if (threadIdx.x == 0 && c < nc) v[i0 + c] = (beta == 0.0) ? alpha * t : beta * v[i0 + c] + alpha * t;
+  }
 }
 
 // ---- elementwise --------------------------------------------------------
@@ -292,12 +368,17 @@ int diag_extrema_f64(const double* L, long long ld, long long stride, int n, int
 int avx_f64(const double* Av, long long K2, int m, const double* x, double alpha, double* out, double b1,
             const double* y1, double b2, const double* y2, double* part, long long part_cap, hipStream_t stream) {
   const int rb = ceil_div(K2, 256);
-  int splits = std::max(1, std::min(m, ceil_div(2048, rb)));
+  const bool vec = K2 % 2 == 0 && (reinterpret_cast<uintptr_t>(Av) & 15u) == 0;
+  const int pb = vec ? ceil_div(K2 / 2, 256) : rb;  // work-groups along the rows
+  int splits = std::max(1, std::min(m, ceil_div(2048, pb)));
   while ((long long)splits * K2 > part_cap && splits > 1) splits = (splits + 1) / 2;
   const int cols = ceil_div(m, splits);
   splits = ceil_div(m, cols);
   if (m > 0) {
-    hipLaunchKernelGGL(avx_partial_kernel, dim3(rb, splits), dim3(256), 0, stream, Av, K2, m, x, cols, part);
+    if (vec)
+      hipLaunchKernelGGL(avx_partial2_kernel, dim3(pb, splits), dim3(256), 0, stream, Av, K2, m, x, cols, part);
+    else
+      hipLaunchKernelGGL(avx_partial_kernel, dim3(rb, splits), dim3(256), 0, stream, Av, K2, m, x, cols, part);
   } else {
     splits = 0;
   }
@@ -310,7 +391,12 @@ int avx_f64(const double* Av, long long K2, int m, const double* x, double alpha
 int avt_f64(const double* Av, long long K2, int m, const double* M, double alpha, double beta, double* v,
             hipStream_t stream) {
   if (m <= 0) return kOk;
-  hipLaunchKernelGGL(avt_kernel, dim3(m), dim3(256), 0, stream, Av, K2, m, M, alpha, beta, v);
+  const bool vec = K2 % 2 == 0 && (reinterpret_cast<uintptr_t>(Av) & 15u) == 0 &&
+                   (reinterpret_cast<uintptr_t>(M) & 15u) == 0;
+  if (vec)
+    hipLaunchKernelGGL(avt4_kernel, dim3(ceil_div(m, kAvtCols)), dim3(256), 0, stream, Av, K2, m, M, alpha, beta, v);
+  else
+    hipLaunchKernelGGL(avt_kernel, dim3(m), dim3(256), 0, stream, Av, K2, m, M, alpha, beta, v);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }
