@@ -177,7 +177,7 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 int sdpx_dense_autotune(sdpx_dense_t* h, double* out);
 /* Applies tiling choices without timing them (no counterpart in the
  * reference): panel shape codes as reported by sdpx_dense_autotune, Gram split
- * in [1, create-time split], tile-order super-block side >= 1, bisection
+ * in [1, workspace capacity], tile-order super-block side >= 1, bisection
  * levels per trial round 1..7.  Lets a caller re-apply a saved autotune
  * result. */
 int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group, int ftb_depth);

@@ -201,7 +201,8 @@ struct sdpx_dense {
   unsigned panel_tiles[2] = {0, 0};  // tile-shape flags of the two P-hat panel products (autotuned)
   int tile_group = 8;        // super-block side of the Gram tile order
   int ftb_depth = 7;         // bisection levels per batched trial round (fraction to the boundary)
-  int split_created = 1;     // split-K factor chosen at create (the workspace holds that many partials)
+  int split_created = 1;     // split-K factor chosen at create
+  int split_max = 1;         // partial sums the workspace holds (autotuner candidates up to this)
   float tune_panel_ms = 0.f, tune_gram_ms = 0.f;
   float tune_ms[2][4] = {};  // panel products x tile shapes (128x128, 64x64, 128x64, 64x128)
   float tune_ftb_ms[4] = {};  // modelled fraction-to-the-boundary search, D = 4..7 levels per round
@@ -655,7 +656,11 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
     }
     h->split = best;
     h->split_created = best;
-    if (best > 1 && (s = dalloc(h, &h->splitws, (size_t)(best * mm2)))) return fail(s);
+    // room for the autotuner's larger candidates (up to 2x), within the same cap
+    h->split_max = best > 1 ? (int)std::min<long long>(2LL * best, std::min<long long>(mem_cap, std::max(1LL, stages / 64)))
+                            : 1;
+    if (h->split_max < best) h->split_max = best;
+    if (h->split_max > 1 && (s = dalloc(h, &h->splitws, (size_t)(h->split_max * mm2)))) return fail(s);
     // Gram tiles visited in 8 x 8 super-blocks: the co-resident groups of one
     // XCD then read few distinct Phat column panels, shared in its L2.
     std::vector<int2> order((size_t)tiles);
@@ -978,8 +983,9 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
 // the operand values, so it can run before any iterate is set; it overwrites
 // the panel and Schur buffers).  Candidates: each of the two P-hat panel
 // products on 128 x 128, 64 x 64, 128 x 64 or 64 x 128 tiles; Gram split-K
-// factor s, 3s/4, s/2 of the create-time choice s (the partial-sum workspace
-// is sized for s); Gram tile order in 4 x 4, 8 x 8 or 16 x 16 super-blocks.
+// factor s/2, 3s/4, s, 3s/2 or 2s of the create-time choice s (the partial-sum
+// workspace holds up to 2s, within the create-time memory cap); Gram tile
+// order in 4 x 4, 8 x 8 or 16 x 16 super-blocks.
 // Also the number of bisection levels per batched trial round of the fraction
 // to the boundary (4..7; any value gives the same result).  out (optional, 20
 // doubles): tile shape code of each panel product (0..3 in the order above),
@@ -1067,7 +1073,8 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
       lower_tile_order(nt, group, order.data());
       SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
       int last = 0;
-      for (int sp : {s0, (3 * s0) / 4, s0 / 2}) {
+      for (int sp : {s0, (3 * s0) / 4, s0 / 2, (3 * s0) / 2, 2 * s0}) {
+        if (sp > h->split_max) continue;
         if (sp < 1 || sp == last) continue;
         last = sp;
         float ms = 0.f;
@@ -1139,13 +1146,13 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
 }
 
 // Applies tiling choices (as reported by sdpx_dense_autotune) without timing:
-// panel shape codes 0..3, split in [1, create-time split], super-block side >= 1.
+// panel shape codes 0..3, split in [1, workspace capacity], super-block side >= 1.
 int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group, int ftb_depth) {
   SDPX_CHECK(check_handle(h));
   const unsigned shapes[4] = {0u, (unsigned)kGemmSmallTiles, (unsigned)kGemmTallTiles, (unsigned)kGemmWideTiles};
-  if (panel0 < 0 || panel0 > 3 || panel1 < 0 || panel1 > 3 || split < 1 || split > h->split_created || group < 1 ||
+  if (panel0 < 0 || panel0 > 3 || panel1 < 0 || panel1 > 3 || split < 1 || split > h->split_max || group < 1 ||
       ftb_depth < 1 || ftb_depth > 7) {
-    set_last_error("sdpx_dense_set_tiling: shape codes 0..3, split 1..create-time split, group >= 1, levels 1..7");
+    set_last_error("sdpx_dense_set_tiling: shape codes 0..3, split 1..workspace capacity, group >= 1, levels 1..7");
     return kInvalidArgument;
   }
   h->panel_tiles[0] = shapes[panel0];

@@ -63,7 +63,8 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     for p, cand in enumerate(t["panel_candidates_ms"]):
         assert cand[t["panel_tiles"][p]] == min(cand.values())
     assert t["default_split"] == default_split
-    assert t["split"] in {default_split, max(1, 3 * default_split // 4), max(1, default_split // 2)}
+    assert t["split"] in {default_split, max(1, 3 * default_split // 4), max(1, default_split // 2),
+                          3 * default_split // 2, 2 * default_split}
     assert b.schur_split() == t["split"] and t["panel_ms"] > 0 and t["gram_ms"] > 0
     b.set_iterate(g.x0, [X], [Y])
     _, _, ok = b.compute_residuals([0.1], affine=True, factor=True)
