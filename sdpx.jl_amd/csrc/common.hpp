@@ -165,6 +165,12 @@ struct CholDesc {
 // (device arrays); matrices with *info != 0 or n <= j are left alone.  The
 // panel and trailing updates of the step are grouped GEMMs built by the caller.
 int potrf_diag_grouped(const CholDesc* d, const int* list, int count, int j, hipStream_t stream);
+// Trailing update t (A22 -= A21 A21', NT, lower output, batch entries skipped
+// when info != 0) fused with the factor of the next diagonal block: jbn x jbn
+// at column jn of each matrix (the top-left block of t's output), whose
+// inverse goes to dinv (block jn / nb of each entry's dinv_stride slice).
+int potrf_trailing_diag(const GemmArgs& t, int jn, int jbn, double* dinv, long long dinv_stride, int* info,
+                        hipStream_t stream);
 
 // Solves L x = b (trans=0) or L' x = b (trans=1) in place for one right-hand
 // side, using the diagonal-block inverses written by potrf_f64.

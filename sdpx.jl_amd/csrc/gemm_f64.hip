@@ -14,6 +14,7 @@
 // barrier.  The MFMA is fed with (B-fragment, A-fragment) so the accumulator
 // holds C transposed: lane%16 indexes a row of C, which makes the
 // column-major epilogue stores 128-byte contiguous.
+#include "chol_diag.hpp"
 #include "common.hpp"
 
 namespace sdpx {
@@ -148,17 +149,26 @@ __device__ __forceinline__ void tile_store(const double* v, double* S, int tid) 
 // K in [kBegin, kEnd); only entries with 0 <= row < M, col < N (and
 // row >= col when lower_out) are written, at C + row + col * ldc or, with
 // brows > 0, at C + row % brows + (row / brows) * bstride + col * ldc.
+// LDS (doubles) one tile of the core stages its operands in
+template <int BM, int BN, bool TA, bool TB>
+constexpr int core_lds() {
+  return 2 * lds_size<BM, TA>() + 2 * lds_size<BN, !TB>();
+}
+
+// The tile computation on caller-provided LDS (core_lds<...>() doubles).
 template <int BM, int BN, bool TA, bool TB, bool VEC>
-__device__ __forceinline__ void gemm_core(const double* __restrict__ A, long long lda, const double* __restrict__ B,
-                                          long long ldb, int M, int N, int kBegin, int kEnd, int m0, int n0,
-                                          bool lower_out, double* C, long long ldc, double alpha, double beta,
-                                          long long brows = 0, long long bstride = 0) {
+__device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restrict__ A, long long lda,
+                                              const double* __restrict__ B, long long ldb, int M, int N, int kBegin,
+                                              int kEnd, int m0, int n0, bool lower_out, double* C, long long ldc,
+                                              double alpha, double beta, long long brows, long long bstride,
+                                              int skip_lead = 0) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int A_ELEMS = BM * BK / 256;
   constexpr int B_ELEMS = BN * BK / 256;
-  __shared__ double As[2][lds_size<BM, TA>()];
-  __shared__ double Bs[2][lds_size<BN, !TB>()];
+  constexpr int LA = lds_size<BM, TA>(), LB = lds_size<BN, !TB>();
+  double(*As)[LA] = reinterpret_cast<double(*)[LA]>(lds);
+  double(*Bs)[LB] = reinterpret_cast<double(*)[LB]>(lds + 2 * LA);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -247,7 +257,7 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int col = n0 + wn0 + j * 16 + (lane >> 4) + 4 * rr;
-        if (row >= 0 && row < M && col < N && (!lower_out || row >= col)) {
+        if (row >= 0 && row < M && col < N && (!lower_out || row >= col) && (row >= skip_lead || col >= skip_lead)) {
           const double v = acc[i][j][rr];
           double* cp = C + roff + (long long)col * ldc;
           *cp = (beta == 0.0) ? alpha * v : alpha * v + beta * (*cp);
@@ -255,6 +265,16 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
       }
     }
   }
+}
+
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+__device__ __forceinline__ void gemm_core(const double* __restrict__ A, long long lda, const double* __restrict__ B,
+                                          long long ldb, int M, int N, int kBegin, int kEnd, int m0, int n0,
+                                          bool lower_out, double* C, long long ldc, double alpha, double beta,
+                                          long long brows = 0, long long bstride = 0) {
+  __shared__ __align__(16) double lds[core_lds<BM, BN, TA, TB>()];
+  gemm_core_lds<BM, BN, TA, TB, VEC>(lds, A, lda, B, ldb, M, N, kBegin, kEnd, m0, n0, lower_out, C, ldc, alpha, beta,
+                                     brows, bstride);
 }
 
 template <int BM, int BN, bool TA, bool TB, bool VEC>
@@ -355,6 +375,42 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g, int split,
 template <bool VEC>
 __global__ __launch_bounds__(256, 2) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
   gemm_f64_body<128, 128, true, false, VEC>(g, split, ws);
+}
+
+// Trailing update of a blocked Cholesky, A22 -= A21 A21' over the lower tiles
+// of A22 (one work-group per tile, batch entries along y), fused with the
+// factor of the next diagonal block: work-group 0 of each entry (dispatched
+// first) updates only that jbn x jbn block (a K = jb product) and factors it
+// at once, while the tile groups update everything else (the tile holding the
+// block leaves it out), so the factor's latency overlaps the update.
+template <int BT, bool VEC>
+__global__ __launch_bounds__(256, 2) void trailing_diag_kernel(GemmArgs t, int jn, int jbn, double* dinv,
+                                                               long long dinv_stride, int* info) {
+  constexpr int kCore = core_lds<BT, BT, false, true>();
+  __shared__ __align__(16) double lds[kCore > chol::kDiagLds ? kCore : chol::kDiagLds];
+  static_assert(core_lds<64, 64, false, true>() <= (kCore > chol::kDiagLds ? kCore : chol::kDiagLds), "LDS");
+  const int b = blockIdx.y;
+  if (info[b] != 0) return;
+  double* C = t.C + (long long)b * t.strideC;
+  const double* A = t.A + (long long)b * t.strideA;
+  const double* B = t.B + (long long)b * t.strideB;
+  if (blockIdx.x == 0) {
+    gemm_core_lds<64, 64, false, true, VEC>(lds, A, t.lda, B, t.ldb, jbn, jbn, 0, t.K, 0, 0, true, C, t.ldc, t.alpha,
+                                            t.beta, 0, 0);
+    __syncthreads();  // the block's update is written by this work-group; LDS is reused
+    chol::potrf_diag_body(C, t.ldc, jn, jbn,
+                          dinv + (long long)b * dinv_stride + (long long)(jn / kCholNb) * kCholNb * kCholNb, info + b,
+                          lds);
+    return;
+  }
+  // lower tiles, row-major: tile = ti (ti + 1) / 2 + tj
+  const int tile = blockIdx.x - 1;
+  int ti = (int)((sqrt(8.0 * (double)tile + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= tile) ++ti;
+  while (ti * (ti + 1) / 2 > tile) --ti;
+  const int tj = tile - ti * (ti + 1) / 2;
+  gemm_core_lds<BT, BT, false, true, VEC>(lds, A, t.lda, B, t.ldb, t.M, t.N, 0, t.K, ti * BT, tj * BT, true, C, t.ldc,
+                                          t.alpha, t.beta, 0, 0, jbn);
 }
 
 // C = sum_s W[s] + beta*C over the (lower) entries, summed in split order.
@@ -502,6 +558,38 @@ int gemm_f64_grouped(const GroupGemm* groups, const GroupTile* tiles, int ntiles
   } else {
     set_last_error("gemm_f64_grouped: TT products are not instantiated");
     return kInvalidArgument;
+  }
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+int potrf_trailing_diag(const GemmArgs& t, int jn, int jbn, double* dinv, long long dinv_stride, int* info,
+                        hipStream_t stream) {
+  if (t.M <= 0 || t.batch <= 0) return kOk;
+  if (t.M != t.N || (t.flags & ~(kGemmTransB | kGemmLowerOut)) != 0 || !(t.flags & kGemmTransB) ||
+      !(t.flags & kGemmLowerOut) || jbn < 1 || jbn > kCholNb || jbn > t.M) {
+    set_last_error("potrf_trailing_diag: expects the NT lower-output trailing update of a blocked Cholesky");
+    return kInvalidArgument;
+  }
+  // the tile side gemm_f64 picks for the same product
+  const bool small = (t.M <= 96) || ((long long)t.M * t.N * t.batch < 256LL * 128 * 128 && t.M <= 256);
+  const bool vec = vec_ok(t, false, true);
+  const int BT = small ? 64 : 128;
+  const int mt = ceil_div(t.M, BT);
+  const dim3 grid((unsigned)((long long)mt * (mt + 1) / 2 + 1), (unsigned)t.batch);  // + the diagonal group
+  if (small) {
+    if (vec)
+      hipLaunchKernelGGL((trailing_diag_kernel<64, true>), grid, dim3(256), 0, stream, t, jn, jbn, dinv, dinv_stride, info);
+    else
+      hipLaunchKernelGGL((trailing_diag_kernel<64, false>), grid, dim3(256), 0, stream, t, jn, jbn, dinv, dinv_stride,
+                         info);
+  } else {
+    if (vec)
+      hipLaunchKernelGGL((trailing_diag_kernel<128, true>), grid, dim3(256), 0, stream, t, jn, jbn, dinv, dinv_stride,
+                         info);
+    else
+      hipLaunchKernelGGL((trailing_diag_kernel<128, false>), grid, dim3(256), 0, stream, t, jn, jbn, dinv, dinv_stride,
+                         info);
   }
   SDPX_HIP(hipGetLastError());
   return kOk;
