@@ -403,6 +403,7 @@ struct sdpx_batch {
   // Schur panel work items
   int2* dpanel = nullptr;
   int npanel = 0;
+  int panel_off[5] = {};  // panel work items of kp = 64, 48, 32, 16 (offsets)
   // member lists (ring of kListRing slots of B ints, host pinned + device)
   int* dlist = nullptr;
   int* hlist = nullptr;
@@ -621,7 +622,7 @@ int set_active(sdpx_batch_t* h, const std::vector<int>& l) {
 int schur_build(sdpx_batch_t* h, const std::vector<int>& l) {
   if (l.empty()) return kOk;
   SDPX_CHECK(set_active(h, l));
-  SDPX_CHECK(launch_panel(h->arr, h->dpanel, h->npanel, kPanelChunk, h->stream));
+  SDPX_CHECK(launch_panel(h->arr, h->dpanel, h->panel_off, kPanelChunk, h->stream));
   SDPX_CHECK(gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream));
   return kOk;
 }
@@ -1324,9 +1325,16 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
     std::vector<int2> work;
     for (int p = 0; p < B; ++p)
       for (int i0 = 0; i0 < m[p]; i0 += kPanelChunk) work.push_back(make_int2(p, i0));
-    // big blocks first: better tail balance
+    // big blocks first (better tail balance), grouped by padded block size for the per-size kernels
     std::stable_sort(work.begin(), work.end(), [&](const int2& x, const int2& y) { return k[x.x] > k[y.x]; });
     h->npanel = (int)work.size();
+    for (int c = 0; c < 5; ++c) h->panel_off[c] = 0;
+    for (const int2& wi : work) {
+      const int kp = h->mem[wi.x].kp;
+      const int c = kp > 48 ? 0 : kp > 32 ? 1 : kp > 16 ? 2 : 3;
+      h->panel_off[c + 1] += 1;
+    }
+    for (int c = 1; c < 5; ++c) h->panel_off[c] += h->panel_off[c - 1];
     if ((s = balloc_n(h, &h->dpanel, h->npanel))) return fail(s);
     if (hipMemcpy(h->dpanel, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice) != hipSuccess)
       return fail(kHipError);

@@ -1076,139 +1076,169 @@ __global__ void set_int_kernel(Arrays a, const int* list, int n, int slot, int v
 }
 
 // ---- Schur panel Phat_i = L_X^-1 A_i L_Y on FP64 MFMA ------------------------------------
-// One work-group per (member, run of `chunk` constraints).  L_X^-1 and L_Y are
-// staged once; each A_i is prefetched into registers one constraint ahead and
-// staged in LDS, T = L_X^-1 A_i goes to LDS and Phat_i = T L_Y to HBM.
-// Products run on 16 x 16 MFMA tiles of the block padded to kp = 16 ceil(k/16);
-// tile (ti, tj) belongs to wave (ti + tj) % 4, so every wave holds tiles of
-// every row block and the triangular K ranges balance across waves.  The
-// operands are fed swapped so the accumulator holds the tile transposed
-// (lane % 16 indexes rows: contiguous column-major stores).
-constexpr int PT = 512;                           // 8 waves: two per SIMD hide MFMA / LDS latency
-constexpr int kPanelPrefetch = KMAX * KMAX / PT;  // doubles of A_i per thread
+// One work-group per (member, run of `chunk` constraints), one instance per
+// padded block size KP = 16 ceil(k/16) (compile-time trip counts and LDS
+// sized to the block).  L_X^-1 and L_Y are staged once; each A_i is
+// prefetched into registers one constraint ahead and staged in LDS,
+// T = L_X^-1 A_i goes to LDS and Phat_i = T L_Y to HBM.  Products run on
+// 16 x 16 MFMA tiles; small blocks put several constraints in flight per
+// work-group (CP slots of WPC waves: KP = 16 eight slots of one wave, KP = 32
+// two slots of four waves, one tile each), larger ones spread the tiles of
+// one constraint over the eight waves (tile (ti, tj) to group (ti + tj) % 4,
+// ranks 0 and 3 of a group to wave g, 1 and 2 to wave g + 4, which balances
+// the triangular K ranges).  The operands are fed swapped so the accumulator
+// holds the tile transposed (lane % 16 indexes rows: contiguous column-major
+// stores).
+constexpr int PT = 512;  // 8 waves: two per SIMD hide MFMA / LDS latency
 
+template <int KP>
+struct PanelShape {
+  static constexpr int NTL = KP / 16;                         // tiles per side
+  static constexpr int CP = KP == 16 ? 8 : KP == 32 ? 2 : 1;  // constraints in flight
+  static constexpr int WPC = 8 / CP;                          // waves per constraint
+  static constexpr int MAXT = (NTL * NTL + WPC - 1) / WPC;    // tiles per wave (at most)
+  // pitches: operands read as (16 consecutive columns x 4 rows) fragments use
+  // LPA with gcd(LPA, 32) = 2 (a half-wave's 8-byte reads land on 32 distinct
+  // bank pairs); (16 consecutive rows x 4 columns) fragments use LPB = 16 mod 32
+  // (the two row groups of a half-wave fall in opposite bank halves)
+  static constexpr int LPA = KP + 2;
+  static constexpr int LPB = KP == 16 ? 16 : KP == 32 ? 48 : KP == 48 ? 48 : 80;
+  static constexpr int NPF = (KP * KP + WPC * 64 - 1) / (WPC * 64);  // doubles of A_i per thread
+};
+
+template <int KP>
 __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, int chunk) {
-  // Two LDS pitches: operands read as (16 consecutive columns x 4 rows) fragments
-  // use LPA (66: 8-byte reads of a half-wave land on 32 distinct bank pairs),
-  // operands read as (16 consecutive rows x 4 columns) use LPB (80 = 16 mod 32
-  // doubles: the two row groups of a half-wave fall in opposite bank halves).
-  constexpr int LPA = 66, LPB = 80;
-  __shared__ double sLi[KMAX * LPB], sT[KMAX * LPB], sA[KMAX * LPA], sLy[KMAX * LPA];
+  using S = PanelShape<KP>;
+  constexpr int LPA = S::LPA, LPB = S::LPB, CP = S::CP, WPC = S::WPC, MAXT = S::MAXT, NPF = S::NPF;
+  __shared__ double sLi[KP * LPB], sLy[KP * LPA], sA[CP][KP * LPA], sT[CP][KP * LPB];
   const int2 w = work[blockIdx.x];
   const int p = w.x;
   const Member M = a.mem[p];
   if (a.si[(long long)p * NISLOT + I_SKIP] != 0) return;
-  const int k = M.k, kp = M.kp, nt = kp / 16;
+  const int k = M.k;
   const int i0 = w.y, i1 = min(M.m, w.y + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lq = lane >> 4;
-  // this wave's tiles: group g = (ti + tj) % 4 holds one tile per row block
-  // (nt = 4); its tiles of rank 0 and 3 (by ti) go to wave g, ranks 1 and 2 to
-  // wave g + 4, which balances the triangular K ranges of both products
-  int r0s[4], c0s[4], nmine = 0;
+  const int slot = wave / WPC, lw = wave % WPC;  // constraint slot, wave within the slot
+  const int st = threadIdx.x - slot * WPC * 64;   // thread within the slot
+  int r0s[MAXT], c0s[MAXT], nmine = 0;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) r0s[u] = c0s[u] = 0;
-  for (int t = 0; t < nt * nt; ++t) {
-    const int ti = t % nt, tj = t / nt;
-    const int g = (ti + tj) % 4;
-    int rank = 0, size = 0;
-    for (int t2 = 0; t2 < nt * nt; ++t2) {
-      const int ti2 = t2 % nt, tj2 = t2 / nt;
-      if ((ti2 + tj2) % 4 != g) continue;
-      ++size;
-      if (ti2 < ti) ++rank;
+  for (int u = 0; u < MAXT; ++u) r0s[u] = c0s[u] = 0;
+  if (WPC < 8) {
+    // one tile per wave
+    if (lw < S::NTL * S::NTL) {
+      r0s[0] = (lw % S::NTL) * 16;
+      c0s[0] = (lw / S::NTL) * 16;
+      nmine = 1;
     }
-    const int half = size == 4 ? (rank == 1 || rank == 2) : (rank & 1);
-    if (g + 4 * half == wave && nmine < 4) {
-      r0s[nmine] = ti * 16;
-      c0s[nmine] = tj * 16;
-      ++nmine;
+  } else {
+    for (int t = 0; t < S::NTL * S::NTL; ++t) {
+      const int ti = t % S::NTL, tj = t / S::NTL;
+      const int g = (ti + tj) % 4;
+      int rank = 0, size = 0;
+      for (int t2 = 0; t2 < S::NTL * S::NTL; ++t2) {
+        const int ti2 = t2 % S::NTL, tj2 = t2 / S::NTL;
+        if ((ti2 + tj2) % 4 != g) continue;
+        ++size;
+        if (ti2 < ti) ++rank;
+      }
+      const int half = size == 4 ? (rank == 1 || rank == 2) : (rank & 1);
+      if (g + 4 * half == wave && nmine < MAXT) {
+        r0s[nmine] = ti * 16;
+        c0s[nmine] = tj * 16;
+        ++nmine;
+      }
     }
-  }
-  int qend = 0, qbeg = kp;
-  for (int u = 0; u < nmine; ++u) {
-    qend = max(qend, r0s[u] + 16);
-    qbeg = min(qbeg, c0s[u]);
   }
   // this thread's share of a staged block: global offset (or -1 for padding) and LDS offset
-  int goff[kPanelPrefetch], loff[kPanelPrefetch];
+  int goff[NPF], loff[NPF];
 #pragma unroll
-  for (int j = 0; j < kPanelPrefetch; ++j) {
-    const int e = threadIdx.x + PT * j;
-    const int r = e % kp, c = e / kp;
-    const bool live = e < kp * kp;
+  for (int j = 0; j < NPF; ++j) {
+    const int e = st + WPC * 64 * j;
+    const int r = e % KP, c = e / KP;
+    const bool live = e < KP * KP;
     loff[j] = live ? r + c * LPA : -1;
     goff[j] = (live && r < k && c < k) ? r + c * k : -1;
   }
-#pragma unroll
-  for (int j = 0; j < kPanelPrefetch; ++j) {
-    const int e = threadIdx.x + PT * j;
-    if (loff[j] < 0) continue;
-    sLi[e % kp + (e / kp) * LPB] = goff[j] >= 0 ? a.LXinv[M.oK + goff[j]] : 0.0;
-    sLy[loff[j]] = goff[j] >= 0 ? a.LY[M.oK + goff[j]] : 0.0;
+  for (int e = threadIdx.x; e < KP * KP; e += PT) {
+    const int r = e % KP, c = e / KP;
+    const int g = (r < k && c < k) ? r + c * k : -1;
+    sLi[r + c * LPB] = g >= 0 ? a.LXinv[M.oK + g] : 0.0;
+    sLy[r + c * LPA] = g >= 0 ? a.LY[M.oK + g] : 0.0;
   }
   const long long k2 = (long long)k * k;
-  double pf[kPanelPrefetch];
+  double pf[NPF];
   auto prefetch = [&](int i) {
     const double* Ai = a.Av + M.oAv + (long long)i * k2;
 #pragma unroll
-    for (int j = 0; j < kPanelPrefetch; ++j) pf[j] = goff[j] >= 0 ? Ai[goff[j]] : 0.0;
+    for (int j = 0; j < NPF; ++j) pf[j] = (goff[j] >= 0 && i < i1) ? Ai[goff[j]] : 0.0;
   };
-  prefetch(i0);
-  for (int i = i0; i < i1; ++i) {
+  double* mA = sA[slot];
+  double* mT = sT[slot];
+  prefetch(i0 + slot);
+  // every slot runs the same number of passes (barriers); a slot past i1 idles
+  for (int base = i0; base < i1; base += CP) {
+    const int i = base + slot;
+    const bool on = i < i1;
 #pragma unroll
-    for (int j = 0; j < kPanelPrefetch; ++j)
-      if (loff[j] >= 0) sA[loff[j]] = pf[j];
-    if (i + 1 < i1) prefetch(i + 1);
+    for (int j = 0; j < NPF; ++j)
+      if (loff[j] >= 0) mA[loff[j]] = pf[j];
+    prefetch(i + CP);
     __syncthreads();
     // T = L_X^-1 A_i: tile rows r0, K range q < r0 + 16 (L_X^-1 lower)
     {
-      double4_t acc[4];
+      double4_t acc[MAXT];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-      for (int q0 = 0; q0 < qend; q0 += 4) {
+      for (int u = 0; u < MAXT; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+      for (int q0 = 0; q0 < KP; q0 += 4) {
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
           if (u >= nmine || q0 >= r0s[u] + 16) continue;
-          const double fb = sA[(q0 + lq) + (c0s[u] + lr) * LPA];   // B'[i'][kq] = A[q0+kq][c0+i']
+          const double fb = mA[(q0 + lq) + (c0s[u] + lr) * LPA];   // B'[i'][kq] = A[q0+kq][c0+i']
           const double fl = sLi[(r0s[u] + lr) + (q0 + lq) * LPB];  // L'[kq][j'] = Li[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < MAXT; ++u) {
         if (u >= nmine) continue;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) sT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LPB] = acc[u][rr];
+        for (int rr = 0; rr < 4; ++rr) mT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LPB] = acc[u][rr];
       }
     }
     __syncthreads();
     // Phat_i = T L_Y: tile columns c0, K range q >= c0 (L_Y lower)
     {
-      double4_t acc[4];
+      double4_t acc[MAXT];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-      for (int q0 = qbeg; q0 < kp; q0 += 4) {
+      for (int u = 0; u < MAXT; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+      for (int q0 = 0; q0 < KP; q0 += 4) {
+#pragma unroll
+        for (int u = 0; u < MAXT; ++u) {
           if (u >= nmine || q0 < c0s[u]) continue;
           const double fb = sLy[(q0 + lq) + (c0s[u] + lr) * LPA];  // Ly[q0+kq][c0+i']
-          const double fl = sT[(r0s[u] + lr) + (q0 + lq) * LPB];   // T[r0+j'][q0+kq]
+          const double fl = mT[(r0s[u] + lr) + (q0 + lq) * LPB];   // T[r0+j'][q0+kq]
           acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
         }
       }
-      double* Pi = a.Ph + M.oPh + (long long)i * M.ldP;
+      if (on) {
+        double* Pi = a.Ph + M.oPh + (long long)i * M.ldP;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (u >= nmine) continue;
-        const int row = r0s[u] + lr;
+        for (int u = 0; u < MAXT; ++u) {
+          if (u >= nmine) continue;
+          const int row = r0s[u] + lr;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int col = c0s[u] + lq + 4 * rr;
-          if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
+          for (int rr = 0; rr < 4; ++rr) {
+            const int col = c0s[u] + lq + 4 * rr;
+            if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
+          }
         }
       }
     }
+    // no barrier here: the next pass writes mA after every wave has passed the
+    // barrier above (its product-1 reads are done) and mT after the next one
   }
 }
 
@@ -1414,9 +1444,13 @@ int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t 
 int launch_block_factor(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(block_factor_kernel, n, a, list);
 }
-int launch_panel(const Arrays& a, const int2* work, int nwork, int chunk, hipStream_t st) {
-  if (nwork <= 0) return kOk;
-  hipLaunchKernelGGL(panel_kernel, dim3((unsigned)nwork), dim3(PT), 0, st, a, work, chunk);
+// work items grouped by padded block size: off[c]..off[c+1] hold kp = 64, 48, 32, 16 for c = 0..3
+int launch_panel(const Arrays& a, const int2* work, const int* off, int chunk, hipStream_t st) {
+  const int n64 = off[1] - off[0], n48 = off[2] - off[1], n32 = off[3] - off[2], n16 = off[4] - off[3];
+  if (n64 > 0) hipLaunchKernelGGL(panel_kernel<64>, dim3((unsigned)n64), dim3(PT), 0, st, a, work + off[0], chunk);
+  if (n48 > 0) hipLaunchKernelGGL(panel_kernel<48>, dim3((unsigned)n48), dim3(PT), 0, st, a, work + off[1], chunk);
+  if (n32 > 0) hipLaunchKernelGGL(panel_kernel<32>, dim3((unsigned)n32), dim3(PT), 0, st, a, work + off[2], chunk);
+  if (n16 > 0) hipLaunchKernelGGL(panel_kernel<16>, dim3((unsigned)n16), dim3(PT), 0, st, a, work + off[3], chunk);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }

@@ -89,7 +89,7 @@ int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t 
 // R = -X Y (+ P_MU I unless P_AFF); L_X, L_Y, their |diag| extrema and info; L_X^-1, X^-1
 int launch_block_factor(const Arrays& a, const int* list, int n, hipStream_t st);
 // Phat_i = L_X^-1 A_i L_Y for the (member, first constraint) items of `work`
-int launch_panel(const Arrays& a, const int2* work, int nwork, int chunk, hipStream_t st);
+int launch_panel(const Arrays& a, const int2* work, const int* off, int chunk, hipStream_t st);
 // Sf = lower(S) + P_REG * max(|S_ii|, 1) on the diagonal; info S reset
 int launch_shift_copy(const Arrays& a, const int* list, int n, hipStream_t st);
 // |diag| extrema of the Schur factor
