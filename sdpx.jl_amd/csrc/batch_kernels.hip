@@ -1096,7 +1096,6 @@ struct PanelShape {
   static constexpr int NTL = KP / 16;                         // tiles per side
   static constexpr int CP = KP == 16 ? 8 : KP == 32 ? 2 : 1;  // constraints in flight
   static constexpr int WPC = 8 / CP;                          // waves per constraint
-  static constexpr int MAXT = (NTL * NTL + WPC - 1) / WPC;    // tiles per wave (at most)
   // pitches: operands read as (16 consecutive columns x 4 rows) fragments use
   // LPA with gcd(LPA, 32) = 2 (a half-wave's 8-byte reads land on 32 distinct
   // bank pairs); (16 consecutive rows x 4 columns) fragments use LPB = 16 mod 32
@@ -1106,10 +1105,102 @@ struct PanelShape {
   static constexpr int NPF = (KP * KP + WPC * 64 - 1) / (WPC * 64);  // doubles of A_i per thread
 };
 
+// Tiles of wave w of a constraint slot, fixed at compile time so that the
+// product loops unroll with constant K ranges (no branches between MFMAs).
+struct PanelTiles {
+  int n;
+  int r0[2], c0[2];
+};
+template <int KP>
+constexpr PanelTiles panel_tiles(int w) {
+  PanelTiles ts{0, {0, 0}, {0, 0}};
+  constexpr int NTL = KP / 16;
+  if (PanelShape<KP>::WPC < 8) {
+    if (w < NTL * NTL) {
+      ts.n = 1;
+      ts.r0[0] = (w % NTL) * 16;
+      ts.c0[0] = (w / NTL) * 16;
+    }
+    return ts;
+  }
+  for (int t = 0; t < NTL * NTL; ++t) {
+    const int ti = t % NTL, tj = t / NTL;
+    const int g = (ti + tj) % 4;
+    int rank = 0, size = 0;
+    for (int t2 = 0; t2 < NTL * NTL; ++t2) {
+      const int ti2 = t2 % NTL, tj2 = t2 / NTL;
+      if ((ti2 + tj2) % 4 != g) continue;
+      ++size;
+      if (ti2 < ti) ++rank;
+    }
+    const int half = size == 4 ? (rank == 1 || rank == 2) : (rank & 1);
+    if (g + 4 * half == w && ts.n < 2) {
+      ts.r0[ts.n] = ti * 16;
+      ts.c0[ts.n] = tj * 16;
+      ++ts.n;
+    }
+  }
+  return ts;
+}
+
+// Both products of one constraint for wave W of its slot (the barrier between
+// them is taken by every wave of the work-group: one instance per wave).
+template <int KP, int W>
+__device__ __forceinline__ void panel_products(const double* mA, double* mT, const double* sLi, const double* sLy,
+                                               double* Pi, int k, int lr, int lq) {
+  using S = PanelShape<KP>;
+  constexpr int LPA = S::LPA, LPB = S::LPB;
+  constexpr PanelTiles ts = panel_tiles<KP>(W);
+  // T = L_X^-1 A_i: tile rows r0, K range q < r0 + 16 (L_X^-1 lower)
+  {
+    double4_t acc[2] = {double4_t{0.0, 0.0, 0.0, 0.0}, double4_t{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int q0 = 0; q0 < KP; q0 += 4) {
+#pragma unroll
+      for (int u = 0; u < ts.n; ++u) {
+        if (q0 >= ts.r0[u] + 16) continue;
+        const double fb = mA[(q0 + lq) + (ts.c0[u] + lr) * LPA];   // B'[i'][kq] = A[q0+kq][c0+i']
+        const double fl = sLi[(ts.r0[u] + lr) + (q0 + lq) * LPB];  // L'[kq][j'] = Li[r0+j'][q0+kq]
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < ts.n; ++u)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) mT[(ts.r0[u] + lr) + (ts.c0[u] + lq + 4 * rr) * LPB] = acc[u][rr];
+  }
+  __syncthreads();
+  // Phat_i = T L_Y: tile columns c0, K range q >= c0 (L_Y lower)
+  {
+    double4_t acc[2] = {double4_t{0.0, 0.0, 0.0, 0.0}, double4_t{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int q0 = 0; q0 < KP; q0 += 4) {
+#pragma unroll
+      for (int u = 0; u < ts.n; ++u) {
+        if (q0 < ts.c0[u]) continue;
+        const double fb = sLy[(q0 + lq) + (ts.c0[u] + lr) * LPA];  // Ly[q0+kq][c0+i']
+        const double fl = mT[(ts.r0[u] + lr) + (q0 + lq) * LPB];   // T[r0+j'][q0+kq]
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
+      }
+    }
+    if (Pi) {
+#pragma unroll
+      for (int u = 0; u < ts.n; ++u) {
+        const int row = ts.r0[u] + lr;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int col = ts.c0[u] + lq + 4 * rr;
+          if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
+        }
+      }
+    }
+  }
+}
+
 template <int KP>
 __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, int chunk) {
   using S = PanelShape<KP>;
-  constexpr int LPA = S::LPA, LPB = S::LPB, CP = S::CP, WPC = S::WPC, MAXT = S::MAXT, NPF = S::NPF;
+  constexpr int LPA = S::LPA, LPB = S::LPB, CP = S::CP, WPC = S::WPC, NPF = S::NPF;
   __shared__ double sLi[KP * LPB], sLy[KP * LPA], sA[CP][KP * LPA], sT[CP][KP * LPB];
   const int2 w = work[blockIdx.x];
   const int p = w.x;
@@ -1121,35 +1212,6 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
   const int lr = lane & 15, lq = lane >> 4;
   const int slot = wave / WPC, lw = wave % WPC;  // constraint slot, wave within the slot
   const int st = threadIdx.x - slot * WPC * 64;   // thread within the slot
-  int r0s[MAXT], c0s[MAXT], nmine = 0;
-#pragma unroll
-  for (int u = 0; u < MAXT; ++u) r0s[u] = c0s[u] = 0;
-  if (WPC < 8) {
-    // one tile per wave
-    if (lw < S::NTL * S::NTL) {
-      r0s[0] = (lw % S::NTL) * 16;
-      c0s[0] = (lw / S::NTL) * 16;
-      nmine = 1;
-    }
-  } else {
-    for (int t = 0; t < S::NTL * S::NTL; ++t) {
-      const int ti = t % S::NTL, tj = t / S::NTL;
-      const int g = (ti + tj) % 4;
-      int rank = 0, size = 0;
-      for (int t2 = 0; t2 < S::NTL * S::NTL; ++t2) {
-        const int ti2 = t2 % S::NTL, tj2 = t2 / S::NTL;
-        if ((ti2 + tj2) % 4 != g) continue;
-        ++size;
-        if (ti2 < ti) ++rank;
-      }
-      const int half = size == 4 ? (rank == 1 || rank == 2) : (rank & 1);
-      if (g + 4 * half == wave && nmine < MAXT) {
-        r0s[nmine] = ti * 16;
-        c0s[nmine] = tj * 16;
-        ++nmine;
-      }
-    }
-  }
   // this thread's share of a staged block: global offset (or -1 for padding) and LDS offset
   int goff[NPF], loff[NPF];
 #pragma unroll
@@ -1179,66 +1241,25 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
   // every slot runs the same number of passes (barriers); a slot past i1 idles
   for (int base = i0; base < i1; base += CP) {
     const int i = base + slot;
-    const bool on = i < i1;
 #pragma unroll
     for (int j = 0; j < NPF; ++j)
       if (loff[j] >= 0) mA[loff[j]] = pf[j];
     prefetch(i + CP);
     __syncthreads();
-    // T = L_X^-1 A_i: tile rows r0, K range q < r0 + 16 (L_X^-1 lower)
-    {
-      double4_t acc[MAXT];
-#pragma unroll
-      for (int u = 0; u < MAXT; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int q0 = 0; q0 < KP; q0 += 4) {
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u) {
-          if (u >= nmine || q0 >= r0s[u] + 16) continue;
-          const double fb = mA[(q0 + lq) + (c0s[u] + lr) * LPA];   // B'[i'][kq] = A[q0+kq][c0+i']
-          const double fl = sLi[(r0s[u] + lr) + (q0 + lq) * LPB];  // L'[kq][j'] = Li[r0+j'][q0+kq]
-          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < MAXT; ++u) {
-        if (u >= nmine) continue;
-#pragma unroll
-        for (int rr = 0; rr < 4; ++rr) mT[(r0s[u] + lr) + (c0s[u] + lq + 4 * rr) * LPB] = acc[u][rr];
-      }
-    }
-    __syncthreads();
-    // Phat_i = T L_Y: tile columns c0, K range q >= c0 (L_Y lower)
-    {
-      double4_t acc[MAXT];
-#pragma unroll
-      for (int u = 0; u < MAXT; ++u) acc[u] = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int q0 = 0; q0 < KP; q0 += 4) {
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u) {
-          if (u >= nmine || q0 < c0s[u]) continue;
-          const double fb = sLy[(q0 + lq) + (c0s[u] + lr) * LPA];  // Ly[q0+kq][c0+i']
-          const double fl = mT[(r0s[u] + lr) + (q0 + lq) * LPB];   // T[r0+j'][q0+kq]
-          acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb, fl, acc[u], 0, 0, 0);
-        }
-      }
-      if (on) {
-        double* Pi = a.Ph + M.oPh + (long long)i * M.ldP;
-#pragma unroll
-        for (int u = 0; u < MAXT; ++u) {
-          if (u >= nmine) continue;
-          const int row = r0s[u] + lr;
-#pragma unroll
-          for (int rr = 0; rr < 4; ++rr) {
-            const int col = c0s[u] + lq + 4 * rr;
-            if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
-          }
-        }
-      }
+    double* Pi = i < i1 ? a.Ph + M.oPh + (long long)i * M.ldP : nullptr;
+    switch (lw) {
+      case 0: panel_products<KP, 0>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 1: panel_products<KP, 1>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 2: panel_products<KP, 2>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 3: panel_products<KP, 3>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 4: panel_products<KP, 4>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 5: panel_products<KP, 5>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 6: panel_products<KP, 6>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      default: panel_products<KP, 7>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
     }
     // no barrier here: the next pass writes mA after every wave has passed the
-    // barrier above (its product-1 reads are done) and mT after the next one
+    // barrier between the products (its product-1 reads are done) and mT after
+    // the next pass's first barrier
   }
 }
 
