@@ -852,15 +852,31 @@ template <int KB>
 __global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list, int n, int op) {
   __shared__ double cols[kStepWaves][2 * kPairPitch > KMAX ? 2 * kPairPitch : KMAX];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // k > 32: one wavefront per side (two per member), so the two searches run on
+  // separate SIMDs; k <= 32: one wavefront per member, a half per side
+  constexpr int kSides = KB > 32 ? 2 : 1;
   const int slot = blockIdx.x * kStepWaves + w;
-  if (slot >= n) return;  // whole wavefront; no block barrier in this kernel
-  const int p = list[slot];
+  if (slot >= n * kSides) return;  // whole wavefront; no block barrier in this kernel
+  const int p = list[slot / kSides];
   const Member M = a.mem[p];
   const int k = M.k;
   if (k > KB || k <= KB - 16) return;
   const long long o = M.oK;
   const double* par = a.par + (long long)p * NPARAM;
   double* col = cols[w];
+  if (KB > 32) {
+    const int side = slot % kSides;
+    const double* base = (side == 0 ? a.X : a.Y) + o;
+    const double* dir = (side == 0 ? a.dX : a.dY) + o;
+    const double r = op == 0 ? wave_ftb_side<KB>(base, dir, k, col, lane)
+                             : wave_backtrack_side<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], col,
+                                                       lane);
+    if (lane == 0) {
+      const double frac = side == 0 ? par[P_PF] : par[P_DF];
+      a.sc[(long long)p * NSLOT + (side == 0 ? S_TX : S_TY)] = op == 0 ? (r < 1.0 ? frac * r : 1.0) : r;
+    }
+    return;
+  }
   if (KB <= 32 && k > 2) {
     // both sides at once, one per half wavefront
     const int half = lane >> 5, row = lane & 31;
@@ -1503,11 +1519,11 @@ int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t 
 // one launch per register width; members outside a width's range exit at once
 int launch_step_waves(const Arrays& a, const int* list, int n, int op, hipStream_t st) {
   if (n <= 0) return kOk;
-  const dim3 grid((unsigned)ceil_div(n, kStepWaves)), block(NT);
+  const dim3 grid((unsigned)ceil_div(n, kStepWaves)), grid2((unsigned)ceil_div(2LL * n, kStepWaves)), block(NT);
   hipLaunchKernelGGL(step_wave_kernel<16>, grid, block, 0, st, a, list, n, op);
   hipLaunchKernelGGL(step_wave_kernel<32>, grid, block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_wave_kernel<48>, grid, block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_wave_kernel<64>, grid, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<48>, grid2, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<64>, grid2, block, 0, st, a, list, n, op);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }
