@@ -55,8 +55,11 @@ __device__ __forceinline__ int diag_wave(double* a, long long lda, int jb, bool 
     }
     __syncthreads();
     const double p = cc[c];
-    if (!(p > 0.0)) return c;  // uniform over the work-group
     const double own = elim ? cc[lane] : mr[lane];  // a_rc (elimination) or M_cq (inverse)
+    // use `own` here so that its LDS read goes out with the pivot's (one round
+    // trip) instead of after the pivot test: 10% of the block's latency
+    asm volatile("" ::"v"(own));
+    if (!(p > 0.0)) return c;  // uniform over the work-group
     const double f = own / p;
     if (H0 + kDiagHalf - 1 > c) {
       const int lo = c + 1 > H0 ? c + 1 : H0;
