@@ -1272,6 +1272,7 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
     M.oG = h->nG;
     h->nG += align_up((long long)nch * k2);
     h->arr.max_chunks = std::max(h->arr.max_chunks, nch);
+    h->arr.max_m = std::max(h->arr.max_m, m[p]);
   }
   int s;
   Arrays& a = h->arr;

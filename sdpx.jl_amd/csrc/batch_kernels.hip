@@ -1375,18 +1375,22 @@ __global__ __launch_bounds__(NT) void solve_kernel(Arrays a, const int* list, in
 // rho = r - S dx (S lower, unfactored); S_RHO = ||rho||_inf, S_RNORM = ||r||_inf.
 // Wave w walks columns c = w, w+4, ...: column entries below the diagonal feed
 // y_r (into the wave's own partial vector) and the column dot feeds y_c.
+// LDS (dynamic, sized to the largest member): dx, four partial vectors, NT reduction slots.
 __global__ __launch_bounds__(NT) void residual_kernel(Arrays a, const int* list) {
-  __shared__ double sdx[MMAX];
-  __shared__ double y[4][MMAX];
-  __shared__ double red[NT];
+  extern __shared__ double dyn[];
+  const int ld = a.max_m;
+  double* sdx = dyn;
+  double* Y = dyn + ld;  // partial vector of wave w at Y + w * ld
+  double* red = dyn + 5 * ld;
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
   const int m = M.m;
   const double* S = a.S + M.oS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* yw = Y + wave * ld;
   for (int i = threadIdx.x; i < m; i += NT) {
     sdx[i] = a.dx[M.oM + i];
-    y[0][i] = y[1][i] = y[2][i] = y[3][i] = 0.0;
+    Y[i] = Y[ld + i] = Y[2 * ld + i] = Y[3 * ld + i] = 0.0;
   }
   __syncthreads();
   for (int c = wave; c < m; c += 4) {
@@ -1395,16 +1399,16 @@ __global__ __launch_bounds__(NT) void residual_kernel(Arrays a, const int* list)
     double dot = 0.0;
     for (int r = c + 1 + lane; r < m; r += 64) {
       const double s = col[r];
-      y[wave][r] += s * xc;
+      yw[r] += s * xc;
       dot += s * sdx[r];
     }
     dot = wave_sum(dot);
-    if (lane == 0) y[wave][c] += col[c] * xc + dot;
+    if (lane == 0) yw[c] += col[c] * xc + dot;
   }
   __syncthreads();
   double rm = 0.0, rr = 0.0;
   for (int i = threadIdx.x; i < m; i += NT) {
-    const double sy = (y[0][i] + y[1][i]) + (y[2][i] + y[3][i]);
+    const double sy = (Y[i] + Y[ld + i]) + (Y[2 * ld + i] + Y[3 * ld + i]);
     const double ri = a.r[M.oM + i];
     const double v = ri - sy;
     a.rho[M.oM + i] = v;
@@ -1504,7 +1508,11 @@ int launch_solve(const Arrays& a, const int* list, int n, int mode, hipStream_t 
   SDPX_LAUNCH(solve_kernel, n, a, list, mode);
 }
 int launch_residual(const Arrays& a, const int* list, int n, hipStream_t st) {
-  SDPX_LAUNCH(residual_kernel, n, a, list);
+  if (n <= 0) return kOk;
+  const size_t lds = sizeof(double) * (5 * (size_t)a.max_m + NT);
+  hipLaunchKernelGGL(residual_kernel, dim3((unsigned)n), dim3(NT), lds, st, a, list);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
 }
 int launch_vec_op(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
   SDPX_LAUNCH(vec_op_kernel, n, a, list, mode);

@@ -63,6 +63,7 @@ struct Arrays {
   double *S, *Sf, *dinv;
   double* gpart;   // Av x partial sums
   int max_chunks;  // most column chunks of any member
+  int max_m;       // most constraints of any member (dynamic LDS of the residual pass)
   // Sparse copy of the constraints when they were given as COO (null otherwise):
   // constraint i of member p owns entries [sp_ptr[oM + i], sp_ptr[oM + i + 1]), each a
   // position r + c k of the k x k block (both mirror positions of an off-diagonal
