@@ -1039,7 +1039,7 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
             },
             &ms));
         h->tune_ms[prod][c] = ms;
-        if (c == 0 || ms < best) {
+        if (c == 0 || ms < 0.99f * best) {  // 1% hysteresis, as for the Gram below
           best = ms;
           h->panel_tiles[prod] = tiles;
         }
@@ -1069,7 +1069,10 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     const int s0 = h->split_created;
     int best_split = s0, best_group = 8;
     bool first = true;
-    for (int group : {4, 8, 16}) {
+    // the create-time choice (split s0, 8 x 8 super-blocks) is timed first and is
+    // replaced only by a candidate at least 1% faster: run-to-run noise between
+    // near-equal candidates does not flip the choice
+    for (int group : {8, 4, 16}) {
       lower_tile_order(nt, group, order.data());
       SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
       int last = 0;
@@ -1079,7 +1082,7 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
         last = sp;
         float ms = 0.f;
         SDPX_CHECK(time_ms([&]() -> int { return gram(sp); }, &ms));
-        if (first || ms < h->tune_gram_ms) {
+        if (first || ms < 0.99f * h->tune_gram_ms) {
           first = false;
           h->tune_gram_ms = ms;
           best_split = sp;

@@ -61,7 +61,8 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     assert all(sh in ("128x128", "64x64", "128x64", "64x128") for sh in t["panel_tiles"])
     assert t["tile_group"] in (4, 8, 16)
     for p, cand in enumerate(t["panel_candidates_ms"]):
-        assert cand[t["panel_tiles"][p]] == min(cand.values())
+        # a candidate replaces the incumbent only when 1% faster
+        assert cand[t["panel_tiles"][p]] <= min(cand.values()) / 0.99 + 1e-3
     assert t["default_split"] == default_split
     assert t["split"] in {default_split, max(1, 3 * default_split // 4), max(1, default_split // 2),
                           3 * default_split // 2, 2 * default_split}
