@@ -93,6 +93,23 @@ end
 
 _concat(blocks) = reduce(vcat, (vec(b) for b in blocks); init=Float64[])
 
+# Plan-time tiling choices (no counterpart in SDPX): time the Schur build's
+# candidates on the device and keep the fastest; returns the 20 reported values
+# (panel tile shape codes, split, super-block side, timings, bisection levels).
+function autotune!(h::DenseHandle)
+    out = zeros(Float64, 20)
+    _check(ccall(_sym(:sdpx_dense_autotune), Cint, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, out),
+           "sdpx_dense_autotune")
+    return out
+end
+
+# Re-applies a saved autotune result without timing it.
+function set_tiling!(h::DenseHandle, panel0::Integer, panel1::Integer, split::Integer, group::Integer,
+                     ftb_levels::Integer)
+    _check(ccall(_sym(:sdpx_dense_set_tiling), Cint, (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint),
+                 h.ptr, panel0, panel1, split, group, ftb_levels), "sdpx_dense_set_tiling")
+end
+
 function set_iterate!(h::DenseHandle, x, X, Y)
     _check(ccall(_sym(:sdpx_dense_set_iterate), Cint,
                  (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
