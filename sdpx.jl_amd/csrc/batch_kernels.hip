@@ -197,18 +197,25 @@ __global__ __launch_bounds__(NT) void gemv_part_kernel(Arrays a, const int* list
 // position at a time: its entries are read 16 at a time (coalesced), each lane
 // keeps a running sum and the group reduces them in a fixed tree, so the
 // result is deterministic; it is written as the member's single partial.
+// Work-groups along y take kSpPositions positions each, so large members spread
+// over several CUs (the member's partial vector is written position by position).
+constexpr int kSpPositions = 1024;
+constexpr int kSpConstraints = 128;  // constraints per work-group of sp_gemvt_kernel
+
 __global__ __launch_bounds__(NT) void sp_gemv_kernel(Arrays a, const int* list, int mode) {
   __shared__ double xs[MMAX];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
   const int k2 = M.k * M.k;
+  const int e0 = blockIdx.y * kSpPositions, e1 = min(k2, e0 + kSpPositions);
+  if (e0 >= k2) return;  // whole work-group
   const double* vec = (mode == 1 || mode == 3 ? a.dx : a.x) + M.oM;
   for (int i = threadIdx.x; i < M.m; i += NT) xs[i] = vec[i];
   __syncthreads();
   const long long* pp = a.sp_pptr + M.oK;
   double* part = a.gpart + M.oG;
   const int g = threadIdx.x >> 4, gl = threadIdx.x & 15;
-  for (int e = g; e < k2; e += NT / 16) {
+  for (int e = e0 + g; e < e1; e += NT / 16) {
     double acc = 0.0;
     const long long q1 = pp[e + 1];
     for (long long q = pp[e] + gl; q < q1; q += 16) acc += a.sp_pval[q] * xs[a.sp_pcon[q]];
@@ -225,12 +232,14 @@ __global__ __launch_bounds__(NT) void sp_gemvt_kernel(Arrays a, const int* list,
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
   const int k2 = M.k * M.k;
+  const int i0 = blockIdx.y * kSpConstraints, i1 = min(M.m, i0 + kSpConstraints);
+  if (i0 >= M.m) return;  // whole work-group
   const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
   for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long long* ptr = a.sp_ptr + M.oM;
-  for (int i = wave; i < M.m; i += NT / 64) {
+  for (int i = i0 + wave; i < i1; i += NT / 64) {
     double s = 0.0;
     for (long long e = ptr[i] + lane; e < ptr[i + 1]; e += 64) s += a.sp_val[e] * W[a.sp_idx[e]];
     s = wave_sum(s);
@@ -1466,7 +1475,8 @@ __global__ void coo_densify_kernel(Arrays a, const long long* cptr, const int* c
 int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
   if (n <= 0) return kOk;
   if (a.sp_ptr)
-    hipLaunchKernelGGL(sp_gemv_kernel, dim3((unsigned)n), dim3(NT), 0, st, a, list, mode);
+    hipLaunchKernelGGL(sp_gemv_kernel, dim3((unsigned)n, (unsigned)ceil_div(KMAX * KMAX, kSpPositions)), dim3(NT), 0,
+                       st, a, list, mode);
   else
     hipLaunchKernelGGL(gemv_part_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
   SDPX_HIP(hipGetLastError());
@@ -1475,7 +1485,8 @@ int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t s
 int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st) {
   if (n <= 0) return kOk;
   if (a.sp_ptr)
-    hipLaunchKernelGGL(sp_gemvt_kernel, dim3((unsigned)n), dim3(NT), 0, st, a, list, mode);
+    hipLaunchKernelGGL(sp_gemvt_kernel, dim3((unsigned)n, (unsigned)ceil_div(a.max_m, kSpConstraints)), dim3(NT), 0,
+                       st, a, list, mode);
   else
     hipLaunchKernelGGL(gemvt_kernel, dim3((unsigned)n, (unsigned)a.max_chunks), dim3(NT), 0, st, a, list, mode);
   SDPX_HIP(hipGetLastError());

@@ -20,21 +20,21 @@
 namespace sdpx {
 namespace {
 
-constexpr int BK = 16;
+constexpr int BK = 16;     // K slice of the general kernels (the Gram kernel stages 32)
 constexpr int PAD = 16;  // [BK][R + 16] layout of operands contiguous along the tile rows
 // Operands contiguous along K are staged row-major, [R][BK + 1]: the 17-double
 // pitch keeps both the transposing 8-byte stores of the K-contiguous loads and
 // the 64-bit fragment reads free of LDS bank conflicts (a [BK][R + 16] layout
 // would put the eight lanes that load one row's 16 K values on one bank pair).
-constexpr int KPITCH = BK + 1;
+// (the K-contiguous pitch is KS + 1 for a K slice of KS)
 
-template <int R, bool KC>
+template <int R, bool KC, int KS = BK>
 __device__ __forceinline__ int lds_at(int kk, int r) {
-  return KC ? r * KPITCH + kk : kk * (R + PAD) + r;
+  return KC ? r * (KS + 1) + kk : kk * (R + PAD) + r;
 }
-template <int R, bool KC>
+template <int R, bool KC, int KS = BK>
 constexpr int lds_size() {
-  return KC ? R * KPITCH : BK * (R + PAD);
+  return KC ? R * (KS + 1) : KS * (R + PAD);
 }
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -43,15 +43,15 @@ typedef double double2_t __attribute__((ext_vector_type(2)));
 // Coordinates (r, kk) inside an R x BK operand tile of element (e, h) owned by
 // thread tid.  KC: the operand is contiguous along K in memory; VEC: pairs of
 // consecutive elements along the contiguous dimension are fetched together.
-template <int R, bool KC, bool VEC>
+template <int R, bool KC, bool VEC, int KS = BK>
 __device__ __forceinline__ void tile_coords(int tid, int e, int h, int& r, int& kk) {
   if (KC) {
     if (VEC) {
-      kk = (tid % (BK / 2)) * 2 + h;
-      r = tid / (BK / 2) + (256 / (BK / 2)) * e;
+      kk = (tid % (KS / 2)) * 2 + h;
+      r = tid / (KS / 2) + (256 / (KS / 2)) * e;
     } else {
-      kk = tid % BK;
-      r = tid / BK + (256 / BK) * e;
+      kk = tid % KS;
+      r = tid / KS + (256 / KS) * e;
     }
   } else {
     if (VEC) {
@@ -69,9 +69,9 @@ __device__ __forceinline__ void tile_coords(int tid, int e, int h, int& r, int& 
 // the epilogue discards), so full slices need no per-element guard: the main
 // K loop is a single basic block of loads, MFMAs and one barrier.  Only the
 // final partial slice (K not a multiple of BK) masks along K.
-template <int R, bool KC, bool VEC>
+template <int R, bool KC, bool VEC, int KS = BK>
 struct TileLoader {
-  static constexpr int NE = R * BK / 256;       // doubles per thread
+  static constexpr int NE = R * KS / 256;       // doubles per thread
   static constexpr int NL = VEC ? NE / 2 : NE;  // loads per thread
   long long rowoff[NL];                         // offset of (clamped row, kk = element's k)
 
@@ -80,7 +80,7 @@ struct TileLoader {
 #pragma unroll
     for (int e = 0; e < NL; ++e) {
       int r, k;
-      tile_coords<R, KC, VEC>(tid, e, 0, r, k);
+      tile_coords<R, KC, VEC, KS>(tid, e, 0, r, k);
       const int gr = max(min(r0 + r, rmax), 0);
       rowoff[e] = KC ? (long long)gr * ld + k : (long long)gr + (long long)k * ld;
     }
@@ -106,7 +106,7 @@ struct TileLoader {
 #pragma unroll
     for (int e = 0; e < NL; ++e) {
       int r, k;
-      tile_coords<R, KC, VEC>(tid, e, 0, r, k);
+      tile_coords<R, KC, VEC, KS>(tid, e, 0, r, k);
       const int gk = k0 + k;
       const long long rowbase = KC ? rowoff[e] - k : rowoff[e] - (long long)k * ld;
       const bool in = gk < ke;
@@ -123,24 +123,24 @@ struct TileLoader {
   }
 };
 
-template <int R, bool KC, bool VEC>
+template <int R, bool KC, bool VEC, int KS = BK>
 __device__ __forceinline__ void tile_store(const double* v, double* S, int tid) {
-  constexpr int NE = R * BK / 256;
+  constexpr int NE = R * KS / 256;
   if (VEC) {
 #pragma unroll
     for (int e = 0; e < NE / 2; ++e)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         int r, kk;
-        tile_coords<R, KC, VEC>(tid, e, h, r, kk);
-        S[lds_at<R, KC>(kk, r)] = v[2 * e + h];
+        tile_coords<R, KC, VEC, KS>(tid, e, h, r, kk);
+        S[lds_at<R, KC, KS>(kk, r)] = v[2 * e + h];
       }
   } else {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       int r, kk;
-      tile_coords<R, KC, VEC>(tid, e, 0, r, kk);
-      S[lds_at<R, KC>(kk, r)] = v[e];
+      tile_coords<R, KC, VEC, KS>(tid, e, 0, r, kk);
+      S[lds_at<R, KC, KS>(kk, r)] = v[e];
     }
   }
 }
@@ -150,13 +150,13 @@ __device__ __forceinline__ void tile_store(const double* v, double* S, int tid) 
 // row >= col when lower_out) are written, at C + row + col * ldc or, with
 // brows > 0, at C + row % brows + (row / brows) * bstride + col * ldc.
 // LDS (doubles) one tile of the core stages its operands in
-template <int BM, int BN, bool TA, bool TB>
+template <int BM, int BN, bool TA, bool TB, int KS = BK>
 constexpr int core_lds() {
-  return 2 * lds_size<BM, TA>() + 2 * lds_size<BN, !TB>();
+  return 2 * lds_size<BM, TA, KS>() + 2 * lds_size<BN, !TB, KS>();
 }
 
 // The tile computation on caller-provided LDS (core_lds<...>() doubles).
-template <int BM, int BN, bool TA, bool TB, bool VEC>
+template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK>
 __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restrict__ A, long long lda,
                                               const double* __restrict__ B, long long ldb, int M, int N, int kBegin,
                                               int kEnd, int m0, int n0, bool lower_out, double* C, long long ldc,
@@ -164,9 +164,9 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
                                               int skip_lead = 0) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int A_ELEMS = BM * BK / 256;
-  constexpr int B_ELEMS = BN * BK / 256;
-  constexpr int LA = lds_size<BM, TA>(), LB = lds_size<BN, !TB>();
+  constexpr int A_ELEMS = BM * KS / 256;
+  constexpr int B_ELEMS = BN * KS / 256;
+  constexpr int LA = lds_size<BM, TA, KS>(), LB = lds_size<BN, !TB, KS>();
   double(*As)[LA] = reinterpret_cast<double(*)[LA]>(lds);
   double(*Bs)[LB] = reinterpret_cast<double(*)[LB]>(lds + 2 * LA);
 
@@ -184,12 +184,12 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
 
   double ra[A_ELEMS], rb[B_ELEMS];
   // op(A)(i, kk): contiguous along K when TA;  op(B)(kk, j): contiguous along K when !TB
-  TileLoader<BM, TA, VEC> la;
-  TileLoader<BN, !TB, VEC> lb;
+  TileLoader<BM, TA, VEC, KS> la;
+  TileLoader<BN, !TB, VEC, KS> lb;
   la.init(lda, m0, M, tid);
   lb.init(ldb, n0, N, tid);
   auto load = [&](int k0) {
-    if (k0 + BK <= kEnd) {
+    if (k0 + KS <= kEnd) {
       la.load(ra, A, lda, k0);
       lb.load(rb, B, ldb, k0);
     } else {
@@ -198,18 +198,18 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
     }
   };
   auto store = [&](int buf) {
-    tile_store<BM, TA, VEC>(ra, As[buf], tid);
-    tile_store<BN, !TB, VEC>(rb, Bs[buf], tid);
+    tile_store<BM, TA, VEC, KS>(ra, As[buf], tid);
+    tile_store<BN, !TB, VEC, KS>(rb, Bs[buf], tid);
   };
   auto compute = [&](int cur) {
 #pragma unroll
-    for (int ks = 0; ks < BK / 4; ++ks) {
+    for (int ks = 0; ks < KS / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
       double fa[FM], fb[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) fa[i] = As[cur][lds_at<BM, TA>(kr, wm0 + i * 16 + (lane & 15))];
+      for (int i = 0; i < FM; ++i) fa[i] = As[cur][lds_at<BM, TA, KS>(kr, wm0 + i * 16 + (lane & 15))];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][lds_at<BN, !TB>(kr, wn0 + j * 16 + (lane & 15))];
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[cur][lds_at<BN, !TB, KS>(kr, wn0 + j * 16 + (lane & 15))];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -223,20 +223,20 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
     store(0);
     __syncthreads();
     int cur = 0;
-    // full slices: prefetch k0 + BK while computing k0 (one barrier per slice)
-    const int kFull = kBegin + ((kEnd - kBegin) / BK) * BK;  // end of the full slices
+    // full slices: prefetch k0 + KS while computing k0 (one barrier per slice)
+    const int kFull = kBegin + ((kEnd - kBegin) / KS) * KS;  // end of the full slices
     int k0 = kBegin;
-    for (; k0 + 2 * BK <= kFull; k0 += BK) {
-      la.load(ra, A, lda, k0 + BK);
-      lb.load(rb, B, ldb, k0 + BK);
+    for (; k0 + 2 * KS <= kFull; k0 += KS) {
+      la.load(ra, A, lda, k0 + KS);
+      lb.load(rb, B, ldb, k0 + KS);
       compute(cur);
       store(cur ^ 1);
       __syncthreads();
       cur ^= 1;
     }
     // remaining one or two slices (the second may be partial)
-    if (k0 + BK < kEnd) {
-      load(k0 + BK);
+    if (k0 + KS < kEnd) {
+      load(k0 + KS);
       compute(cur);
       store(cur ^ 1);
       __syncthreads();
@@ -277,8 +277,8 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
                                      brows, bstride);
 }
 
-template <int BM, int BN, bool TA, bool TB, bool VEC>
-__device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws) {
+template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK>
+__device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws, double* lds) {
   const int b = blockIdx.y;
   if (g.skip && g.skip[(long long)b * g.skip_stride] != 0) return;
 
@@ -331,21 +331,21 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   if (g.flags & kGemmALower) kEnd = min(kEnd, m0 + BM);
   if (g.flags & kGemmBLower) kBegin = max(kBegin, n0);
   if (g.flags & kGemmAUpper) kBegin = max(kBegin, m0);
-  kBegin = (kBegin / BK) * BK;
+  kBegin = (kBegin / KS) * KS;
   if (split > 1) {
-    const int stages = (kEnd - kBegin + BK - 1) / BK;
+    const int stages = (kEnd - kBegin + KS - 1) / KS;
     const int per = (stages + split - 1) / split;
-    const int sb = kBegin + s * per * BK;
-    const int se = min(kEnd, sb + per * BK);
+    const int sb = kBegin + s * per * KS;
+    const int se = min(kEnd, sb + per * KS);
     kBegin = sb;
     kEnd = se;
   }
   // split-K partials go to W (M x M per split, no beta); otherwise straight to C
   const bool part = split > 1;
   double* out = part ? ws + ((long long)b * split + s) * (long long)g.M * g.N : g.C + (long long)b * g.strideC;
-  gemm_core<BM, BN, TA, TB, VEC>(A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
-                                 part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta,
-                                 part ? 0 : g.block_rows, g.block_stride);
+  gemm_core_lds<BM, BN, TA, TB, VEC, KS>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
+                                         part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta,
+                                         part ? 0 : g.block_rows, g.block_stride);
 }
 
 // ---- grouped launches: one work-group per (GEMM, tile) item ------------------
@@ -367,14 +367,23 @@ __global__ __launch_bounds__(256, 2) void gemm_grouped_kernel(const GroupGemm* _
 
 template <int BM, int BN, bool TA, bool TB, bool VEC>
 __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(GemmArgs g, int split, double* ws) {
-  gemm_f64_body<BM, BN, TA, TB, VEC>(g, split, ws);
+  __shared__ __align__(16) double lds[core_lds<BM, BN, TA, TB>()];
+  gemm_f64_body<BM, BN, TA, TB, VEC>(g, split, ws, lds);
 }
+
+// Gram kernel configuration: K slice and resident work-groups per CU (a 32-deep
+// slice with one resident group per CU, 135 KB of LDS, measured 55 ms against
+// 42-44 ms: the second resident group hides more latency than the halved
+// barrier count saves)
+constexpr int kGramKs = BK;
+constexpr int kGramWgPerCu = 2;
 
 // The Schur Gram product S = Phat' Phat (lower tiles) under its own symbol,
 // so profiles attribute the dominant kernel of the Newton step unambiguously.
 template <bool VEC>
-__global__ __launch_bounds__(256, 2) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
-  gemm_f64_body<128, 128, true, false, VEC>(g, split, ws);
+__global__ __launch_bounds__(256, kGramWgPerCu) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
+  __shared__ __align__(16) double lds[core_lds<128, 128, true, false, kGramKs>()];
+  gemm_f64_body<128, 128, true, false, VEC, kGramKs>(g, split, ws, lds);
 }
 
 // Trailing update of a blocked Cholesky, A22 -= A21 A21' over the lower tiles
