@@ -854,6 +854,123 @@ __device__ __forceinline__ double pair_backtrack(const double* base, const doubl
 
 constexpr int kStepWaves = NT / 64;  // members per work-group in the wavefront kernels
 
+// 2 < k <= 32: one work-group per member and eight half-wavefront trial slots,
+// slots 0-3 for the X side and 4-7 for the Y side.  Each round a side tests
+// the midpoints of the next two bisection levels (three nodes; the first
+// round tests the full step and the first two levels: four nodes), or the
+// next four backtracking candidates, and every thread replays the sequential
+// search from the outcomes (the same midpoints by the same arithmetic, the
+// same adjacency stop and 64-level cap), so the result is the sequential one.
+template <int KB>
+__global__ __launch_bounds__(NT) void step_quad_kernel(Arrays a, const int* list, int n, int op) {
+  __shared__ double cols[NT / 64][2 * kPairPitch];
+  __shared__ int pass[8];
+  static_assert(NT == 256, "four wavefronts: eight half-wavefront slots");
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  if (k > KB || k <= KB - 16 || k <= 2) return;  // whole work-group
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, row = lane & 31;
+  const int slot = 2 * w + half, side = slot >> 2, node = slot & 3;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  const double* base = (side == 0 ? a.X : a.Y) + o;
+  const double* dir = (side == 0 ? a.dX : a.dY) + o;
+  double* hcol = &cols[w][half * kPairPitch];
+  double res[2] = {1.0, 1.0};
+  bool active[2] = {true, true};
+  if (op == 0) {
+    double lo[2] = {0.0, 0.0}, hi[2] = {1.0, 1.0};
+    int levels[2] = {0, 0};
+    bool first = true;
+    while (active[0] || active[1]) {
+      // this slot's candidate
+      double t = 0.0;
+      if (active[side]) {
+        if (first) {
+          t = node == 0 ? 1.0 : node == 1 ? 0.5 : node == 2 ? 0.25 : 0.75;
+        } else if (node < 3) {
+          const double mid = (lo[side] + hi[side]) / 2.0;
+          t = node == 0 ? mid : node == 1 ? (lo[side] + mid) / 2.0 : (mid + hi[side]) / 2.0;
+        }
+      }
+      const bool ok = pair_trial_ok<KB>(base, dir, t, 0.0, k, hcol, row);
+      __syncthreads();  // the previous round's outcomes have been read
+      if (row == 0) pass[slot] = ok ? 1 : 0;
+      __syncthreads();
+      for (int sd = 0; sd < 2; ++sd) {
+        if (!active[sd]) continue;
+        const int* ps = pass + 4 * sd;
+        int nb = 0;  // node of the next level's midpoint
+        if (first) {
+          if (ps[0]) {
+            res[sd] = 1.0;
+            active[sd] = false;
+            continue;
+          }
+          nb = 1;
+        }
+        for (int lvl = 0; lvl < 2 && active[sd]; ++lvl) {
+          const double mid = (lo[sd] + hi[sd]) / 2.0;
+          if (levels[sd] >= 64 || mid == lo[sd] || mid == hi[sd]) {
+            active[sd] = false;
+            break;
+          }
+          ++levels[sd];
+          const bool up = ps[nb] != 0;
+          if (up)
+            lo[sd] = mid;
+          else
+            hi[sd] = mid;
+          // next level's node: (lo + mid) / 2 after a failure, (mid + hi) / 2 after a pass
+          nb = first ? (up ? 3 : 2) : (up ? 2 : 1);
+        }
+        if (!active[sd] || levels[sd] >= 64) {
+          active[sd] = false;
+          res[sd] = lo[sd];
+        }
+      }
+      first = false;
+    }
+  } else {
+    const double gamma = par[P_GAMMA], min_step = par[P_MINSTEP], ratio = par[P_RATIO];
+    double tn[2] = {1.0, 1.0};  // next candidate of each side
+    while (active[0] || active[1]) {
+      double t = 0.0;
+      if (active[side]) {
+        t = tn[side];
+        for (int j = 0; j < node; ++j) t *= gamma;  // t_j = t_0 gamma^j by repeated multiplication
+      }
+      const bool ok = pair_trial_ok<KB>(base, dir, t, ratio, k, hcol, row);
+      __syncthreads();
+      if (row == 0) pass[slot] = ok ? 1 : 0;
+      __syncthreads();
+      for (int sd = 0; sd < 2; ++sd) {
+        if (!active[sd]) continue;
+        double tc = tn[sd];
+        for (int j = 0; j < 4; ++j) {
+          if (pass[4 * sd + j] || tc < min_step || !(tc > 0.0)) {
+            res[sd] = tc;
+            active[sd] = false;
+            break;
+          }
+          tc *= gamma;
+        }
+        tn[sd] = tc;  // t reaches 0 after ~1100 candidates: the loop always ends
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    if (op == 0) {
+      a.sc[(long long)p * NSLOT + S_TX] = res[0] < 1.0 ? par[P_PF] * res[0] : 1.0;
+      a.sc[(long long)p * NSLOT + S_TY] = res[1] < 1.0 ? par[P_DF] * res[1] : 1.0;
+    } else {
+      a.sc[(long long)p * NSLOT + S_TX] = res[0];
+      a.sc[(long long)p * NSLOT + S_TY] = res[1];
+    }
+  }
+}
+
 // op 0: fraction to the boundary (S_TX, S_TY scaled by P_PF / P_DF); op 1: backtracking.
 // One instance per register width KB handles the listed members with
 // KB - 16 < k <= KB (the KB = 16 instance also the closed forms of k <= 2).
@@ -886,7 +1003,8 @@ __global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list
     }
     return;
   }
-  if (KB <= 32 && k > 2) {
+  if (KB <= 32 && k > 2) return;  // step_quad_kernel
+  if (false) {
     // both sides at once, one per half wavefront
     const int half = lane >> 5, row = lane & 31;
     const double* base = (half == 0 ? a.X : a.Y) + o;
@@ -1539,8 +1657,9 @@ int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t 
 int launch_step_waves(const Arrays& a, const int* list, int n, int op, hipStream_t st) {
   if (n <= 0) return kOk;
   const dim3 grid((unsigned)ceil_div(n, kStepWaves)), grid2((unsigned)ceil_div(2LL * n, kStepWaves)), block(NT);
-  hipLaunchKernelGGL(step_wave_kernel<16>, grid, block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_wave_kernel<32>, grid, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_wave_kernel<16>, grid, block, 0, st, a, list, n, op);  // k <= 2: closed forms
+  hipLaunchKernelGGL(step_quad_kernel<16>, dim3((unsigned)n), block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_quad_kernel<32>, dim3((unsigned)n), block, 0, st, a, list, n, op);
   hipLaunchKernelGGL(step_wave_kernel<48>, grid2, block, 0, st, a, list, n, op);
   hipLaunchKernelGGL(step_wave_kernel<64>, grid2, block, 0, st, a, list, n, op);
   SDPX_HIP(hipGetLastError());
