@@ -732,44 +732,12 @@ __device__ __forceinline__ bool wave_trial_ok(const double* base, const double* 
   return isfinite(mn) && isfinite(mx) && mx > 0.0 && mn > ratio * mx;
 }
 
-// fraction_to_boundary_bound! for k >= 3: full step, else bisection.  Once lo
-// and hi are adjacent doubles every remaining level of the 64 re-tests one of
-// them with a known outcome, so the search stops there (same result).
-template <int KB>
-__device__ __forceinline__ double wave_ftb_side(const double* base, const double* dir, int k, double* col, int lane) {
-  if (wave_trial_ok<KB>(base, dir, 1.0, 0.0, k, col, lane)) return 1.0;
-  double lo = 0.0, hi = 1.0;
-  for (int it = 0; it < 64; ++it) {
-    const double mid = (lo + hi) / 2.0;
-    if (mid == lo || mid == hi) break;
-    if (wave_trial_ok<KB>(base, dir, mid, 0.0, k, col, lane))
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-template <int KB>
-__device__ __forceinline__ double wave_backtrack_side(const double* base, const double* dir, int k, double gamma, double min_step,
-                                      double ratio, double* col, int lane) {
-  double t = 1.0;
-  while (true) {
-    const bool pass = wave_trial_ok<KB>(base, dir, t, ratio, k, col, lane);
-    if (pass || t < min_step || !(t > 0.0)) break;  // t reaches 0 after ~1100 steps: the loop always ends
-    t *= gamma;
-  }
-  return t;
-}
-
-// ---- both sides in one wavefront (k <= 32) --------------------------------------------------
-// Lanes 0-31 run the X-side search and lanes 32-63 the Y-side search in lock
-// step, one trial per side at a time: each half factors its own trial exactly
-// as wave_trial_chol does (same operations and order, so every decision is
-// the same), publishing its pivot column through its own LDS row; the two rows
-// are 33 doubles apart, so the halves' reads fall in different banks.  A half
-// whose search has ended still steps through the trials of the other half
-// (on t = 0, the base block itself) and ignores the outcome.
+// ---- half-wavefront trials (k <= 32) -----------------------------------------------------------
+// A half wavefront (32 lanes, one row each) factors one trial exactly as
+// wave_trial_chol does (same operations and order, so every decision is the
+// same), publishing its pivot column through its own LDS row; the two halves
+// of a wavefront use rows 33 doubles apart, so their reads fall in different
+// banks.
 constexpr int kPairPitch = 33;
 
 template <int KB>
@@ -811,48 +779,98 @@ __device__ __forceinline__ bool pair_trial_ok(const double* base, const double* 
   return isfinite(dmin) && isfinite(dmax) && dmax > 0.0 && dmin > ratio * dmax;
 }
 
-// fraction_to_boundary_bound! of this half's side (full step, then the
-// bisection that stops at adjacent bounds), as wave_ftb_side.
+// 32 < k <= 64: one work-group per (member, side), four wavefront trials per
+// round (node = wave): two bisection levels (the first round: the full step
+// and the first two levels) or four backtracking candidates, replayed as in
+// step_quad_kernel below.
 template <int KB>
-__device__ __forceinline__ double pair_ftb(const double* base, const double* dir, int k, double* col, int row) {
-  const bool full = pair_trial_ok<KB>(base, dir, 1.0, 0.0, k, col, row);
-  bool active = !full;
-  double lo = 0.0, hi = 1.0;
-  for (int it = 0; it < 64; ++it) {
-    const double mid = (lo + hi) / 2.0;
-    if (mid == lo || mid == hi) active = false;
-    if (!__any(active)) break;
-    const bool pass = pair_trial_ok<KB>(base, dir, active ? mid : 0.0, 0.0, k, col, row);
-    if (active) {
-      if (pass)
-        lo = mid;
-      else
-        hi = mid;
-    }
-  }
-  return full ? 1.0 : lo;
-}
-
-// line_search! backtracking of this half's side, as wave_backtrack_side.
-template <int KB>
-__device__ __forceinline__ double pair_backtrack(const double* base, const double* dir, int k, double gamma,
-                                                 double min_step, double ratio, double* col, int row) {
-  double t = 1.0;
+__global__ __launch_bounds__(NT) void step_quad_wide_kernel(Arrays a, const int* list, int n, int op) {
+  __shared__ double cols[NT / 64][KMAX];
+  __shared__ int pass[4];
+  static_assert(NT == 256, "four wavefronts: one trial each");
+  const int p = list[blockIdx.x >> 1];
+  const int side = blockIdx.x & 1;
+  const Member M = a.mem[p];
+  const int k = M.k;
+  if (k > KB || k <= KB - 16) return;  // whole work-group
+  const int node = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long o = M.oK;
+  const double* par = a.par + (long long)p * NPARAM;
+  const double* base = (side == 0 ? a.X : a.Y) + o;
+  const double* dir = (side == 0 ? a.dX : a.dY) + o;
+  double* col = cols[node];
+  double res = 1.0;
   bool active = true;
-  while (true) {  // each half's t reaches 0 after ~1100 steps: the loop always ends
-    const bool pass = pair_trial_ok<KB>(base, dir, active ? t : 0.0, ratio, k, col, row);
-    if (active) {
-      if (pass || t < min_step || !(t > 0.0))
+  if (op == 0) {
+    double lo = 0.0, hi = 1.0;
+    int levels = 0;
+    bool first = true;
+    while (active) {
+      double t = 0.0;
+      if (first) {
+        t = node == 0 ? 1.0 : node == 1 ? 0.5 : node == 2 ? 0.25 : 0.75;
+      } else if (node < 3) {
+        const double mid = (lo + hi) / 2.0;
+        t = node == 0 ? mid : node == 1 ? (lo + mid) / 2.0 : (mid + hi) / 2.0;
+      }
+      const bool ok = wave_trial_ok<KB>(base, dir, t, 0.0, k, col, lane);
+      __syncthreads();
+      if (lane == 0) pass[node] = ok ? 1 : 0;
+      __syncthreads();
+      int nb = 0;
+      if (first && pass[0]) {
+        res = 1.0;
         active = false;
-      else
-        t *= gamma;
+        break;
+      }
+      if (first) nb = 1;
+      for (int lvl = 0; lvl < 2; ++lvl) {
+        const double mid = (lo + hi) / 2.0;
+        if (levels >= 64 || mid == lo || mid == hi) {
+          active = false;
+          break;
+        }
+        ++levels;
+        const bool up = pass[nb] != 0;
+        if (up)
+          lo = mid;
+        else
+          hi = mid;
+        nb = first ? (up ? 3 : 2) : (up ? 2 : 1);
+      }
+      if (!active || levels >= 64) {
+        active = false;
+        res = lo;
+      }
+      first = false;
     }
-    if (!__any(active)) break;
+  } else {
+    const double gamma = par[P_GAMMA], min_step = par[P_MINSTEP], ratio = par[P_RATIO];
+    double tn = 1.0;
+    while (active) {
+      double t = tn;
+      for (int j = 0; j < node; ++j) t *= gamma;
+      const bool ok = wave_trial_ok<KB>(base, dir, t, ratio, k, col, lane);
+      __syncthreads();
+      if (lane == 0) pass[node] = ok ? 1 : 0;
+      __syncthreads();
+      double tc = tn;
+      for (int j = 0; j < 4; ++j) {
+        if (pass[j] || tc < min_step || !(tc > 0.0)) {
+          res = tc;
+          active = false;
+          break;
+        }
+        tc *= gamma;
+      }
+      tn = tc;  // t reaches 0 after ~1100 candidates: the loop always ends
+    }
   }
-  return t;
+  if (threadIdx.x == 0) {
+    const double frac = side == 0 ? par[P_PF] : par[P_DF];
+    a.sc[(long long)p * NSLOT + (side == 0 ? S_TX : S_TY)] = op == 0 ? (res < 1.0 ? frac * res : 1.0) : res;
+  }
 }
-
-constexpr int kStepWaves = NT / 64;  // members per work-group in the wavefront kernels
 
 // 2 < k <= 32: one work-group per member and eight half-wavefront trial slots,
 // slots 0-3 for the X side and 4-7 for the Y side.  Each round a side tests
@@ -972,94 +990,50 @@ __global__ __launch_bounds__(NT) void step_quad_kernel(Arrays a, const int* list
 }
 
 // op 0: fraction to the boundary (S_TX, S_TY scaled by P_PF / P_DF); op 1: backtracking.
-// One instance per register width KB handles the listed members with
-// KB - 16 < k <= KB (the KB = 16 instance also the closed forms of k <= 2).
-template <int KB>
-__global__ __launch_bounds__(NT) void step_wave_kernel(Arrays a, const int* list, int n, int op) {
-  __shared__ double cols[kStepWaves][2 * kPairPitch > KMAX ? 2 * kPairPitch : KMAX];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // k > 32: one wavefront per side (two per member), so the two searches run on
-  // separate SIMDs; k <= 32: one wavefront per member, a half per side
-  constexpr int kSides = KB > 32 ? 2 : 1;
-  const int slot = blockIdx.x * kStepWaves + w;
-  if (slot >= n * kSides) return;  // whole wavefront; no block barrier in this kernel
-  const int p = list[slot / kSides];
+// k <= 2: closed forms on the 2 x 2 (or 1 x 1) block, one thread per member.
+__global__ __launch_bounds__(NT) void step_small_kernel(Arrays a, const int* list, int n, int op) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  const int p = list[i];
   const Member M = a.mem[p];
   const int k = M.k;
-  if (k > KB || k <= KB - 16) return;
+  if (k > 2) return;
   const long long o = M.oK;
   const double* par = a.par + (long long)p * NPARAM;
-  double* col = cols[w];
-  if (KB > 32) {
-    const int side = slot % kSides;
-    const double* base = (side == 0 ? a.X : a.Y) + o;
-    const double* dir = (side == 0 ? a.dX : a.dY) + o;
-    const double r = op == 0 ? wave_ftb_side<KB>(base, dir, k, col, lane)
-                             : wave_backtrack_side<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], col,
-                                                       lane);
-    if (lane == 0) {
-      const double frac = side == 0 ? par[P_PF] : par[P_DF];
-      a.sc[(long long)p * NSLOT + (side == 0 ? S_TX : S_TY)] = op == 0 ? (r < 1.0 ? frac * r : 1.0) : r;
-    }
-    return;
-  }
-  if (KB <= 32 && k > 2) return;  // step_quad_kernel
-  if (false) {
-    // both sides at once, one per half wavefront
-    const int half = lane >> 5, row = lane & 31;
-    const double* base = (half == 0 ? a.X : a.Y) + o;
-    const double* dir = (half == 0 ? a.dX : a.dY) + o;
-    double* hcol = col + half * kPairPitch;
-    const double r = op == 0 ? pair_ftb<KB>(base, dir, k, hcol, row)
-                             : pair_backtrack<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], hcol, row);
-    if (row == 0) {
-      const double frac = half == 0 ? par[P_PF] : par[P_DF];
-      a.sc[(long long)p * NSLOT + (half == 0 ? S_TX : S_TY)] = op == 0 ? (r < 1.0 ? frac * r : 1.0) : r;
-    }
-    return;
-  }
   double res[2];
   for (int side = 0; side < 2; ++side) {
     const double* base = (side == 0 ? a.X : a.Y) + o;
     const double* dir = (side == 0 ? a.dX : a.dY) + o;
-    if (k <= 2) {
-      // closed forms on the 2 x 2 (or 1 x 1) block, LP-strided as in the LDS kernels
-      double sb[LP + 2] = {}, sd[LP + 2] = {};
-      sb[0] = base[0];
-      sd[0] = dir[0];
-      if (k == 2) {
-        sb[1] = base[1];
-        sb[LP] = base[2];
-        sb[LP + 1] = base[3];
-        sd[1] = dir[1];
-        sd[LP] = dir[2];
-        sd[LP + 1] = dir[3];
-      }
-      if (op == 0) {
-        res[side] = small_ftb(sb, sd, k);
-      } else {
-        double t = 1.0;
-        while (true) {
-          const bool pass = small_margin(sb, sd, k, t, par[P_RATIO]);
-          if (pass || t < par[P_MINSTEP] || !(t > 0.0)) break;
-          t *= par[P_GAMMA];
-        }
-        res[side] = t;
-      }
-      continue;
+    // LP-strided as in the LDS kernels
+    double sb[LP + 2] = {}, sd[LP + 2] = {};
+    sb[0] = base[0];
+    sd[0] = dir[0];
+    if (k == 2) {
+      sb[1] = base[1];
+      sb[LP] = base[2];
+      sb[LP + 1] = base[3];
+      sd[1] = dir[1];
+      sd[LP] = dir[2];
+      sd[LP + 1] = dir[3];
     }
-    res[side] = op == 0 ? wave_ftb_side<KB>(base, dir, k, col, lane)
-                        : wave_backtrack_side<KB>(base, dir, k, par[P_GAMMA], par[P_MINSTEP], par[P_RATIO], col,
-                                                  lane);
-  }
-  if (lane == 0) {
     if (op == 0) {
-      a.sc[(long long)p * NSLOT + S_TX] = res[0] < 1.0 ? par[P_PF] * res[0] : 1.0;
-      a.sc[(long long)p * NSLOT + S_TY] = res[1] < 1.0 ? par[P_DF] * res[1] : 1.0;
+      res[side] = small_ftb(sb, sd, k);
     } else {
-      a.sc[(long long)p * NSLOT + S_TX] = res[0];
-      a.sc[(long long)p * NSLOT + S_TY] = res[1];
+      double t = 1.0;
+      while (true) {
+        const bool pass = small_margin(sb, sd, k, t, par[P_RATIO]);
+        if (pass || t < par[P_MINSTEP] || !(t > 0.0)) break;  // t reaches 0 after ~1100 steps
+        t *= par[P_GAMMA];
+      }
+      res[side] = t;
     }
+  }
+  if (op == 0) {
+    a.sc[(long long)p * NSLOT + S_TX] = res[0] < 1.0 ? par[P_PF] * res[0] : 1.0;
+    a.sc[(long long)p * NSLOT + S_TY] = res[1] < 1.0 ? par[P_DF] * res[1] : 1.0;
+  } else {
+    a.sc[(long long)p * NSLOT + S_TX] = res[0];
+    a.sc[(long long)p * NSLOT + S_TY] = res[1];
   }
 }
 
@@ -1656,12 +1630,12 @@ int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t 
 // one launch per register width; members outside a width's range exit at once
 int launch_step_waves(const Arrays& a, const int* list, int n, int op, hipStream_t st) {
   if (n <= 0) return kOk;
-  const dim3 grid((unsigned)ceil_div(n, kStepWaves)), grid2((unsigned)ceil_div(2LL * n, kStepWaves)), block(NT);
-  hipLaunchKernelGGL(step_wave_kernel<16>, grid, block, 0, st, a, list, n, op);  // k <= 2: closed forms
+  const dim3 block(NT);
+  hipLaunchKernelGGL(step_small_kernel, dim3((unsigned)ceil_div(n, NT)), block, 0, st, a, list, n, op);  // k <= 2
   hipLaunchKernelGGL(step_quad_kernel<16>, dim3((unsigned)n), block, 0, st, a, list, n, op);
   hipLaunchKernelGGL(step_quad_kernel<32>, dim3((unsigned)n), block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_wave_kernel<48>, grid2, block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_wave_kernel<64>, grid2, block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_quad_wide_kernel<48>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
+  hipLaunchKernelGGL(step_quad_wide_kernel<64>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }
