@@ -1,0 +1,88 @@
+"""Bit-exact Schur assembly on the reference's integer-pattern dense problem.
+
+docs/evidence/bench/threading/benchmark_schur_scheduler.jl:23-54 builds
+A_i[r, c] = (mod(17 i + 31 l + 13 r + 7 c, 41) - 20) / 23 (1-based, r <= c,
+mirrored).  Scaled by 23 the entries are integers in [-20, 20], and at
+X = Y = I (the cold start, interior_point.jl:710-721) the Schur matrix is the
+Gram matrix <A_i, A_j>: every product and every partial sum is an integer
+below 2^53, so any correct FP64 evaluation order gives it exactly.  The GPU
+assembly (panel products with L_X^-1 = M_Y = I, split-K Gram, tile order,
+ragged and triangular tiles) must therefore equal the CPU Gram bit for bit,
+at the benchmark's own size too: a wrong index, tile offset or dropped partial
+changes integers.  The FP32 and BF16x3 Gram kernels are exact on the same
+data while every partial sum stays below 2^24 (k <= 130 here): integers up to
+256 are exact in bf16, so the lo plane is zero and H'H carries the whole sum.
+"""
+
+import numpy as np
+import pytest
+
+import sdpx_oracle as O
+from sdpx_mi355x import MI355XDenseBackend, ingest
+
+from _helpers import rel_fro_lower
+
+
+def scheduler_panel(k: int, m: int, block: int = 1) -> np.ndarray:
+    """23 x the A_i of benchmark_schur_scheduler.jl:23-54, as an (m, k, k) array of integers."""
+    i = np.arange(1, m + 1)[:, None, None]
+    r = np.arange(1, k + 1)[None, :, None]
+    c = np.arange(1, k + 1)[None, None, :]
+    lo, hi = np.minimum(r, c), np.maximum(r, c)  # the formula is evaluated on the upper triangle, then mirrored
+    return (np.mod(17 * i + 31 * block + 13 * lo + 7 * hi, 41) - 20).astype(np.float64)
+
+
+def test_oracle_cold_start_gram_is_exact():
+    k, m = 7, 12
+    A = scheduler_panel(k, m)
+    prob = O.ingest(np.ones(m), [A], [np.zeros((k, k))])
+    S = O.schur_build(prob, [np.eye(k)], [np.eye(k)])
+    V = A.reshape(m, k * k)
+    assert np.array_equal(np.tril(S), np.tril(V @ V.T))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,m", [(65, 300), (130, 700), (400, 4000)])
+def test_gpu_cold_start_gram_is_bit_exact(gpu_lib, k, m):
+    A = scheduler_panel(k, m)
+    prob = ingest(np.ones(m), [A], [np.zeros((k, k))])
+    b = MI355XDenseBackend(prob)
+    try:
+        I = np.eye(k)
+        b.set_iterate(np.zeros(m), [I], [I])
+        _, _, ok = b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        assert ok
+        b.schur_build()
+        S = np.tril(b.get_schur())
+    finally:
+        b.close()
+    V = A.reshape(m, k * k)
+    ref = np.tril(V @ V.T)
+    bad = np.argwhere(S != ref)
+    print(f"k={k} m={m}: {len(bad)} entries differ; rel {rel_fro_lower(S, ref):.1e}", flush=True)
+    assert len(bad) == 0, bad[:5]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("k,m", [(65, 300), (130, 700)])
+def test_gpu_reduced_precision_cold_start_gram_is_bit_exact(gpu_lib, k, m, precision):
+    A = scheduler_panel(k, m)
+    assert 400.0 * k * k < 2.0 ** 24  # every FP32 partial sum is an exact integer
+    prob = ingest(np.ones(m), [A], [np.zeros((k, k))])
+    b = MI355XDenseBackend(prob)
+    try:
+        I = np.eye(k)
+        b.set_iterate(np.zeros(m), [I], [I])
+        _, _, ok = b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        assert ok
+        b.set_precision(precision)
+        b.schur_build()
+        S = np.tril(b.get_schur())
+    finally:
+        b.close()
+    V = A.reshape(m, k * k)
+    ref = np.tril(V @ V.T)
+    bad = np.argwhere(S != ref)
+    print(f"precision {precision} k={k} m={m}: {len(bad)} entries differ", flush=True)
+    assert len(bad) == 0, bad[:5]
