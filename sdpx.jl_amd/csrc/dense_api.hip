@@ -604,11 +604,15 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
     const long long dn = (long long)b.nblk * kCholNb * kCholNb;
     int s = kOk;
     if ((s = dalloc(h, &b.Av, (size_t)(b.k2 * m)))) return fail(s);
-    double** mats[] = {&b.C, &b.X, &b.Y, &b.LX, &b.MY, &b.LXinv, &b.Xinv, &b.P, &b.R, &b.dX, &b.dY, &b.W1, &b.W2};
+    double** mats[] = {&b.C, &b.X, &b.Y, &b.LXinv, &b.Xinv, &b.P, &b.R, &b.dX, &b.dY, &b.W1, &b.W2};
     for (double** p : mats)
       if ((s = dalloc(h, p, (size_t)b.k2))) return fail(s);
-    if ((s = dalloc(h, &b.dinvX, (size_t)dn))) return fail(s);
-    if ((s = dalloc(h, &b.dinvY, (size_t)dn))) return fail(s);
+    // L_X and M_Y (and their diagonal-block inverses) side by side: both
+    // factors run as one batch of two
+    if ((s = dalloc(h, &b.LX, (size_t)(2 * b.k2)))) return fail(s);
+    b.MY = b.LX + b.k2;
+    if ((s = dalloc(h, &b.dinvX, (size_t)(2 * dn)))) return fail(s);
+    b.dinvY = b.dinvX + dn;
     if (b.k > 2) {
       if ((s = dalloc(h, &b.trial, (size_t)(b.k2 * kTrialCap)))) return fail(s);
       if ((s = dalloc(h, &b.trial_dinv, (size_t)(dn * kTrialCap)))) return fail(s);
@@ -827,8 +831,8 @@ int sdpx_dense_residuals(sdpx_dense_t* h, const double* mu, int affine, int fact
       Block& b = h->blk[l];
       SDPX_HIP(hipMemcpyAsync(b.LX, b.X, sizeof(double) * b.k2, hipMemcpyDeviceToDevice, h->stream));
       SDPX_HIP(hipMemcpyAsync(b.MY, b.Y, sizeof(double) * b.k2, hipMemcpyDeviceToDevice, h->stream));
-      SDPX_CHECK(potrf_f64(b.LX, b.k, b.k2, b.k, 1, b.dinvX, 0, h->dinfo + 2 * l, h->stream));
-      SDPX_CHECK(potrf_f64(b.MY, b.k, b.k2, b.k, 1, b.dinvY, 0, h->dinfo + 2 * l + 1, h->stream));
+      // L_X and M_Y in one batch of two (MY = LX + k2, dinvY = dinvX + dn; info 2l, 2l+1)
+      SDPX_CHECK(potrf_f64(b.LX, b.k, b.k2, b.k, 2, b.dinvX, b.dinvY - b.dinvX, h->dinfo + 2 * l, h->stream));
     }
     nint = 2 * L;
   }
