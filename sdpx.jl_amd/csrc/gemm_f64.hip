@@ -531,8 +531,13 @@ int gemm_f64(const GemmArgs& g0, hipStream_t stream, int split_k, double* worksp
   }
   // Small problems use 64 x 64 tiles so that k x k block products launch enough groups.
   // (and so does a product with a side of at most 64, which would leave half of a 128-wide tile idle)
+  // A full-output product that would launch fewer 128 x 128 groups than half
+  // the CUs (the k x k block products) takes 64 x 64 tiles too: four times the
+  // groups for the same K loop (every entry's K order is unchanged).
+  const long long tiles128 = (long long)ceil_div(g.M, 128) * ceil_div(g.N, 128) * g.batch;
   const bool small = (g.flags & kGemmSmallTiles) || (g.M <= 96 && g.N <= 96) || g.M <= 64 || g.N <= 64 ||
-                     ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256);
+                     ((long long)g.M * g.N * g.batch < 256LL * 128 * 128 && g.M <= 256 && g.N <= 256) ||
+                     (!(g.flags & kGemmLowerOut) && tiles128 < 128);
   if (small) {
     return vec ? dispatch_trans<64, 64, true>(g, stream, split_k, workspace, ta, tb, after_main)
                : dispatch_trans<64, 64, false>(g, stream, split_k, workspace, ta, tb, after_main);
