@@ -294,14 +294,20 @@ __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16*
       bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][wn0 + 16 * j + fr][fk]);
       bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][wn0 + 16 * j + fr][fk]);
     }
+    // the three products of one accumulator are issued 16 MFMAs apart, so that
+    // consecutive MFMAs never depend on each other (same order per accumulator)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
-      }
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah[i], acc[i][j], 0, 0, 0);
   };
   if (kBegin < kEnd) {
     load(kBegin);
