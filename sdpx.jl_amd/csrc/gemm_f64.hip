@@ -156,7 +156,11 @@ constexpr int core_lds() {
 }
 
 // The tile computation on caller-provided LDS (core_lds<...>() doubles).
-template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK>
+// FENCE keeps every MFMA of a slice ahead of the stores of the next one, so
+// the prefetch loads have the whole slice to arrive (left to the compiler,
+// half the MFMAs are moved past the barrier and the stores wait on the loads
+// after 32 of 64; worth 3% on long-K products, 1% worse on short K ranges).
+template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK, bool FENCE = false>
 __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restrict__ A, long long lda,
                                               const double* __restrict__ B, long long ldb, int M, int N, int kBegin,
                                               int kEnd, int m0, int n0, bool lower_out, double* C, long long ldc,
@@ -230,6 +234,7 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
       la.load(ra, A, lda, k0 + KS);
       lb.load(rb, B, ldb, k0 + KS);
       compute(cur);
+      if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
       store(cur ^ 1);
       __syncthreads();
       cur ^= 1;
@@ -277,7 +282,11 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
                                      brows, bstride);
 }
 
-template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK>
+// GRAM (the Schur Gram kernel): a partial last tile row with at most 64 rows
+// in range runs as flat strips of the same MFMA count per wave, 32 x 256 (two
+// tiles in one group; the group of the odd tile exits) or 64 x 128, instead
+// of 128-row tiles that compute 2 to 4 times the rows they write.
+template <int BM, int BN, bool TA, bool TB, bool VEC, int KS = BK, bool GRAM = false>
 __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, double* ws, double* lds) {
   const int b = blockIdx.y;
   if (g.skip && g.skip[(long long)b * g.skip_stride] != 0) return;
@@ -343,7 +352,23 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   // split-K partials go to W (M x M per split, no beta); otherwise straight to C
   const bool part = split > 1;
   double* out = part ? ws + ((long long)b * split + s) * (long long)g.M * g.N : g.C + (long long)b * g.strideC;
-  gemm_core_lds<BM, BN, TA, TB, VEC, KS>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
+  if constexpr (GRAM) {
+    const int rows = g.M - m0;  // < BM only in the last tile row
+    if (rows <= 32) {
+      if (tj & 1) return;  // covered by the strip of tile column tj - 1
+      gemm_core_lds<32, 2 * BN, TA, TB, VEC, KS, true>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
+                                                       lower_out, out, part ? (long long)g.M : g.ldc, g.alpha,
+                                                       part ? 0.0 : g.beta, 0, 0);
+      return;
+    }
+    if (rows <= 64) {
+      gemm_core_lds<64, BN, TA, TB, VEC, KS, true>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
+                                                   lower_out, out, part ? (long long)g.M : g.ldc, g.alpha,
+                                                   part ? 0.0 : g.beta, 0, 0);
+      return;
+    }
+  }
+  gemm_core_lds<BM, BN, TA, TB, VEC, KS, GRAM>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0, lower_out, out,
                                          part ? (long long)g.M : g.ldc, g.alpha, part ? 0.0 : g.beta,
                                          part ? 0 : g.block_rows, g.block_stride);
 }
@@ -380,10 +405,13 @@ constexpr int kGramWgPerCu = 2;
 
 // The Schur Gram product S = Phat' Phat (lower tiles) under its own symbol,
 // so profiles attribute the dominant kernel of the Newton step unambiguously.
+constexpr int kGramLds = std::max({core_lds<128, 128, true, false, kGramKs>(), core_lds<32, 256, true, false, kGramKs>(),
+                                   core_lds<64, 128, true, false, kGramKs>()});
+static_assert(kGramLds * 8 * kGramWgPerCu <= 160 * 1024, "two Gram groups per CU");
 template <bool VEC>
 __global__ __launch_bounds__(256, kGramWgPerCu) void schur_gram_f64_kernel(GemmArgs g, int split, double* ws) {
-  __shared__ __align__(16) double lds[core_lds<128, 128, true, false, kGramKs>()];
-  gemm_f64_body<128, 128, true, false, VEC, kGramKs>(g, split, ws, lds);
+  __shared__ __align__(16) double lds[kGramLds];
+  gemm_f64_body<128, 128, true, false, VEC, kGramKs, true>(g, split, ws, lds);
 }
 
 // Trailing update of a blocked Cholesky, A22 -= A21 A21' over the lower tiles

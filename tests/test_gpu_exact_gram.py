@@ -7,7 +7,8 @@ X = Y = I (the cold start, interior_point.jl:710-721) the Schur matrix is the
 Gram matrix <A_i, A_j>: every product and every partial sum is an integer
 below 2^53, so any correct FP64 evaluation order gives it exactly.  The GPU
 assembly (panel products with L_X^-1 = M_Y = I, split-K Gram, tile order,
-ragged and triangular tiles) must therefore equal the CPU Gram bit for bit,
+ragged and triangular tiles, the flat strips of a short last tile row) must
+therefore equal the CPU Gram bit for bit,
 at the benchmark's own size too: a wrong index, tile offset or dropped partial
 changes integers.  The FP32 and BF16x3 Gram kernels are exact on the same
 data while every partial sum stays below 2^24 (k <= 130 here): integers up to
@@ -42,8 +43,11 @@ def test_oracle_cold_start_gram_is_exact():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k,m", [(65, 300), (130, 700), (400, 4000)])
+@pytest.mark.parametrize("k,m", [(65, 300), (130, 700), (400, 4000), (65, 160), (65, 276), (33, 20), (33, 64)])
 def test_gpu_cold_start_gram_is_bit_exact(gpu_lib, k, m):
+    """m mod 128 picks the shape of the last tile row: 32 x 256 strips (m = 160,
+    276, 4000, 20: the last tile column even or odd), 64 x 128 strips (300, 700,
+    64) or full tiles."""
     A = scheduler_panel(k, m)
     prob = ingest(np.ones(m), [A], [np.zeros((k, k))])
     b = MI355XDenseBackend(prob)
