@@ -282,6 +282,107 @@ __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long lon
                                      brows, bstride);
 }
 
+// Diagonal 128 x 128 tile of the Gram product S = P' P (m0 == n0, all rows in
+// range).  Both operands are the same K slice of P, staged once, and only the
+// 36 of the 64 16 x 16 blocks on or below the diagonal are computed, nine per
+// wave, so the group takes 9/16 of a full tile's MFMA time.  Blocks are named
+// by their 16-row block index (0..7) along each side:
+//   waves 0-2 (SQ): 3 x 3 squares, rows r0..r0+2 by columns c0..c0+2 with
+//                   (r0, c0) = (5, 0), (5, 3), (2, 0);
+//   wave 3:         the 2 x 2 lower triangles {(a,a), (a+1,a), (a+1,a+1)}
+//                   at a = 0, 3, 6.
+// Each accumulator sees the same operands in the same K order as in the full
+// tile, so the written entries are bitwise those of the general path.
+template <bool VEC, int KS, bool SQ>
+__device__ __forceinline__ void gram_diag_core(double* lds, const double* __restrict__ P, long long ld, int M,
+                                               int kBegin, int kEnd, int m0, double* C, long long ldc, double alpha,
+                                               double beta, int wave) {
+  constexpr int LA = lds_size<128, true, KS>();
+  double(*As)[LA] = reinterpret_cast<double(*)[LA]>(lds);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  // fragment block indices fb[0..5] and the nine (row, column) pairs into them
+  int fbk[6];
+  if (SQ) {
+    const int r0 = wave == 2 ? 2 : 5, c0 = wave == 1 ? 3 : 0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      fbk[t] = r0 + t;
+      fbk[3 + t] = c0 + t;
+    }
+  } else {
+    fbk[0] = 0, fbk[1] = 1, fbk[2] = 3, fbk[3] = 4, fbk[4] = 6, fbk[5] = 7;
+  }
+  constexpr int pr_sq[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2}, pc_sq[9] = {3, 4, 5, 3, 4, 5, 3, 4, 5};
+  constexpr int pr_tr[9] = {0, 1, 1, 2, 3, 3, 4, 5, 5}, pc_tr[9] = {0, 0, 1, 2, 2, 3, 4, 4, 5};
+
+  double4_t acc[9];
+#pragma unroll
+  for (int b = 0; b < 9; ++b) acc[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  constexpr int NE = 128 * KS / 256;
+  double ra[NE];
+  TileLoader<128, true, VEC, KS> la;
+  la.init(ld, m0, M, tid);
+  auto load = [&](int k0) {
+    if (k0 + KS <= kEnd) la.load(ra, P, ld, k0);
+    else la.load_tail(ra, P, ld, k0, kEnd, tid);
+  };
+  auto compute = [&](int cur) {
+#pragma unroll
+    for (int ks = 0; ks < KS / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double f[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) f[q] = As[cur][lds_at<128, true, KS>(kr, fbk[q] * 16 + (lane & 15))];
+#pragma unroll
+      for (int b = 0; b < 9; ++b) {
+        const int r = SQ ? pr_sq[b] : pr_tr[b], c = SQ ? pc_sq[b] : pc_tr[b];
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[c], f[r], acc[b], 0, 0, 0);
+      }
+    }
+  };
+
+  if (kBegin < kEnd) {
+    load(kBegin);
+    tile_store<128, true, VEC, KS>(ra, As[0], tid);
+    __syncthreads();
+    int cur = 0;
+    const int kFull = kBegin + ((kEnd - kBegin) / KS) * KS;
+    int k0 = kBegin;
+    for (; k0 + 2 * KS <= kFull; k0 += KS) {
+      la.load(ra, P, ld, k0 + KS);
+      compute(cur);
+      __builtin_amdgcn_sched_barrier(0);
+      tile_store<128, true, VEC, KS>(ra, As[cur ^ 1], tid);
+      __syncthreads();
+      cur ^= 1;
+    }
+    if (k0 + KS < kEnd) {
+      load(k0 + KS);
+      compute(cur);
+      tile_store<128, true, VEC, KS>(ra, As[cur ^ 1], tid);
+      __syncthreads();
+      cur ^= 1;
+    }
+    compute(cur);
+  }
+
+#pragma unroll
+  for (int b = 0; b < 9; ++b) {
+    const int rb = fbk[SQ ? pr_sq[b] : pr_tr[b]], cb = fbk[SQ ? pc_sq[b] : pc_tr[b]];
+    const int row = m0 + rb * 16 + (lane & 15);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int col = m0 + cb * 16 + (lane >> 4) + 4 * rr;
+      if (row >= col) {
+        double* cp = C + row + (long long)col * ldc;
+        *cp = (beta == 0.0) ? alpha * acc[b][rr] : alpha * acc[b][rr] + beta * (*cp);
+      }
+    }
+  }
+}
+
 // GRAM (the Schur Gram kernel): a partial last tile row with at most 64 rows
 // in range runs as flat strips of the same MFMA count per wave, 32 x 256 (two
 // tiles in one group; the group of the odd tile exits) or 64 x 128, instead
@@ -365,6 +466,14 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
       gemm_core_lds<64, BN, TA, TB, VEC, KS, true>(lds, A, g.lda, B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
                                                    lower_out, out, part ? (long long)g.M : g.ldc, g.alpha,
                                                    part ? 0.0 : g.beta, 0, 0);
+      return;
+    }
+    if (ti == tj && rows >= BM && A == B && g.lda == g.ldb) {
+      const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+      const long long ldo = part ? (long long)g.M : g.ldc;
+      const double bo = part ? 0.0 : g.beta;
+      if (wave < 3) gram_diag_core<VEC, KS, true>(lds, A, g.lda, g.M, kBegin, kEnd, m0, out, ldo, g.alpha, bo, wave);
+      else gram_diag_core<VEC, KS, false>(lds, A, g.lda, g.M, kBegin, kEnd, m0, out, ldo, g.alpha, bo, wave);
       return;
     }
   }
