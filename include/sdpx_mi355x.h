@@ -165,9 +165,11 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split);
 /* Plan-time autotuner for the Schur build of this handle's shape (no
  * counterpart in the reference, whose BLAS calls take no tiling choices): runs
  * and times on the device each of the two P-hat panel products on 128x128,
- * 64x64, 128x64 or 64x128 tiles and the Schur Gram product at three split-K
- * factors and three tile-order super-block sides, and keeps the fastest.
- * It also picks the bisection levels per batched trial round of
+ * 64x64, 128x64 or 64x128 tiles and the Schur Gram product at up to five
+ * split-K factors and three tile-order super-block sides, and keeps the
+ * fastest.  The products run on the handle's A_i with dense lower-triangular
+ * stand-ins for L_X^-1 and M_Y (restored afterwards), so the operands carry
+ * values like a solve's.  It also picks the bisection levels per batched trial round of
  * sdpx_dense_fraction_to_boundary (4..7; the result is the same for any
  * value).  Overwrites the panel, Schur and trial buffers (call before solving
  * or between solves).  out (optional, 20 doubles): tile shape code of each

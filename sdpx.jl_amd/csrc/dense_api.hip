@@ -152,6 +152,24 @@ __global__ void identity_trials_kernel(long long k2, int k, int B, double* trial
   }
 }
 
+// Dense lower-triangular k x k stand-in for L_X^-1 and M_Y while the autotuner
+// times the panel and Gram products: entries (0.5 + u) / sqrt(k) with u in
+// [0, 1) from an integer hash of (i, j), zeros above the diagonal.  With these
+// the panels and the Gram operands hold O(|A_i|) values, as in a solve (the
+// power drawn, and so the clock, depends on the operand bits: products of
+// zeros run several percent faster than the same launch on real data).
+__global__ void tune_lower_kernel(int k, double* a) {
+  const long long n = (long long)k * k;
+  for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+    const int i = (int)(idx % k), j = (int)(idx / k);
+    unsigned h = (unsigned)i * 2654435761u ^ ((unsigned)j + 0x9e3779b9u) * 40503u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    a[idx] = i >= j ? (0.5 + (double)(h >> 8) * (1.0 / 16777216.0)) * rsqrt((double)k) : 0.0;
+  }
+}
+
 // corrector target (threaded.jl:839-854 / 943-1003) from the block dot in *dotv:
 //   mode 0 (fixed):     max(rl < 1 ? rl^2 : rl, beta) * mu,  rl = dot / mu / k
 //   mode 1 (local):     sigma * dot / k
@@ -1019,7 +1037,20 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     SDPX_HIP(hipEventElapsedTime(ms, ev[0], ev[1]));
     return kOk;
   };
+  // L_X^-1 and M_Y are replaced by dense stand-ins while timing, and restored
+  double* saved = nullptr;
+  const size_t kk = (size_t)k * k;
+  if (hipMalloc(&saved, 2 * kk * sizeof(double)) != hipSuccess) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    set_last_error("sdpx_dense_autotune: hipMalloc failed");
+    return kOutOfMemory;
+  }
   auto body = [&]() -> int {
+    SDPX_HIP(hipMemcpyAsync(saved, b.LXinv, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(saved + kk, b.MY, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(tune_lower_kernel, dim3(ceil_div((long long)kk, 256)), dim3(256), 0, h->stream, k, b.LXinv);
+    hipLaunchKernelGGL(tune_lower_kernel, dim3(ceil_div((long long)kk, 256)), dim3(256), 0, h->stream, k, b.MY);
+    SDPX_HIP(hipGetLastError());
     h->tune_panel_ms = 0.f;
     // the two panel products, each on four tile shapes
     const unsigned shapes[4] = {0u, (unsigned)kGemmSmallTiles, (unsigned)kGemmTallTiles, (unsigned)kGemmWideTiles};
@@ -1133,8 +1164,18 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     return kOk;
   };
   const int rc = body();
+  // L_X^-1 and M_Y back (also after a failed timing)
+  const bool restored =
+      hipMemcpyAsync(b.LXinv, saved, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess &&
+      hipMemcpyAsync(b.MY, saved + kk, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess &&
+      hipStreamSynchronize(h->stream) == hipSuccess;
+  (void)hipFree(saved);
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc != kOk) return rc;
+  if (!restored) {
+    set_last_error("sdpx_dense_autotune: restoring L_X^-1 and M_Y failed");
+    return kHipError;
+  }
   if (out) {
     for (int prod = 0; prod < 2; ++prod) {
       const unsigned t = h->panel_tiles[prod];

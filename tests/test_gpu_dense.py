@@ -56,7 +56,7 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     X, Y = _random_pd(rng, k), _random_pd(rng, k)
     b = MI355XDenseBackend(prob)
     default_split = b.schur_split()
-    t = b.autotune()  # before the iterate: timings do not depend on values
+    t = b.autotune()  # before the iterate
     print(t, flush=True)
     assert all(sh in ("128x128", "64x64", "128x64", "64x128") for sh in t["panel_tiles"])
     assert t["tile_group"] in (4, 8, 16)
@@ -76,6 +76,31 @@ def test_autotuned_schur_matches_oracle(gpu_lib, k, m):
     MY, _ = O.kchol(Y)
     assert _rel_fro_lower(S, O.schur_build(oprob, [LX], [MY])) <= 1e-12
     b.close()
+
+
+def test_autotune_restores_the_iterate_factors(gpu_lib):
+    """The autotuner times its candidates with stand-ins for L_X^-1 and M_Y and puts
+    the iterate's back: with the earlier tiling re-applied, a Schur build from the
+    same iterate (no new residual pass) is bitwise the one before tuning."""
+    k, m = 130, 300
+    rng = np.random.default_rng(12)
+    g, prob, _ = _dense_case(k, m, seed=7)
+    X, Y = _random_pd(rng, k), _random_pd(rng, k)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_tiling(("64x64", "64x64"), split=b.schur_split(), tile_group=8)
+        split = b.schur_split()
+        b.set_iterate(g.x0, [X], [Y])
+        _, _, ok = b.compute_residuals([0.1], affine=True, factor=True)
+        assert ok
+        b.schur_build()
+        S0 = np.tril(b.get_schur())
+        b.autotune()
+        b.set_tiling(("64x64", "64x64"), split=split, tile_group=8)
+        b.schur_build()
+        assert np.array_equal(np.tril(b.get_schur()), S0)
+    finally:
+        b.close()
 
 
 @pytest.mark.parametrize("shapes", [("64x64", "64x64"), ("128x64", "64x128"), ("64x128", "128x64"),
