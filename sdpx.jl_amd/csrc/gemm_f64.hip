@@ -272,14 +272,14 @@ __device__ __forceinline__ void gemm_core_lds(double* lds, const double* __restr
   }
 }
 
-template <int BM, int BN, bool TA, bool TB, bool VEC>
+template <int BM, int BN, bool TA, bool TB, bool VEC, bool FENCE = false>
 __device__ __forceinline__ void gemm_core(const double* __restrict__ A, long long lda, const double* __restrict__ B,
                                           long long ldb, int M, int N, int kBegin, int kEnd, int m0, int n0,
                                           bool lower_out, double* C, long long ldc, double alpha, double beta,
                                           long long brows = 0, long long bstride = 0) {
   __shared__ __align__(16) double lds[core_lds<BM, BN, TA, TB>()];
-  gemm_core_lds<BM, BN, TA, TB, VEC>(lds, A, lda, B, ldb, M, N, kBegin, kEnd, m0, n0, lower_out, C, ldc, alpha, beta,
-                                     brows, bstride);
+  gemm_core_lds<BM, BN, TA, TB, VEC, BK, FENCE>(lds, A, lda, B, ldb, M, N, kBegin, kEnd, m0, n0, lower_out, C, ldc,
+                                                alpha, beta, brows, bstride);
 }
 
 // Diagonal 128 x 128 tile of the Gram product S = P' P (m0 == n0, all rows in
@@ -495,6 +495,8 @@ __global__ __launch_bounds__(256, 2) void gemm_grouped_kernel(const GroupGemm* _
   if (g.flags & kGemmBLower) kBegin = max(kBegin, n0);
   if (g.flags & kGemmAUpper) kBegin = max(kBegin, m0);
   kBegin = (kBegin / BK) * BK;
+  // (the slice fence of the Gram kernel measured 0.7% slower on the batched
+  // Schur Gram products here: 64 x 64 tiles issue 16 MFMAs per slice)
   gemm_core<BM, BN, TA, TB, VEC>(g.A, g.lda, g.B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
                                  (g.flags & kGemmLowerOut) != 0, g.C, g.ldc, g.alpha, g.beta);
 }
