@@ -82,6 +82,9 @@ struct GemmArgs {
   const int* skip;        // optional per-batch flag: skip the batch entry when *skip != 0
   long long skip_stride;  // stride between per-batch skip flags
   const int2* tile_order; // optional (ti, tj) visiting order of the lower tiles (kGemmLowerOut)
+  // optional Schur Gram work list (kGemmSchurGram), one entry per work item in
+  // the XCD-remapped launch order: x = split index, y = ti << 16 | tj
+  const int2* item_order;
   // Optional blocked output: when block_rows > 0, C(row, col) is stored at
   // C + row % block_rows + (row / block_rows) * block_stride + col * ldc
   // (the rows of a vertically stacked product go back to separate blocks).
@@ -114,6 +117,13 @@ int gemm_f64_grouped(const GroupGemm* groups, const GroupTile* tiles, int ntiles
 // Lower-tile visiting order in square super-blocks of `group` tile rows and
 // columns (row-major inside), so that co-resident work-groups share panels.
 void lower_tile_order(int nt, int group, int2* out);
+// Work list of the FP64 Schur Gram launch for an m x m lower output on 128-row
+// tiles, `group` super-blocks and `split` K ranges (nt(nt+1)/2 * split entries,
+// GemmArgs::item_order layout).  Each XCD's contiguous run of items lists its
+// full tiles first and the shorter items (diagonal tiles, strips of a short last
+// tile row, groups that exit) last, so that the co-resident groups of an XCD,
+// all of one length, start and finish together and share Phat panels in L2.
+void gram_item_order(int m, int group, int split, int2* out);
 
 // Launches the GEMM on `stream`. `split_k` > 1 requires `workspace` of
 // split_k * M * N * batch doubles; partial sums are reduced in a fixed order

@@ -212,7 +212,8 @@ struct sdpx_dense {
   double *T = nullptr, *Ph = nullptr;  // Schur panels (max over blocks)
   double* splitws = nullptr;
   int split = 1;
-  int2* tile_order = nullptr;  // Schur Gram lower-tile visiting order
+  int2* tile_order = nullptr;  // Schur Gram lower-tile visiting order (FP32 / BF16x3 kernels)
+  int2* gram_items = nullptr;  // FP64 Schur Gram work list (tiles x split_max entries)
   // mixed precision (sdpx_dense_set_precision): FP32 copy of the Schur panel
   // (precision 1) or its bf16 hi / lo planes (precision 2)
   int precision = 0;
@@ -268,6 +269,20 @@ int dmalloc(sdpx_dense_t* h, void** p, size_t bytes) {
 template <typename T>
 int dalloc(sdpx_dense_t* h, T** p, size_t count) {
   return dmalloc(h, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+// Gram visiting orders for `group` x `group` super-blocks and `split` K ranges:
+// the lower-tile order of the FP32 / BF16x3 kernels and the FP64 kernel's work
+// list (full tiles of each XCD's run first, shorter items last).
+int set_gram_order(sdpx_dense_t* h, int group, int split) {
+  const int nt = ceil_div(h->m, 128);
+  std::vector<int2> order((size_t)nt * (nt + 1) / 2), items(order.size() * (size_t)split);
+  lower_tile_order(nt, group, order.data());
+  gram_item_order(h->m, group, split, items.data());
+  SDPX_HIP(hipStreamSynchronize(h->stream));  // no launch in flight reads the old lists
+  SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+  SDPX_HIP(hipMemcpy(h->gram_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return kOk;
 }
 
 int sync_scalars(sdpx_dense_t* h, int ndoubles, int nints = 0) {
@@ -685,13 +700,9 @@ int sdpx_dense_create(int device, int L, const int* k, int m, sdpx_dense_t** out
     if (h->split_max > 1 && (s = dalloc(h, &h->splitws, (size_t)(h->split_max * mm2)))) return fail(s);
     // Gram tiles visited in 8 x 8 super-blocks: the co-resident groups of one
     // XCD then read few distinct Phat column panels, shared in its L2.
-    std::vector<int2> order((size_t)tiles);
-    lower_tile_order((int)nt, 8, order.data());
-    if ((s = dalloc(h, &h->tile_order, order.size()))) return fail(s);
-    if (hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
-      set_last_error("hipMemcpy(tile order) failed");
-      return fail(kHipError);
-    }
+    if ((s = dalloc(h, &h->tile_order, (size_t)tiles))) return fail(s);
+    if ((s = dalloc(h, &h->gram_items, (size_t)(tiles * h->split_max)))) return fail(s);
+    if ((s = set_gram_order(h, 8, h->split))) return fail(s);
   }
   h->gemv_part_cap = std::max<long long>(maxk2 * 64, 4096);
   if ((s = dalloc(h, &h->gemv_part, (size_t)h->gemv_part_cap))) return fail(s);
@@ -905,6 +916,7 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);
     s.tile_order = h->tile_order;
+    s.item_order = h->gram_items;
     if (h->precision == 2) {
       // mixed precision: Phat split into bf16 hi + lo planes, Gram product as three
       // bf16 MFMA products accumulated in FP32, S kept in FP64
@@ -1097,28 +1109,25 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
       GemmArgs g = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, 0.0, h->S, m,
                         kGemmTransA | kGemmLowerOut | kGemmSchurGram);
       g.tile_order = h->tile_order;
+      g.item_order = h->gram_items;
       return gemm_f64(g, h->stream, sp, h->splitws);
     };
-    const int nt = ceil_div(m, 128);
-    std::vector<int2> order((size_t)nt * (nt + 1) / 2);
     const int s0 = h->split_created;
     int best_split = s0, best_group = 8;
     bool first = true;
     // the create-time choice (split s0, 8 x 8 super-blocks) is timed first and is
     // replaced only by a candidate at least 1% faster (3% for another tile
     // order): run-to-run noise between near-equal candidates does not flip the
-    // choice.  The kernel is MFMA-bound, so the orders time alike while their L2
-    // misses differ: at n = 400, m = 4000 the 64 co-resident groups of an XCD
-    // span one 8 x 8 super-block (16 panels of Phat) or four 4 x 4 ones (up to
-    // 32), and the Gram fetched 39 GB per launch against 129 GB.
+    // choice.  The kernel is MFMA-bound, so the orders time alike; the 64
+    // co-resident groups of an XCD span one 8 x 8 super-block (16 panels of
+    // Phat) or up to four 4 x 4 ones (32), and fewer panels mean fewer L2 misses.
     for (int group : {8, 4, 16}) {
-      lower_tile_order(nt, group, order.data());
-      SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
       int last = 0;
       for (int sp : {s0, (3 * s0) / 4, s0 / 2, (3 * s0) / 2, 2 * s0}) {
         if (sp > h->split_max) continue;
         if (sp < 1 || sp == last) continue;
         last = sp;
+        SDPX_CHECK(set_gram_order(h, group, sp));
         float ms = 0.f;
         SDPX_CHECK(time_ms([&]() -> int { return gram(sp); }, &ms));
         if (first || ms < (group == 8 ? 0.99f : 0.97f) * h->tune_gram_ms) {
@@ -1131,8 +1140,7 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     }
     h->split = best_split;
     h->tile_group = best_group;
-    lower_tile_order(nt, best_group, order.data());
-    SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
+    SDPX_CHECK(set_gram_order(h, best_group, best_split));
     // fraction-to-the-boundary rounds: wall time of one batched trial round
     // (trial fill, batched Cholesky, read-back, host wait) at the batch sizes
     // of D = 4..7 levels per round; a search of ~53 levels costs round 0
@@ -1212,11 +1220,7 @@ int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, in
   h->split = split;
   h->tile_group = group;
   h->ftb_depth = ftb_depth;
-  const int nt = ceil_div(h->m, 128);
-  std::vector<int2> order((size_t)nt * (nt + 1) / 2);
-  lower_tile_order(nt, group, order.data());
-  SDPX_HIP(hipMemcpy(h->tile_order, order.data(), order.size() * sizeof(int2), hipMemcpyHostToDevice));
-  return kOk;
+  return set_gram_order(h, group, split);
 }
 
 int sdpx_dense_predictor_rhs(sdpx_dense_t* h) {

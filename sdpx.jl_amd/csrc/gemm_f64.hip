@@ -17,6 +17,8 @@
 #include "chol_diag.hpp"
 #include "common.hpp"
 
+#include <vector>
+
 namespace sdpx {
 namespace {
 
@@ -403,12 +405,17 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
   const int q = total / kNumXcd, rq = total % kNumXcd;
   const int lin = xcd * q + (xcd < rq ? xcd : rq) + slot;
   const int per_split = total / split;
-  const int s = lin / per_split;
+  int s = lin / per_split;
   const int tile = lin % per_split;
 
   int ti, tj;
   const bool lower_out = (g.flags & kGemmLowerOut) != 0;
-  if (lower_out && g.tile_order) {
+  if (GRAM && g.item_order) {
+    const int2 it = g.item_order[lin];
+    s = it.x;
+    ti = it.y >> 16;
+    tj = it.y & 0xffff;
+  } else if (lower_out && g.tile_order) {
     ti = g.tile_order[tile].x;
     tj = g.tile_order[tile].y;
   } else if (lower_out) {
@@ -754,6 +761,32 @@ void lower_tile_order(int nt, int group, int2* out) {
     for (int sj = 0; sj <= si; sj += group)
       for (int ti = si; ti < std::min(nt, si + group); ++ti)
         for (int tj = sj; tj < std::min(sj + group, ti + 1); ++tj) out[n++] = make_int2(ti, tj);
+}
+
+void gram_item_order(int m, int group, int split, int2* out) {
+  const int nt = ceil_div(m, 128);
+  const int per_split = nt * (nt + 1) / 2;
+  std::vector<int2> order((size_t)per_split);
+  lower_tile_order(nt, group, order.data());
+  // length class of a tile in the Gram kernel: 0 full, 1 diagonal (9/16),
+  // 2 strip of a short last tile row (1/2), 3 exits at once
+  auto cls = [&](int2 t) {
+    const int rows = m - t.x * 128;
+    if (rows <= 32) return (t.y & 1) ? 3 : 2;
+    if (rows <= 64) return 2;
+    return (t.x == t.y && rows >= 128) ? 1 : 0;
+  };
+  const int total = per_split * split;
+  const int q = total / kNumXcd, rq = total % kNumXcd;
+  for (int x = 0; x < kNumXcd; ++x) {
+    const int lo = x * q + std::min(x, rq), n = q + (x < rq ? 1 : 0);
+    int w = lo;
+    for (int c = 0; c < 4; ++c)
+      for (int lin = lo; lin < lo + n; ++lin) {
+        const int2 t = order[lin % per_split];
+        if (cls(t) == c) out[w++] = make_int2(lin / per_split, t.x << 16 | t.y);
+      }
+  }
 }
 
 }  // namespace sdpx
