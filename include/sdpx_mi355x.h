@@ -183,6 +183,14 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out);
  * levels per trial round 1..7.  Lets a caller re-apply a saved autotune
  * result. */
 int sdpx_dense_set_tiling(sdpx_dense_t* h, int panel0, int panel1, int split, int group, int ftb_depth);
+/* Host-only (no device call; no counterpart in the reference): the work list
+ * of the FP64 Schur Gram launch for an m x m Schur matrix, `group` x `group`
+ * tile super-blocks and `split` K ranges, as the kernel reads it: for each
+ * work item in launch order (XCD-remapped), out[2i] = split index and
+ * out[2i+1] = ti << 16 | tj of its 128 x 128 lower tile; ceil(m/128) *
+ * (ceil(m/128)+1) / 2 * split items.  Each XCD's run lists its full tiles
+ * first and its diagonal tiles, short-last-row strips and empty items after. */
+int sdpx_gram_work_list(int m, int group, int split, int* out);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 

@@ -1006,6 +1006,20 @@ int sdpx_dense_kernel_times(sdpx_dense_t* h, int reset, double* out) {
   return kOk;
 }
 
+int sdpx_gram_work_list(int m, int group, int split, int* out) {
+  if (m < 1 || group < 1 || split < 1 || !out) {
+    set_last_error("sdpx_gram_work_list: m, group and split must be positive, out non-null");
+    return kInvalidArgument;
+  }
+  const long long nt = ceil_div(m, 128);
+  if (nt > 32767 || nt * (nt + 1) / 2 * split > (1LL << 30)) {
+    set_last_error("sdpx_gram_work_list: too many work items");
+    return kInvalidArgument;
+  }
+  gram_item_order(m, group, split, reinterpret_cast<int2*>(out));
+  return kOk;
+}
+
 int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
   SDPX_CHECK(check_handle(h));
   if (split) *split = h->split;
@@ -1013,9 +1027,9 @@ int sdpx_dense_schur_split(sdpx_dense_t* h, int* split) {
 }
 
 // Plan-time tuning of the Schur build for this handle's shape: every candidate
-// is run on the device and timed with events (kernel timings do not depend on
-// the operand values, so it can run before any iterate is set; it overwrites
-// the panel and Schur buffers).  Candidates: each of the two P-hat panel
+// is run on the device and timed with events, on the handle's A_i with dense
+// stand-ins for L_X^-1 and M_Y (restored afterwards), so it can run before any
+// iterate is set; it overwrites the panel and Schur buffers.  Candidates: each of the two P-hat panel
 // products on 128 x 128, 64 x 64, 128 x 64 or 64 x 128 tiles; Gram split-K
 // factor s/2, 3s/4, s, 3s/2 or 2s of the create-time choice s (the partial-sum
 // workspace holds up to 2s, within the create-time memory cap); Gram tile

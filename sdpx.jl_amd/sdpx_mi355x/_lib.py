@@ -57,6 +57,7 @@ SIGNATURES = {
     "sdpx_dense_schur_split": [_h, _ip],
     "sdpx_dense_autotune": [_h, _dp],
     "sdpx_dense_set_tiling": [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "sdpx_gram_work_list": [C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_dense_set_precision": [_h, C.c_int],
     "sdpx_dense_exact_residual": [_h, _dp],
     "sdpx_dense_refine_exact": [_h, C.c_double, C.c_int, _ip, _dp],
