@@ -59,7 +59,7 @@ __device__ __forceinline__ int diag_wave(double* a, long long lda, int jb, bool 
     // use `own` here so that its LDS read goes out with the pivot's (one round
     // trip) instead of after the pivot test: 10% of the block's latency
     asm volatile("" ::"v"(own));
-    if (!(p > 0.0)) return c;  // uniform over the work-group
+    if (!(p > 0.0 && p <= 1.7976931348623157e308)) return c;  // not positive or not finite; uniform
     const double f = own / p;
     if (H0 + kDiagHalf - 1 > c) {
       const int lo = c + 1 > H0 ? c + 1 : H0;
