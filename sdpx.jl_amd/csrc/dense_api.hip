@@ -1071,9 +1071,14 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
     set_last_error("sdpx_dense_autotune: hipMalloc failed");
     return kOutOfMemory;
   }
+  const bool saved_ok =
+      hipMemcpyAsync(saved, b.LXinv, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess &&
+      hipMemcpyAsync(saved + kk, b.MY, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
   auto body = [&]() -> int {
-    SDPX_HIP(hipMemcpyAsync(saved, b.LXinv, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    SDPX_HIP(hipMemcpyAsync(saved + kk, b.MY, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (!saved_ok) {
+      set_last_error("sdpx_dense_autotune: saving L_X^-1 and M_Y failed");
+      return kHipError;
+    }
     hipLaunchKernelGGL(tune_lower_kernel, dim3(ceil_div((long long)kk, 256)), dim3(256), 0, h->stream, k, b.LXinv);
     hipLaunchKernelGGL(tune_lower_kernel, dim3(ceil_div((long long)kk, 256)), dim3(256), 0, h->stream, k, b.MY);
     SDPX_HIP(hipGetLastError());
@@ -1191,7 +1196,7 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
   };
   const int rc = body();
   // L_X^-1 and M_Y back (also after a failed timing)
-  const bool restored =
+  const bool restored = saved_ok &&
       hipMemcpyAsync(b.LXinv, saved, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess &&
       hipMemcpyAsync(b.MY, saved + kk, kk * sizeof(double), hipMemcpyDeviceToDevice, h->stream) == hipSuccess &&
       hipStreamSynchronize(h->stream) == hipSuccess;
