@@ -213,6 +213,7 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
     out = {"workload": f"dense SDP n={k}, m={m}, rel gap {tol:g}"}
     b = MI355XDenseBackend(prob)
     try:
+        out["autotune"] = b.autotune()  # plan time, as for the FP64 leg
         # Schur Gram launch time per assembly precision at the start point (HIP events)
         import numpy as np
 
