@@ -258,7 +258,7 @@ class MI355XDenseBackend:
                     ftb_levels_per_round=int(out[15]),
                     ftb_model_ms={d: round(float(out[16 + d - 4]), 4) for d in range(4, 8)})
 
-    def set_tiling(self, panel_tiles=("128x128", "128x128"), split: int = 1, tile_group: int = 8,
+    def set_tiling(self, panel_tiles=("64x64", "64x64"), split: int = 1, tile_group: int = 8,
                    ftb_levels_per_round: int = 7) -> None:
         """Apply tiling choices (e.g. a saved autotune result) without timing them."""
         shapes = ("128x128", "64x64", "128x64", "64x128")
