@@ -489,6 +489,108 @@ __device__ __forceinline__ void gemm_f64_body(const GemmArgs& g, int split, doub
                                          part ? 0 : g.block_rows, g.block_stride);
 }
 
+// Diagonal 64 x 64 tile of a grouped Gram product C = P' P (lower output,
+// m0 == n0): the 10 of 16 blocks of 16 x 16 on or below the diagonal, from one
+// staged operand slice.  Blocks are named by 16-row block index (0..3); wave W
+// reads the fragments of blocks fb[] and computes the (row, column) pairs
+// (fb[pr[b]], fb[pc[b]]):
+//   wave 0: (2,0) (3,0) (3,1)    wave 1: (2,1) (2,2) (3,2)
+//   wave 2: (1,0) (1,1)          wave 3: (0,0) (3,3)
+// so the group takes 3/4 of a full tile's MFMA time.  Each accumulator sees the
+// same operands in the same K order as in the full tile.  Rows at or beyond M
+// (a partial tile) are computed on clamped rows and not written.
+template <int W>
+struct DiagPlan64;
+template <>
+struct DiagPlan64<0> {
+  static constexpr int nf = 4, np = 3;
+  static constexpr int fb[4] = {2, 3, 0, 1}, pr[3] = {0, 1, 1}, pc[3] = {2, 2, 3};
+};
+template <>
+struct DiagPlan64<1> {
+  static constexpr int nf = 3, np = 3;
+  static constexpr int fb[3] = {1, 2, 3}, pr[3] = {1, 1, 2}, pc[3] = {0, 1, 1};
+};
+template <>
+struct DiagPlan64<2> {
+  static constexpr int nf = 2, np = 2;
+  static constexpr int fb[2] = {0, 1}, pr[2] = {1, 1}, pc[2] = {0, 1};
+};
+template <>
+struct DiagPlan64<3> {
+  static constexpr int nf = 2, np = 2;
+  static constexpr int fb[2] = {0, 3}, pr[2] = {0, 1}, pc[2] = {0, 1};
+};
+
+template <int W, bool VEC>
+__device__ __forceinline__ void grouped_diag_core(double* lds, const double* __restrict__ P, long long ld, int M,
+                                                  int kBegin, int kEnd, int m0, double* C, long long ldc,
+                                                  double alpha, double beta) {
+  using Plan = DiagPlan64<W>;
+  constexpr int LA = lds_size<64, true, BK>();
+  double(*As)[LA] = reinterpret_cast<double(*)[LA]>(lds);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  double4_t acc[Plan::np];
+#pragma unroll
+  for (int b = 0; b < Plan::np; ++b) acc[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+  constexpr int NE = 64 * BK / 256;
+  double ra[NE];
+  TileLoader<64, true, VEC, BK> la;
+  la.init(ld, m0, M, tid);
+  auto load = [&](int k0) {
+    if (k0 + BK <= kEnd) la.load(ra, P, ld, k0);
+    else la.load_tail(ra, P, ld, k0, kEnd, tid);
+  };
+  auto compute = [&](int cur) {
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double f[Plan::nf];
+#pragma unroll
+      for (int q = 0; q < Plan::nf; ++q) f[q] = As[cur][lds_at<64, true, BK>(kr, Plan::fb[q] * 16 + (lane & 15))];
+#pragma unroll
+      for (int b = 0; b < Plan::np; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[Plan::pc[b]], f[Plan::pr[b]], acc[b], 0, 0, 0);
+    }
+  };
+  if (kBegin < kEnd) {
+    load(kBegin);
+    tile_store<64, true, VEC, BK>(ra, As[0], tid);
+    __syncthreads();
+    int cur = 0;
+    const int kFull = kBegin + ((kEnd - kBegin) / BK) * BK;
+    int k0 = kBegin;
+    for (; k0 + 2 * BK <= kFull; k0 += BK) {
+      la.load(ra, P, ld, k0 + BK);
+      compute(cur);
+      tile_store<64, true, VEC, BK>(ra, As[cur ^ 1], tid);
+      __syncthreads();
+      cur ^= 1;
+    }
+    if (k0 + BK < kEnd) {
+      load(k0 + BK);
+      compute(cur);
+      tile_store<64, true, VEC, BK>(ra, As[cur ^ 1], tid);
+      __syncthreads();
+      cur ^= 1;
+    }
+    compute(cur);
+  }
+#pragma unroll
+  for (int b = 0; b < Plan::np; ++b) {
+    const int row = m0 + Plan::fb[Plan::pr[b]] * 16 + (lane & 15);
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int col = m0 + Plan::fb[Plan::pc[b]] * 16 + (lane >> 4) + 4 * rr;
+      if (row < M && row >= col) {
+        double* cp = C + row + (long long)col * ldc;
+        *cp = (beta == 0.0) ? alpha * acc[b][rr] : alpha * acc[b][rr] + beta * (*cp);
+      }
+    }
+  }
+}
+
 // ---- grouped launches: one work-group per (GEMM, tile) item ------------------
 template <int BM, int BN, bool TA, bool TB, bool VEC>
 __global__ __launch_bounds__(256, 2) void gemm_grouped_kernel(const GroupGemm* __restrict__ G,
@@ -502,10 +604,24 @@ __global__ __launch_bounds__(256, 2) void gemm_grouped_kernel(const GroupGemm* _
   if (g.flags & kGemmBLower) kBegin = max(kBegin, n0);
   if (g.flags & kGemmAUpper) kBegin = max(kBegin, m0);
   kBegin = (kBegin / BK) * BK;
+  __shared__ __align__(16) double lds[core_lds<BM, BN, TA, TB>()];
+  if constexpr (TA && !TB && BM == 64 && BN == 64) {
+    // diagonal tiles of the batched Schur Gram products
+    if ((g.flags & kGemmLowerOut) && t.ti == t.tj && g.A == g.B && g.lda == g.ldb) {
+      static_assert(lds_size<64, true, BK>() * 2 <= core_lds<BM, BN, TA, TB>(), "LDS");
+      switch (__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) {
+        case 0: grouped_diag_core<0, VEC>(lds, g.A, g.lda, g.M, kBegin, kEnd, m0, g.C, g.ldc, g.alpha, g.beta); break;
+        case 1: grouped_diag_core<1, VEC>(lds, g.A, g.lda, g.M, kBegin, kEnd, m0, g.C, g.ldc, g.alpha, g.beta); break;
+        case 2: grouped_diag_core<2, VEC>(lds, g.A, g.lda, g.M, kBegin, kEnd, m0, g.C, g.ldc, g.alpha, g.beta); break;
+        default: grouped_diag_core<3, VEC>(lds, g.A, g.lda, g.M, kBegin, kEnd, m0, g.C, g.ldc, g.alpha, g.beta);
+      }
+      return;
+    }
+  }
   // (the slice fence of the Gram kernel measured 0.7% slower on the batched
   // Schur Gram products here: 64 x 64 tiles issue 16 MFMAs per slice)
-  gemm_core<BM, BN, TA, TB, VEC>(g.A, g.lda, g.B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
-                                 (g.flags & kGemmLowerOut) != 0, g.C, g.ldc, g.alpha, g.beta);
+  gemm_core_lds<BM, BN, TA, TB, VEC>(lds, g.A, g.lda, g.B, g.ldb, g.M, g.N, kBegin, kEnd, m0, n0,
+                                     (g.flags & kGemmLowerOut) != 0, g.C, g.ldc, g.alpha, g.beta, 0, 0);
 }
 
 template <int BM, int BN, bool TA, bool TB, bool VEC>
