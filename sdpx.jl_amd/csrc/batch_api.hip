@@ -409,6 +409,12 @@ struct sdpx_batch {
   int* hlist = nullptr;
   int ring_pos = 0;
   int* dall = nullptr;  // 0..B-1
+  // results packed member after member (unpadded) for one device-to-host copy:
+  // offsets of each member's k x k block and m-vector, and the staging buffer
+  long long* dpackK = nullptr;
+  long long* dpackM = nullptr;
+  double* dpack = nullptr;
+  long long packK = 0, packM = 0;
   // host mirrors of the scalar arrays (pinned)
   double* hsc = nullptr;
   int* hsi = nullptr;
@@ -1296,6 +1302,22 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
   if ((s = balloc_n(h, &a.par, (long long)B * NPARAM))) return fail(s);
   if ((s = balloc_n(h, &h->dlist, (long long)B * kListRing))) return fail(s);
   if ((s = balloc_n(h, &h->dall, B))) return fail(s);
+  if ((s = balloc_n(h, &h->dpackK, B)) || (s = balloc_n(h, &h->dpackM, B))) return fail(s);
+  {
+    std::vector<long long> ok(B), om(B);
+    for (int p = 0; p < B; ++p) {
+      ok[p] = h->packK;
+      om[p] = h->packM;
+      h->packK += (long long)k[p] * k[p];
+      h->packM += m[p];
+    }
+    if ((s = balloc_n(h, &h->dpack, std::max(h->packK, h->packM)))) return fail(s);
+    if (hipMemcpy(h->dpackK, ok.data(), sizeof(long long) * B, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->dpackM, om.data(), sizeof(long long) * B, hipMemcpyHostToDevice) != hipSuccess) {
+      set_last_error("hipMemcpy(pack offsets) failed");
+      return fail(kHipError);
+    }
+  }
   if (hipHostMalloc(reinterpret_cast<void**>(&h->hlist), sizeof(int) * B * kListRing) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&h->hsc), sizeof(double) * B * NSLOT) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&h->hsi), sizeof(int) * B * NISLOT) != hipSuccess ||
@@ -1496,28 +1518,52 @@ void set_scales(sdpx_batch_t* h, const double* C, const double* c) {
   }
 }
 
-// host (members concatenated, unpadded) -> device (aligned member offsets)
+// inverse of pack_blocks_kernel: an unpadded member-after-member buffer to the
+// members' aligned device offsets
+__global__ __launch_bounds__(256) void unpack_blocks_kernel(const Member* __restrict__ mem,
+                                                           const double* __restrict__ src,
+                                                           const long long* __restrict__ off, int kblock,
+                                                           double* __restrict__ dst) {
+  const Member M = mem[blockIdx.x];
+  const long long n = kblock ? (long long)M.k * M.k : M.m;
+  const double* s = src + off[blockIdx.x];
+  double* d = dst + (kblock ? M.oK : M.oM);
+  for (long long i = threadIdx.x; i < n; i += 256) d[i] = s[i];
+}
+
+// host (members concatenated, unpadded) -> device (aligned member offsets): one
+// copy into the staging buffer, then scattered on the device
 int upload_blocks(sdpx_batch_t* h, double* dst, const double* src, bool kblock) {
-  long long o = 0;
-  for (int p = 0; p < h->B; ++p) {
-    const Member& M = h->mem[p];
-    const long long n = kblock ? (long long)M.k * M.k : M.m;
-    SDPX_HIP(hipMemcpyAsync(dst + (kblock ? M.oK : M.oM), src + o, sizeof(double) * n, hipMemcpyHostToDevice,
-                            h->stream));
-    o += n;
-  }
+  if (h->B <= 0) return kOk;
+  SDPX_HIP(hipMemcpyAsync(h->dpack, src, sizeof(double) * (kblock ? h->packK : h->packM), hipMemcpyHostToDevice,
+                          h->stream));
+  hipLaunchKernelGGL(unpack_blocks_kernel, dim3(h->B), dim3(256), 0, h->stream, h->dmem, h->dpack,
+                     kblock ? h->dpackK : h->dpackM, kblock ? 1 : 0, dst);
+  SDPX_HIP(hipGetLastError());
   return kOk;
 }
 
+// members' k x k blocks (or m-vectors) from their aligned device offsets into
+// one unpadded buffer, member after member (one work-group per member)
+__global__ __launch_bounds__(256) void pack_blocks_kernel(const Member* __restrict__ mem, const double* __restrict__ src,
+                                                         const long long* __restrict__ off, int kblock,
+                                                         double* __restrict__ dst) {
+  const Member M = mem[blockIdx.x];
+  const long long n = kblock ? (long long)M.k * M.k : M.m;
+  const double* s = src + (kblock ? M.oK : M.oM);
+  double* d = dst + off[blockIdx.x];
+  for (long long i = threadIdx.x; i < n; i += 256) d[i] = s[i];
+}
+
+// device (aligned member offsets) -> host (members concatenated, unpadded):
+// packed on the device, then one copy (a copy per member costs ~4 us each)
 int download_blocks(sdpx_batch_t* h, double* dst, const double* src, bool kblock) {
-  long long o = 0;
-  for (int p = 0; p < h->B; ++p) {
-    const Member& M = h->mem[p];
-    const long long n = kblock ? (long long)M.k * M.k : M.m;
-    SDPX_HIP(hipMemcpyAsync(dst + o, src + (kblock ? M.oK : M.oM), sizeof(double) * n, hipMemcpyDeviceToHost,
-                            h->stream));
-    o += n;
-  }
+  if (h->B <= 0) return kOk;
+  hipLaunchKernelGGL(pack_blocks_kernel, dim3(h->B), dim3(256), 0, h->stream, h->dmem, src,
+                     kblock ? h->dpackK : h->dpackM, kblock ? 1 : 0, h->dpack);
+  SDPX_HIP(hipGetLastError());
+  SDPX_HIP(hipMemcpyAsync(dst, h->dpack, sizeof(double) * (kblock ? h->packK : h->packM), hipMemcpyDeviceToHost,
+                          h->stream));
   return kOk;
 }
 }  // namespace
