@@ -641,15 +641,55 @@ def symmetrize(M):
 # --------------------------------------------------------------------------
 # Constraint contractions and Schur assembly (schur.jl)
 # --------------------------------------------------------------------------
+# Summation order of the two A-contractions below.  "blas" is OpenBLAS dgemv,
+# the reference's own call (schur.jl:36-44 through LinearAlgebra).  The others
+# are equally valid Float64 evaluations of the same sums, used by the tests to
+# show which termination statuses are decided by rounding: "pairwise" is
+# numpy's pairwise reduction over the summed index, "strided:N" adds N
+# interleaved partial GEMVs (terms j = s mod N) in order s = 0..N-1.
+_ORDER = {"contract": "blas", "build": "blas"}
+
+
+class summation_order:
+    """Context manager selecting the order of ``contract`` / ``build_P`` sums."""
+
+    def __init__(self, contract: str = "blas", build: str = "blas"):
+        for v in (contract, build):
+            if not (v in ("blas", "pairwise") or (v.startswith("strided:") and int(v[8:]) >= 1)):
+                raise ValueError(f"unknown summation order {v!r}")
+        self.new = {"contract": contract, "build": build}
+
+    def __enter__(self):
+        self.old = dict(_ORDER)
+        _ORDER.update(self.new)
+        return self
+
+    def __exit__(self, *exc):
+        _ORDER.update(self.old)
+
+
+def _ordered_matvec(A: np.ndarray, v: np.ndarray, order: str) -> np.ndarray:
+    """A @ v summed over A's columns in the named order."""
+    if order == "blas":
+        return A @ v
+    if order == "pairwise":
+        return np.sum(A * v[None, :], axis=1)
+    n = int(order[8:])
+    s = A[:, 0::n] @ v[0::n]
+    for i in range(1, n):
+        s = s + A[:, i::n] @ v[i::n]
+    return s
+
+
 def build_P(prob: Problem, l: int, x) -> np.ndarray:
     """P = sum_i x_i A_i (schur.jl:36-39)."""
     k = prob.k[l]
-    return (prob.Av[l] @ x).reshape(k, k, order="F")
+    return _ordered_matvec(prob.Av[l], np.asarray(x, dtype=np.float64), _ORDER["build"]).reshape(k, k, order="F")
 
 
 def contract(prob: Problem, l: int, M) -> np.ndarray:
     """[<A_i, M>]_i (schur.jl:41-44)."""
-    return prob.Av[l].T @ np.asarray(M).ravel(order="F")
+    return _ordered_matvec(prob.Av[l].T, np.asarray(M).ravel(order="F"), _ORDER["contract"])
 
 
 def schur_build(prob: Problem, LX: list, MY: list) -> np.ndarray:
@@ -1005,6 +1045,7 @@ class Result:
     d_res: float
     gap_rel: float
     trace: dict | None = None
+    history: list | None = None
 
 
 def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=False) -> Result:
@@ -1036,6 +1077,7 @@ def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=Fals
     centering = 0
     status = "NotStarted"
     gap_rel = math.inf
+    history = []
     while True:
         gap = pobj - dobj
         gap_rel = abs(gap) / max(1.0, (abs(pobj) + abs(dobj)) / 2)
@@ -1149,10 +1191,14 @@ def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=Fals
         pobj = float(prob.c @ x)
         dobj = dual_objective(prob, Y)
         iteration += 1
+        history.append(dict(iteration=iteration, pobj=pobj, dobj=dobj, p_res=res["p_res"], d_res=res["d_res"],
+                            tX=tX, tY=tY, sigma=params.sigma, mu=res["mu"], mu_aff=res["mu_aff"],
+                            reg_attempts=res["reg_attempts"], refine_steps=res["refine_steps"],
+                            fallback=ctrl.fallback))
         p_res, d_res = p_after, d_after
     if status != "Optimal" and best is not None:
         x, X, Y, pobj, dobj, p_res, d_res, gap_rel = best
-    return Result(status, x, X, Y, pobj, dobj, iteration, p_res, d_res, gap_rel, trace)
+    return Result(status, x, X, Y, pobj, dobj, iteration, p_res, d_res, gap_rel, trace, history)
 
 
 def certificate(prob: Problem, result: Result, tol_primal, tol_dual, tol_gap):

@@ -134,18 +134,21 @@ def sparse_feasible(k: int, m: int, seed: int) -> SparseSDP:
     return SparseSDP(k=k, m=m, rows=list(r), cols=list(cc), vals=list(-v), C=-Cstd, c=-b, x0=y0)
 
 
-def batched_sparse(B: int, seed: int = 0, kmin: int = 8, kmax: int = 64) -> list:
+def batched_sparse(B: int, seed: int = 0, kmin: int = 8, kmax: int = 64, only=None) -> list:
     """The uneven batched workload: B problems, k ~ U{kmin..kmax}, m ~ U{2k..floor(k(k+1)/4)},
-    3k nonzero positions per A_i (sparse_feasible), one seed per problem."""
+    3k nonzero positions per A_i (sparse_feasible), one seed per problem.
+
+    only: optional member indices; then just those members are built (in that order), from the
+    same draws as the whole batch."""
     rng = np.random.default_rng(seed)
     ks = rng.integers(kmin, kmax + 1, size=B)
-    out = []
-    for p, k in enumerate(ks):
+    shapes = []
+    for k in ks:
         k = int(k)
         mlo, mhi = 2 * k, max(2 * k, (k * (k + 1)) // 4)
-        m = int(rng.integers(mlo, mhi + 1))
-        out.append(sparse_feasible(k, m, seed=seed * 1_000_003 + p + 1))
-    return out
+        shapes.append((k, int(rng.integers(mlo, mhi + 1))))
+    members = range(B) if only is None else [int(p) for p in only]
+    return [sparse_feasible(*shapes[p], seed=seed * 1_000_003 + p + 1) for p in members]
 
 
 def batched_dense_layer(B: int, seed: int = 0, k: int = 32, m: int = 128, noise: float = 0.1) -> list:
