@@ -270,6 +270,32 @@ int sdpx_batch_get_result(sdpx_batch_t* h, int* status, int* iterations, double*
  * are copied. */
 #define SDPX_BATCH_TRACE_COLS 13
 int sdpx_batch_get_trace(sdpx_batch_t* h, int member, double* rows, int cap, int* n);
+/* ---- host-only hooks on the per-member controller (no device call) -----------
+ * sdpx_batch_policy_select replaces select_iteration_parameters!
+ * (adaptive_parameters.jl:511-548) over select_parameters (:188-194, 253-453):
+ * a controller built from `o` (AdaptiveIPMController(opts), :455-476), with
+ * nhist history rows of (sigma, unstable != 0) and, when preset_fallback != 0,
+ * a controller already in fallback, selects the parameters for the
+ * diagnostics `diag` (17 doubles in IterationDiagnostics order, :63-83:
+ * iteration, primal_residual, dual_residual, relative_gap, mu, mu_aff,
+ * affine_primal_step, affine_dual_step, previous_primal_step,
+ * previous_dual_step, backtracking_count, regularization, refinement_count,
+ * factorization_quality, achieved_residual_reduction, primal_psd_margin,
+ * dual_psd_margin).  out (16): sigma, primal and dual fraction, backtracking
+ * factor, refinement tolerance, refinement max count, minimum step, restart
+ * scale, regularization, fallback, fallback reason (0 none, 1
+ * nonfinite_diagnostics, 2 degraded_factorization, 3 repeated_instability, 4
+ * previous_instability), the controller's fallback flag, beta, gamma after the
+ * selection, and the policy's sigma_max, sigma_min.
+ * sdpx_batch_policy_record replaces record_and_update! (:597-700) on a fresh
+ * controller: rec (11) = iteration, complementarity before / after, primal and
+ * dual residual, primal and dual step, primal and dual residual after,
+ * backtracking count, refinement count; out (4) = achieved residual
+ * reduction, unstable, the controller's fallback flag, instability count. */
+int sdpx_batch_policy_select(const sdpx_batch_options_t* o, const double* diag, int nhist, const double* hist,
+                             int preset_fallback, double* out);
+int sdpx_batch_policy_record(const sdpx_batch_options_t* o, const double* rec, double* out);
+
 /* ---- differentiation (no counterpart in the reference) -----------------------
  * For the user's dual-form model  max b'y  s.t.  C - sum_i y_i A_i >= 0  with
  * primal matrix X* (the core Y) and multipliers y* (the core x), and the loss

@@ -24,9 +24,14 @@ SDPX.jl repository root).  Parity pins (see ``tests/test_oracle.py``):
 * ``solve`` -- reference known answers T1 (p* = 2 sqrt 6, 19 +- 2 iterations,
   test/correctness.jl:71-94, 230-243) and strong duality of the seeded
   feasible generator.
-* ``AdaptiveController`` -- parity unpinned against the reference
-  trajectory (Julia is unavailable); pinned only through the end-to-end
-  objective and certificate checks.
+* ``AdaptivePolicy``/``Controller`` -- the reference's own known values of
+  test/adaptive_parameter_policy.jl:20-285 (sigma = 0.15 under the cap, 3/25,
+  fallback reasons and the restored fixed policy, achieved reduction 0.75),
+  checked on this oracle and on both product controllers
+  (tests/test_adaptive_policy.py).  The whole-solve trajectory stays unpinned
+  against the reference (Julia is unavailable); near the gap target some
+  statuses depend on the summation order (``summation_order``,
+  tests/test_status_rounding.py).
 """
 
 from __future__ import annotations

@@ -46,6 +46,9 @@ struct Params {
   int rmax = 0;
   double min_step = 0, restart_scale = 0, reg = 0;
   bool fallback = false;
+  // fallback_reason: 0 none, 1 nonfinite_diagnostics, 2 degraded_factorization,
+  // 3 repeated_instability, 4 previous_instability (adaptive_parameters.jl:253-280, 511-548)
+  int reason = 0;
 };
 
 struct FixedPolicy {  // adaptive_parameters.jl:21-32, 112-131
@@ -57,7 +60,7 @@ struct FixedPolicy {  // adaptive_parameters.jl:21-32, 112-131
     const int count = o.refine_policy == 1 ? o.refine_steps : o.refine_max_steps;
     return {o.beta, o.gamma, o.gamma, o.gamma, tol, count, o.min_step, o.omega_step};
   }
-  Params params(bool fallback) const {
+  Params params(bool fallback, int reason = 0) const {
     Params p;
     p.sigma = sigma;
     p.pf = pf;
@@ -68,6 +71,7 @@ struct FixedPolicy {  // adaptive_parameters.jl:21-32, 112-131
     p.min_step = min_step;
     p.restart_scale = restart_scale;
     p.fallback = fallback;
+    p.reason = reason;
     return p;
   }
 };
@@ -166,11 +170,11 @@ struct Controller {  // AdaptiveIPMController (adaptive_parameters.jl:440-782)
 
   Params select_adaptive(const Diag& d) const {  // adaptive_parameters.jl:188-194, 253-453
     const AdaptivePolicy& p = ap;
-    if (!d.finite()) return p.fb.params(true);
-    if (d.factorization_quality <= 0) return p.fb.params(true);
+    if (!d.finite()) return p.fb.params(true, 1);
+    if (d.factorization_quality <= 0) return p.fb.params(true, 2);
     int unstable = 0;
     for (auto it = history.rbegin(); it != history.rend() && it->unstable; ++it) ++unstable;
-    if (unstable >= p.instability_limit) return p.fb.params(true);
+    if (unstable >= p.instability_limit) return p.fb.params(true, 3);
     if (std::max(d.primal_residual, std::max(d.dual_residual, d.relative_gap)) >= 1) return p.fb.params(false);
     const double ratio = clampd(d.mu_aff / d.mu, 0.0, 1.0);
     const double amin = std::min(d.affine_primal_step, d.affine_dual_step);
@@ -220,7 +224,7 @@ struct Controller {  // AdaptiveIPMController (adaptive_parameters.jl:440-782)
     if (fallback) {
       FixedPolicy fx{default_beta, default_gamma, default_gamma, default_gamma, parameters.rtol, parameters.rmax,
                      parameters.min_step, parameters.restart_scale};
-      prm = fx.params(true);
+      prm = fx.params(true, 4);
     } else if (!adaptive) {
       prm = fixed.params(false);
     } else {
@@ -1890,6 +1894,65 @@ int sdpx_batch_schur_at_start(sdpx_batch_t* h) {
     }
   SDPX_CHECK(schur_build(h, all));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+/* Host-only hooks on the per-member controller (no device call). */
+int sdpx_batch_policy_select(const sdpx_batch_options_t* o, const double* diag, int nhist, const double* hist,
+                             int preset_fallback, double* out) {
+  if (!o || !diag || !out || nhist < 0 || (nhist > 0 && !hist)) {
+    set_last_error("sdpx_batch_policy_select: invalid arguments");
+    return kInvalidArgument;
+  }
+  Controller c;
+  c.init(*o);
+  for (int i = 0; i < nhist; ++i) {
+    Hist r{};
+    r.sigma = hist[2 * i];
+    r.unstable = hist[2 * i + 1] != 0.0;
+    c.history.push_back(r);
+  }
+  c.fallback = preset_fallback != 0;
+  Diag d;
+  d.iteration = (int)diag[0];
+  d.primal_residual = diag[1];
+  d.dual_residual = diag[2];
+  d.relative_gap = diag[3];
+  d.mu = diag[4];
+  d.mu_aff = diag[5];
+  d.affine_primal_step = diag[6];
+  d.affine_dual_step = diag[7];
+  d.previous_primal_step = diag[8];
+  d.previous_dual_step = diag[9];
+  d.backtracking_count = (int)diag[10];
+  d.regularization = diag[11];
+  d.refinement_count = (int)diag[12];
+  d.factorization_quality = diag[13];
+  d.achieved_residual_reduction = diag[14];
+  d.primal_psd_margin = diag[15];
+  d.dual_psd_margin = diag[16];
+  const Params p = c.select(d);
+  const double v[] = {p.sigma, p.pf, p.df, p.bt, p.rtol, (double)p.rmax, p.min_step, p.restart_scale, p.reg,
+                      p.fallback ? 1.0 : 0.0, (double)p.reason, c.fallback ? 1.0 : 0.0, c.beta, c.gamma,
+                      c.ap.sigma_max, c.ap.sigma_min};
+  for (int i = 0; i < 16; ++i) out[i] = v[i];
+  return kOk;
+}
+
+int sdpx_batch_policy_record(const sdpx_batch_options_t* o, const double* rec, double* out) {
+  if (!o || !rec || !out) {
+    set_last_error("sdpx_batch_policy_record: invalid arguments");
+    return kInvalidArgument;
+  }
+  Controller c;
+  c.init(*o);
+  c.record((int)rec[0], rec[1], rec[2], rec[3], rec[4], rec[5], rec[6], rec[7], rec[8], (int)rec[9], (int)rec[10],
+           c.parameters);
+  const Hist& h = c.history.back();
+  out[0] = h.achieved;
+  out[1] = h.unstable ? 1.0 : 0.0;
+  out[2] = c.fallback ? 1.0 : 0.0;
+  out[3] = (double)c.instability_count;
   return kOk;
 }
 

@@ -76,6 +76,8 @@ SIGNATURES = {
     "sdpx_batch_center": [_h, C.c_int],
     "sdpx_batch_restore_checkpoint": [_h, _ip, _dp],
     "sdpx_batch_schur_at_start": [_h],
+    "sdpx_batch_policy_select": [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp],
+    "sdpx_batch_policy_record": [C.c_void_p, _dp, _dp],
     "sdpx_soc_create": [C.c_int, C.c_int, _ip, C.POINTER(_h)],
     "sdpx_soc_destroy": [_h],
     "sdpx_soc_memory": [_h, C.POINTER(C.c_longlong)],
