@@ -369,6 +369,23 @@ int sdpx_soc_inverse(sdpx_soc_t* h, int src, int dst, int* nbad);
 /* _soc_fraction_to_boundary (soc.jl:148-214) per cone for s + t ds; *min_bound
  * = the minimum over cones (NaN propagates); per_cone (ncones) optional. */
 int sdpx_soc_fraction_to_boundary(sdpx_soc_t* h, int s, int ds, double* min_bound, double* per_cone);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sdpx_soc_fraction_to_boundary on caller-owned device
+ * buffers, with no handle and no allocation.  offsets: device, ncones + 1
+ * int64 (offsets[0] = 0, cone c occupies [offsets[c], offsets[c+1]), every
+ * cone at least 1 long); s, ds: device, offsets[ncones] doubles each;
+ * min_bound: device, one double, receives the minimum over cones (NaN
+ * propagates); per_cone: optional device array of ncones doubles for the
+ * per-cone bounds.  Without per_cone the bounds go to a static device array
+ * of SDPX_SOC_NOALLOC_MAX_CONES doubles (32 MiB per device, held until the
+ * process ends); more cones are rejected with SDPX_INVALID_ARGUMENT.  The
+ * launches are queued on `stream` (a hipStream_t; null = the default stream)
+ * and the call returns without waiting.  Calls that overlap on one device
+ * share that array and race. */
+#define SDPX_SOC_NOALLOC_MAX_CONES (1 << 22)
+int sdpx_soc_fraction_to_boundary_noalloc(int device, int ncones, const long long* offsets, const double* s,
+                                          const double* ds, double* min_bound, double* per_cone, void* stream);
 /* Per-cone margins v0 - ||v_tail|| (_soc_margin, soc_lorentz_kernels.jl:36)
  * and the number of strictly interior cones (_soc_is_interior, soc.jl:122). */
 int sdpx_soc_margins(sdpx_soc_t* h, int slot, double* margins, int* ninterior);

@@ -13,6 +13,7 @@
 // compiled without floating-point contraction, so results are reproducible
 // operation for operation.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -642,6 +643,40 @@ int read_counter(sdpx_soc_t* h, int* out, int n = 1) {
 
 }  // namespace
 
+// Scratch of sdpx_soc_fraction_to_boundary_noalloc: the per-cone bounds that
+// the fixed-order minimum reads.  Each device holds its own copy from module
+// load until the process ends; every call on a device shares it.
+static __device__ __align__(256) double scratch[SDPX_SOC_NOALLOC_MAX_CONES];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved once per device;
+// the array's host-side name is not a device address).
+int noalloc_scratch(int device, double** out) {
+  if (device < 0 || device >= kNoAllocMaxDevices) {
+    set_last_error("sdpx_soc_fraction_to_boundary_noalloc: device index out of range");
+    return kInvalidArgument;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return kHipError;
+    }
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return kOk;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sdpx_soc_create(int device, int ncones, const int* dims, sdpx_soc_t** out) {
@@ -985,6 +1020,30 @@ int sdpx_soc_last_kernel_ms(sdpx_soc_t* h, double* ms) {
 int sdpx_soc_synchronize(sdpx_soc_t* h) {
   SDPX_CHECK(soc_check(h));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_soc_fraction_to_boundary_noalloc(int device, int ncones, const long long* offsets, const double* s,
+                                          const double* ds, double* min_bound, double* per_cone, void* stream) {
+  if (ncones <= 0 || !offsets || !s || !ds || !min_bound) {
+    set_last_error("sdpx_soc_fraction_to_boundary_noalloc: invalid arguments");
+    return kInvalidArgument;
+  }
+  if (!per_cone && ncones > SDPX_SOC_NOALLOC_MAX_CONES) {
+    set_last_error("sdpx_soc_fraction_to_boundary_noalloc: " + std::to_string(ncones) +
+                   " cones exceed the scratch capacity of " + std::to_string(SDPX_SOC_NOALLOC_MAX_CONES) +
+                   " (pass per_cone to use caller memory instead)");
+    return kInvalidArgument;
+  }
+  SDPX_HIP(hipSetDevice(device));
+  double* bound = per_cone;
+  if (!bound) SDPX_CHECK(noalloc_scratch(device, &bound));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Cone cn{offsets, ncones};
+  hipLaunchKernelGGL(ftb_kernel, soc_grid(ncones), dim3(256), 0, st, cn, s, ds, bound);
+  SDPX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(min_reduce_kernel, dim3(1), dim3(1024), 0, st, bound, ncones, min_bound);
+  SDPX_HIP(hipGetLastError());
   return kOk;
 }
 

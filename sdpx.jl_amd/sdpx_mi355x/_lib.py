@@ -91,6 +91,8 @@ SIGNATURES = {
     "sdpx_soc_jordan_solve": [_h, C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_soc_inverse": [_h, C.c_int, C.c_int, _ip],
     "sdpx_soc_fraction_to_boundary": [_h, C.c_int, C.c_int, _dp, _dp],
+    "sdpx_soc_fraction_to_boundary_noalloc": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p],
     "sdpx_soc_margins": [_h, C.c_int, _dp, _ip],
     "sdpx_soc_strict_step": [_h, C.c_int, C.c_int, C.c_double, C.c_int, _dp],
     "sdpx_soc_hkm_metric": [_h, C.c_int, C.c_int, C.c_int, _ip],
