@@ -707,11 +707,17 @@ def schur_build(prob: Problem, LX: list, MY: list) -> np.ndarray:
     S = np.zeros((m, m))
     for l in range(prob.L):
         k = prob.k[l]
-        panel = prob.Av[l].T.reshape(m, k, k).transpose(0, 2, 1)  # panel[i] = A_i
-        T = sla.solve_triangular(LX[l], np.concatenate(list(panel), axis=1), lower=True, check_finite=False)
-        T = T.reshape(k, m, k).transpose(1, 0, 2)  # T[i] = L_X^-1 A_i
-        Ph = T @ np.tril(MY[l])                     # Phat_i = L_X^-1 A_i M_Y
-        Pm = Ph.transpose(0, 2, 1).reshape(m, k * k)  # row i = vec(Phat_i)
+        MYl = np.tril(MY[l])
+        Pm = np.empty((m, k * k))  # row i = vec(Phat_i)
+        # constraints in chunks of ~256 MB: every column is computed as in one call, with less memory
+        chunk = max(1, (1 << 25) // (k * k))
+        for i0 in range(0, m, chunk):
+            n = min(m, i0 + chunk) - i0
+            panel = prob.Av[l][:, i0:i0 + n].T.reshape(n, k, k).transpose(0, 2, 1)  # panel[i] = A_i
+            T = sla.solve_triangular(LX[l], np.concatenate(list(panel), axis=1), lower=True, check_finite=False)
+            T = T.reshape(k, n, k).transpose(1, 0, 2)  # T[i] = L_X^-1 A_i
+            Ph = T @ MYl                                # Phat_i = L_X^-1 A_i M_Y
+            Pm[i0:i0 + n] = Ph.transpose(0, 2, 1).reshape(n, k * k)
         S += Pm @ Pm.T
     return S
 

@@ -1574,6 +1574,7 @@ int download_blocks(sdpx_batch_t* h, double* dst, const double* src, bool kblock
 
 int sdpx_batch_set_dense(sdpx_batch_t* h, const double* Av, const double* C, const double* c) {
   SDPX_CHECK(check_batch(h));
+  h->have_data = false;  // set again only when the whole upload succeeds
   if (!Av || !C || !c) {
     set_last_error("sdpx_batch_set_dense: null data");
     return kInvalidArgument;
@@ -1597,13 +1598,28 @@ int sdpx_batch_set_dense(sdpx_batch_t* h, const double* Av, const double* C, con
 int sdpx_batch_set_coo(sdpx_batch_t* h, const long long* ptr, const int* row, const int* col, const double* val,
                        const double* C, const double* c) {
   SDPX_CHECK(check_batch(h));
+  h->have_data = false;  // set again only when the whole upload succeeds
   if (!ptr || !C || !c) {
     set_last_error("sdpx_batch_set_coo: null data");
     return kInvalidArgument;
   }
   long long ncons = 0;
   for (int p = 0; p < h->B; ++p) ncons += h->mem[p].m;
+  if (ptr[0] != 0) {
+    set_last_error("sdpx_batch_set_coo: ptr[0] must be 0");
+    return kInvalidArgument;
+  }
+  for (long long q = 0; q < ncons; ++q) {
+    if (ptr[q + 1] < ptr[q]) {
+      set_last_error("sdpx_batch_set_coo: ptr must be non-decreasing (constraint " + std::to_string(q) + ")");
+      return kInvalidArgument;
+    }
+  }
   const long long nnz = ptr[ncons];
+  if (nnz > 0 && (!row || !col || !val)) {
+    set_last_error("sdpx_batch_set_coo: null row, col or val with entries present");
+    return kInvalidArgument;
+  }
   // column offsets and block sides per constraint; entries checked on the host
   std::vector<long long> cons(2 * ncons);
   long long j = 0;

@@ -68,6 +68,20 @@ class BatchResult:
         return np.abs(self.pobj - self.dobj) / np.maximum(1.0, (np.abs(self.pobj) + np.abs(self.dobj)) / 2)
 
 
+def _members(name, arrays, sizes, B):
+    """Per-member arrays flattened after checking each against its expected element count."""
+    arrays = list(arrays)
+    if len(arrays) != B:
+        raise ValueError(f"{name}: {len(arrays)} members given, the batch has {B}")
+    out = []
+    for p, (a, n) in enumerate(zip(arrays, sizes)):
+        a = np.asarray(a, dtype=np.float64)
+        if a.size != int(n):
+            raise ValueError(f"{name}: member {p} has {a.size} values, expected {int(n)}")
+        out.append(a)
+    return out
+
+
 class BatchSolver:
     """Device-resident batch of single-block SDPs (block sides ``k``, constraint counts ``m``)."""
 
@@ -109,13 +123,25 @@ class BatchSolver:
     # ---- data -------------------------------------------------------------------------------
     def set_dense(self, Av_list, C_list, c_list) -> None:
         """Av_list[p]: (m_p, k_p^2) rows = vec(A_i) (column-major), C_list[p]: (k_p, k_p), c_list[p]: (m_p,)."""
-        Av = np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in Av_list])
-        Cc = np.concatenate([np.asarray(c_, dtype=np.float64).ravel(order="F") for c_ in C_list])
-        cc = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in c_list])
+        k2 = self.k.astype(np.int64) ** 2
+        Av_list = _members("set_dense Av", Av_list, k2 * self.m, self.B)
+        C_list = _members("set_dense C", C_list, k2, self.B)
+        c_list = _members("set_dense c", c_list, self.m, self.B)
+        Av = np.concatenate([np.ascontiguousarray(a).ravel() for a in Av_list])
+        Cc = np.concatenate([c_.ravel(order="F") for c_ in C_list])
+        cc = np.concatenate([v.ravel() for v in c_list])
         check(self._lib.sdpx_batch_set_dense(self._h, dptr(Av), dptr(Cc), dptr(cc)), "sdpx_batch_set_dense")
 
     def set_coo(self, rows, cols, vals, C_list, c_list) -> None:
         """rows/cols/vals: per constraint of every member (concatenated), upper-triangle COO."""
+        k2 = self.k.astype(np.int64) ** 2
+        C_list = _members("set_coo C", C_list, k2, self.B)
+        c_list = _members("set_coo c", c_list, self.m, self.B)
+        if not (len(rows) == len(cols) == len(vals) == int(self.m.sum())):
+            raise ValueError(f"set_coo: rows, cols and vals need one entry list per constraint ({int(self.m.sum())})")
+        for i, (r_, c_, v_) in enumerate(zip(rows, cols, vals)):
+            if not (len(r_) == len(c_) == len(v_)):
+                raise ValueError(f"set_coo: constraint {i} has rows, cols and vals of different lengths")
         counts = np.array([len(r) for r in rows], dtype=np.int64)
         ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         r = np.concatenate(rows).astype(np.int32) if rows else np.zeros(0, np.int32)
@@ -128,6 +154,10 @@ class BatchSolver:
                                            dptr(Cc), dptr(cc)), "sdpx_batch_set_coo")
 
     def set_start(self, x_list, X_list, Y_list) -> None:
+        k2 = self.k.astype(np.int64) ** 2
+        x_list = _members("set_start x", x_list, self.m, self.B)
+        X_list = _members("set_start X", X_list, k2, self.B)
+        Y_list = _members("set_start Y", Y_list, k2, self.B)
         x = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in x_list])
         X = np.concatenate([np.asarray(v, dtype=np.float64).ravel(order="F") for v in X_list])
         Y = np.concatenate([np.asarray(v, dtype=np.float64).ravel(order="F") for v in Y_list])
@@ -173,9 +203,11 @@ class BatchSolver:
 
         Returns (dC list of symmetric k_p x k_p, db list of m_p vectors); members that could not be
         factored get NaN."""
-        G = np.concatenate([np.asarray(g, dtype=np.float64).ravel(order="F") for g in G_list])
-        hv = np.concatenate([np.asarray(v, dtype=np.float64).ravel() for v in h_list])
         k2 = self.k.astype(np.int64) ** 2
+        G_list = _members("adjoint G", G_list, k2, self.B)
+        h_list = _members("adjoint h", h_list, self.m, self.B)
+        G = np.concatenate([g.ravel(order="F") for g in G_list])
+        hv = np.concatenate([v.ravel() for v in h_list])
         dC = np.empty(int(k2.sum()))
         db = np.empty(int(self.m.sum()))
         check(self._lib.sdpx_batch_adjoint(self._h, dptr(G), dptr(hv), dptr(dC), dptr(db)), "sdpx_batch_adjoint")

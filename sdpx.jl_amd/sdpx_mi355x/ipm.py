@@ -38,7 +38,7 @@ class SDPResult:
     message: str = ""
     phase_times: dict = field(default_factory=dict)
     history: list = field(default_factory=list)
-    schur_trace: list = field(default_factory=list)
+    schur_trace: list = field(default_factory=list)  # first trace_schur steps: S and the iterate it was built at
     fp32_iterations: int = 0   # Newton steps whose Schur matrix was assembled in FP32 or BF16x3 (mixed precision)
     fp64_iterations: int = 0
 
@@ -91,7 +91,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
     b.schur_build()
     clock.lap("schur_assembly")
     if trace_schur is not None:
-        trace_schur.append(b.get_schur())
+        x_it, X_it, Y_it = b.get_iterate()
+        trace_schur.append(dict(S=b.get_schur(), x=x_it, X=X_it, Y=Y_it))
     fok, reg_attempts = b.factorize()
     clock.lap("kkt_factorization")
     if not fok:

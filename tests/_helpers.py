@@ -49,3 +49,27 @@ def regularization_matrix(m, smallest, seed=9):
     ev = np.linspace(1.0, 2.0, m)
     ev[0] = smallest
     return (Q * ev) @ Q.T
+
+
+# Refinement cases (kkt.jl:3891-3963): the factor is of S, the refinement runs against
+# M = S + rho*G with G = Q diag(ev*u) Q' sharing S's eigenvectors Q, so each correction
+# multiplies the error by -rho*u (componentwise in that basis): the contraction per step is
+# chosen, not left to rounding.  Each entry: (name, rho, u mode, fixed policy, max steps, expected steps).
+REFINE_CASES = [
+    ("cap", 0.1, "spread", False, 8, 8),        # >= 2x decrease every step, never reaches 64 eps
+    ("slow", 0.7, "ones", False, 8, 1),         # 0.7x decrease: stops after the first correction
+    ("diverge", 1.5, "ones", False, 8, 0),      # the correction grows the residual: rolled back
+    ("fixed", 0.1, "spread", True, 3, 3),       # fixed policy: exactly refine_steps passes
+    ("fixed_diverge", 1.5, "ones", True, 3, 0),  # fixed policy rolls back a worsening pass too
+]
+
+
+def refinement_case(rho, mode, m=200, seed=21):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    ev = np.exp(rng.uniform(0.0, np.log(100.0), m))
+    u = np.ones(m) if mode == "ones" else rng.uniform(0.2, 1.0, m)
+    S = (Q * ev) @ Q.T
+    M = (Q * (ev * (1.0 + rho * u))) @ Q.T
+    r = rng.standard_normal(m)
+    return (S + S.T) / 2, (M + M.T) / 2, r

@@ -53,3 +53,29 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "absent.so")
     with pytest.raises(_lib.SDPXError):
         _lib.load()
+
+
+def test_batch_inputs_checked_before_the_abi():
+    """Mis-shaped member arrays raise ValueError on the host, before any copy reads past them."""
+    import numpy as np
+
+    from sdpx_mi355x.batch import BatchSolver
+
+    bs = BatchSolver.__new__(BatchSolver)  # no device handle: every check below runs first
+    bs.k, bs.m, bs.B = np.array([3, 4], np.int32), np.array([5, 6], np.int32), 2
+    good_Av = [np.zeros((5, 9)), np.zeros((6, 16))]
+    good_C = [np.eye(3), np.eye(4)]
+    good_c = [np.zeros(5), np.zeros(6)]
+    with pytest.raises(ValueError, match="member 1 has 80 values"):
+        bs.set_dense([good_Av[0], np.zeros((5, 16))], good_C, good_c)
+    with pytest.raises(ValueError, match="1 members given"):
+        bs.set_dense(good_Av[:1], good_C, good_c)
+    with pytest.raises(ValueError, match="set_start X"):
+        bs.set_start(good_c, [np.eye(3), np.eye(3)], good_C)
+    with pytest.raises(ValueError, match="adjoint h"):
+        bs.adjoint(good_C, [np.zeros(5), np.zeros(5)])
+    rows = [np.array([0], np.int32)] * 11
+    with pytest.raises(ValueError, match="different lengths"):
+        bs.set_coo(rows, rows, [np.array([1.0, 2.0])] * 11, good_C, good_c)
+    with pytest.raises(ValueError, match="one entry list per constraint"):
+        bs.set_coo(rows[:10], rows[:10], [np.array([1.0])] * 10, good_C, good_c)

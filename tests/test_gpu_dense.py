@@ -9,6 +9,8 @@ import sdpx_oracle as O
 from sdpx_mi355x import MI355XDenseBackend, SolverOptions, ingest, solve
 from sdpx_mi355x import generators as G
 
+import _helpers as _H
+
 pytestmark = pytest.mark.gpu
 
 
@@ -304,3 +306,80 @@ def test_multi_block(gpu_lib):
     assert res.status == ref.status == "Optimal"
     assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
     assert abs(res.iterations - ref.iterations) <= 2
+
+
+@pytest.mark.parametrize("name,rho,mode,fixed,cap,expected", _H.REFINE_CASES)
+def test_refine_matches_oracle(gpu_lib, name, rho, mode, fixed, cap, expected):
+    """refine! (kkt_backend.jl:476-485) -> _refine_native_direction! (kkt.jl:3891-3963): steps, final
+    residual and refined dx against the oracle's refine_direction, on a factor of S applied to
+    M = S + rho G (a chosen contraction per correction, so no decision sits at rounding level)."""
+    S, M, r = _H.refinement_case(rho, mode)
+    m = S.shape[0]
+    _, prob, _ = _dense_case(4, m, seed=2)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_schur(S)
+        assert b.factorize() == (True, 0)
+        b.set_schur(M)  # the factor stays that of S
+        b.set_rhs(r)
+        b.solve_direction()
+        rn = float(np.max(np.abs(r)))
+        steps, res = b.refine(64 * O.EPS * max(rn, 1.0), cap, fixed)
+        dx, _, _ = b.get_direction()
+    finally:
+        b.close()
+    Lf, _, _ = O.factor_dense_kkt(S)
+    o = O.Options(refine_policy="fixed" if fixed else "auto", refine_steps=cap, refine_max_steps=cap)
+    dx_o, steps_o, res_o = O.refine_direction(M, Lf, r, O.solve_dense_kkt(Lf, r), o)
+    print(f"{name}: gpu steps {steps} res {res:.6e} | oracle steps {steps_o} res {res_o:.6e}", flush=True)
+    assert steps == steps_o == expected
+    assert abs(res - res_o) <= 1e-8 * res_o + 1e-14 * max(rn, 1.0)
+    assert np.max(np.abs(dx - dx_o)) <= 1e-12 * max(1.0, np.max(np.abs(dx_o)))
+
+
+def test_diagnostics_match_oracle(gpu_lib):
+    """a10 phase by phase on a two-block problem: per-block complementarity <X_l, Y_l> and the trial
+    <X_l + tX dX_l, Y_l + tY dY_l> (_predictor_complementarity_diagnostics!, step.jl:373-544; the
+    oracle's complementarity_diagnostics), solution_residuals (validation.jl:15-58) and the objectives
+    (interior_point.jl:7-14)."""
+    m = 48
+    rng = np.random.default_rng(8)
+    g1, g2 = G.dense_feasible(9, m, seed=31), G.dense_feasible(14, m, seed=32)
+    A, Cs = [g1.A_blocks(), g2.A_blocks()], [g1.C, g2.C]
+    prob, oprob = ingest(g1.c, A, Cs), O.ingest(g1.c, A, Cs)
+    X = [_random_pd(rng, 9, 5.0), _random_pd(rng, 14, 5.0)]
+    Y = [_random_pd(rng, 9, 5.0), _random_pd(rng, 14, 5.0)]
+    x = g1.x0 + 0.01 * rng.standard_normal(m)
+    mu = [0.1 * float(np.sum(X[l] * Y[l])) / X[l].shape[0] for l in range(2)]
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(x, X, Y)
+        b.compute_residuals(mu, affine=True, factor=True)
+        b.schur_build()
+        assert b.factorize() == (True, 0)
+        b.predictor_rhs()
+        b.solve_direction()
+        b.direction_blocks()
+        _, dX, dY = b.get_direction()
+        st = O.StepState()
+        o = O.Options()
+        O.compute_residuals(oprob, st, x, X, Y, mu, o, factor=True)
+        st.dX, st.dY = dX, dY  # the GPU's own direction: only the diagnostics are compared
+        for tX, tY in ((1.0, 1.0), (0.37, 0.81), (0.0, 0.5)):
+            comp, trial = b.complementarity(tX, tY)
+            comp_o, aff_o, _ = O.complementarity_diagnostics(oprob, st, X, Y, tX, tY, False)
+            blocks = [float(np.sum(X[l] * Y[l])) for l in range(2)]
+            trial_ref = [float(np.sum((X[l] + tX * dX[l]) * (Y[l] + tY * dY[l]))) for l in range(2)]
+            print(f"t=({tX},{tY}): gpu {comp} {trial} | oracle {comp_o} {aff_o}", flush=True)
+            assert np.allclose(comp, blocks, rtol=1e-13, atol=0)
+            assert abs(float(np.sum(comp)) - comp_o) <= 1e-13 * abs(comp_o)
+            assert abs(float(np.sum(trial)) - aff_o) <= 1e-12 * max(abs(aff_o), float(np.sum(comp)))
+            assert np.allclose(trial, trial_ref, rtol=1e-12, atol=1e-12 * max(blocks))
+        p_res, d_res = b.solution_residuals()
+        p_o, d_o = O.solution_residuals(oprob, x, X, Y)
+        assert abs(p_res - p_o) <= 1e-13 * max(1.0, p_o) and abs(d_res - d_o) <= 1e-13 * max(1.0, d_o)
+        pobj, dobj = b.objectives()
+        assert abs(pobj - float(oprob.c @ x)) <= 1e-13 * max(1.0, abs(pobj))
+        assert abs(dobj - O.dual_objective(oprob, Y)) <= 1e-13 * max(1.0, abs(dobj))
+    finally:
+        b.close()

@@ -1,7 +1,9 @@
 """The dense route at the benchmark's own size (one PSD block n = 400, m = 4000).
 
-The oracle cannot assemble this Schur matrix in test time, so the full size is
-pinned through properties that do not depend on it: the assembled FP64 Schur
+The Schur matrix of the first three Newton steps is compared with the oracle's
+at the GPU's own iterate, and the solve with the oracle's solve of the same
+problem (a committed fixture).  Properties that do not need the oracle pin the
+rest: the assembled FP64 Schur
 matrix against the matrix-free FP64 operator Av' vec(X^-1 (sum dx_i A_i) Y)
 (two independent GPU computations of S dx), bitwise repeatability of the
 assembly, agreement between different tilings (another split-K, tile order
@@ -11,15 +13,21 @@ factor with no regularisation, and a full solve to rel gap 1e-8 whose
 certificate the oracle checks on the CPU.
 """
 
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import sdpx_oracle as O
 from sdpx_mi355x import MI355XDenseBackend, SolverOptions, generators, ingest, solve
 
+from _helpers import rel_fro_lower
+
 pytestmark = pytest.mark.gpu
 
 K, M = 400, 4000
+FIXTURES = Path(__file__).resolve().parent / "golden" / "oracle_solves.json"
 
 
 @pytest.fixture(scope="module")
@@ -80,19 +88,36 @@ def test_fullsize_schur_properties(gpu_lib, case):
         b.close()
 
 
-def test_fullsize_solve_certificate(gpu_lib, case):
+def test_fullsize_solve_matches_oracle(gpu_lib, case):
+    """The full-size FP64 solve against the oracle: the Schur matrix of its first three Newton steps
+    against the oracle's schur_build at the GPU's own iterate (<= 1e-12 relative Frobenius), and the
+    oracle's own solve of the same problem (tests/golden/oracle_solves.json, written by
+    make_solve_fixtures.py): same status, iterations +-2, objective 1e-8 relative, valid certificate."""
     g, prob, A = case
     tol = 1e-8
     o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
     b = MI355XDenseBackend(prob)
     try:
         b.autotune()
-        res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
+        res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b, trace_schur=3)
     finally:
         b.close()
     print(f"{res.status} iters={res.iterations} pobj={res.pobj:.12e} gap={res.gap_rel:.2e}", flush=True)
-    assert res.status == "Optimal" and res.gap_rel <= tol
+    ref = json.loads(FIXTURES.read_text())["dense_n400_m4000"]
+    print(f"oracle {ref['status']} iters={ref['iterations']} pobj={ref['pobj']:.12e}", flush=True)
+    assert res.status == ref["status"] == "Optimal" and res.gap_rel <= tol
+    assert abs(res.iterations - ref["iterations"]) <= 2
+    assert abs(res.pobj - ref["pobj"]) <= 1e-8 * max(1.0, abs(ref["pobj"]))
     oprob = O.ingest(g.c, [A], [g.C])
     cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
                                          res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
     assert cert["valid"], cert
+    assert len(res.schur_trace) == 3
+    for it, t in enumerate(res.schur_trace, 1):
+        LX, ok1 = O.kchol(t["X"][0])
+        MY, ok2 = O.kchol(t["Y"][0])
+        assert ok1 and ok2
+        S_ref = O.schur_build(oprob, [LX], [MY])
+        rel = rel_fro_lower(t["S"], S_ref)
+        print(f"iteration {it}: Schur rel Frobenius vs oracle {rel:.2e}", flush=True)
+        assert rel <= 1e-12

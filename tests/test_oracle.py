@@ -7,6 +7,7 @@ import pytest
 import scipy.linalg as sla
 
 import sdpx_oracle as O
+import _helpers as _H
 from _helpers import golden, owner_problem, random_pd, regularization_matrix, rel_fro_lower
 from sdpx_mi355x import generators as G
 
@@ -179,3 +180,22 @@ def test_adaptive_policy_cold_start_uses_fallback_values():
     # ratio 0.5, affine 0.5 -> sigma = clamp(0.8*max(0.25, 0.08) + 0.2*0.1)
     assert math.isclose(par.sigma, 0.8 * 0.25 + 0.2 * 0.1)
     assert par.primal_fraction_to_boundary == 19 / 20
+
+
+@pytest.mark.parametrize("name,rho,mode,fixed,cap,expected", _H.REFINE_CASES)
+def test_refine_direction_steps_follow_the_contraction(name, rho, mode, fixed, cap, expected):
+    """_refine_native_direction! (kkt.jl:3891-3963) on a factor of S applied to M = S + rho G: the
+    error contracts by rho*u per correction, so the stop rules decide with wide margins."""
+    S, M, r = _H.refinement_case(rho, mode)
+    Lf, ok, _ = O.factor_dense_kkt(S)
+    dx0 = O.solve_dense_kkt(Lf, r)
+    res0, _ = O.direction_residual(M, r, dx0)
+    o = O.Options(refine_policy="fixed" if fixed else "auto", refine_steps=cap, refine_max_steps=cap)
+    dx, steps, res = O.refine_direction(M, Lf, r, dx0, o)
+    assert steps == expected
+    if expected == 0:
+        assert np.array_equal(dx, dx0) and res == res0
+    else:
+        assert res < res0 * (0.71 if mode == "ones" else 0.5 ** expected)
+        # the exact solution of M dx = r is approached, not S's
+        assert np.linalg.norm(dx - np.linalg.solve(M, r)) < np.linalg.norm(dx0 - np.linalg.solve(M, r))
