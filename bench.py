@@ -10,8 +10,8 @@ Prints one JSON line:
                (2k^3 m + k^2 m(m+1) + m^3/3) / wall time of the timed solves
   roofline   = the Schur Gram kernel (S = Phat' Phat, k^2 m(m+1) flops per
                launch) timed with HIP events on the library's stream
-  cpu_baseline = the oracle's Float64 Schur assemble+factor on the host cores
-               for a bounded sample (same n, fewer constraints)
+  cpu_baseline = the oracle's Float64 Newton step on the host cores for a
+               bounded sample (one step of the same problem, median of 3)
 """
 
 from __future__ import annotations
@@ -40,30 +40,104 @@ def newton_flops(k: int, m: int) -> float:
     return 2.0 * k ** 3 * m + float(k) ** 2 * m * (m + 1) + m ** 3 / 3.0
 
 
-def cpu_baseline(k: int, m_sample: int, seed: int = 1) -> dict:
-    """Oracle Schur assemble+factor (OpenBLAS via numpy/scipy) on a bounded sample."""
+def _cpu_threads() -> int:
+    """Host threads this run may use (OMP_NUM_THREADS when set: the GPU host's share of cores)."""
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def _timed_median(fn, reps: int = 3) -> float:
+    fn()  # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baseline(k: int, m: int, seed: int = 0) -> dict:
+    """The oracle (the reference algorithm restated on numpy/OpenBLAS) on a bounded sample of the
+    FP64 workload: one complete Newton step (residuals, Schur assembly, Cholesky with the
+    regularisation ladder, both solves with refinement, direction recovery) at the start point of
+    the n x m problem, median of 3 after a warm-up, with BLAS threads = the host threads and the
+    Cholesky capped at clamp(m / 256, 1, threads) as the reference does (step.jl:154-157)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     import sdpx_oracle as O
     from sdpx_mi355x import generators as G
 
-    g = G.dense_feasible(k, m_sample, seed=seed)
+    nthreads = _cpu_threads()
+    chol_threads = min(max(m // 256, 1), nthreads)
+    g = G.dense_feasible(k, m, seed=seed)
     prob = O.ingest(g.c, [g.A_blocks()], [g.C])
-    rng = np.random.default_rng(seed)
-    Q = rng.standard_normal((k, k))
-    X = Q @ Q.T / k + np.eye(k)
-    LX, _ = O.kchol(X)
-    MY, _ = O.kchol(np.eye(k) * 2.0)
-    O.schur_build(prob, [LX], [MY])  # warm
-    t0 = time.perf_counter()
-    S = O.schur_build(prob, [LX], [MY])
-    O.factor_dense_kkt(S)
-    dt = time.perf_counter() - t0
-    flops = newton_flops(k, m_sample)
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": os.cpu_count(), "kind": "port",
-            "sample": f"oracle FP64 Schur assemble+factor, n={k}, m={m_sample} (one Newton step's "
-                      f"assembly+Cholesky), {dt:.2f} s, OpenBLAS threads={os.environ.get('OMP_NUM_THREADS', 'default')}"}
+    x0 = g.x0
+    del g
+    o = O.Options(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8)
+    X, Y = [np.eye(k)], [np.eye(k)]
+    mu = [o.beta * float(np.sum(X[0] * Y[0])) / k]
+
+    def step():
+        res = O.newton_step(prob, o, o, x0, X, Y, mu, O.Controller(o), 1, 1.0, 1.0, 1.0)
+        assert res["status"] == "ok"
+
+    with threadpool_limits(limits=nthreads):
+        O.CHOLESKY_THREADS = chol_threads
+        try:
+            dt = _timed_median(step)
+        finally:
+            O.CHOLESKY_THREADS = None
+    return {"value": newton_flops(k, m) / dt / 1e12, "unit": "TFLOP/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle FP64 Newton step (all phases) at the start point of n={k}, m={m}: median "
+                      f"{dt:.2f} s of 3 after a warm-up; BLAS threads {nthreads}, Cholesky threads {chol_threads}",
+            "blas_threads": nthreads, "cholesky_threads": chol_threads}
+
+
+def cpu_reference_solves() -> dict:
+    """The reference's own CPU case run by the oracle: full solves of n=50, m=200 to gap 1e-8 at one
+    thread and one BLAS thread, median of 3 after a warm-up (problem generation excluded)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    import sdpx_oracle as O
+    from sdpx_mi355x import generators as G
+
+    k, m, tol = 50, 200, 1e-8
+    g = G.dense_feasible(k, m, seed=0)
+    prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    o = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    box = {}
+
+    def one():
+        box["r"] = O.solve(prob, o, x0=g.x0, X0=[np.eye(k)], Y0=[np.eye(k)])
+
+    with threadpool_limits(limits=1):
+        dt = _timed_median(one)
+    r = box["r"]
+    return {"workload": f"dense SDP n={k}, m={m}, rel gap {tol:g}", "threads": 1, "blas_threads": 1,
+            "status": r.status, "iterations": r.iterations, "seconds_per_solve": round(dt, 4),
+            "solves_per_s": round(1 / dt, 3), "schur_tflops": round(r.iterations * newton_flops(k, m) / dt / 1e12, 5)}
+
+
+ORACLE_SOLVES = ROOT / "tests" / "golden" / "oracle_solves.json"
+
+
+def oracle_solve_record(name: str):
+    """The oracle's solve of a dense workload (written by tests/golden/make_solve_fixtures.py)."""
+    if not ORACLE_SOLVES.exists():
+        return None
+    return json.loads(ORACLE_SOLVES.read_text()).get(name)
+
+
+def vs_oracle(name: str, pobj: float, iterations: int, status: str) -> dict | None:
+    ref = oracle_solve_record(name)
+    if ref is None:
+        return None
+    return {"oracle_status": ref["status"], "status_matches": status == ref["status"],
+            "oracle_iterations": ref["iterations"], "iterations_diff": iterations - ref["iterations"],
+            "objective_rel_err": abs(pobj - ref["pobj"]) / max(1.0, abs(ref["pobj"]))}
 
 
 def soc_leg(ncones: int = 16_000_000, q: int = 4, reps: int = 10) -> dict:
@@ -153,14 +227,22 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
                 Gs.append(E / np.linalg.norm(E))
             hs = [v / np.linalg.norm(v) for v in (rng.standard_normal(m) for m in ms)]
             t0 = time.perf_counter()
-            restored, _ = bs.restore_checkpoint()
+            restored, ck_mu = bs.restore_checkpoint()
             bs.center(4)
             dC, db = bs.adjoint(Gs, hs)
             ta = time.perf_counter() - t0
+            # a member without a checkpoint (NaN mu) would be differentiated at its final iterate,
+            # which is not a central-path point: its gradients are withheld
+            masked = [p for p in range(B) if not np.isfinite(ck_mu[p])]
+            for p in masked:
+                dC[p][:] = np.nan
+                db[p][:] = np.nan
             finite = sum(bool(np.all(np.isfinite(a)) and np.all(np.isfinite(b_))) for a, b_ in zip(dC, db))
             adjoint = {"seconds": round(ta, 4), "gradients_per_s": round(B / ta, 1), "checkpoints": restored,
-                       "finite_gradients": finite,
-                       "note": "dL/dC, dL/db at the central-path point of the first iterate with mu <= 1e-6"}
+                       "masked_no_checkpoint": masked, "finite_gradients": finite,
+                       "note": "dL/dC, dL/db at the central-path point of the first iterate with mu <= 1e-6; "
+                               "measured distance from the optimum's derivative (mu -> 0 limit): dL/db ~1e-4, "
+                               "dL/dC ~1e-3 relative (tests/test_adjoint.py)"}
     it = np.asarray(res.iterations, dtype=np.float64)
     opt = np.array([s == "Optimal" for s in res.status])
     dense_F = sum(it[p] * newton_flops(ks[p], ms[p]) for p in range(B))
@@ -169,9 +251,14 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
     out = {"workload": desc, "problems": B, "rel_gap": tol, "device_GB": round(dev_gb, 2),
            "solves_per_s": round(B / per, 1), "newton_iterations_per_s": round(float(it.sum()) / per, 1),
            "seconds_per_batch": round(per, 4), "batched_steps_launched": res.launched,
-           "optimal": int(opt.sum()), "statuses": {s: res.status.count(s) for s in sorted(set(res.status))},
+           "optimal": int(opt.sum()), "all_optimal": bool(opt.all()),
+           "non_optimal_members": [[p, res.status[p], int(res.iterations[p])] for p in range(B) if not opt[p]],
+           "statuses": {s: res.status.count(s) for s in sorted(set(res.status))},
            "max_gap_of_optimal": float(np.max(res.gap_rel[opt])) if opt.any() else None,
            "schur_tflops_dense_count": round(dense_F / per / 1e12, 3)}
+    if not opt.all():
+        out["non_optimal_note"] = ("the oracle ends the members listed in tests/test_status_rounding.py with "
+                                   "these statuses too, under another valid Float64 summation order")
     if which == "sparse":
         out["schur_tflops_reference_sparse_count"] = round(sparse_F / per / 1e12, 3)
     if adjoint is not None:
@@ -247,6 +334,8 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
             out[prec] = {"status": r.status, "iterations": r.iterations, "fp32_iterations": r.fp32_iterations,
                          "seconds": round(dt, 3), "schur_tflops": round(r.iterations * F / dt / 1e12, 2),
                          "pobj": r.pobj, "gap": r.gap_rel}
+        for prec, (r, dt) in res.items():
+            out[prec]["vs_oracle"] = vs_oracle(f"dense_n{k}_m{m}", r.pobj, r.iterations, r.status)
         p64, pm, pb = res["fp64"][0].pobj, res["mixed"][0].pobj, res["mixed_bf16x3"][0].pobj
         out["objective_rel_diff"] = abs(pm - p64) / max(1.0, abs(p64))
         out["objective_rel_diff_bf16x3"] = abs(pb - p64) / max(1.0, abs(p64))
@@ -294,7 +383,7 @@ def main():
     ap.add_argument("--m", type=int, default=4000)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample-m", type=int, default=1500)
+    ap.add_argument("--cpu-sample-m", type=int, default=4000, help="constraints of the oracle's Newton-step sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-soc", action="store_true", help="skip the Lorentz-cone kernel leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched legs")
@@ -387,6 +476,7 @@ def main():
         "newton_iterations_per_s": round(iters / wall, 3),
         "iterations_per_solve": iters / args.steps,
         "all_optimal": ok,
+        "vs_oracle": vs_oracle(f"dense_n{k}_m{m}", results[-1].pobj, results[-1].iterations, results[-1].status),
         "schur_phase_tflops": round(rp.iterations * F / asm_fac_s / 1e12, 3) if asm_fac_s > 0 else None,
         "phase_seconds_per_solve": {kk: round(v, 5) for kk, v in phase.items()},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -413,6 +503,13 @@ def main():
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(k, args.cpu_sample_m)
+            log(f"cpu_baseline: {out['cpu_baseline']}")
+            out["cpu_reference_solves"] = cpu_reference_solves()
+            rec = oracle_solve_record(f"dense_n{k}_m{m}")
+            if rec is not None:
+                out["cpu_reference_solves"]["full_solve_of_this_workload"] = {
+                    "seconds": rec["seconds"], "iterations": rec["iterations"], "blas_threads": rec["blas_threads"],
+                    "where": "the build container (8 cores), when the oracle fixture was written"}
         except Exception as e:  # the baseline is reported, never the measurement
             log(f"cpu baseline failed: {e}")
             out["cpu_baseline"] = None

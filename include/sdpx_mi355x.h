@@ -253,6 +253,23 @@ int sdpx_batch_set_dense(sdpx_batch_t* h, const double* Av, const double* C, con
  * panels are densified on the device (SparseBlockCOO, types/constraints.jl:59-109). */
 int sdpx_batch_set_coo(sdpx_batch_t* h, const long long* ptr, const int* row, const int* col, const double* val,
                        const double* C, const double* c);
+/* The reference's SparseBlockCOO as stored (types/constraints.jl:59-109, built by
+ * build_block_coo :79-109): per member, m_p + 1 Int32 positions ptr (1-based,
+ * ptr[0] = 1) followed in the concatenation by the next member's; the
+ * member's entries (lin, row, col Int32 1-based, val) follow the previous
+ * member's.  Every stored entry of the symmetric A_i is given (both
+ * triangles), lin = (col - 1) k + row.  Each lower entry must mirror an upper
+ * entry of the same value (else SDPX_INVALID_ARGUMENT); the upper ones are
+ * loaded as by sdpx_batch_set_coo. */
+int sdpx_batch_set_block_coo(sdpx_batch_t* h, const int* ptr, const int* lin, const int* row, const int* col,
+                             const double* val, const double* C, const double* c);
+/* Host-only (no device call): one block's SparseBlockCOO (side k, m positions,
+ * layout as above) converted to this ABI's COO (out_ptr m + 1 int64 0-based,
+ * out_row <= out_col 0-based, out_val); *out_nnz = entries kept.  Null
+ * outputs skip; the outputs need at most ptr[m] - 1 entries. */
+int sdpx_block_coo_convert(int k, int m, const int* ptr, const int* lin, const int* row, const int* col,
+                           const double* val, long long* out_ptr, int* out_row, int* out_col, double* out_val,
+                           long long* out_nnz);
 /* Start point, members concatenated: x (m_p), X and Y (k_p^2 each, column-major). */
 int sdpx_batch_set_start(sdpx_batch_t* h, const double* x, const double* X, const double* Y);
 /* Runs the interior-point loop (interior_point.jl:1323-2420) of every member to

@@ -731,11 +731,25 @@ def schur_mul(S: np.ndarray, x) -> np.ndarray:
 # --------------------------------------------------------------------------
 # KKT factor / solve / refinement (kkt.jl, kkt_backend.jl)
 # --------------------------------------------------------------------------
+# BLAS threads of the Schur Cholesky (None: leave the process setting).  The reference
+# caps them at clamp(m / 256, 1, threads) around this phase (step.jl:124-157).
+CHOLESKY_THREADS = None
+
+
 def factor_dense_kkt(S: np.ndarray):
     """Schur Cholesky with the regularisation ladder (kkt.jl:1404-1464).
 
     Returns (L, ok, reg_attempts).
     """
+    if CHOLESKY_THREADS is not None:
+        from threadpoolctl import threadpool_limits
+
+        with threadpool_limits(limits=int(CHOLESKY_THREADS)):
+            return _factor_dense_kkt(S)
+    return _factor_dense_kkt(S)
+
+
+def _factor_dense_kkt(S: np.ndarray):
     m = S.shape[0]
     Sl = np.tril(S)
     c, info = lapack.dpotrf(np.asfortranarray(Sl), lower=1, clean=1)

@@ -1678,6 +1678,121 @@ int sdpx_batch_set_coo(sdpx_batch_t* h, const long long* ptr, const int* row, co
   return kOk;
 }
 
+// SparseBlockCOO as stored (types/constraints.jl:59-109): positions 1..m with 1-based Int32
+// ptr (ptr[0] = 1), every stored entry of the symmetric A_i (both triangles), 1-based row /
+// col and lin = (col - 1) k + row.  Converted to this ABI's COO: 0-based, row <= col, in
+// stored order; a lower entry must mirror an upper entry of the same position with the same
+// value (checked), and is then dropped.
+static int block_coo_convert(int k, int m, const int* ptr, const int* lin, const int* row, const int* col,
+                             const double* val, std::vector<long long>& optr, std::vector<int>& orow,
+                             std::vector<int>& ocol, std::vector<double>& oval, const char* who) {
+  if (k < 1 || m < 1 || !ptr) {
+    set_last_error(std::string(who) + ": invalid block side, count or null ptr");
+    return kInvalidArgument;
+  }
+  if (ptr[0] != 1) {
+    set_last_error(std::string(who) + ": ptr[1] must be 1 (1-based positions)");
+    return kInvalidArgument;
+  }
+  for (int i = 0; i < m; ++i)
+    if (ptr[i + 1] < ptr[i]) {
+      set_last_error(std::string(who) + ": ptr must be non-decreasing");
+      return kInvalidArgument;
+    }
+  const long long total = (long long)ptr[m] - 1;
+  if (total > 0 && (!lin || !row || !col || !val)) {
+    set_last_error(std::string(who) + ": null lin, row, col or val with entries present");
+    return kInvalidArgument;
+  }
+  for (int i = 0; i < m; ++i) {
+    optr.push_back((long long)orow.size());
+    std::vector<std::pair<long long, double>> upper, lower;  // (0-based lin of the upper twin, value)
+    for (long long e = (long long)ptr[i] - 1; e < (long long)ptr[i + 1] - 1; ++e) {
+      const int r = row[e], c = col[e];
+      if (r < 1 || r > k || c < 1 || c > k || lin[e] != (c - 1) * k + r) {
+        set_last_error(std::string(who) + ": entry " + std::to_string(e + 1) + " of position " +
+                       std::to_string(i + 1) + " has row/col outside the block or lin != (col-1)k+row");
+        return kInvalidArgument;
+      }
+      if (r <= c) {
+        orow.push_back(r - 1);
+        ocol.push_back(c - 1);
+        oval.push_back(val[e]);
+        if (r < c) upper.push_back({(long long)(c - 1) * k + (r - 1), val[e]});
+      } else {
+        lower.push_back({(long long)(r - 1) * k + (c - 1), val[e]});
+      }
+    }
+    // each stored lower entry has an upper twin with the same value (sums of duplicates compared)
+    auto merge = [](std::vector<std::pair<long long, double>>& v) {
+      std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+      std::vector<std::pair<long long, double>> out;
+      for (const auto& x : v) {
+        if (!out.empty() && out.back().first == x.first)
+          out.back().second += x.second;
+        else
+          out.push_back(x);
+      }
+      v.swap(out);
+    };
+    merge(upper);
+    merge(lower);
+    if (upper.size() != lower.size()) {
+      set_last_error(std::string(who) + ": position " + std::to_string(i + 1) +
+                     " does not store both triangles of a symmetric matrix");
+      return kInvalidArgument;
+    }
+    for (size_t q = 0; q < upper.size(); ++q)
+      if (upper[q].first != lower[q].first || upper[q].second != lower[q].second) {
+        set_last_error(std::string(who) + ": position " + std::to_string(i + 1) + " is not symmetric");
+        return kInvalidArgument;
+      }
+  }
+  return kOk;
+}
+
+int sdpx_block_coo_convert(int k, int m, const int* ptr, const int* lin, const int* row, const int* col,
+                           const double* val, long long* out_ptr, int* out_row, int* out_col, double* out_val,
+                           long long* out_nnz) {
+  std::vector<long long> optr;
+  std::vector<int> orow, ocol;
+  std::vector<double> oval;
+  SDPX_CHECK(block_coo_convert(k, m, ptr, lin, row, col, val, optr, orow, ocol, oval, "sdpx_block_coo_convert"));
+  optr.push_back((long long)orow.size());
+  if (out_nnz) *out_nnz = (long long)orow.size();
+  if (out_ptr) std::copy(optr.begin(), optr.end(), out_ptr);
+  if (out_row) std::copy(orow.begin(), orow.end(), out_row);
+  if (out_col) std::copy(ocol.begin(), ocol.end(), out_col);
+  if (out_val) std::copy(oval.begin(), oval.end(), out_val);
+  return kOk;
+}
+
+int sdpx_batch_set_block_coo(sdpx_batch_t* h, const int* ptr, const int* lin, const int* row, const int* col,
+                             const double* val, const double* C, const double* c) {
+  SDPX_CHECK(check_batch(h));
+  h->have_data = false;
+  if (!ptr || !C || !c) {
+    set_last_error("sdpx_batch_set_block_coo: null data");
+    return kInvalidArgument;
+  }
+  std::vector<long long> optr;
+  std::vector<int> orow, ocol;
+  std::vector<double> oval;
+  long long pofs = 0, eofs = 0;
+  for (int p = 0; p < h->B; ++p) {
+    const Member& M = h->mem[p];
+    const int* pp = ptr + pofs;
+    const long long ne = (long long)pp[M.m] - 1;
+    SDPX_CHECK(block_coo_convert(M.k, M.m, pp, lin ? lin + eofs : nullptr, row ? row + eofs : nullptr,
+                                 col ? col + eofs : nullptr, val ? val + eofs : nullptr, optr, orow, ocol, oval,
+                                 "sdpx_batch_set_block_coo"));
+    pofs += M.m + 1;
+    eofs += ne;
+  }
+  optr.push_back((long long)orow.size());
+  return sdpx_batch_set_coo(h, optr.data(), orow.data(), ocol.data(), oval.data(), C, c);
+}
+
 int sdpx_batch_set_start(sdpx_batch_t* h, const double* x, const double* X, const double* Y) {
   SDPX_CHECK(check_batch(h));
   if (!x || !X || !Y) {

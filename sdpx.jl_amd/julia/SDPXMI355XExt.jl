@@ -17,6 +17,10 @@
     934-936; interior_point.jl:1962, 2221) route them to the methods at the end
     of this file when `ws.backend isa MI355XDenseBackend`.
 
+    The batched route (sdpx_batch_*) and the differentiable layer with its
+    ChainRulesCore.rrule (the adjoint, sdpx_batch_adjoint) have no counterpart
+    in SDPX; they are bound below as well.
+
     One device handle per SDPX Workspace, called sequentially (the reference's
     one-provider-per-workspace rule, ext/SDPXMultiFloatLinearAlgebraExt.jl:14-20).
     Device buffers are owned by the handle; every host array passed here is
@@ -27,6 +31,7 @@ module SDPXMI355XExt
 using SDPX
 using LinearAlgebra
 using Libdl
+using ChainRulesCore
 
 const LIB = Ref{Ptr{Cvoid}}(C_NULL)
 
@@ -133,15 +138,285 @@ function get_iterate!(h::DenseHandle, x, X, Y)
 end
 
 # ---------------------------------------------------------------------------
-# LA provider seam
+# LA provider seam (src/la_backend.jl:11-22, 283-506; types/backends.jl:26-48)
 # ---------------------------------------------------------------------------
-const _CAPABILITIES = (:chol, :cholesky_factor!, :solve, :trsv_lower, :trsv_transpose,
-                       :syrk, :mul_owned, :dot, :multi_rhs)
+# Operation symbols as the planner checks them (la_backend.jl:24-31, 80-97); the
+# capability facts below are what the device path implements for Float64.
+const _CAPABILITIES = (:chol, :cholesky_factor!, :solve, :multi_rhs, :trsm, :trsv_lower, :trsv_transpose,
+                       :syrk, :mul_owned, :dot, :norminf)
+
+const _CAPABILITY_MODEL = SDPX.LAProviderCapabilities(
+    cholesky=true,
+    factor_solve=true,
+    multi_rhs=true,
+    iterative_refinement=true,
+    mixed_precision_residual=true,   # FP32 / BF16x3 Schur assembly refined against the FP64 operator
+    threading=false,                 # one device stream per handle; insensitive to BLAS threads
+    dot=true,
+    norminf=true,
+    mul_owned=true,
+    syrk=true,
+    triangular_solve=true,
+)
 
 function SDPX.la_provider_descriptor(::Type{Float64}, ::Int=1)
-    available = _device_count() > 0
-    return (available=available, provider=:mi355x,
-            capabilities=available ? _CAPABILITIES : ())
+    ndev = _device_count()
+    ndev > 0 || return (available=false, provider=:none, capabilities=(),
+                        capability_model=SDPX.LAProviderCapabilities(), reason=:no_device)
+    return (available=true, provider=:mi355x, capabilities=_CAPABILITIES,
+            capability_model=_CAPABILITY_MODEL)
+end
+
+"""Provider payload of a planned :mi355x LA backend: the device it runs on.  The
+dense handle itself is created with the Workspace (`DenseHandle(prob)`), because
+it owns the problem's constraint panel in HBM."""
+struct MI355XProvider
+    device::Int
+end
+
+SDPX.la_provider_capability_model(::MI355XProvider) = _CAPABILITY_MODEL
+
+"""Setup hook of the planner branch for `requested === :mi355x` (INTEGRATION.md,
+modelled on instantiate_multifloat_la_backend, la_backend.jl:264-276)."""
+function instantiate_mi355x_la_backend(::Type{Float64}, config::SDPX.LABackendConfiguration, threads::Int=1)
+    config.provider === :mi355x || return nothing
+    _device_count() > 0 || return nothing
+    return MI355XProvider(0)
+end
+instantiate_mi355x_la_backend(::Type, ::SDPX.LABackendConfiguration, ::Int=1) = nothing
+
+"""The instantiated :mi355x LA backend.  On the dense Schur route its operations run inside
+MI355XDenseBackend on the device (ws.S, its factor and the directions never leave HBM), so the
+host-buffer la_* entry points are fail-closed here (la_backend.jl:1695-1745)."""
+struct MI355XLABackend <: SDPX.AbstractLABackend
+    arithmetic::Symbol
+    provider::MI355XProvider
+end
+
+SDPX.la_backend_capabilities(b::MI355XLABackend) = _CAPABILITY_MODEL
+
+for op in (:la_chol!, :la_cholesky_factor!, :la_trsm!, :la_trsv_lower!, :la_trsv_transpose!, :la_syrk!,
+           :la_mul_owned!, :la_axpby_owned!, :la_dot, :la_norminf)
+    @eval SDPX.$op(::MI355XLABackend, args...) = throw(ArgumentError(
+        "MI355X provider: $($(QuoteNode(op))) on host buffers is not provided; the dense Schur route runs it " *
+        "on the device inside MI355XDenseBackend"))
+end
+
+# ---------------------------------------------------------------------------
+# Mixed precision (kernels/mixed_precision_kkt.jl template, FP64 target)
+# ---------------------------------------------------------------------------
+"""Schur Gram precision of the handle: 0 FP64, 1 FP32 MFMA, 2 BF16x3 (bf16 hi/lo
+split, three products accumulated in FP32)."""
+set_precision!(h::DenseHandle, precision::Integer) =
+    _check(ccall(_sym(:sdpx_dense_set_precision), Cint, (Ptr{Cvoid}, Cint), h.ptr, precision),
+           "sdpx_dense_set_precision")
+
+"""rho = r - S dx against the matrix-free FP64 Schur operator -> (||rho||_inf, ||r||_inf)."""
+function exact_residual(h::DenseHandle)
+    out = zeros(2)
+    _check(ccall(_sym(:sdpx_dense_exact_residual), Cint, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, out),
+           "sdpx_dense_exact_residual")
+    return out[1], out[2]
+end
+
+"""Refinement of dx against the FP64 operator with the current (low-precision-assembled)
+factor as preconditioner (_refine_mixed_direction!, mixed_precision_kkt.jl:1634-1713)."""
+function refine_exact!(h::DenseHandle, abstol::Float64, max_steps::Integer)
+    steps = Ref{Cint}(0)
+    residual = Ref{Float64}(0.0)
+    _check(ccall(_sym(:sdpx_dense_refine_exact), Cint, (Ptr{Cvoid}, Float64, Cint, Ref{Cint}, Ref{Float64}),
+                 h.ptr, abstol, max_steps, steps, residual), "sdpx_dense_refine_exact")
+    return Int(steps[]), residual[]
+end
+
+# ---------------------------------------------------------------------------
+# Batched route: many single-block SDPs in lock step (no counterpart in SDPX)
+# ---------------------------------------------------------------------------
+"""sdpx_batch_options_t, field for field."""
+mutable struct BatchOptions
+    beta::Float64; gamma::Float64; eps_gap::Float64; eps_primal::Float64; eps_dual::Float64
+    min_step::Float64; omega_step::Float64; stall_tolerance::Float64; refine_tol::Float64
+    adaptive_sigma_max::Float64
+    iter_max::Cint; max_restarts::Cint; max_centering::Cint; stall_iterations::Cint; restart::Cint
+    refine_policy::Cint; refine_steps::Cint; refine_max_steps::Cint; adaptive::Cint; step_rule::Cint
+    precision::Cint
+    cond_limit::Float64; refine_target::Float64; adjoint_mu::Float64
+    BatchOptions() = new()
+end
+
+"""Reference defaults (SolverOptions{Float64}(), types/workspaces.jl:101-228) at tolerance tol,
+overridden by the options SDPX passes."""
+function BatchOptions(opts::SDPX.SolverOptions{Float64}; precision::Integer=0)
+    o = BatchOptions()
+    _check(ccall(_sym(:sdpx_batch_default_options), Cint, (Ref{BatchOptions}, Float64), o, opts.ϵ_gap),
+           "sdpx_batch_default_options")
+    o.beta, o.gamma = opts.β, opts.γ
+    o.eps_gap, o.eps_primal, o.eps_dual = opts.ϵ_gap, opts.ϵ_primal, opts.ϵ_dual
+    o.min_step, o.omega_step = opts.min_step, opts.omega_step
+    o.iter_max, o.refine_steps, o.refine_max_steps = opts.iter_max, opts.refine_steps, opts.refine_max_steps
+    o.refine_policy = opts.refine_policy === :fixed ? 1 : 0
+    o.adaptive = opts.parameter_strategy === :adaptive ? 1 : 0
+    o.adaptive_sigma_max = opts.adaptive_sigma_max
+    o.precision = precision
+    return o
+end
+
+mutable struct BatchHandle
+    ptr::Ptr{Cvoid}
+    k::Vector{Int}
+    m::Vector{Int}
+end
+
+function BatchHandle(k::AbstractVector{<:Integer}, m::AbstractVector{<:Integer}; device::Int=0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall(_sym(:sdpx_batch_create), Cint, (Cint, Cint, Ptr{Cint}, Ptr{Cint}, Ref{Ptr{Cvoid}}),
+                 device, length(k), Cint.(k), Cint.(m), out), "sdpx_batch_create")
+    h = BatchHandle(out[], collect(Int, k), collect(Int, m))
+    finalizer(h) do x
+        x.ptr == C_NULL || ccall(_sym(:sdpx_batch_destroy), Cint, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    return h
+end
+
+"""Members given as SDPX problems with one dense PSD block each (DenseCons.Av[1] as the panel)."""
+function set_data!(h::BatchHandle, probs::AbstractVector{<:SDPX.SDPProblem{Float64}})
+    Av = reduce(vcat, (vec((p.cons::SDPX.DenseCons{Float64}).Av[1]) for p in probs))
+    C = reduce(vcat, (vec(p.C[1]) for p in probs))
+    c = reduce(vcat, (p.c for p in probs))
+    _check(ccall(_sym(:sdpx_batch_set_dense), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                 h.ptr, Av, C, c), "sdpx_batch_set_dense")
+end
+
+"""Members' sparse constraints as SDPX stores them: one SparseBlockCOO per member
+(types/constraints.jl:59-109), passed as stored (1-based Int32, both triangles, lin)."""
+function set_data!(h::BatchHandle, coos::AbstractVector{<:SDPX.SparseBlockCOO{Float64}}, Cs, cs)
+    ptr = reduce(vcat, (c.ptr for c in coos))
+    lin = reduce(vcat, (c.lin for c in coos); init=Int32[])
+    row = reduce(vcat, (c.row for c in coos); init=Int32[])
+    col = reduce(vcat, (c.col for c in coos); init=Int32[])
+    val = reduce(vcat, (c.val for c in coos); init=Float64[])
+    _check(ccall(_sym(:sdpx_batch_set_block_coo), Cint,
+                 (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}, Ptr{Float64},
+                  Ptr{Float64}),
+                 h.ptr, ptr, lin, row, col, val, reduce(vcat, vec.(Cs)), reduce(vcat, cs)),
+           "sdpx_batch_set_block_coo")
+end
+
+function set_start!(h::BatchHandle, xs, Xs, Ys)
+    _check(ccall(_sym(:sdpx_batch_set_start), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                 h.ptr, reduce(vcat, xs), reduce(vcat, vec.(Xs)), reduce(vcat, vec.(Ys))),
+           "sdpx_batch_set_start")
+end
+
+const BATCH_STATUS = (:Optimal, :IterLimit, :NumericalBreakdown, :Stalled, :InsufficientPrecision,
+                      :MaxRestartsExceeded)
+
+"""Solve every member; returns (status, iterations, pobj, dobj, x, X, Y) per member."""
+function batch_solve!(h::BatchHandle, o::BatchOptions)
+    launched = Ref{Cint}(0)
+    _check(ccall(_sym(:sdpx_batch_solve), Cint, (Ptr{Cvoid}, Ref{BatchOptions}, Ref{Cint}), h.ptr, o, launched),
+           "sdpx_batch_solve")
+    B = length(h.k)
+    status, iters = zeros(Cint, B), zeros(Cint, B)
+    pobj, dobj = zeros(B), zeros(B)
+    x = zeros(sum(h.m))
+    X, Y = zeros(sum(abs2, h.k)), zeros(sum(abs2, h.k))
+    _check(ccall(_sym(:sdpx_batch_get_result), Cint,
+                 (Ptr{Cvoid}, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                  Ptr{Float64}), h.ptr, status, iters, pobj, dobj, x, X, Y), "sdpx_batch_get_result")
+    xs = _split(x, h.m)
+    Xs = [reshape(b, k, k) for (b, k) in zip(_split(X, abs2.(h.k)), h.k)]
+    Ys = [reshape(b, k, k) for (b, k) in zip(_split(Y, abs2.(h.k)), h.k)]
+    return (status=[BATCH_STATUS[s+1] for s in status], iterations=Int.(iters), pobj=pobj, dobj=dobj,
+            x=xs, X=Xs, Y=Ys)
+end
+
+function _split(v, sizes)
+    out = Vector{Vector{Float64}}(undef, length(sizes))
+    o = 0
+    for (i, n) in enumerate(sizes)
+        out[i] = v[o+1:o+n]
+        o += n
+    end
+    return out
+end
+
+"""Backward pass of the layer at the solved batch: restore the differentiation checkpoints,
+centre them (`iterations` Newton steps at fixed mu) and run the adjoint for cotangents
+G (of X*) and hv (of y*).  Returns (dC, db) per member (NaN where no checkpoint)."""
+function batch_adjoint!(h::BatchHandle, Gs, hs; iterations::Integer=4)
+    B = length(h.k)
+    restored = Ref{Cint}(0)
+    mu = zeros(B)
+    _check(ccall(_sym(:sdpx_batch_restore_checkpoint), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Float64}),
+                 h.ptr, restored, mu), "sdpx_batch_restore_checkpoint")
+    _check(ccall(_sym(:sdpx_batch_center), Cint, (Ptr{Cvoid}, Cint), h.ptr, iterations), "sdpx_batch_center")
+    dC, db = zeros(sum(abs2, h.k)), zeros(sum(h.m))
+    _check(ccall(_sym(:sdpx_batch_adjoint), Cint,
+                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                 h.ptr, reduce(vcat, vec.(Gs)), reduce(vcat, hs), dC, db), "sdpx_batch_adjoint")
+    dCs = [reshape(b, k, k) for (b, k) in zip(_split(dC, abs2.(h.k)), h.k)]
+    dbs = _split(db, h.m)
+    for p in 1:B
+        if isnan(mu[p])
+            fill!(dCs[p], NaN)
+            fill!(dbs[p], NaN)
+        end
+    end
+    return dCs, dbs
+end
+
+# ---------------------------------------------------------------------------
+# Differentiable SDP layer (no counterpart in SDPX: it has no differentiation)
+# ---------------------------------------------------------------------------
+"""A batch of user-model SDPs  max b'y  s.t.  C - sum_i y_i A_i >= 0  with fixed A
+(one PSD block each) whose data (C, b) is the layer's input and whose optimum
+(X*, y*) is its output.  In SDPX's core form (ir/lower_sdp.jl:244-300) x = y,
+A_core = -A, C_core = -C, c = -b, and X* is the core Y."""
+struct SDPLayer
+    handle::BatchHandle
+    Av::Vector{Matrix{Float64}}   # per member k^2 x m panel of the core A (DenseCons.Av[1])
+    x0::Vector{Vector{Float64}}
+    options::BatchOptions
+end
+
+function SDPLayer(Av::AbstractVector, x0::AbstractVector, opts::SDPX.SolverOptions{Float64}; device::Int=0)
+    k = [isqrt(size(a, 1)) for a in Av]
+    m = [size(a, 2) for a in Av]
+    return SDPLayer(BatchHandle(k, m; device=device), [Matrix{Float64}(a) for a in Av],
+                    [Vector{Float64}(v) for v in x0], BatchOptions(opts))
+end
+
+"""Forward solve: (X*_p, y*_p) for every member at data (C_p, b_p) of the user's model."""
+function (layer::SDPLayer)(Cs::AbstractVector, bs::AbstractVector)
+    h = layer.handle
+    Av = reduce(vcat, vec.(layer.Av))
+    Ccore = reduce(vcat, (vec(-C) for C in Cs))
+    ccore = reduce(vcat, (-b for b in bs))
+    _check(ccall(_sym(:sdpx_batch_set_dense), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                 h.ptr, Av, Ccore, ccore), "sdpx_batch_set_dense")
+    eyes = [Matrix{Float64}(I, k, k) for k in h.k]
+    set_start!(h, layer.x0, eyes, eyes)
+    res = batch_solve!(h, layer.options)
+    return res.Y, res.x   # X* (the core Y) and y* (the core x)
+end
+
+"""The layer's reverse rule: for cotangents (Xbar_p, ybar_p) of (X*_p, y*_p) the adjoint
+of L = sum_p <Xbar_p, X*_p> + ybar_p'y*_p gives dL/dC_p and dL/db_p, with the solve's
+Schur Cholesky refactored at the re-centred differentiation checkpoint (sdpx_batch_adjoint)."""
+function ChainRulesCore.rrule(layer::SDPLayer, Cs::AbstractVector, bs::AbstractVector)
+    Xs, ys = layer(Cs, bs)
+    function sdp_layer_pullback(ȳ)
+        Xbar, ybar = ChainRulesCore.unthunk(ȳ)
+        G = [Xbar isa ChainRulesCore.AbstractZero ? zeros(size(X)) : Matrix{Float64}(Xbar[p])
+             for (p, X) in enumerate(Xs)]
+        hv = [ybar isa ChainRulesCore.AbstractZero ? zeros(length(y)) : Vector{Float64}(ybar[p])
+              for (p, y) in enumerate(ys)]
+        dCs, dbs = batch_adjoint!(layer.handle, G, hv)
+        return ChainRulesCore.NoTangent(), dCs, dbs
+    end
+    return (Xs, ys), sdp_layer_pullback
 end
 
 # ---------------------------------------------------------------------------

@@ -67,6 +67,9 @@ SIGNATURES = {
     "sdpx_batch_memory": [_h, C.POINTER(C.c_longlong)],
     "sdpx_batch_set_dense": [_h, _dp, _dp, _dp],
     "sdpx_batch_set_coo": [_h, C.POINTER(C.c_longlong), _ip, _ip, _dp, _dp, _dp],
+    "sdpx_batch_set_block_coo": [_h, _ip, _ip, _ip, _ip, _dp, _dp, _dp],
+    "sdpx_block_coo_convert": [C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp, C.POINTER(C.c_longlong), _ip, _ip, _dp,
+                               C.POINTER(C.c_longlong)],
     "sdpx_batch_set_start": [_h, _dp, _dp, _dp],
     "sdpx_batch_solve": [_h, C.c_void_p, _ip],
     "sdpx_batch_get_result": [_h, _ip, _ip, _dp, _dp, _dp, _dp, _dp],

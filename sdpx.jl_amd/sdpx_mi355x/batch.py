@@ -153,6 +153,33 @@ class BatchSolver:
                                            r.ctypes.data_as(_lib._ip), c.ctypes.data_as(_lib._ip), dptr(v),
                                            dptr(Cc), dptr(cc)), "sdpx_batch_set_coo")
 
+    def set_block_coo(self, coo_list, C_list, c_list) -> None:
+        """coo_list[p] = (ptr, lin, row, col, val): member p's SparseBlockCOO as the reference stores it
+        (types/constraints.jl:59-109: 1-based Int32, both triangles, lin = (col-1) k + row)."""
+        k2 = self.k.astype(np.int64) ** 2
+        C_list = _members("set_block_coo C", C_list, k2, self.B)
+        c_list = _members("set_block_coo c", c_list, self.m, self.B)
+        coo_list = list(coo_list)
+        if len(coo_list) != self.B:
+            raise ValueError(f"set_block_coo: {len(coo_list)} members given, the batch has {self.B}")
+        parts = [[], [], [], [], []]
+        for p, (ptr, lin, row, col, val) in enumerate(coo_list):
+            ptr = np.asarray(ptr, dtype=np.int32)
+            n = int(ptr[-1]) - 1 if ptr.size else -1
+            if ptr.size != int(self.m[p]) + 1 or not (len(lin) == len(row) == len(col) == len(val) == n):
+                raise ValueError(f"set_block_coo: member {p} needs m+1 positions and ptr[end]-1 entries")
+            for dst, a in zip(parts, (ptr, lin, row, col)):
+                dst.append(np.asarray(a, dtype=np.int32))
+            parts[4].append(np.asarray(val, dtype=np.float64))
+        ptr, lin, row, col = (np.concatenate(x) for x in parts[:4])
+        val = np.concatenate(parts[4])
+        Cc = np.concatenate([c_.ravel(order="F") for c_ in C_list])
+        cc = np.concatenate([v.ravel() for v in c_list])
+        ip = _lib._ip
+        check(self._lib.sdpx_batch_set_block_coo(self._h, ptr.ctypes.data_as(ip), lin.ctypes.data_as(ip),
+                                                 row.ctypes.data_as(ip), col.ctypes.data_as(ip), dptr(val),
+                                                 dptr(Cc), dptr(cc)), "sdpx_batch_set_block_coo")
+
     def set_start(self, x_list, X_list, Y_list) -> None:
         k2 = self.k.astype(np.int64) ** 2
         x_list = _members("set_start x", x_list, self.m, self.B)

@@ -126,3 +126,36 @@ def test_gpu_adjoint_matches_oracle_and_central_differences(gpu_lib):
         assert _rel(aC, fC, nC) <= 1e-6 and _rel(ab, fb, nb) <= 1e-6
         checked += 1
     assert checked >= 12
+
+
+@pytest.mark.parametrize("p", [1, 4, 8])
+def test_central_path_gradient_approaches_the_optimum(p):
+    """The adjoint is the exact derivative of the central-path point of mu (pinned above by central
+    differences); the optimum's derivative is its mu -> 0 limit.  On the differentiable-layer shape
+    (n=32, m=128, C and b perturbed) the oracle stalls near gap 1e-11 in FP64, so central differences
+    of its optimum map with step 1e-6 carry the solve error 1e-11 / 1e-6, and the limit is measured
+    instead: dL/db changes 10x less per decade of mu (linear convergence, ~1e-4 relative left at
+    mu = 1e-6), and dL/dC changes ~1e-3 between mu = 1e-5 and 1e-6, below which FP64 cancellation
+    (lambda's error, kappa(S) eps, amplified by 1/mu in X^-1 A_lambda Y) sets its floor."""
+    g = G.batched_dense_layer(16, seed=11)[p]
+    prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    rng = np.random.default_rng(5 + p)
+    Gm, h = _unit_sym(rng, 32), _unit(rng, 128)
+    r = O.solve(prob, O.Options(eps_gap=1e-9, eps_primal=1e-9, eps_dual=1e-9), x0=g.x0, X0=[np.eye(32)],
+                Y0=[np.eye(32)])
+    assert r.status == "Optimal"
+    D = {}
+    for mu in (1e-4, 1e-5, 1e-6, 1e-7):
+        x, X, Y = AO.center(prob, r.x, r.X[0], r.Y[0], mu, iterations=12)
+        assert abs(float(np.sum(X * Y)) / 32 - mu) <= 1e-9 * mu
+        D[mu] = AO.adjoint(prob, x, X, Y, Gm, h)
+
+    def change(a, b, i):
+        return float(np.linalg.norm(D[a][i] - D[b][i]) / np.linalg.norm(D[b][i]))
+
+    b_changes = [change(1e-4, 1e-5, 1), change(1e-5, 1e-6, 1), change(1e-6, 1e-7, 1)]
+    c_changes = [change(1e-4, 1e-5, 0), change(1e-5, 1e-6, 0), change(1e-6, 1e-7, 0)]
+    print(f"problem {p}: db change per decade {b_changes}, dC change per decade {c_changes}")
+    assert 0.05 <= b_changes[1] / b_changes[0] <= 0.2 and 0.05 <= b_changes[2] / b_changes[1] <= 0.2
+    assert b_changes[2] <= 1e-3
+    assert c_changes[1] <= 3e-3 and c_changes[1] < c_changes[0]
