@@ -169,7 +169,41 @@ def soc_leg(ncones: int = 16_000_000, q: int = 4, reps: int = 10) -> dict:
             gbs = per_cone * ncones / avg / 1e9
             out[name] = {"avg_launch_ms": round(avg * 1e3, 4), "bytes_per_launch": per_cone * ncones,
                          "achieved_GBps": round(gbs, 1), "frac_of_8TBps": round(gbs / 8000.0, 4)}
+    out["metric_assembly"] = soc_metric_leg()
     return out
+
+
+def soc_metric_leg(ncones: int = 2048, q: int = 4, n: int = 4096, reps: int = 5) -> dict:
+    """H = sum_c A_c' (W_c'W_c)^-1 A_c over 2048 cones of dimension 4 with dense coefficient rows on
+    4096 variables (soc_native.jl:1119-1218): one W^-1 pass over the slab, then the FP64 Gram kernel.
+    MFMA-bound: the algorithmic work counted is the Gram product's n(n+1)N flops (N = total dims)."""
+    import numpy as np
+
+    from sdpx_mi355x.soc import LorentzCones, random_interior
+
+    rng = np.random.default_rng(3)
+    dims = np.full(ncones, q, dtype=np.int32)
+    s = random_interior(rng, dims, 1.5)
+    z = random_interior(rng, dims, 0.7)
+    A = rng.standard_normal((ncones * q, n)) / math.sqrt(n)
+    with LorentzCones(dims) as c:
+        c.set(0, s)
+        c.set(1, z)
+        c.set_constraints(A)
+        c.nt_scaling(0, 1, 2, 3)
+        c.metric(2, download=False)
+        c.set_profiling(True)
+        ms = []
+        for _ in range(reps):
+            c.metric(2, download=False)
+            ms.append(c.last_kernel_ms())
+        c.set_profiling(False)
+    avg = sum(ms) / len(ms) / 1e3
+    flops = float(n) * (n + 1) * ncones * q
+    tf = flops / avg / 1e12
+    return {"workload": f"{ncones} cones of dimension {q}, dense coefficient rows on {n} variables",
+            "avg_ms": round(avg * 1e3, 4), "flops": flops, "achieved_TFLOPs": round(tf, 2),
+            "frac_of_fp64_mfma_peak": round(tf / FP64_PEAK_TFLOPS, 4)}
 
 
 def sparse_reference_flops(k: int, m: int) -> float:

@@ -194,6 +194,38 @@ int sdpx_gram_work_list(int m, int group, int split, int* out);
 /* Wait for the handle's stream. */
 int sdpx_dense_synchronize(sdpx_dense_t* h);
 
+/* ---- setup on the device -------------------------------------------------------
+ * equilibrate(prob::SDPProblem) dense path (ingest.jl:1432-1521): Ruiz passes
+ * per block, A_i <- E A_i E and C <- E C E with e = 1/sqrt(row infinity-norms
+ * over every A_i and C) (ruiz_iters < 0: :auto = 2..8 passes, stop when
+ * max |log rownorm| <= 1/20; otherwise exactly ruiz_iters passes), then the
+ * per-variable scale s_i = max_l ||A_i^(l)||_inf (1 when zero), A_i /= s_i,
+ * c <- (c ./ s) / objective_scale with objective_scale = ||c ./ s||_inf
+ * (_equilibration_variable_scales, ingest.jl:1252-1270).  Scales the handle's
+ * panels, C and c in place; E (sum k_l, per-block row scales), s (m),
+ * objective_scale and passes (L) are returned (null skips).  A solve of the
+ * scaled problem maps back by unequilibrate (ingest.jl:1523-1560):
+ * x = x^ ./ s, X = E^-1 X^ E^-1, Y = E Y^ E objective_scale. */
+int sdpx_dense_equilibrate(sdpx_dense_t* h, int ruiz_iters, double* E, double* s, double* objective_scale,
+                           int* passes);
+/* The handle's current C blocks (concatenated) and c (after equilibration). */
+int sdpx_dense_get_data(sdpx_dense_t* h, double* C, double* c);
+/* _kkt_cold_start_initialization (solver/interior_point.jl:699-1287), n_eq = 0:
+ * X = Y = I, one Schur build (the Gram of the A_i) and one factor; S x =
+ * [<A_i, C>] and S v = c solved with that factor and refined when the
+ * normalized residual exceeds max(sqrt(eps), eps_primal, eps_dual); X = A(x) - C,
+ * Y = A(v); the normalized residuals gated; every block lifted into the PD
+ * interior (_cold_start_psd_shift!), the identity mass floor and the
+ * cross-centering shifts applied (cold_start.jl:288-428).  Leaves (x, X, Y) as
+ * the handle's iterate; mu[l] = beta <X_l, Y_l> / k_l.  out (16): ok, reason (0
+ * ok, 1 identity_kkt_factorization, 2 kkt_cold_start_solve, 3
+ * kkt_cold_start_residuals, 4 psd shift, 5 identity mass, 6 centering), Schur
+ * regularization attempts, primal and dual normalized KKT residuals,
+ * refinement steps, normalized p_res and d_res, largest primal and dual
+ * shifts, primal and dual margins, kappa after / degree, kappa after the mass
+ * floor / degree. */
+int sdpx_dense_cold_start(sdpx_dense_t* h, double beta, double eps_primal, double eps_dual, double* mu, double* out);
+
 /* ---- a13: mixed precision ----------------------------------------------------
  * The reference's mixed-precision template (kernels/mixed_precision_kkt.jl:
  * low-precision factor, target-precision residuals and refinement,
@@ -421,6 +453,20 @@ int sdpx_soc_hkm_metric(sdpx_soc_t* h, int primal, int dual, int dst, int* nfail
  * subtracts the affine product of slots affine_primal / affine_dual. */
 int sdpx_soc_hkm_contract(sdpx_soc_t* h, int mode, int primal, int dual, int v, int affine_primal,
                           int affine_dual, double target, int include_affine, int dst);
+/* ---- SOC metric assembly: the dense branch of _native_soc_add_metric!
+ * (soc_native.jl:1119-1218) over every cone at once.  A: (sum_c q_c) x n
+ * column-major (cone c's coefficient rows A_c, q_c x n, at its coordinate
+ * offset; n = the Hessian's variables), uploaded once per handle. */
+int sdpx_soc_set_constraints(sdpx_soc_t* h, int n, const double* A);
+/* H = H0 + sum_c A_c' (W_c'W_c)^-1 A_c with the NT scaling point in slot w and
+ * the eta of the last sdpx_soc_nt_scaling (n x n, both triangles written; H0
+ * null -> 0; H null -> kept on the device only).  Evaluated as Atil' Atil,
+ * Atil_c = W_c^-1 A_c (W^-1 is symmetric, so W^-1 W^-1 = (W'W)^-1), on the
+ * FP64 MFMA Gram kernel; the reference's two GEMMs M A_c and A_c'(M A_c) agree
+ * with it to rounding.  Returns 1 with *nfailed cones when a cone's last NT
+ * scaling failed (nothing assembled; the reference reports such a cone as
+ * NumericalBreakdown). */
+int sdpx_soc_metric(sdpx_soc_t* h, int w, const double* H0, double* H, int* nfailed);
 /* Device-event timing of each following call's main kernel (one event sync per call). */
 int sdpx_soc_set_profiling(sdpx_soc_t* h, int enabled);
 int sdpx_soc_last_kernel_ms(sdpx_soc_t* h, double* ms);

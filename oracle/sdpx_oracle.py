@@ -1052,6 +1052,269 @@ def solution_residuals(prob: Problem, x, X, Y):
     return p_res, knrm_inf(d)
 
 
+# --------------------------------------------------------------------------
+# Setup: Ruiz equilibration (ingest.jl:1141-1270, 1432-1560) and the
+# identity-metric cold start (interior_point.jl:699-1287, cold_start.jl)
+# --------------------------------------------------------------------------
+@dataclass
+class Equilibration:
+    """ingest.jl:1120-1139"""
+
+    E: list
+    s: np.ndarray
+    objective_scale: float
+    passes: list
+
+
+def equilibrate(prob: Problem, ruiz_iters="auto"):
+    """Dense equilibrate (ingest.jl:1432-1521) -> (scaled problem, Equilibration)."""
+    if ruiz_iters == "auto":
+        min_it, max_it, tol, adaptive = 2, 8, 1.0 / 20.0, True   # _ruiz_control, ingest.jl:1141-1160
+    else:
+        min_it = max_it = int(ruiz_iters)
+        tol, adaptive = 0.0, False
+    m = prob.m
+    Av = [a.copy() for a in prob.Av]
+    C2 = [c.copy() for c in prob.C]
+    E, passes = [], []
+    for l in range(prob.L):
+        k = prob.k[l]
+        El = np.ones(k)
+        done = 0
+        # T3[i][c, r] = A_i[r, c] (column i of the panel is vec(A_i), column-major)
+        T3 = np.array(Av[l].T).reshape(m, k, k)
+        for it in range(1, max_it + 1):
+            done = it
+            rn = np.max(np.abs(C2[l]), axis=1)                       # _ruiz_absorb_c_rowmax!
+            if m:
+                rn = np.maximum(rn, np.max(np.abs(T3), axis=(0, 1)))  # row r of every A_i
+            converged = adaptive and it >= min_it and _ruiz_converged(rn, tol)
+            e = np.array([1.0 / math.sqrt(v) if v > 0 else 1.0 for v in rn])   # _ruiz_step
+            F = np.outer(e, e)                                           # F[r, c] = e[r] * e[c]
+            C2[l] = C2[l] * F                                            # _ruiz_congruence!
+            T3 *= F.T[None, :, :]                                        # A_i[r, c] *= e[r] * e[c]
+            El = El * e
+            if converged:
+                break
+        Av[l] = np.ascontiguousarray(T3.reshape(m, k * k).T)
+        E.append(El)
+        passes.append(done)
+    maxima = np.zeros(m)
+    for l in range(prob.L):
+        if m:
+            maxima = np.maximum(maxima, np.max(np.abs(Av[l]), axis=0))
+    s = np.where(maxima > 0, maxima, 1.0)                             # _equilibration_variable_scales
+    cc = prob.c / s
+    osc = float(np.max(np.abs(cc))) if m else 0.0
+    if not (osc > 0 and math.isfinite(osc)):
+        osc = 1.0
+    cc = cc / osc
+    for l in range(prob.L):
+        Av[l] /= s[None, :]
+    return Problem(c=cc, C=C2, Av=Av, k=list(prob.k)), Equilibration(E, s, osc, passes)
+
+
+def _ruiz_converged(rn, tol) -> bool:
+    """ingest.jl:1162-1176"""
+    dev = 0.0
+    for v in rn:
+        if not v > 0:
+            continue
+        if not math.isfinite(v):
+            return False
+        dev = max(dev, abs(math.log(v)))
+    return dev <= tol
+
+
+def unequilibrate(eq: Equilibration, x, X, Y):
+    """ingest.jl:1523-1560 (n_eq = 0): x = x^ ./ s, X = E^-1 X^ E^-1, Y = E Y^ E objective_scale."""
+    x = np.asarray(x) / eq.s
+    Xo = [Xl / np.outer(E, E) for Xl, E in zip(X, eq.E)]
+    Yo = [Yl * np.outer(E, E) for Yl, E in zip(Y, eq.E)]
+    if eq.objective_scale != 1.0:
+        Yo = [Yl * eq.objective_scale for Yl in Yo]
+    return x, Xo, Yo
+
+
+def cold_safety(scale):
+    """cold_start.jl:24-26"""
+    return SQRT_EPS * max(1.0, scale)
+
+
+def cold_rounding_slack(scale):
+    """cold_start.jl:47-49"""
+    return 8.0 * EPS * max(1.0, scale)
+
+
+def cold_psd_lower_bound(A) -> float:
+    """cold_start.jl:141-181: exact for 1x1 and 2x2, symmetric Gershgorin on the lower triangle above."""
+    n = A.shape[0]
+    if n == 1:
+        return float(A[0, 0])
+    if n == 2:
+        center = (A[0, 0] + A[1, 1]) / 2.0
+        return float(center - math.hypot((A[0, 0] - A[1, 1]) / 2.0, A[1, 0]))
+    bound = 0.0
+    for row in range(n):
+        radius = 0.0
+        for col in range(n):
+            if col != row:
+                radius += abs(A[max(row, col), min(row, col)])
+        cand = A[row, row] - radius
+        bound = cand if row == 0 else min(bound, cand)
+    return float(bound)
+
+
+def cold_psd_stats(A):
+    """cold_start.jl:184-201 -> (finite, scale, margin)"""
+    n = A.shape[0]
+    scale = 0.0
+    for c in range(n):
+        for r in range(c, n):
+            v = A[r, c]
+            if not math.isfinite(v):
+                return False, scale, 0.0
+            scale = max(scale, abs(v))
+    return True, scale, cold_psd_lower_bound(A)
+
+
+def cold_psd_shift(A):
+    """_cold_start_psd_shift! (cold_start.jl:288-312), in place -> (ok, shift, margin, scale)."""
+    n = A.shape[0]
+    if A.ndim != 2 or A.shape[1] != n:
+        raise ValueError("cold-start PSD shift requires a square matrix")
+    if n == 0:
+        return True, 0.0, 0.0, 1.0
+    finite, scale, margin = cold_psd_stats(A)
+    if not finite:
+        return False, 0.0, margin, scale
+    safety = cold_safety(scale)
+    slack = cold_rounding_slack(scale)
+    if margin > safety:
+        return True, 0.0, margin, scale
+    applied = 0.0
+    for _ in range(16):
+        shift = safety - margin + slack
+        if not math.isfinite(shift):
+            return False, applied, margin, scale
+        A[np.diag_indices(n)] += shift
+        applied += shift
+        finite, scale, margin = cold_psd_stats(A)
+        if not finite:
+            return False, applied, margin, scale
+        safety = cold_safety(scale)
+        if margin > safety:
+            return True, applied, margin, scale
+    return False, applied, margin, scale
+
+
+def cold_identity_mass_shifts(primal, dual, rho):
+    """cold_start.jl:314-350 -> (ok, primal_shift, dual_shift)"""
+    if not (rho > 0 and math.isfinite(primal) and math.isfinite(dual)):
+        return False, 0.0, 0.0
+    target = float(rho)
+    p = max(0.0, (target - primal) / target)
+    d = max(0.0, (target - dual) / target)
+    ok = math.isfinite(p) and math.isfinite(d)
+    return ok, (p if ok else 0.0), (d if ok else 0.0)
+
+
+def cold_centering_shifts(kappa, primal, dual):
+    """cold_start.jl:409-428 -> (ok, primal_shift, dual_shift)"""
+    ok = (math.isfinite(kappa) and kappa > 0 and math.isfinite(primal) and primal > 0
+          and math.isfinite(dual) and dual > 0)
+    ps = kappa / (2.0 * dual) if ok else 0.0
+    ds = kappa / (2.0 * primal) if ok else 0.0
+    ok = ok and math.isfinite(ps) and math.isfinite(ds)
+    return ok, (ps if ok else 0.0), (ds if ok else 0.0)
+
+
+def cold_start(prob: Problem, o: Options):
+    """_kkt_cold_start_initialization (interior_point.jl:699-1287), n_eq = 0.
+
+    Returns (ok, reason, x, X, Y, mu, record)."""
+    m, L, k = prob.m, prob.L, prob.k
+    degree = sum(k)
+    eye = [np.eye(kl) for kl in k]
+    S = schur_build(prob, eye, eye)
+    Lf, ok, reg = factor_dense_kkt(S)
+    rec = {"regularization_attempts": reg}
+    if not ok:
+        return False, "identity_kkt_factorization", None, None, None, None, rec
+    threshold = max(SQRT_EPS, o.eps_primal, o.eps_dual)
+    refine_total = 0
+
+    def solve_gated(r):
+        nonlocal refine_total
+        dx = solve_dense_kkt(Lf, r)
+        res, _ = direction_residual(S, r, dx)
+        scale = max(knrm_inf(r), 1.0)
+        normalized = res / scale
+        if math.isfinite(normalized) and normalized > threshold:
+            dx, steps, _ = refine_direction(S, Lf, r, dx, replace(o, refine_policy="auto"))
+            refine_total += steps
+            res, _ = direction_residual(S, r, dx)
+            normalized = res / scale
+        return dx, normalized
+
+    primal_rhs = np.zeros(m)
+    for l in range(L):
+        primal_rhs += contract(prob, l, prob.C[l])
+    x, pn = solve_gated(primal_rhs)
+    rec["primal_kkt_residual"] = pn
+    if not (math.isfinite(pn) and pn <= threshold):
+        return False, "kkt_cold_start_solve", None, None, None, None, rec
+    v, dn = solve_gated(prob.c.copy())
+    rec["dual_kkt_residual"] = dn
+    rec["refinement_steps"] = refine_total
+    if not (math.isfinite(dn) and dn <= threshold):
+        return False, "kkt_cold_start_solve", None, None, None, None, rec
+    X = [build_P(prob, l, x) - prob.C[l] for l in range(L)]
+    Y = [build_P(prob, l, v) for l in range(L)]
+    p_res, d_res = solution_residuals(prob, x, X, Y)
+    scale_p = 1.0 + max((knrm_inf(Cl) for Cl in prob.C), default=0.0)
+    scale_d = 1.0 + knrm_inf(prob.c)
+    rec["primal_residual"], rec["dual_residual"] = p_res / scale_p, d_res / scale_d
+    if not (math.isfinite(p_res / scale_p) and math.isfinite(d_res / scale_d)
+            and p_res / scale_p <= threshold and d_res / scale_d <= threshold):
+        return False, "kkt_cold_start_residuals", None, None, None, None, rec
+    pshift, dshift = [], []
+    for l in range(L):
+        okp, ps, _, _ = cold_psd_shift(X[l])
+        okd, ds, _, _ = cold_psd_shift(Y[l])
+        if not (okp and okd):
+            return False, "psd_shift", None, None, None, None, rec
+        pshift.append(ps)
+        dshift.append(ds)
+    pmass = sum(float(np.trace(Xl)) for Xl in X)
+    dmass = sum(float(np.trace(Yl)) for Yl in Y)
+    okf, pf, df = cold_identity_mass_shifts(pmass, dmass, degree)
+    if not okf:
+        return False, "identity_mass", None, None, None, None, rec
+    for l in range(L):
+        if pf > 0:
+            X[l][np.diag_indices(k[l])] += pf
+        if df > 0:
+            Y[l][np.diag_indices(k[l])] += df
+    kappa_floor = sum(float(np.sum(X[l] * Y[l])) for l in range(L))
+    pmass = sum(float(np.trace(Xl)) for Xl in X)
+    dmass = sum(float(np.trace(Yl)) for Yl in Y)
+    okc, pc, dc = cold_centering_shifts(kappa_floor, pmass, dmass)
+    if not okc:
+        return False, "centering", None, None, None, None, rec
+    for l in range(L):
+        if pc > 0:
+            X[l][np.diag_indices(k[l])] += pc
+        if dc > 0:
+            Y[l][np.diag_indices(k[l])] += dc
+    rec["primal_margin"] = min(cold_psd_stats(Xl)[2] for Xl in X)
+    rec["dual_margin"] = min(cold_psd_stats(Yl)[2] for Yl in Y)
+    rec["largest_primal_shift"] = max(pshift) + max(pf, 0.0) + max(pc, 0.0)
+    rec["largest_dual_shift"] = max(dshift) + max(df, 0.0) + max(dc, 0.0)
+    mu = [o.beta * float(np.sum(X[l] * Y[l])) / k[l] for l in range(L)]
+    return True, "none", x, X, Y, mu, rec
+
+
 def dual_objective(prob: Problem, Y) -> float:
     """interior_point.jl:7-14"""
     return float(sum(np.sum(prob.C[l] * Y[l]) for l in range(prob.L)))
@@ -1073,11 +1336,18 @@ class Result:
     history: list | None = None
 
 
-def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=False) -> Result:
-    """_solve_sdp_core! warm-start path and loop (interior_point.jl:1323-2420), n_eq = 0, scaling none."""
+def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=False, start: str = "given") -> Result:
+    """_solve_sdp_core! warm-start path and loop (interior_point.jl:1323-2420), n_eq = 0, scaling none.
+
+    start="cold": the identity-metric KKT cold start (interior_point.jl:699-1287) replaces x0/X0/Y0."""
     m, L, k = prob.m, prob.L, prob.k
     x = np.zeros(m) if x0 is None else np.asarray(x0, dtype=np.float64).copy()
-    if X0 is None:
+    if start == "cold":
+        ok, reason, x, X, Y, mu0, _ = cold_start(prob, o)
+        if not ok:
+            return Result("NumericalBreakdown", np.zeros(m), [np.eye(kl) for kl in k], [np.eye(kl) for kl in k],
+                          0.0, 0.0, 0, math.inf, math.inf, math.inf, None, [])
+    elif X0 is None:
         X = [o.omega_p * np.eye(kl) for kl in k]
         Y = [o.omega_d * np.eye(kl) for kl in k]
     else:
@@ -1224,6 +1494,25 @@ def solve(prob: Problem, o: Options, x0=None, X0=None, Y0=None, trace_schur=Fals
     if status != "Optimal" and best is not None:
         x, X, Y, pobj, dobj, p_res, d_res, gap_rel = best
     return Result(status, x, X, Y, pobj, dobj, iteration, p_res, d_res, gap_rel, trace, history)
+
+
+def solve_equilibrated(prob: Problem, o: Options, ruiz_iters="auto", start: str = "cold", x0=None, X0=None,
+                       Y0=None) -> tuple:
+    """equilibrate -> solve -> unequilibrate (the reference's :sdp_ruiz scaling policy around
+    _solve_sdp_core!); objectives are recomputed on the original problem.  Returns (Result, Equilibration)."""
+    sp, eq = equilibrate(prob, ruiz_iters)
+    if start != "cold" and X0 is not None:
+        # a warm start mapped into the scaled space (interior_point.jl:606-661)
+        x0 = np.asarray(x0) * eq.s
+        X0 = [Xl * np.outer(E, E) for Xl, E in zip(X0, eq.E)]
+        Y0 = [Yl / np.outer(E, E) / eq.objective_scale for Yl, E in zip(Y0, eq.E)]
+    r = solve(sp, o, x0=x0, X0=X0, Y0=Y0, start=start)
+    x, X, Y = unequilibrate(eq, r.x, r.X, r.Y)
+    pobj = float(prob.c @ x)
+    dobj = dual_objective(prob, Y)
+    gap = abs(pobj - dobj) / max(1.0, (abs(pobj) + abs(dobj)) / 2)
+    p_res, d_res = solution_residuals(prob, x, X, Y)
+    return Result(r.status, x, X, Y, pobj, dobj, r.iterations, p_res, d_res, gap, None, r.history), eq
 
 
 def certificate(prob: Problem, result: Result, tol_primal, tol_dual, tol_gap):

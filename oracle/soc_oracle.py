@@ -33,6 +33,8 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
+
 
 def tail_norm(v):
     """_soc_tail_norm (soc_lorentz_kernels.jl:22-34): max-scaled two-pass norm."""
@@ -211,6 +213,27 @@ def apply_hs_inverse(w, eta2, src):
     for i in range(1, q):
         out[i] = (-2.0 * w[i] * ld + src[i]) / eta2
     return out
+
+
+def metric_matrix(w, eta2):
+    """(W'W)^-1 of one cone as a q x q matrix: _soc_nt_apply_hs_inverse! on each unit vector, the
+    dense branch of _native_soc_add_metric! (soc_native.jl:1180-1205)."""
+    q = len(w)
+    M = np.zeros((q, q))
+    basis = [0.0] * q
+    for j in range(q):
+        basis[j] = 1.0
+        M[:, j] = apply_hs_inverse(w, eta2, basis)
+        basis[j] = 0.0
+    return M
+
+
+def add_metric(H, A, w, eta2):
+    """H += A' (M A) for one cone with coefficient rows A (q x n): the dense single-cone assembly of
+    _native_soc_add_metric! (soc_native.jl:1173-1217)."""
+    M = metric_matrix(w, eta2)
+    H += A.T @ (M @ A)
+    return H
 
 
 def jordan(left, right):

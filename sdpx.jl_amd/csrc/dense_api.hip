@@ -8,7 +8,9 @@
 //   X, Y, LX, MY, LXinv, Xinv, P, R, dX, dY, W1, W2   k x k blocks
 //   trial  cap x k^2 batched line-search trial matrices
 // Host code only sequences kernels and reads back the scalars of each phase.
+#include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -193,6 +195,79 @@ __global__ void corrector_target_kernel(const double* dotv, int mode, double mu,
 __global__ void neg_sum_kernel(const double* a, const double* b, double* out, int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) out[i] = -(a[i] + b[i]);
+}
+
+// ---- Ruiz equilibration (ingest.jl:1287-1521, dense path :1432-1521) ----------
+// One pass over a block's panel (k x k x m, every A_i column-major) and C: with
+// e given, every entry is scaled in place, a(r, c) *= e[r] * e[c] (the
+// reference's order: the factor product first); then, or without e, the row
+// maxima |a(r, .)| over every A_i and C are folded into rowmax (the bit
+// patterns of non-negative doubles order like unsigned integers, so the
+// device-wide max is exact and order-free).  A wavefront owns a fixed slab of
+// kRuizRows rows (8 per lane, kept in registers) and walks the columns.
+constexpr int kRuizRows = 512;
+
+__global__ void __launch_bounds__(256) ruiz_pass_kernel(double* Av, double* C, int k, int m, const double* e,
+                                                        unsigned long long* rowmax, int nslab) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long waves = (long long)gridDim.x * 4;  // a multiple of nslab
+  const int slab = (int)(wave % nslab);
+  const long long ncol = (long long)k * m + k;  // the m panels' columns, then C's
+  const int r0 = slab * kRuizRows + lane;
+  const int rend = min(k, (slab + 1) * kRuizRows);
+  double mx[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) mx[j] = 0.0;
+  for (long long col = wave / nslab; col < ncol; col += waves / nslab) {
+    const int c = (int)(col % k);
+    double* p = col < (long long)k * m ? Av + col * k : C + (col - (long long)k * m) * k;
+    const double ec = e ? e[c] : 1.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = r0 + 64 * j;
+      if (r < rend) {
+        double v = p[r];
+        if (e) {
+          v = v * (e[r] * ec);
+          p[r] = v;
+        }
+        mx[j] = fmax(mx[j], fabs(v));
+      }
+    }
+  }
+  if (rowmax) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = r0 + 64 * j;
+      if (r < rend) atomicMax(rowmax + r, (unsigned long long)__double_as_longlong(mx[j]));
+    }
+  }
+}
+
+// maxima[i] = max(maxima[i], max |A_i|) over one block (one work-group per constraint;
+// first != 0 starts from 0)
+__global__ void __launch_bounds__(256) ruiz_colmax_kernel(const double* Av, long long k2, double* maxima,
+                                                          int first) {
+  __shared__ double red[256];
+  const long long i = blockIdx.x;
+  const double* a = Av + i * k2;
+  double mx = 0.0;
+  for (long long j = threadIdx.x; j < k2; j += 256) mx = fmax(mx, fabs(a[j]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) maxima[i] = first ? red[0] : fmax(maxima[i], red[0]);
+}
+
+// A_i ./= s[i] (the per-variable scale)
+__global__ void div_columns_kernel(double* Av, long long k2, int m, const double* s) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= k2 * m) return;
+  Av[idx] = Av[idx] / s[idx / k2];
 }
 
 }  // namespace
@@ -573,6 +648,113 @@ int backtrack_both(sdpx_dense_t* h, double gamma, double min_step, double ratio,
   *tY = res[1];
   return kOk;
 }
+
+// ---- cold-start helpers on host (cold_start.jl:24-49, 141-312, 314-428) --------------
+namespace cold {
+
+double safety(double scale) { return std::sqrt(2.220446049250313e-16) * std::max(1.0, scale); }
+double rounding_slack(double scale) { return 8.0 * 2.220446049250313e-16 * std::max(1.0, scale); }
+
+// certified lower bound on lambda_min from the lower triangle (cold_start.jl:141-181)
+double psd_lower_bound(const double* A, int n) {
+  if (n == 1) return A[0];
+  if (n == 2) {
+    const double first = A[0], second = A[3], off = A[1];
+    const double center = (first + second) / 2.0;
+    const double radius = std::hypot((first - second) / 2.0, off);
+    return center - radius;
+  }
+  double bound = 0.0;
+  for (int row = 0; row < n; ++row) {
+    double radius = 0.0;
+    for (int col = 0; col < n; ++col) {
+      if (col == row) continue;
+      const int lr = std::max(row, col), lc = std::min(row, col);
+      radius += std::fabs(A[lr + (long long)lc * n]);
+    }
+    const double cand = A[row + (long long)row * n] - radius;
+    bound = row == 0 ? cand : std::min(bound, cand);
+  }
+  return bound;
+}
+
+// finite max-norm of the lower triangle and the certified margin (cold_start.jl:184-201)
+bool psd_stats(const double* A, int n, double& scale, double& margin) {
+  scale = 0.0;
+  for (int c = 0; c < n; ++c)
+    for (int r = c; r < n; ++r) {
+      const double v = A[r + (long long)c * n];
+      if (!std::isfinite(v)) {
+        margin = 0.0;
+        return false;
+      }
+      scale = std::max(scale, std::fabs(v));
+    }
+  margin = psd_lower_bound(A, n);
+  return true;
+}
+
+// _cold_start_psd_shift! (cold_start.jl:288-312): ok, applied shift, margin, scale
+bool psd_shift(double* A, int n, double& applied, double& margin, double& scale) {
+  applied = 0.0;
+  if (n == 0) {
+    margin = 0.0;
+    scale = 1.0;
+    return true;
+  }
+  if (!psd_stats(A, n, scale, margin)) return false;
+  double sf = safety(scale);
+  const double slack = rounding_slack(scale);
+  if (margin > sf) return true;
+  for (int it = 0; it < 16; ++it) {
+    const double shift = sf - margin + slack;
+    if (!std::isfinite(shift)) return false;
+    for (int i = 0; i < n; ++i) A[i + (long long)i * n] += shift;
+    applied += shift;
+    if (!psd_stats(A, n, scale, margin)) return false;
+    sf = safety(scale);
+    if (margin > sf) return true;
+  }
+  return false;
+}
+
+// _cold_start_identity_mass_shifts (cold_start.jl:314-350)
+bool mass_shifts(double primal, double dual, long long rho, double& dp, double& dd) {
+  dp = dd = 0.0;
+  if (!(rho > 0 && std::isfinite(primal) && std::isfinite(dual))) return false;
+  const double target = (double)rho;
+  const double p = std::max(0.0, (target - primal) / target);
+  const double d = std::max(0.0, (target - dual) / target);
+  if (!(std::isfinite(p) && std::isfinite(d))) return false;
+  dp = p;
+  dd = d;
+  return true;
+}
+
+// _cold_start_centering_shifts (cold_start.jl:409-428)
+bool centering_shifts(double kappa, double primal, double dual, double& ps, double& ds) {
+  bool ok = std::isfinite(kappa) && kappa > 0.0 && std::isfinite(primal) && primal > 0.0 && std::isfinite(dual) &&
+            dual > 0.0;
+  ps = ok ? kappa / (2.0 * dual) : 0.0;
+  ds = ok ? kappa / (2.0 * primal) : 0.0;
+  ok = ok && std::isfinite(ps) && std::isfinite(ds);
+  if (!ok) ps = ds = 0.0;
+  return ok;
+}
+
+double kdot(const std::vector<double>& a, const std::vector<double>& b) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+double trace(const std::vector<double>& a, int n) {
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += a[i + (long long)i * n];
+  return t;
+}
+
+}  // namespace cold
 
 }  // namespace
 
@@ -1565,6 +1747,275 @@ int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out) {
   out[1] = dm;
   out[2] = quality(h->hsc[4 * L], h->hsc[4 * L + 1]);
   return kOk;
+}
+
+int sdpx_dense_equilibrate(sdpx_dense_t* h, int ruiz_iters, double* E, double* s_out, double* objective_scale,
+                           int* passes) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L, m = h->m;
+  // _ruiz_control (ingest.jl:1141-1160): :auto = 2..8 passes, log tolerance 1/20
+  const bool adaptive = ruiz_iters < 0;
+  const int min_it = adaptive ? 2 : ruiz_iters, max_it = adaptive ? 8 : ruiz_iters;
+  const double log_tol = adaptive ? 1.0 / 20.0 : 0.0;
+  unsigned long long* rowmax = nullptr;
+  double *e_dev = nullptr, *s_dev = nullptr, *maxima = nullptr;
+  int kmax = 1;
+  for (const Block& b : h->blk) kmax = std::max(kmax, b.k);
+  SDPX_CHECK(dalloc(h, &rowmax, (size_t)kmax));
+  SDPX_CHECK(dalloc(h, &e_dev, (size_t)kmax));
+  SDPX_CHECK(dalloc(h, &s_dev, (size_t)m));
+  SDPX_CHECK(dalloc(h, &maxima, (size_t)m));
+  std::vector<unsigned long long> rn_bits(kmax);
+  std::vector<double> rn(kmax), e(kmax);
+  long long eoff = 0;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    const int k = b.k;
+    const int nslab = ceil_div(k, kRuizRows);
+    const int groups = ceil_div(1024, nslab) * nslab;  // 4 waves each, so waves % nslab == 0
+    std::vector<double> El(k, 1.0);
+    // row maxima of the unscaled block (the first pass reads only)
+    SDPX_HIP(hipMemsetAsync(rowmax, 0, sizeof(unsigned long long) * k, h->stream));
+    hipLaunchKernelGGL(ruiz_pass_kernel, dim3(groups), dim3(256), 0, h->stream, b.Av, b.C, k, m, nullptr, rowmax,
+                       nslab);
+    SDPX_HIP(hipGetLastError());
+    int done = 0;
+    for (int it = 1; it <= max_it; ++it) {
+      done = it;
+      SDPX_HIP(hipMemcpyAsync(rn_bits.data(), rowmax, sizeof(unsigned long long) * k, hipMemcpyDeviceToHost,
+                              h->stream));
+      SDPX_HIP(hipStreamSynchronize(h->stream));
+      // _ruiz_step / _ruiz_converged (ingest.jl:1162-1176, 1224-1231)
+      bool converged = adaptive && it >= min_it;
+      double dev = 0.0;
+      for (int r = 0; r < k; ++r) {
+        double v;
+        std::memcpy(&v, &rn_bits[r], sizeof(double));
+        rn[r] = v;
+        if (!(v > 0.0)) continue;
+        if (!std::isfinite(v)) converged = false;
+        else dev = std::max(dev, std::fabs(std::log(v)));
+      }
+      converged = converged && dev <= log_tol;
+      for (int r = 0; r < k; ++r) e[r] = rn[r] > 0.0 ? 1.0 / std::sqrt(rn[r]) : 1.0;
+      for (int r = 0; r < k; ++r) El[r] *= e[r];
+      // scale the block and C, and take the next pass's row maxima in the same sweep
+      const bool last = converged || it == max_it;
+      SDPX_HIP(hipMemcpyAsync(e_dev, e.data(), sizeof(double) * k, hipMemcpyHostToDevice, h->stream));
+      if (!last) SDPX_HIP(hipMemsetAsync(rowmax, 0, sizeof(unsigned long long) * k, h->stream));
+      hipLaunchKernelGGL(ruiz_pass_kernel, dim3(groups), dim3(256), 0, h->stream, b.Av, b.C, k, m, e_dev,
+                         last ? nullptr : rowmax, nslab);
+      SDPX_HIP(hipGetLastError());
+      if (last) break;
+    }
+    if (max_it == 0) done = 0;
+    if (passes) passes[l] = done;
+    if (E) std::copy(El.begin(), El.end(), E + eoff);
+    eoff += k;
+    // per-constraint maxima over the blocks (ingest.jl:1470-1481)
+    hipLaunchKernelGGL(ruiz_colmax_kernel, dim3((unsigned)m), dim3(256), 0, h->stream, b.Av, b.k2, maxima, l == 0);
+    SDPX_HIP(hipGetLastError());
+  }
+  // _equilibration_variable_scales (ingest.jl:1252-1270)
+  std::vector<double> mx(m), sv(m), cc(m);
+  SDPX_HIP(hipMemcpyAsync(mx.data(), maxima, sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipMemcpyAsync(cc.data(), h->c, sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  double osc = 0.0;
+  for (int i = 0; i < m; ++i) {
+    sv[i] = mx[i] > 0.0 ? mx[i] : 1.0;
+    cc[i] = cc[i] / sv[i];
+    osc = std::max(osc, std::fabs(cc[i]));
+  }
+  if (!(osc > 0.0 && std::isfinite(osc))) osc = 1.0;
+  for (int i = 0; i < m; ++i) cc[i] = cc[i] / osc;
+  SDPX_HIP(hipMemcpyAsync(s_dev, sv.data(), sizeof(double) * m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->c, cc.data(), sizeof(double) * m, hipMemcpyHostToDevice, h->stream));
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    hipLaunchKernelGGL(div_columns_kernel, dim3((unsigned)ceil_div(b.k2 * m, 256)), dim3(256), 0, h->stream, b.Av,
+                       b.k2, m, s_dev);
+    SDPX_HIP(hipGetLastError());
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  if (s_out) std::copy(sv.begin(), sv.end(), s_out);
+  if (objective_scale) *objective_scale = osc;
+  return kOk;
+}
+
+int sdpx_dense_get_data(sdpx_dense_t* h, double* C, double* c) {
+  SDPX_CHECK(check_handle(h));
+  long long off = 0;
+  for (const Block& b : h->blk) {
+    if (C) SDPX_HIP(hipMemcpyAsync(C + off, b.C, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    off += b.k2;
+  }
+  if (c) SDPX_HIP(hipMemcpyAsync(c, h->c, sizeof(double) * h->m, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_cold_start(sdpx_dense_t* h, double beta, double eps_primal, double eps_dual, double* mu,
+                          double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int L = h->L, m = h->m;
+  double rec[16] = {0.0};
+  auto finish = [&](int reason) {
+    rec[0] = reason == 0 ? 1.0 : 0.0;
+    rec[1] = reason;
+    if (out) std::copy(rec, rec + 16, out);
+    return kOk;
+  };
+  long long degree = 0;
+  for (const Block& b : h->blk) degree += b.k;
+  const double threshold = std::max(std::sqrt(2.220446049250313e-16), std::max(eps_primal, eps_dual));
+  // identity metric: X = Y = I, one Schur build (the Gram of the A_i) and one factor
+  for (Block& b : h->blk) {
+    SDPX_CHECK(fill_f64(b.k2, 0.0, b.X, h->stream));
+    SDPX_CHECK(add_diag_f64(b.X, b.k, b.k, 1.0, nullptr, 0.0, h->stream));
+    SDPX_HIP(hipMemcpyAsync(b.Y, b.X, sizeof(double) * b.k2, hipMemcpyDeviceToDevice, h->stream));
+  }
+  SDPX_CHECK(fill_f64(m, 0.0, h->x, h->stream));
+  std::vector<double> zmu(L, 0.0);
+  double tmp[3];
+  SDPX_CHECK(sdpx_dense_residuals(h, zmu.data(), 1, 1, tmp));
+  SDPX_CHECK(sdpx_dense_schur_build(h));
+  int reg = 0;
+  const int fst = sdpx_dense_schur_factor(h, &reg);
+  if (fst < 0) return fst;
+  rec[2] = reg;
+  if (fst != 0) return finish(1);  // :identity_kkt_factorization
+  // the two right-hand sides with the one factor, each checked against the unfactored
+  // operator and refined when its normalized residual exceeds the gate
+  int refine_total = 0;
+  auto solve_gated = [&](double& normalized) -> int {
+    SDPX_CHECK(solve_into(h, h->r, h->dx));
+    SDPX_CHECK(direction_residual_dev(h, 0));
+    SDPX_CHECK(sync_scalars(h, 2));
+    const double scale = std::max(h->hsc[1], 1.0);
+    normalized = h->hsc[0] / scale;
+    if (std::isfinite(normalized) && normalized > threshold) {
+      int steps = 0;
+      double res = 0.0;
+      SDPX_CHECK(sdpx_dense_refine(h, 64.0 * 2.220446049250313e-16 * scale, 8, 0, &steps, &res));
+      refine_total += steps;
+      SDPX_CHECK(direction_residual_dev(h, 0));
+      SDPX_CHECK(sync_scalars(h, 1));
+      normalized = h->hsc[0] / scale;
+    }
+    return kOk;
+  };
+  // primal: S x = [<A_i, C>]
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(avt_f64(b.Av, b.k2, m, b.C, 1.0, l == 0 ? 0.0 : 1.0, h->r, h->stream));
+  }
+  double pn = 0.0, dn = 0.0;
+  SDPX_CHECK(solve_gated(pn));
+  rec[3] = pn;
+  if (!(std::isfinite(pn) && pn <= threshold)) return finish(2);  // :kkt_cold_start_solve
+  SDPX_HIP(hipMemcpyAsync(h->x, h->dx, sizeof(double) * m, hipMemcpyDeviceToDevice, h->stream));
+  // dual: S v = c
+  SDPX_HIP(hipMemcpyAsync(h->r, h->c, sizeof(double) * m, hipMemcpyDeviceToDevice, h->stream));
+  SDPX_CHECK(solve_gated(dn));
+  rec[4] = dn;
+  rec[5] = refine_total;
+  if (!(std::isfinite(dn) && dn <= threshold)) return finish(2);
+  // X = A(x) - C, Y = A(v)
+  for (Block& b : h->blk) {
+    SDPX_CHECK(avx_f64(b.Av, b.k2, m, h->x, 1.0, b.X, -1.0, b.C, 0.0, nullptr, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+    SDPX_CHECK(avx_f64(b.Av, b.k2, m, h->dx, 1.0, b.Y, 0.0, nullptr, 0.0, nullptr, h->gemv_part, h->gemv_part_cap,
+                       h->stream));
+  }
+  // pre-shift residuals, normalized as for termination
+  double p_res = 0.0, d_res = 0.0;
+  SDPX_CHECK(sdpx_dense_solution_residuals(h, &p_res, &d_res));
+  double cmax = 0.0;
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_CHECK(absmax_f64(b.C, b.k2, h->red_part, h->dsc + l, h->stream));
+  }
+  SDPX_CHECK(absmax_f64(h->c, m, h->red_part, h->dsc + L, h->stream));
+  SDPX_CHECK(sync_scalars(h, L + 1));
+  for (int l = 0; l < L; ++l) cmax = std::max(cmax, h->hsc[l]);
+  const double npres = p_res / (1.0 + cmax), ndres = d_res / (1.0 + h->hsc[L]);
+  rec[6] = npres;
+  rec[7] = ndres;
+  if (!(std::isfinite(npres) && std::isfinite(ndres) && npres <= threshold && ndres <= threshold))
+    return finish(3);  // :kkt_cold_start_residuals
+  // PSD shifts, identity mass floor and cross-centering on the host (k x k blocks)
+  std::vector<std::vector<double>> X(L), Y(L);
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    X[l].resize(b.k2);
+    Y[l].resize(b.k2);
+    SDPX_HIP(hipMemcpyAsync(X[l].data(), b.X, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipMemcpyAsync(Y[l].data(), b.Y, sizeof(double) * b.k2, hipMemcpyDeviceToHost, h->stream));
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  double max_ps = 0.0, max_ds = 0.0, pmarg = 1.0, dmarg = 1.0;
+  for (int l = 0; l < L; ++l) {
+    const int k = h->blk[l].k;
+    double ps, dsh, pm, dm, sc;
+    const bool pok = cold::psd_shift(X[l].data(), k, ps, pm, sc);
+    const bool dok = cold::psd_shift(Y[l].data(), k, dsh, dm, sc);
+    if (!(pok && dok)) return finish(4);  // :psd_shift
+    max_ps = std::max(max_ps, ps);
+    max_ds = std::max(max_ds, dsh);
+  }
+  double pmass = 0.0, dmass = 0.0;
+  for (int l = 0; l < L; ++l) {
+    pmass += cold::trace(X[l], h->blk[l].k);
+    dmass += cold::trace(Y[l], h->blk[l].k);
+  }
+  double pfloor = 0.0, dfloor = 0.0;
+  if (!cold::mass_shifts(pmass, dmass, degree, pfloor, dfloor)) return finish(5);  // :identity_mass
+  for (int l = 0; l < L; ++l) {
+    const int k = h->blk[l].k;
+    for (int i = 0; i < k; ++i) {
+      if (pfloor > 0.0) X[l][i + (long long)i * k] += pfloor;
+      if (dfloor > 0.0) Y[l][i + (long long)i * k] += dfloor;
+    }
+  }
+  double kappa_floor = 0.0;
+  pmass = dmass = 0.0;
+  for (int l = 0; l < L; ++l) {
+    kappa_floor += cold::kdot(X[l], Y[l]);
+    pmass += cold::trace(X[l], h->blk[l].k);
+    dmass += cold::trace(Y[l], h->blk[l].k);
+  }
+  double pc = 0.0, dc = 0.0;
+  if (!cold::centering_shifts(kappa_floor, pmass, dmass, pc, dc)) return finish(6);  // :centering
+  double kappa_after = 0.0;
+  for (int l = 0; l < L; ++l) {
+    const int k = h->blk[l].k;
+    for (int i = 0; i < k; ++i) {
+      if (pc > 0.0) X[l][i + (long long)i * k] += pc;
+      if (dc > 0.0) Y[l][i + (long long)i * k] += dc;
+    }
+    double mg, sc;
+    cold::psd_stats(X[l].data(), k, sc, mg);
+    pmarg = l == 0 ? mg : std::min(pmarg, mg);
+    cold::psd_stats(Y[l].data(), k, sc, mg);
+    dmarg = l == 0 ? mg : std::min(dmarg, mg);
+    const double kd = cold::kdot(X[l], Y[l]);
+    kappa_after += kd;
+    if (mu) mu[l] = beta * kd / k;
+  }
+  rec[8] = max_ps + std::max(pfloor, 0.0) + std::max(pc, 0.0);
+  rec[9] = max_ds + std::max(dfloor, 0.0) + std::max(dc, 0.0);
+  rec[10] = pmarg;
+  rec[11] = dmarg;
+  rec[12] = kappa_after / (double)std::max(degree, 1LL);
+  rec[13] = kappa_floor / (double)std::max(degree, 1LL);
+  for (int l = 0; l < L; ++l) {
+    Block& b = h->blk[l];
+    SDPX_HIP(hipMemcpyAsync(b.X, X[l].data(), sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(b.Y, Y[l].data(), sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+  }
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return finish(0);
 }
 
 }  // extern "C"

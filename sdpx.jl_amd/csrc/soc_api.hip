@@ -542,6 +542,38 @@ __global__ void zero_ints_kernel(int* p, int n, int value) {
   if (i < n) p[i] = value;
 }
 
+// ---- metric assembly (soc_native.jl:1119-1218) --------------------------------
+// Atil = W^-1 A for every cone c and column j of the coefficient slab A
+// ((sum q) x n, column-major, cone c on rows [ptr[c], ptr[c+1])).  The Gram
+// product Atil' Atil then equals sum_c A_c' (W'W)^-1 A_c: W^-1 is symmetric,
+// so (W'W)^-1 = W^-1 W^-1 = W^-T W^-1.  Cones vary fastest within a column so
+// a wavefront reads consecutive coordinates.
+__global__ void winv_columns_kernel(Cone cn, const double* __restrict__ W, const double* __restrict__ eta,
+                                    const double* __restrict__ A, long long lda, long long ncols,
+                                    double* __restrict__ At) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)cn.ncones * ncols) return;
+  const int c = (int)(idx % cn.ncones);
+  const long long j = idx / cn.ncones;
+  long long b;
+  int q;
+  cone_range(cn.ptr, c, b, q);
+  apply_cone(1, W + b, eta[c], 0.0, A + b + j * lda, q, At + b + j * lda);
+}
+
+// strict upper triangle of the n x n matrix := its lower triangle
+__global__ void mirror_lower_kernel(double* H, int n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  if (r < c) H[r + (long long)c * n] = H[c + (long long)r * n];
+}
+
+__global__ void count_not_ok_kernel(const int* ok, int n, int* count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n && !ok[c]) atomicAdd(count, 1);
+}
+
 }  // namespace
 }  // namespace sdpx
 
@@ -559,6 +591,11 @@ struct sdpx_soc {
   double* vec[kSocSlots] = {};
   double *eta = nullptr, *eta2 = nullptr, *bound = nullptr, *steps = nullptr, *dmin = nullptr;
   int* ok = nullptr;
+  // metric assembly: coefficient slab A ((sum q) x nvars), W^-1 A, the Hessian
+  // (nvars x nvars) and the Gram kernel's split-K partials and work list
+  int nvars = 0, metric_split = 1;
+  double *A = nullptr, *At = nullptr, *H = nullptr, *metric_ws = nullptr;
+  int2* metric_items = nullptr;
   int* counters = nullptr;  // [0] failures / bad cones / interior count, [1..25] strict-step flags
   int* hcounters = nullptr;
   double* hscalar = nullptr;
@@ -745,6 +782,11 @@ int sdpx_soc_create(int device, int ncones, const int* dims, sdpx_soc_t** out) {
   }
   for (int i = 0; i < kSocSlots; ++i)
     if (hipMemset(h->vec[i], 0, sizeof(double) * (size_t)h->total) != hipSuccess) return fail(kHipError);
+  // no NT scaling yet: every cone reads as not scaled
+  if (hipMemset(h->ok, 0, sizeof(int) * (size_t)ncones) != hipSuccess ||
+      hipMemset(h->eta, 0, sizeof(double) * (size_t)ncones) != hipSuccess ||
+      hipMemset(h->eta2, 0, sizeof(double) * (size_t)ncones) != hipSuccess)
+    return fail(kHipError);
   *out = h;
   return kOk;
 }
@@ -1044,6 +1086,79 @@ int sdpx_soc_fraction_to_boundary_noalloc(int device, int ncones, const long lon
   SDPX_HIP(hipGetLastError());
   hipLaunchKernelGGL(min_reduce_kernel, dim3(1), dim3(1024), 0, st, bound, ncones, min_bound);
   SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+int sdpx_soc_set_constraints(sdpx_soc_t* h, int n, const double* A) {
+  SDPX_CHECK(soc_check(h));
+  if (n < 1 || !A || h->nvars != 0) {
+    set_last_error("sdpx_soc_set_constraints: n >= 1 and A required, once per handle");
+    return kInvalidArgument;
+  }
+  const long long N = h->total;
+  const int nt = ceil_div(n, 128);
+  const long long tiles = (long long)nt * (nt + 1) / 2;
+  // split-K so that the Gram launch fills two rounds of resident groups, each K range >= 256
+  int split = (int)std::max<long long>(1, std::min<long long>(16, (1024 + tiles - 1) / tiles));
+  split = (int)std::max<long long>(1, std::min<long long>(split, N / 256));
+  while (split > 1 && (long long)split * n * n > (1LL << 28)) --split;  // partials within 2 GiB
+  SDPX_CHECK(soc_alloc(h, (void**)&h->A, sizeof(double) * (size_t)(N * n)));
+  SDPX_CHECK(soc_alloc(h, (void**)&h->At, sizeof(double) * (size_t)(N * n)));
+  SDPX_CHECK(soc_alloc(h, (void**)&h->H, sizeof(double) * (size_t)n * n));
+  if (split > 1) SDPX_CHECK(soc_alloc(h, (void**)&h->metric_ws, sizeof(double) * (size_t)split * n * n));
+  std::vector<int2> items((size_t)tiles * split);
+  gram_item_order(n, 8, split, items.data());
+  SDPX_CHECK(soc_alloc(h, (void**)&h->metric_items, sizeof(int2) * items.size()));
+  SDPX_HIP(hipMemcpyAsync(h->A, A, sizeof(double) * (size_t)(N * n), hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->metric_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+                          h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  h->nvars = n;
+  h->metric_split = split;
+  return kOk;
+}
+
+int sdpx_soc_metric(sdpx_soc_t* h, int w, const double* H0, double* H, int* nfailed) {
+  SDPX_CHECK(soc_check(h, w));
+  if (h->nvars == 0) {
+    set_last_error("sdpx_soc_metric: no constraints (sdpx_soc_set_constraints)");
+    return kInvalidArgument;
+  }
+  const int n = h->nvars;
+  const long long N = h->total;
+  // cones whose last NT scaling failed have no metric: report them, assemble nothing
+  SDPX_CHECK(reset_counters(h, 1, 0));
+  hipLaunchKernelGGL(count_not_ok_kernel, soc_grid(h->ncones), dim3(256), 0, h->stream, h->ok, h->ncones,
+                     h->counters);
+  int bad = 0;
+  SDPX_CHECK(read_counter(h, &bad));
+  if (nfailed) *nfailed = bad;
+  if (bad) return 1;
+  if (H0) SDPX_HIP(hipMemcpyAsync(h->H, H0, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, h->stream));
+  SDPX_CHECK(begin(h));
+  Cone cn{h->ptr, h->ncones};
+  const long long work = (long long)h->ncones * n;
+  hipLaunchKernelGGL(winv_columns_kernel, dim3((unsigned)ceil_div(work, 256)), dim3(256), 0, h->stream, cn,
+                     h->vec[w], h->eta, h->A, N, (long long)n, h->At);
+  SDPX_HIP(hipGetLastError());
+  GemmArgs g{};
+  g.M = g.N = n;
+  g.K = (int)N;
+  g.alpha = 1.0;
+  g.beta = H0 ? 1.0 : 0.0;
+  g.A = g.B = h->At;
+  g.lda = g.ldb = N;
+  g.C = h->H;
+  g.ldc = n;
+  g.batch = 1;
+  g.flags = kGemmTransA | kGemmLowerOut | kGemmSchurGram;
+  g.item_order = h->metric_items;
+  SDPX_CHECK(gemm_f64(g, h->stream, h->metric_split, h->metric_ws, nullptr));
+  hipLaunchKernelGGL(mirror_lower_kernel, dim3((unsigned)ceil_div((long long)n * n, 256)), dim3(256), 0, h->stream,
+                     h->H, n);
+  SDPX_CHECK(end(h));
+  if (H) SDPX_HIP(hipMemcpyAsync(H, h->H, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
   return kOk;
 }
 

@@ -59,6 +59,9 @@ SIGNATURES = {
     "sdpx_dense_set_tiling": [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
     "sdpx_gram_work_list": [C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_dense_set_precision": [_h, C.c_int],
+    "sdpx_dense_equilibrate": [_h, C.c_int, _dp, _dp, _dp, _ip],
+    "sdpx_dense_get_data": [_h, _dp, _dp],
+    "sdpx_dense_cold_start": [_h, C.c_double, C.c_double, C.c_double, _dp, _dp],
     "sdpx_dense_exact_residual": [_h, _dp],
     "sdpx_dense_refine_exact": [_h, C.c_double, C.c_int, _ip, _dp],
     "sdpx_batch_default_options": [C.c_void_p, C.c_double],
@@ -94,6 +97,8 @@ SIGNATURES = {
     "sdpx_soc_jordan_solve": [_h, C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_soc_inverse": [_h, C.c_int, C.c_int, _ip],
     "sdpx_soc_fraction_to_boundary": [_h, C.c_int, C.c_int, _dp, _dp],
+    "sdpx_soc_set_constraints": [_h, C.c_int, _dp],
+    "sdpx_soc_metric": [_h, C.c_int, _dp, _dp, _ip],
     "sdpx_soc_fraction_to_boundary_noalloc": [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p],
     "sdpx_soc_margins": [_h, C.c_int, _dp, _ip],

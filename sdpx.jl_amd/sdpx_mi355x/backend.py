@@ -55,6 +55,30 @@ def ingest(c, A, C) -> SDPProblem:
     return SDPProblem(c=c, C=Cs, Av=Av, k=ks)
 
 
+COLD_START_REASONS = {0: "none", 1: "identity_kkt_factorization", 2: "kkt_cold_start_solve",
+                      3: "kkt_cold_start_residuals", 4: "psd_shift", 5: "identity_mass", 6: "centering"}
+
+
+@dataclass
+class Equilibration:
+    """Ruiz row scales per block, per-variable scales and the objective scale (ingest.jl:1120-1139)."""
+
+    E: list
+    s: np.ndarray
+    objective_scale: float
+    passes: list
+
+    def unequilibrate(self, x, X, Y, y=None):
+        """unequilibrate (ingest.jl:1523-1560): the original problem's (x, X, y, Y)."""
+        x = np.asarray(x) / self.s
+        Xo = [Xl / np.outer(E, E) for Xl, E in zip(X, self.E)]
+        Yo = [Yl * np.outer(E, E) for Yl, E in zip(Y, self.E)]
+        if self.objective_scale != 1.0:
+            Yo = [Yl * self.objective_scale for Yl in Yo]
+            y = None if y is None else np.asarray(y) * self.objective_scale
+        return x, Xo, y, Yo
+
+
 def _fortran_concat(blocks) -> np.ndarray:
     return np.concatenate([np.asarray(b, dtype=np.float64).ravel(order="F") for b in blocks])
 
@@ -92,6 +116,40 @@ class MI355XDenseBackend:
             self.close()
         except Exception:
             pass
+
+    # ---- setup on the device ------------------------------------------------
+    def equilibrate(self, ruiz_iters: int | str = "auto"):
+        """equilibrate (ingest.jl:1432-1521) on the device-resident problem -> Equilibration.
+
+        The handle's panels, C and c are scaled in place; ``self.prob`` is replaced by the scaled
+        problem's host copy (C and c downloaded; the panels stay on the device only)."""
+        n = -1 if ruiz_iters == "auto" else int(ruiz_iters)
+        E = np.empty(int(sum(self.k)))
+        s = np.empty(self.m)
+        osc = C.c_double(0.0)
+        passes = np.zeros(self.L, dtype=np.int32)
+        check(self.lib.sdpx_dense_equilibrate(self.h, n, dptr(E), dptr(s), C.byref(osc),
+                                              passes.ctypes.data_as(_lib._ip)), "equilibrate")
+        Cc = np.empty(int(self.offsets[-1]))
+        cc = np.empty(self.m)
+        check(self.lib.sdpx_dense_get_data(self.h, dptr(Cc), dptr(cc)), "get_data")
+        self.prob = SDPProblem(c=cc, C=self._split(Cc), Av=[None] * self.L, k=list(self.k))
+        Es = np.split(E, np.cumsum(self.k)[:-1])
+        return Equilibration(E=Es, s=s, objective_scale=float(osc.value), passes=passes.tolist())
+
+    def cold_start(self, beta: float, eps_primal: float, eps_dual: float) -> dict:
+        """_kkt_cold_start_initialization (interior_point.jl:699-1287): sets the iterate; returns the record."""
+        mu = np.empty(self.L)
+        out = np.zeros(16)
+        check(self.lib.sdpx_dense_cold_start(self.h, beta, eps_primal, eps_dual, dptr(mu), dptr(out)), "cold_start")
+        names = ("ok", "reason", "regularization_attempts", "primal_kkt_residual", "dual_kkt_residual",
+                 "refinement_steps", "primal_residual", "dual_residual", "largest_primal_shift",
+                 "largest_dual_shift", "primal_margin", "dual_margin", "kappa_after", "kappa_after_mass_floor")
+        rec = dict(zip(names, out.tolist()))
+        rec["ok"] = bool(rec["ok"])
+        rec["reason"] = COLD_START_REASONS[int(rec["reason"])]
+        rec["mu"] = mu
+        return rec
 
     def memory_bytes(self) -> int:
         b = C.c_longlong()

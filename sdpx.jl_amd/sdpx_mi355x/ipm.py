@@ -41,6 +41,7 @@ class SDPResult:
     schur_trace: list = field(default_factory=list)  # first trace_schur steps: S and the iterate it was built at
     fp32_iterations: int = 0   # Newton steps whose Schur matrix was assembled in FP32 or BF16x3 (mixed precision)
     fp64_iterations: int = 0
+    equilibration: object = None  # the Ruiz scaling the solve ran under (equilibrate=True)
 
 
 def resolved_step_rule(k, rule: str) -> str:
@@ -203,21 +204,49 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
 
 
 def solve(prob: SDPProblem, o: SolverOptions = SolverOptions(), x0=None, X0=None, Y0=None, device: int = 0,
-          timing: bool = False, trace_schur: int = 0, backend: MI355XDenseBackend | None = None) -> SDPResult:
-    """Solve a core-form SDP on the GPU (interior_point.jl:1323-2420 warm-start path)."""
+          timing: bool = False, trace_schur: int = 0, backend: MI355XDenseBackend | None = None,
+          start: str = "given", equilibrate: bool = False, ruiz_iters="auto") -> SDPResult:
+    """Solve a core-form SDP on the GPU (interior_point.jl:1323-2420).
+
+    start="given" takes x0/X0/Y0 (the warm-start path; Omega-scaled identities when X0 is None);
+    start="cold" runs the identity-metric KKT cold start on the device (interior_point.jl:699-1287).
+    equilibrate=True scales the device-resident problem first (Ruiz, ingest.jl:1432-1521) and maps
+    the result back (unequilibrate, ingest.jl:1523-1560); the objectives are then the original
+    problem's.  A backend passed in is scaled in place."""
     own = backend is None
     b = backend if backend is not None else MI355XDenseBackend(prob, device)
     try:
-        return _solve(b, prob, o, x0, X0, Y0, timing, trace_schur)
+        if not equilibrate:
+            return _solve(b, prob, o, x0, X0, Y0, timing, trace_schur, start)
+        eq = b.equilibrate(ruiz_iters)
+        if start != "cold" and X0 is not None:  # warm start into the scaled space (interior_point.jl:606-661)
+            x0 = np.asarray(x0) * eq.s
+            X0 = [Xl * np.outer(E, E) for Xl, E in zip(X0, eq.E)]
+            Y0 = [Yl / np.outer(E, E) / eq.objective_scale for Yl, E in zip(Y0, eq.E)]
+        r = _solve(b, b.prob, o, x0, X0, Y0, timing, trace_schur, start)
+        x, X, _, Y = eq.unequilibrate(r.x, r.X, r.Y)
+        r.x, r.X, r.Y = x, X, Y
+        r.pobj = float(prob.c @ x)
+        r.dobj = float(sum(np.sum(prob.C[l] * Y[l]) for l in range(prob.L)))
+        r.gap_rel = abs(r.pobj - r.dobj) / max(1.0, (abs(r.pobj) + abs(r.dobj)) / 2)
+        r.equilibration = eq
+        return r
     finally:
         if own:
             b.close()
 
 
-def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
+def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur, start="given") -> SDPResult:
     m, L, k = prob.m, prob.L, prob.k
     x = np.zeros(m) if x0 is None else np.asarray(x0, dtype=np.float64)
-    if X0 is None:
+    if start == "cold":
+        rec = b.cold_start(o.beta, o.eps_primal, o.eps_dual)
+        if not rec["ok"]:
+            X = [np.eye(kl) for kl in k]
+            return SDPResult("NumericalBreakdown", x, X, X, 0.0, 0.0, 0, math.inf, math.inf, math.inf,
+                             f"cold start failed: {rec['reason']}")
+        x, X, Y = b.get_iterate()
+    elif X0 is None:
         X = [o.omega_p * np.eye(kl) for kl in k]
         Y = [o.omega_d * np.eye(kl) for kl in k]
     else:
@@ -228,7 +257,8 @@ def _solve(b, prob, o, x0, X0, Y0, timing, trace_schur) -> SDPResult:
             except np.linalg.LinAlgError:
                 return SDPResult("NumericalBreakdown", x, X, Y, 0.0, 0.0, 0, math.inf, math.inf, math.inf,
                                  "initial X0/Y0 must be positive definite")
-    b.set_iterate(x, X, Y)
+    if start != "cold":
+        b.set_iterate(x, X, Y)
     mu = np.array([o.beta * float(np.sum(X[l] * Y[l])) / k[l] for l in range(L)])
     ctrl = AdaptiveIPMController.from_options(o)
     clock = _Clock(b, timing)

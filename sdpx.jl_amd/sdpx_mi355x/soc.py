@@ -102,6 +102,27 @@ class LorentzCones:
               "sdpx_soc_fraction_to_boundary")
         return (mn.value, pc) if per_cone else mn.value
 
+    def set_constraints(self, A) -> None:
+        """A: (sum q) x n coefficient rows of every cone, stacked in cone order (uploaded once)."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        if A.ndim != 2 or A.shape[0] != self.total:
+            raise ValueError(f"A must have {self.total} rows (one per cone coordinate)")
+        self.nvars = int(A.shape[1])
+        check(self._lib.sdpx_soc_set_constraints(self._h, self.nvars, dptr(A)), "sdpx_soc_set_constraints")
+
+    def metric(self, w: int, H0=None, download: bool = True):
+        """H0 + sum_c A_c' (W_c'W_c)^-1 A_c -> (H or None, failed cones)."""
+        n = self.nvars
+        h0 = None if H0 is None else np.asfortranarray(H0, dtype=np.float64)
+        if h0 is not None and h0.shape != (n, n):
+            raise ValueError(f"H0 must be {n} x {n}")
+        H = np.empty((n, n), order="F") if download else None
+        nf = C.c_int(0)
+        rc = self._lib.sdpx_soc_metric(self._h, w, dptr(h0), dptr(H), C.byref(nf))
+        if rc < 0:
+            check(rc, "sdpx_soc_metric")
+        return H, nf.value
+
     def margins(self, slot: int):
         mg = np.empty(self.ncones)
         ni = C.c_int(0)

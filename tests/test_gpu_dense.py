@@ -383,3 +383,88 @@ def test_diagnostics_match_oracle(gpu_lib):
         assert abs(dobj - O.dual_objective(oprob, Y)) <= 1e-13 * max(1.0, abs(dobj))
     finally:
         b.close()
+
+
+def _scaled_two_block(seed=1):
+    """Two blocks (k = 12 and 7) with rows scaled by powers of ten: Ruiz has work to do in each."""
+    rng = np.random.default_rng(seed)
+    m = 30
+    g1, g2 = G.dense_feasible(12, m, seed=seed), G.dense_feasible(7, m, seed=seed + 50)
+    d1, d2 = 10.0 ** rng.integers(-3, 4, 12), 10.0 ** rng.integers(-2, 3, 7)
+    A = [g1.A_blocks() * np.outer(d1, d1)[None], g2.A_blocks() * np.outer(d2, d2)[None]]
+    Cs = [g1.C * np.outer(d1, d1), g2.C * np.outer(d2, d2) - 0.5 * np.eye(7)]
+    c = g1.c * 10.0 ** rng.integers(-2, 3, m)
+    return ingest(c, A, Cs), O.ingest(c, A, Cs)
+
+
+@pytest.mark.parametrize("ruiz_iters", ["auto", 3, 0])
+def test_equilibrate_matches_oracle_bitwise(gpu_lib, ruiz_iters):
+    """equilibrate (ingest.jl:1432-1521) on the device: maxima, 1/sqrt, products of two factors and
+    divisions are correctly rounded, so E, s, the objective scale and the scaled C and c equal the
+    oracle's bit for bit; the scaled panels are compared through the Gram (the Schur matrix at X = Y = I)."""
+    prob, oprob = _scaled_two_block()
+    b = MI355XDenseBackend(prob)
+    try:
+        eq = b.equilibrate(ruiz_iters)
+        sp, oeq = O.equilibrate(oprob, ruiz_iters)
+        print(f"passes gpu {eq.passes} oracle {oeq.passes}, objective scale {eq.objective_scale}", flush=True)
+        assert eq.passes == oeq.passes
+        for El, Eo in zip(eq.E, oeq.E):
+            assert np.array_equal(El, Eo)
+        assert np.array_equal(eq.s, oeq.s) and eq.objective_scale == oeq.objective_scale
+        assert np.array_equal(b.prob.c, sp.c)
+        for Cl, Co in zip(b.prob.C, sp.C):
+            assert np.array_equal(Cl, Co)
+        eyes = [np.eye(12), np.eye(7)]
+        b.set_iterate(np.zeros(prob.m), eyes, eyes)
+        b.compute_residuals([0.0, 0.0], affine=True, factor=True)
+        b.schur_build()
+        gram = sum(Av.T @ Av for Av in sp.Av)
+        assert _rel_fro_lower(b.get_schur(), gram) <= 1e-14
+    finally:
+        b.close()
+
+
+def test_cold_start_matches_oracle(gpu_lib):
+    """_kkt_cold_start_initialization (interior_point.jl:699-1287) on the device against the oracle:
+    the identity-metric solves, the residual gate and the shifts give the same start."""
+    prob, oprob = _scaled_two_block(seed=3)
+    o = O.Options()
+    b = MI355XDenseBackend(prob)
+    try:
+        b.equilibrate()
+        sp, _ = O.equilibrate(oprob)
+        rec = b.cold_start(o.beta, o.eps_primal, o.eps_dual)
+        x, X, Y = b.get_iterate()
+    finally:
+        b.close()
+    ok, reason, xo, Xo, Yo, muo, reco = O.cold_start(sp, o)
+    print(rec, reco, flush=True)
+    assert rec["ok"] and ok and rec["reason"] == reason == "none"
+    assert rec["regularization_attempts"] == reco["regularization_attempts"]
+    assert np.max(np.abs(x - xo)) <= 1e-11 * max(1.0, np.max(np.abs(xo)))
+    for l in range(2):
+        assert np.max(np.abs(X[l] - Xo[l])) <= 1e-11 * max(1.0, np.max(np.abs(Xo[l])))
+        assert np.max(np.abs(Y[l] - Yo[l])) <= 1e-11 * max(1.0, np.max(np.abs(Yo[l])))
+    assert np.allclose(rec["mu"], muo, rtol=1e-11)
+    assert abs(rec["largest_primal_shift"] - reco["largest_primal_shift"]) <= 1e-11 * reco["largest_primal_shift"]
+
+
+def test_equilibrated_cold_solve_matches_oracle(gpu_lib):
+    """The reference's default SDP setup on the device (Ruiz, then the identity-metric cold start) and
+    the solve, mapped back to the original problem, against the oracle doing the same: status,
+    objective 1e-8 relative, iterations +-2, and the certificate on the original problem."""
+    prob, oprob = _scaled_two_block(seed=5)
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    res = solve(prob, o, start="cold", equilibrate=True)
+    ref, _ = O.solve_equilibrated(oprob, oo, start="cold")
+    print(f"gpu {res.status} it={res.iterations} pobj={res.pobj:.12e} | oracle {ref.status} it={ref.iterations} "
+          f"pobj={ref.pobj:.12e}", flush=True)
+    assert res.status == ref.status == "Optimal"
+    assert abs(res.iterations - ref.iterations) <= 2
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
+                                         0.0, 0.0, res.gap_rel), 1e-7, 1e-7, 1e-7)
+    assert cert["valid"], cert

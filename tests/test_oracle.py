@@ -199,3 +199,117 @@ def test_refine_direction_steps_follow_the_contraction(name, rho, mode, fixed, c
         assert res < res0 * (0.71 if mode == "ones" else 0.5 ** expected)
         # the exact solution of M dx = r is approached, not S's
         assert np.linalg.norm(dx - np.linalg.solve(M, r)) < np.linalg.norm(dx0 - np.linalg.solve(M, r))
+
+
+# ---------------------------------------------------------------- setup: cold start and Ruiz
+def test_cold_start_shift_helpers_reference_values():
+    """test/cold_start.jl:134-330 (the Float64 instances)."""
+    A = np.array([[1.0]])
+    assert O.cold_psd_shift(A)[:3] == (True, 0.0, 1.0)
+    A = np.array([[-1.0]])
+    ok, shift, margin, scale = O.cold_psd_shift(A)
+    assert ok and shift > 0 and margin == A[0, 0] and margin > O.SQRT_EPS * max(1.0, scale)
+    for entry, expect_zero in (([[2, 1], [1, 2]], True), ([[2, 2], [2, 2]], False), ([[1, -3], [-3, 1]], False)):
+        M = np.array(entry, dtype=float)
+        ok, shift, margin, scale = O.cold_psd_shift(M)
+        assert ok
+        if expect_zero:
+            assert shift == 0.0 and abs(margin - 1.0) <= O.SQRT_EPS * 8
+        else:
+            expected = (M[0, 0] + M[1, 1]) / 2 - math.hypot((M[0, 0] - M[1, 1]) / 2, (M[0, 1] + M[1, 0]) / 2)
+            assert shift > 0 and abs(margin - expected) <= O.SQRT_EPS * 16 * scale
+            assert margin > O.SQRT_EPS * max(1.0, scale)
+    definite = np.array([[5, 1, 0, 0, 1, 0], [1, 6, 1, 0, 0, 0], [0, 1, 5, 1, 0, 0], [0, 0, 1, 6, 1, 0],
+                         [1, 0, 0, 1, 5, 1], [0, 0, 0, 0, 1, 6]], dtype=float)
+    ok, shift, margin, scale = O.cold_psd_shift(definite.copy())
+    assert ok and shift == 0.0 and margin > O.SQRT_EPS * max(1.0, scale)
+    indefinite = definite.copy()
+    indefinite[0, 0] = -10.0
+    ok, shift, margin, scale = O.cold_psd_shift(indefinite)
+    assert ok and shift > 0 and margin > O.SQRT_EPS * max(1.0, scale)
+    np.linalg.cholesky(indefinite)  # positive definite after the shift
+    poisoned = definite.copy()
+    poisoned[np.triu_indices(6, 1)] = np.nan  # lower-authoritative storage
+    assert O.cold_psd_shift(poisoned)[:2] == (True, 0.0)
+    nonfinite = np.ones((2, 2))
+    nonfinite[1, 1] = np.inf
+    assert not O.cold_psd_shift(nonfinite)[0]
+    assert O.cold_psd_shift(np.zeros((0, 0))) == (True, 0.0, 0.0, 1.0)
+    # centering shifts (cold_start.jl:240-262)
+    ok, ps, ds = O.cold_centering_shifts(1.0, 2.0, 4.0)
+    assert ok and math.isclose(ps, 0.125) and math.isclose(ds, 0.25)
+    for bad in (0.0, -1.0):
+        assert O.cold_centering_shifts(1.0, bad, 4.0) == (False, 0.0, 0.0)
+    assert not O.cold_centering_shifts(0.0, 2.0, 4.0)[0]
+    assert not O.cold_centering_shifts(math.inf, 2.0, 4.0)[0]
+    assert not O.cold_centering_shifts(1.0, 2.0, math.inf)[0]
+    # identity mass floor (cold_start.jl:296-330)
+    tiny = O.SQRT_EPS
+    ok, ps, ds = O.cold_identity_mass_shifts(tiny, tiny, 1)
+    assert ok and ps > 0 and ds > 0 and tiny + ps >= 1.0 and tiny + ds >= 1.0
+    assert O.cold_identity_mass_shifts(2.0, 3.0, 2) == (True, 0.0, 0.0)
+    assert O.cold_identity_mass_shifts(1.0, 4.0, 2) == (True, 0.5, 0.0)
+    for bad in (math.inf, math.nan):
+        assert not O.cold_identity_mass_shifts(bad, 1.0, 1)[0]
+
+
+def test_cold_start_kkt_solutions():
+    """The identity-metric solves project C onto span{A_i}: X = A(x) - C is orthogonal to every A_i,
+    and Y = A(v) satisfies <A_i, Y> = c_i; the shifted point is strictly interior."""
+    g = G.dense_feasible(9, 20, seed=4)
+    prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    o = O.Options()
+    eye = [np.eye(9)]
+    S = O.schur_build(prob, eye, eye)
+    Lf, ok, _ = O.factor_dense_kkt(S)
+    x = O.solve_dense_kkt(Lf, O.contract(prob, 0, prob.C[0]))
+    v = O.solve_dense_kkt(Lf, prob.c)
+    Xp = O.build_P(prob, 0, x) - prob.C[0]
+    Yp = O.build_P(prob, 0, v)
+    assert np.max(np.abs(O.contract(prob, 0, Xp))) <= 1e-12 * np.max(np.abs(prob.C[0])) * 20
+    assert np.max(np.abs(O.contract(prob, 0, Yp) - prob.c)) <= 1e-12 * np.max(np.abs(prob.c)) * 20
+    ok, reason, xc, X, Y, mu, rec = O.cold_start(prob, o)
+    assert ok and reason == "none"
+    assert np.array_equal(xc, x)
+    np.linalg.cholesky(X[0])
+    np.linalg.cholesky(Y[0])
+    assert math.isclose(mu[0], o.beta * float(np.sum(X[0] * Y[0])) / 9)
+
+
+def _badly_scaled(k=12, m=30, seed=1):
+    rng = np.random.default_rng(0)
+    g = G.dense_feasible(k, m, seed=seed)
+    d = 10.0 ** rng.integers(-3, 4, k)
+    A = g.A_blocks() * np.outer(d, d)[None]
+    return O.ingest(g.c * 10.0 ** rng.integers(-2, 3, m), [A], [g.C * np.outer(d, d)])
+
+
+def test_equilibrate_identity_on_balanced_data():
+    """Row norms already 1 (entries +-1 on every row of C = I and every A_i): e = 1 on every pass, so
+    E = 1 after the two minimum passes and s_i = max|A_i| = 1."""
+    k, m = 5, 7
+    rng = np.random.default_rng(2)
+    A = np.sign(rng.standard_normal((m, k, k)))
+    A = np.triu(A) + np.triu(A, 1).transpose(0, 2, 1)
+    c = rng.standard_normal(m)
+    sp, eq = O.equilibrate(O.ingest(c, [A], [np.eye(k)]))
+    assert eq.passes == [2] and np.array_equal(eq.E[0], np.ones(k)) and np.array_equal(eq.s, np.ones(m))
+    assert eq.objective_scale == np.max(np.abs(c)) and np.array_equal(sp.c, c / np.max(np.abs(c)))
+
+
+def test_equilibrate_balances_and_preserves_the_optimum():
+    """A badly scaled problem (rows scaled by 1e-3..1e3): after the Ruiz passes every block row has
+    infinity-norm near 1, and the equilibrated solve maps back to an optimum of the original problem
+    (the unscaled solve stalls on it)."""
+    prob = _badly_scaled()
+    sp, eq = O.equilibrate(prob)
+    T = sp.Av[0].T.reshape(prob.m, 12, 12)
+    rn = np.maximum(np.max(np.abs(T), axis=(0, 1)), np.max(np.abs(sp.C[0]), axis=1))
+    # rows before the per-variable scale: within 1/20 in log after convergence, then the A_i are divided by s
+    assert np.all(rn <= 1.0 + 1e-12) and np.all(rn >= 0.5)
+    o = O.Options(eps_gap=1e-9, eps_primal=1e-9, eps_dual=1e-9)
+    r, _ = O.solve_equilibrated(prob, o, start="cold")
+    assert r.status == "Optimal"
+    cert = O.certificate(prob, r, 1e-7, 1e-7, 1e-7)
+    assert cert["valid"], cert
+    assert O.solve(prob, o).status != "Optimal"

@@ -382,3 +382,85 @@ def test_gpu_full_size_properties(cones):
     c.apply(APPLY_W, 2, 6, 7)
     c.apply(APPLY_WINV, 2, 7, 7)
     np.testing.assert_allclose(c.get(7), x, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- metric assembly (soc_native.jl:1119-1218)
+@pytest.mark.parametrize("q", [2, 3, 4, 7])
+def test_metric_matrix_maps_s_to_z(q):
+    """The NT scaling is defined by W z = W^-1 s = lambda, so (W'W)^-1 s = z (W symmetric): the oracle's
+    metric block is pinned by the NT equations, and it is symmetric positive definite."""
+    rng = np.random.default_rng(q)
+    s, z = _cone(rng, q, 1.5), _cone(rng, q, 0.7)
+    ok, w, lam, eta, eta2 = O.nt_scaling(s, z)
+    assert ok
+    M = O.metric_matrix(w, eta2)
+    assert np.allclose(M @ np.array(s), z, rtol=1e-13, atol=1e-13 * np.max(np.abs(z)))
+    assert np.allclose(M, M.T, rtol=0, atol=1e-14 * np.max(np.abs(M)))
+    assert np.min(np.linalg.eigvalsh((M + M.T) / 2)) > 0
+
+
+def test_add_metric_block_structure():
+    """Cones whose coefficient rows select disjoint variables give a block-diagonal Hessian whose
+    blocks are the metric matrices (each pinned by the NT equations above)."""
+    rng = np.random.default_rng(3)
+    dims = [3, 2, 4]
+    n = sum(dims)
+    H = np.zeros((n, n))
+    blocks, off = [], 0
+    for q in dims:
+        s, z = _cone(rng, q, 1.2), _cone(rng, q, 0.9)
+        ok, w, _, _, eta2 = O.nt_scaling(s, z)
+        A = np.zeros((q, n))
+        A[:, off:off + q] = np.eye(q)
+        O.add_metric(H, A, w, eta2)
+        blocks.append((off, q, O.metric_matrix(w, eta2)))
+        off += q
+    for o, q, M in blocks:  # 0/1 selections: every product is exact
+        assert np.array_equal(H[o:o + q, o:o + q], M)
+    mask = np.ones((n, n), bool)
+    for o, q, _ in blocks:
+        mask[o:o + q, o:o + q] = False
+    assert np.all(H[mask] == 0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims_kind", ["mixed", "uniform4"])
+def test_gpu_metric_matches_oracle(gpu_lib, dims_kind):
+    """sdpx_soc_metric (one W^-1 pass over the coefficient slab, then the FP64 Gram kernel) against the
+    oracle's per-cone dense assembly, accumulating onto H0; a cone whose NT scaling failed is reported."""
+    from sdpx_mi355x.soc import LorentzCones
+
+    rng = np.random.default_rng(17)
+    ncones, n = 700, 333
+    dims = rng.choice([2, 3, 4, 5, 8, 13], size=ncones) if dims_kind == "mixed" else np.full(ncones, 4)
+    s = random_interior(rng, dims, 1.5)
+    z = random_interior(rng, dims, 0.7)
+    A = rng.standard_normal((int(dims.sum()), n)) / np.sqrt(n)
+    H0 = rng.standard_normal((n, n))
+    H0 = H0 + H0.T
+    with LorentzCones(dims) as c:
+        c.set(0, s)
+        c.set(1, z)
+        c.set_constraints(A)
+        assert c.metric(2)[1] == ncones  # before any NT scaling every cone is unscaled
+        assert c.nt_scaling(0, 1, 2, 3) == 0
+        H, nf = c.metric(2, H0=H0)
+        assert nf == 0
+        eta, eta2, ok = c.scaling()
+        w = c.get(2)
+        ref = H0.copy()
+        off = 0
+        for ci, q in enumerate(dims):
+            O.add_metric(ref, A[off:off + q], list(w[off:off + q]), float(eta2[ci]))
+            off += q
+        rel = np.linalg.norm(H - ref) / np.linalg.norm(ref - H0)
+        print(f"{dims_kind}: |H - oracle| / |sum A'MA| = {rel:.2e}", flush=True)
+        assert rel <= 1e-13
+        assert np.array_equal(H, H.T)
+        # a cone outside the interior: its NT scaling fails and the metric reports it
+        s_bad = s.copy()
+        s_bad[0] = -1.0
+        c.set(0, s_bad)
+        assert c.nt_scaling(0, 1, 2, 3) == 1
+        H2, nf = c.metric(2, download=True)
+        assert nf == 1 and H2 is not None
