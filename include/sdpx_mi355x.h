@@ -226,6 +226,29 @@ int sdpx_dense_get_data(sdpx_dense_t* h, double* C, double* c);
  * floor / degree. */
 int sdpx_dense_cold_start(sdpx_dense_t* h, double beta, double eps_primal, double eps_dual, double* mu, double* out);
 
+/* ---- equality-constrained KKT (n_eq > 0) ---------------------------------------
+ * The KKT system  S dx - B dy = r,  B' dx = p  of the dense route with n
+ * equality constraints (B: m x n column-major, uploaded once).
+ * sdpx_dense_equality_factor replaces the n > 0 branch of
+ * _factor_dense_kkt_native! (kkt.jl:1466-1560) after sdpx_dense_schur_factor:
+ * B~ = L_S^-1 B (blocked substitution with the Schur factor), the
+ * column normalization B^ = B~ D, D_jj = 1/||B~_j||_inf
+ * (_normalize_equality_panel_columns!, kkt.jl:281-304), Q = B^'B^ on MFMA and
+ * its Cholesky, accepted when min diag > tau max diag with
+ * tau = max(max(m, n) eps, min(sqrt(eps), requested)) and no exactly
+ * duplicated columns (_cholesky_has_numerical_rank, kkt.jl:76-176).
+ * out (4): status (0 ok, 1 Gram factor failed, 2 numerically rank deficient),
+ * min and max |diag| of the factor of Q, tau.  Returns 1 unless status 0.  The
+ * reference's pivoted-QR fallback for rank-deficient B (kkt.jl:1517-1560) is
+ * not provided: status 2 is returned instead. */
+int sdpx_dense_set_equalities(sdpx_dense_t* h, int n, const double* B);
+int sdpx_dense_equality_factor(sdpx_dense_t* h, double requested, double* out);
+/* _solve_dense_kkt_owned! (kkt.jl:2570-2595) for n > 0 with the handle's r:
+ * dx (left in the handle, sdpx_dense_get_direction) and dy (n, host). */
+int sdpx_dense_solve_equality(sdpx_dense_t* h, const double* p, double* dy);
+/* B~ D (m x n), Q (n x n, lower) and D (n) of the last equality factor. */
+int sdpx_dense_get_equality(sdpx_dense_t* h, double* Bt, double* Q, double* scale);
+
 /* ---- a13: mixed precision ----------------------------------------------------
  * The reference's mixed-precision template (kernels/mixed_precision_kkt.jl:
  * low-precision factor, target-precision residuals and refinement,

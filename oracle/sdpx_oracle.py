@@ -773,6 +773,58 @@ def solve_dense_kkt(Lf: np.ndarray, r) -> np.ndarray:
     return sla.solve_triangular(Lf, t, lower=True, trans="T", check_finite=False)
 
 
+def equality_rank_tau(m, n, requested):
+    """_equality_cholesky_rank_tau, Float64 branch (kkt.jl:103-112)."""
+    req = requested if requested > 0 else SQRT_EPS
+    req = min(SQRT_EPS, req)
+    return max(max(m, n) * EPS, min(SQRT_EPS, req))
+
+
+def factor_equality(Lf: np.ndarray, B: np.ndarray, requested: float):
+    """n > 0 branch of _factor_dense_kkt_native! (kkt.jl:1466-1510) on the Schur factor Lf:
+    B~ = L^-1 B, B^ = B~ D with D_jj = 1/||B~_j||_inf (kkt.jl:281-304), Q = B^'B^, its Cholesky and the
+    numerical-rank gate (kkt.jl:122-176).  Returns (status 0 ok / 1 failed / 2 rank deficient, Bh, D, LQ, tau)."""
+    m, n = B.shape
+    Bt = sla.solve_triangular(Lf, B, lower=True, check_finite=False)
+    D = np.ones(n)
+    for j in range(n):
+        norm = float(np.max(np.abs(Bt[:, j]))) if m else 0.0
+        sc = 1.0 if norm == 0.0 else 1.0 / norm
+        if not math.isfinite(sc):
+            sc = 1.0
+        D[j] = sc
+        if sc != 1.0:
+            Bt[:, j] *= sc
+    Q = Bt.T @ Bt
+    tau = equality_rank_tau(m, n, requested)
+    c, info = lapack.dpotrf(np.asfortranarray(np.tril(Q)), lower=1, clean=1)
+    if info != 0:
+        return 1, Bt, D, None, tau
+    dg = np.abs(np.diag(c))
+    mn, mx = float(dg.min()), float(dg.max())
+    if not (math.isfinite(mn) and math.isfinite(mx) and mx > 0):
+        return 1, Bt, D, None, tau
+    if not mn > tau * mx:
+        return 2, Bt, D, None, tau
+    if mn <= math.sqrt(tau) * mx:
+        for a in range(n):
+            for b2 in range(a + 1, n):
+                if np.array_equal(Bt[:, a], Bt[:, b2]):
+                    return 2, Bt, D, None, tau
+    return 0, Bt, D, np.tril(c), tau
+
+
+def solve_equality(Lf, Bh, D, LQ, r, p):
+    """_solve_dense_kkt_owned! n > 0 branch (kkt.jl:2578-2587): the solution of
+    S dx - B dy = r, B' dx = p."""
+    rt = sla.solve_triangular(Lf, r, lower=True, check_finite=False)
+    q = D * p - Bh.T @ rt
+    t = sla.solve_triangular(LQ, q, lower=True, check_finite=False)
+    dyh = sla.solve_triangular(LQ, t, lower=True, trans="T", check_finite=False)
+    dx = sla.solve_triangular(Lf, rt + Bh @ dyh, lower=True, trans="T", check_finite=False)
+    return dx, D * dyh
+
+
 def direction_residual(S, r, dx):
     """rho_r = r - S dx; returns (||rho_r||_inf, rho_r) (kkt.jl:3689-3708)."""
     rho = r - schur_mul(S, dx)

@@ -263,6 +263,43 @@ __global__ void __launch_bounds__(256) ruiz_colmax_kernel(const double* Av, long
   if (threadIdx.x == 0) maxima[i] = first ? red[0] : fmax(maxima[i], red[0]);
 }
 
+// ---- equality-constrained KKT (kkt.jl:1466-1665, 2570-2595) ----------------------
+// dst(r, c) = src(r, c) for an rows x cols block
+__global__ void copy_block_kernel(const double* src, long long lds, double* dst, long long ldd, int rows, int cols) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)rows * cols) return;
+  const int r = (int)(idx % rows), c = (int)(idx / rows);
+  dst[r + c * ldd] = src[r + c * lds];
+}
+
+// _normalize_equality_panel_columns! (kkt.jl:281-304): per column, scale = 1 / ||col||_inf
+// (1 for a zero column or a non-finite reciprocal), applied in place
+__global__ void __launch_bounds__(256) normalize_columns_kernel(double* B, long long ld, int rows, double* scale) {
+  __shared__ double red[256];
+  double* col = B + (long long)blockIdx.x * ld;
+  double mx = 0.0;
+  for (int r = threadIdx.x; r < rows; r += 256) mx = fmax(mx, fabs(col[r]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  double sc = red[0] == 0.0 ? 1.0 : 1.0 / red[0];
+  if (!isfinite(sc)) sc = 1.0;
+  if (threadIdx.x == 0) scale[blockIdx.x] = sc;
+  if (sc == 1.0) return;
+  for (int r = threadIdx.x; r < rows; r += 256) col[r] *= sc;
+}
+
+// q = scale .* p - q   (_form_scaled_equality_rhs!, kkt.jl:346-357), or dy = scale .* dy
+__global__ void scaled_rhs_kernel(const double* scale, const double* p, double* q, int n, int mode) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  if (mode == 0) q[i] = scale[i] * p[i] - q[i];
+  else q[i] = q[i] * scale[i];
+}
+
 // A_i ./= s[i] (the per-variable scale)
 __global__ void div_columns_kernel(double* Av, long long k2, int m, const double* s) {
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -322,6 +359,12 @@ struct sdpx_dense {
   long long bytes = 0;
   std::vector<void*> allocs;
   int last_reg_attempts = 0;
+  // equality constraints B' x = p (n_eq columns of B, m x n_eq): B, B-tilde = L_S^-1 B D,
+  // Q = B-tilde' B-tilde and its factor, the column scales D, solve vectors
+  int neq = 0;
+  bool eq_factored = false;
+  double *B = nullptr, *Bt = nullptr, *Btmp = nullptr, *Q = nullptr, *Qf = nullptr, *dinvQ = nullptr;
+  double *eqscale = nullptr, *qrhs = nullptr, *qy = nullptr, *rtil = nullptr, *peq = nullptr;
   // device-event kernel timing (sdpx_dense_set_profiling)
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -2016,6 +2059,140 @@ int sdpx_dense_cold_start(sdpx_dense_t* h, double beta, double eps_primal, doubl
   }
   SDPX_HIP(hipStreamSynchronize(h->stream));
   return finish(0);
+}
+
+int sdpx_dense_set_equalities(sdpx_dense_t* h, int n, const double* B) {
+  SDPX_CHECK(check_handle(h));
+  if (n < 1 || !B || h->neq != 0) {
+    set_last_error("sdpx_dense_set_equalities: n >= 1 and B (m x n) required, once per handle");
+    return kInvalidArgument;
+  }
+  const long long m = h->m;
+  const long long nbq = (long long)ceil_div(n, kCholNb) * kCholNb * kCholNb;
+  SDPX_CHECK(dalloc(h, &h->B, (size_t)(m * n)));
+  SDPX_CHECK(dalloc(h, &h->Bt, (size_t)(m * n)));
+  SDPX_CHECK(dalloc(h, &h->Btmp, (size_t)kCholNb * n));
+  SDPX_CHECK(dalloc(h, &h->Q, (size_t)n * n));
+  SDPX_CHECK(dalloc(h, &h->Qf, (size_t)n * n));
+  SDPX_CHECK(dalloc(h, &h->dinvQ, (size_t)nbq));
+  SDPX_CHECK(dalloc(h, &h->eqscale, (size_t)n));
+  SDPX_CHECK(dalloc(h, &h->qrhs, (size_t)n));
+  SDPX_CHECK(dalloc(h, &h->qy, (size_t)n));
+  SDPX_CHECK(dalloc(h, &h->peq, (size_t)n));
+  SDPX_CHECK(dalloc(h, &h->rtil, (size_t)m));
+  SDPX_HIP(hipMemcpyAsync(h->B, B, sizeof(double) * (size_t)(m * n), hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  h->neq = n;
+  return kOk;
+}
+
+int sdpx_dense_equality_factor(sdpx_dense_t* h, double requested, double* out) {
+  SDPX_CHECK(check_handle(h));
+  const int m = h->m, n = h->neq;
+  if (n == 0) {
+    set_last_error("sdpx_dense_equality_factor: no equalities (sdpx_dense_set_equalities)");
+    return kInvalidArgument;
+  }
+  h->eq_factored = false;
+  // B-tilde = L_S^-1 B (la_trsm!, kkt.jl:1471): blocked forward substitution with the Schur
+  // factor, each 64-row block = D_j (B_j - L(j, <j) B-tilde_<j) with the factor's diagonal-block inverses
+  SDPX_HIP(hipMemcpyAsync(h->Bt, h->B, sizeof(double) * (size_t)m * n, hipMemcpyDeviceToDevice, h->stream));
+  for (int j0 = 0; j0 < m; j0 += kCholNb) {
+    const int jb = std::min(kCholNb, m - j0);
+    if (j0 > 0)
+      SDPX_CHECK(gemm_f64(gemm(jb, n, j0, -1.0, h->Sf + j0, m, h->Bt, m, 1.0, h->Bt + j0, m, 0), h->stream));
+    SDPX_CHECK(gemm_f64(gemm(jb, n, jb, 1.0, h->dinvS + (long long)(j0 / kCholNb) * kCholNb * kCholNb, kCholNb,
+                             h->Bt + j0, m, 0.0, h->Btmp, kCholNb, kGemmALower),
+                        h->stream));
+    hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)ceil_div((long long)jb * n, 256)), dim3(256), 0, h->stream,
+                       h->Btmp, (long long)kCholNb, h->Bt + j0, (long long)m, jb, n);
+    SDPX_HIP(hipGetLastError());
+  }
+  // D: per-column infinity-norm normalization (kkt.jl:1472)
+  hipLaunchKernelGGL(normalize_columns_kernel, dim3((unsigned)n), dim3(256), 0, h->stream, h->Bt, (long long)m, m,
+                     h->eqscale);
+  SDPX_HIP(hipGetLastError());
+  // Q = B-hat' B-hat (lower) and its Cholesky (kkt.jl:1495-1510)
+  SDPX_CHECK(gemm_f64(gemm(n, n, m, 1.0, h->Bt, m, h->Bt, m, 0.0, h->Q, n, kGemmTransA | kGemmLowerOut), h->stream));
+  SDPX_CHECK(copy_lower_shift_f64(h->Q, h->Qf, n, 0.0, h->stream));
+  SDPX_CHECK(potrf_f64(h->Qf, n, 0, n, 1, h->dinvQ, 0, h->dinfo, h->stream));
+  SDPX_CHECK(diag_extrema_f64(h->Qf, n, 0, n, 1, h->dsc, h->stream));
+  SDPX_CHECK(sync_scalars(h, 2, 1));
+  const int info = h->hinfo[0];
+  const double mn = h->hsc[0], mx = h->hsc[1];
+  // _equality_cholesky_rank_tau / _cholesky_has_numerical_rank (kkt.jl:76-113, 122-176)
+  const double eps = 2.220446049250313e-16;
+  double req = requested > 0.0 ? requested : std::sqrt(eps);
+  req = std::min(std::sqrt(eps), req);
+  const double tau = std::max((double)std::max(m, n) * eps, std::min(std::sqrt(eps), req));
+  int status = 0;
+  if (info != 0 || !std::isfinite(mn) || !std::isfinite(mx) || !(mx > 0.0)) {
+    status = 1;  // the Gram factor failed
+  } else if (!(mn > tau * mx)) {
+    status = 2;  // numerically rank deficient
+  } else if (mn <= std::sqrt(tau) * mx) {
+    // suspicious regime: POTRF may round the zero pivot of exactly duplicated columns up
+    // to O(sqrt(eps)); scan B-tilde for exact duplicate columns (kkt.jl:168-175, 406-440)
+    std::vector<double> bt((size_t)m * n);
+    SDPX_HIP(hipMemcpyAsync(bt.data(), h->Bt, sizeof(double) * bt.size(), hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    for (int a = 0; a < n && status == 0; ++a)
+      for (int b = a + 1; b < n && status == 0; ++b) {
+        bool same = true;
+        for (int r = 0; r < m && same; ++r) {
+          const double x = bt[r + (size_t)a * m], y = bt[r + (size_t)b * m];
+          same = (x == y) || (x == 0.0 && y == 0.0);
+        }
+        if (same) status = 2;
+      }
+  }
+  h->eq_factored = status == 0;
+  if (out) {
+    out[0] = status;
+    out[1] = mn;
+    out[2] = mx;
+    out[3] = tau;
+  }
+  return status == 0 ? kOk : kNotPositiveDefinite;
+}
+
+int sdpx_dense_solve_equality(sdpx_dense_t* h, const double* p, double* dy) {
+  SDPX_CHECK(check_handle(h));
+  const int m = h->m, n = h->neq;
+  if (!h->eq_factored || !p) {
+    set_last_error("sdpx_dense_solve_equality: a successful sdpx_dense_equality_factor and p are required");
+    return kInvalidArgument;
+  }
+  // _solve_dense_kkt_owned! (kkt.jl:2570-2595): r~ = L^-1 r; q = D p - B-hat' r~;
+  // dy^ = Q^-1 q; dx = L^-T (r~ + B-hat dy^); dy = D dy^
+  SDPX_HIP(hipMemcpyAsync(h->rtil, h->r, sizeof(double) * m, hipMemcpyDeviceToDevice, h->stream));
+  SDPX_CHECK(trsv_f64_steps(h->Sf, m, m, h->dinvS, h->rtil, h->trsv_y, 0, h->stream));
+  SDPX_CHECK(gemm_f64(gemm(n, 1, m, 1.0, h->Bt, m, h->rtil, m, 0.0, h->qrhs, n, kGemmTransA), h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->peq, p, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(scaled_rhs_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, h->eqscale, h->peq, h->qrhs,
+                     n, 0);
+  SDPX_HIP(hipGetLastError());
+  SDPX_CHECK(trsv_f64_steps(h->Qf, n, n, h->dinvQ, h->qrhs, h->qy, 0, h->stream));
+  SDPX_CHECK(trsv_f64_steps(h->Qf, n, n, h->dinvQ, h->qrhs, h->qy, 1, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->dx, h->rtil, sizeof(double) * m, hipMemcpyDeviceToDevice, h->stream));
+  SDPX_CHECK(gemm_f64(gemm(m, 1, n, 1.0, h->Bt, m, h->qrhs, n, 1.0, h->dx, m, 0), h->stream));
+  SDPX_CHECK(trsv_f64_steps(h->Sf, m, m, h->dinvS, h->dx, h->trsv_y, 1, h->stream));
+  hipLaunchKernelGGL(scaled_rhs_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, h->eqscale, nullptr,
+                     h->qrhs, n, 1);
+  SDPX_HIP(hipGetLastError());
+  if (dy) SDPX_HIP(hipMemcpyAsync(dy, h->qrhs, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_get_equality(sdpx_dense_t* h, double* Bt, double* Q, double* scale) {
+  SDPX_CHECK(check_handle(h));
+  const long long m = h->m, n = h->neq;
+  if (Bt) SDPX_HIP(hipMemcpyAsync(Bt, h->Bt, sizeof(double) * m * n, hipMemcpyDeviceToHost, h->stream));
+  if (Q) SDPX_HIP(hipMemcpyAsync(Q, h->Q, sizeof(double) * n * n, hipMemcpyDeviceToHost, h->stream));
+  if (scale) SDPX_HIP(hipMemcpyAsync(scale, h->eqscale, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
 }
 
 }  // extern "C"

@@ -60,6 +60,10 @@ SIGNATURES = {
     "sdpx_gram_work_list": [C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_dense_set_precision": [_h, C.c_int],
     "sdpx_dense_equilibrate": [_h, C.c_int, _dp, _dp, _dp, _ip],
+    "sdpx_dense_set_equalities": [_h, C.c_int, _dp],
+    "sdpx_dense_equality_factor": [_h, C.c_double, _dp],
+    "sdpx_dense_solve_equality": [_h, _dp, _dp],
+    "sdpx_dense_get_equality": [_h, _dp, _dp, _dp],
     "sdpx_dense_get_data": [_h, _dp, _dp],
     "sdpx_dense_cold_start": [_h, C.c_double, C.c_double, C.c_double, _dp, _dp],
     "sdpx_dense_exact_residual": [_h, _dp],

@@ -151,6 +151,36 @@ class MI355XDenseBackend:
         rec["mu"] = mu
         return rec
 
+    # ---- equality-constrained KKT (kkt.jl:1466-1665, 2570-2595) ------------
+    def set_equalities(self, B) -> None:
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim != 2 or B.shape[0] != self.m:
+            raise ValueError(f"B must be m x n_eq with m = {self.m}")
+        self.neq = int(B.shape[1])
+        check(self.lib.sdpx_dense_set_equalities(self.h, self.neq, dptr(B.ravel(order="F"))), "set_equalities")
+
+    def equality_factor(self, requested: float):
+        """-> (status 0 ok / 1 factor failed / 2 rank deficient, min diag, max diag, tau)."""
+        out = np.zeros(4)
+        rc = self.lib.sdpx_dense_equality_factor(self.h, requested, dptr(out))
+        if rc < 0:
+            check(rc, "equality_factor")
+        return int(out[0]), float(out[1]), float(out[2]), float(out[3])
+
+    def solve_equality(self, p):
+        """dx (left on the device; get_direction) and dy for S dx - B dy = r, B' dx = p."""
+        dy = np.empty(self.neq)
+        check(self.lib.sdpx_dense_solve_equality(self.h, dptr(np.ascontiguousarray(p, dtype=np.float64)), dptr(dy)),
+              "solve_equality")
+        return dy
+
+    def get_equality(self):
+        Bt = np.empty(self.m * self.neq)
+        Q = np.empty(self.neq * self.neq)
+        sc = np.empty(self.neq)
+        check(self.lib.sdpx_dense_get_equality(self.h, dptr(Bt), dptr(Q), dptr(sc)), "get_equality")
+        return Bt.reshape(self.m, self.neq, order="F"), Q.reshape(self.neq, self.neq, order="F"), sc
+
     def memory_bytes(self) -> int:
         b = C.c_longlong()
         check(self.lib.sdpx_dense_memory(self.h, C.byref(b)), "memory")

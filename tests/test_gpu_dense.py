@@ -468,3 +468,55 @@ def test_equilibrated_cold_solve_matches_oracle(gpu_lib):
     cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
                                          0.0, 0.0, res.gap_rel), 1e-7, 1e-7, 1e-7)
     assert cert["valid"], cert
+
+
+@pytest.mark.parametrize("m,n", [(200, 7), (1030, 130)])
+def test_equality_kkt_matches_oracle(gpu_lib, m, n):
+    """n_eq > 0 (kkt.jl:1466-1560, 2570-2595) on the device: B~ D, Q, the rank gate and the bordered
+    solve against the oracle (which is pinned by a direct solve of [S -B; B' 0])."""
+    rng = np.random.default_rng(m + n)
+    M = rng.standard_normal((m, m))
+    S = M @ M.T / m + np.eye(m)
+    B = rng.standard_normal((m, n)) * 10.0 ** rng.integers(-3, 4, n)[None, :]
+    r, p = rng.standard_normal(m), rng.standard_normal(n)
+    _, prob, _ = _dense_case(3, m, seed=2)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_schur(S)
+        assert b.factorize() == (True, 0)
+        b.set_equalities(B)
+        st, mn, mx, tau = b.equality_factor(1e-8)
+        Bh, Q, D = b.get_equality()
+        b.set_rhs(r)
+        dy = b.solve_equality(p)
+        dx, _, _ = b.get_direction()
+        # a duplicated equality column is refused by the rank gate
+        B2 = B.copy()
+        B2[:, n - 1] = B2[:, 0]
+        b2 = MI355XDenseBackend(prob)
+        try:
+            b2.set_schur(S)
+            b2.factorize()
+            b2.set_equalities(B2)
+            st2 = b2.equality_factor(1e-8)[0]
+        finally:
+            b2.close()
+    finally:
+        b.close()
+    Lf, _, _ = O.factor_dense_kkt(S)
+    sto, Bho, Do, LQo, tauo = O.factor_equality(Lf, B, 1e-8)
+    dxo, dyo = O.solve_equality(Lf, Bho, Do, LQo, r, p)
+    print(f"m={m} n={n}: status {st} (oracle {sto}), tau {tau:.3e}; |Bh| rel "
+          f"{np.linalg.norm(Bh - Bho) / np.linalg.norm(Bho):.2e}; dx rel {np.linalg.norm(dx - dxo) / np.linalg.norm(dxo):.2e}"
+          f" dy rel {np.linalg.norm(dy - dyo) / np.linalg.norm(dyo):.2e}; duplicate -> {st2}", flush=True)
+    assert st == sto == 0 and tau == tauo
+    assert np.array_equal(D, Do) or np.allclose(D, Do, rtol=1e-13)
+    assert np.linalg.norm(Bh - Bho) <= 1e-12 * np.linalg.norm(Bho)
+    assert np.linalg.norm(np.tril(Q) - np.tril(Bho.T @ Bho)) <= 1e-12 * np.linalg.norm(Bho.T @ Bho)
+    assert np.linalg.norm(dx - dxo) <= 1e-10 * np.linalg.norm(dxo)
+    assert np.linalg.norm(dy - dyo) <= 1e-10 * np.linalg.norm(dyo)
+    assert np.max(np.abs(S @ dx - B @ dy - r)) <= 1e-9 * np.max(np.abs(r))
+    assert np.max(np.abs(B.T @ dx - p)) <= 1e-9 * max(1.0, np.max(np.abs(p)))
+    # exactly singular Q: refused on both sides.  Whether POTRF fails (1) or rounds the zero pivot up
+    # and the rank gate refuses it (2) depends on rounding (kkt.jl:168-171 anticipates exactly this)
+    assert st2 in (1, 2) and O.factor_equality(Lf, B2, 1e-8)[0] in (1, 2)

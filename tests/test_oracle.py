@@ -313,3 +313,53 @@ def test_equilibrate_balances_and_preserves_the_optimum():
     cert = O.certificate(prob, r, 1e-7, 1e-7, 1e-7)
     assert cert["valid"], cert
     assert O.solve(prob, o).status != "Optimal"
+
+
+# ---------------------------------------------------------------- equality-constrained KKT (n_eq > 0)
+def _eq_case(m=60, n=7, seed=3):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((m, m))
+    S = M @ M.T / m + np.eye(m)
+    B = rng.standard_normal((m, n)) * 10.0 ** rng.integers(-3, 4, n)[None, :]
+    return S, B, rng.standard_normal(m), rng.standard_normal(n)
+
+
+def test_equality_kkt_solves_the_bordered_system():
+    """dx, dy against a direct solve of [S -B; B' 0] [dx; dy] = [r; p] (LAPACK, independent of the
+    Schur-complement route), and the residual identities."""
+    S, B, r, p = _eq_case()
+    Lf, ok, _ = O.factor_dense_kkt(S)
+    st, Bh, D, LQ, tau = O.factor_equality(Lf, B, 1e-8)
+    assert st == 0
+    dx, dy = O.solve_equality(Lf, Bh, D, LQ, r, p)
+    m, n = B.shape
+    K = np.block([[S, -B], [B.T, np.zeros((n, n))]])
+    ref = np.linalg.solve(K, np.concatenate([r, p]))
+    assert np.allclose(dx, ref[:m], rtol=1e-9, atol=1e-11) and np.allclose(dy, ref[m:], rtol=1e-9, atol=1e-11)
+    assert np.max(np.abs(S @ dx - B @ dy - r)) <= 1e-10 * np.max(np.abs(r))
+    assert np.max(np.abs(B.T @ dx - p)) <= 1e-10 * max(1.0, np.max(np.abs(p)))
+    # the normalized columns have infinity-norm one; a rescaled equality leaves dx unchanged
+    assert np.allclose(np.max(np.abs(Bh), axis=0), 1.0, rtol=1e-15)
+    B2, p2 = B.copy(), p.copy()
+    B2[:, 2] *= 1e3
+    p2[2] *= 1e3
+    st2, Bh2, D2, LQ2, _ = O.factor_equality(Lf, B2, 1e-8)
+    dx2, dy2 = O.solve_equality(Lf, Bh2, D2, LQ2, r, p2)
+    assert np.allclose(dx2, dx, rtol=1e-10, atol=1e-12) and math.isclose(dy2[2] * 1e3, dy[2], rel_tol=1e-9)
+
+
+def test_equality_rank_gate():
+    """A duplicated or proportional equality is numerically rank deficient (status 2); tau follows
+    _equality_cholesky_rank_tau (kkt.jl:103-112)."""
+    S, B, r, p = _eq_case(n=5)
+    Lf, _, _ = O.factor_dense_kkt(S)
+    assert math.isclose(O.equality_rank_tau(60, 5, 1e-8), 1e-8)
+    assert O.equality_rank_tau(60, 5, 1e-20) == 60 * O.EPS
+    assert O.equality_rank_tau(60, 5, 0.0) == O.SQRT_EPS
+    dup = B.copy()
+    dup[:, 4] = dup[:, 1]
+    assert O.factor_equality(Lf, dup, 1e-8)[0] in (1, 2)
+    prop = B.copy()
+    prop[:, 3] = -2.5 * prop[:, 0]
+    assert O.factor_equality(Lf, prop, 1e-8)[0] in (1, 2)
+    assert O.factor_equality(Lf, B, 1e-8)[0] == 0
