@@ -57,6 +57,17 @@ int sdpx_dense_memory(sdpx_dense_t* h, long long* bytes);
 /* Av: k_l^2 x m column-major (bytes of DenseCons.Av[l]); C: k_l x k_l. */
 int sdpx_dense_set_block(sdpx_dense_t* h, int l, const double* Av, const double* C);
 int sdpx_dense_set_objective(sdpx_dense_t* h, const double* c);
+/* Block l given as the reference's SparseBlockCOO (types/constraints.jl:59-109), as
+ * stored (1-based): positions p = 1..na touch constraints ids[p] (here 0-based, strictly
+ * increasing), entries ptr[p]..ptr[p+1]-1 with row, col, lin = (col-1)k + row and val,
+ * every stored entry of the symmetric A (both triangles).  Replaces the block's dense
+ * panel: the Schur build assembles its pairs as sparse_schur_block! does
+ * (schur.jl:936-1032: W1 = Y A_p X^-1, S_pr = <W1, A_r>) and scatters them
+ * (reduce_sparse_schur!, schur.jl:1229-1318); the Av passes read a densified copy.
+ * Set once per block; with a sparse block, equilibration and the reduced-precision
+ * Gram routes are rejected (the block assembles in FP64 as given). */
+int sdpx_dense_set_block_sparse(sdpx_dense_t* h, int l, int na, const int* ids, const int* ptr, const int* lin,
+                                const int* row, const int* col, const double* val, const double* C);
 
 /* ---- iterate I/O (interior_point.jl:1528-1650 starts; Checkpoint problems.jl:27-38) */
 int sdpx_dense_set_iterate(sdpx_dense_t* h, const double* x, const double* X, const double* Y);

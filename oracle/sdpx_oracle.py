@@ -19,6 +19,10 @@ SDPX.jl repository root).  Parity pins (see ``tests/test_oracle.py``):
 * ``schur_build`` -- brute-force tr(A_i X^-1 A_j Y) in 50-digit mpmath and the
   X=Y=I Gram closed form.
 * ``solve_dense_kkt``/``refine_direction`` -- S dx = r identities.
+* ``sparse_schur_block``/``scatter_sparse_schur`` -- the dense Gram route on the same
+  blocks (the identity tr(A_p X^-1 A_r Y)), the reference's own "general sparse
+  outer-product Schur equals dense path" case (test/sparse.jl:191-258) and its analytic
+  sparse solve (test/sparse.jl:330-352, p* = 2 sqrt 6).
 * ``fraction_to_boundary_bound`` -- eigenvalue bound (scipy eigh) and the 2x2
   determinant closed form.
 * ``solve`` -- reference known answers T1 (p* = 2 sqrt 6, 19 +- 2 iterations,
@@ -73,6 +77,9 @@ class Problem:
     C: list
     Av: list
     k: list
+    # blocks assembled by the sparse Schur route: l -> (ids, SparseBlockCOO); Av[l] stays the
+    # dense panel the residual and direction passes read
+    sparse: dict = field(default_factory=dict)
 
     @property
     def m(self) -> int:
@@ -697,16 +704,22 @@ def contract(prob: Problem, l: int, M) -> np.ndarray:
     return _ordered_matvec(prob.Av[l].T, np.asarray(M).ravel(order="F"), _ORDER["contract"])
 
 
-def schur_build(prob: Problem, LX: list, MY: list) -> np.ndarray:
+def schur_build(prob: Problem, LX: list, MY: list, Y: list | None = None) -> np.ndarray:
     """Dense Schur S = sum_l Phat_l' Phat_l, Phat_i = L_X^-1 A_i M_Y (schur.jl:511-547).
 
-    Returns the full symmetric matrix; the reference keeps only the lower
-    triangle authoritative (workspace.jl:1166-1172).
+    Blocks in ``prob.sparse`` take the sparse pair route (sparse_schur_block, reading Y;
+    M_Y M_Y' when Y is not given).  Returns the full symmetric matrix; the reference keeps
+    only the lower triangle authoritative (workspace.jl:1166-1172).
     """
     m = prob.m
     S = np.zeros((m, m))
     for l in range(prob.L):
         k = prob.k[l]
+        if l in prob.sparse:
+            ids, coo = prob.sparse[l]
+            Yl = Y[l] if Y is not None else MY[l] @ MY[l].T
+            scatter_sparse_schur(S, ids, sparse_schur_block(ids, coo, LX[l], Yl))
+            continue
         MYl = np.tril(MY[l])
         Pm = np.empty((m, k * k))  # row i = vec(Phat_i)
         # constraints in chunks of ~256 MB: every column is computed as in one call, with less memory
@@ -720,6 +733,105 @@ def schur_build(prob: Problem, LX: list, MY: list) -> np.ndarray:
             Pm[i0:i0 + n] = Ph.transpose(0, 2, 1).reshape(n, k * k)
         S += Pm @ Pm.T
     return S
+
+
+# --------------------------------------------------------------------------
+# Sparse blocks (types/constraints.jl:59-109, schur.jl:566-640, 936-1032, 1229-1318)
+# --------------------------------------------------------------------------
+@dataclass
+class SparseBlockCOO:
+    """SparseBlockCOO (types/constraints.jl:59-65): 1-based Int32 ptr / lin / row / col and val."""
+
+    ptr: np.ndarray
+    lin: np.ndarray
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+
+def block_active(Al: np.ndarray) -> np.ndarray:
+    """Variables whose coefficient on the block is structurally nonzero, ascending: the
+    reference's ``active`` / ``schur_order`` of a sparse block (test/sparse.jl:206-216)."""
+    m = Al.shape[0]
+    return np.flatnonzero(np.abs(np.asarray(Al).reshape(m, -1)).max(axis=1) > 0.0) if m else np.zeros(0, int)
+
+
+def build_block_coo(Al: np.ndarray, order) -> SparseBlockCOO:
+    """build_block_coo (types/constraints.jl:79-109): positions in ``order``, each
+    coefficient's stored entries in column-major (CSC) order, both triangles, 1-based."""
+    k = Al.shape[1]
+    ptr, lin, row, col, val = [1], [], [], [], []
+    for i in order:
+        A = Al[i]
+        for cc in range(k):
+            for rr in range(k):
+                if A[rr, cc] != 0.0:
+                    row.append(rr + 1)
+                    col.append(cc + 1)
+                    lin.append(cc * k + rr + 1)
+                    val.append(A[rr, cc])
+        ptr.append(len(val) + 1)
+    i32 = lambda v: np.asarray(v, dtype=np.int32)  # noqa: E731
+    return SparseBlockCOO(i32(ptr), i32(lin), i32(row), i32(col), np.asarray(val, dtype=np.float64))
+
+
+def ingest_sparse(c, A, C, blocks) -> Problem:
+    """``ingest(c, A, C)`` with the listed blocks on the sparse route (ingest.jl sparse=true
+    for those blocks): ids = block_active, coo = build_block_coo over ids."""
+    prob = ingest(c, A, C)
+    for l in blocks:
+        Al = np.asarray(A[l], dtype=np.float64)
+        ids = block_active(Al)
+        prob.sparse[l] = (ids, build_block_coo(Al, ids))
+    return prob
+
+
+def two_sided_coo_product(left, coo: SparseBlockCOO, p: int, right) -> np.ndarray:
+    """_two_sided_coo_product! (schur.jl:601-640): left * A_p * right as a sum of rank-one terms
+    val_t left[:, row_t] right[col_t, :], first entry writing, later ones accumulating."""
+    k = left.shape[0]
+    t0, t1 = int(coo.ptr[p]) - 1, int(coo.ptr[p + 1]) - 1
+    dest = np.zeros((k, k))
+    for t in range(t0, t1):
+        r, c = int(coo.row[t]) - 1, int(coo.col[t]) - 1
+        term = np.outer(left[:, r], coo.val[t] * right[c, :])
+        dest = term if t == t0 else dest + term
+    return dest
+
+
+def dot_dense_coo(M, coo: SparseBlockCOO, p: int) -> float:
+    """_dot_dense_coo (schur.jl:566-579): sum of M[lin_t] val_t over p's entries, stored order."""
+    flat = np.asarray(M).ravel(order="F")
+    acc = 0.0
+    for t in range(int(coo.ptr[p]) - 1, int(coo.ptr[p + 1]) - 1):
+        acc += flat[int(coo.lin[t]) - 1] * coo.val[t]
+    return acc
+
+
+def sparse_schur_block(ids, coo: SparseBlockCOO, LX, Y) -> np.ndarray:
+    """sparse_schur_block! (schur.jl:936-1032), general COO branch: W2 = X^-1 from L_X,
+    W1_p = Y A_p W2, Svals packed over p <= r as <W1_p, A_r>."""
+    k = LX.shape[0]
+    W2 = kcholsolve(LX, np.eye(k))
+    na = len(ids)
+    out = []
+    for p in range(na):
+        W1 = two_sided_coo_product(Y, coo, p, W2)
+        for r in range(p, na):
+            out.append(dot_dense_coo(W1, coo, r))
+    return np.asarray(out)
+
+
+def scatter_sparse_schur(S, ids, Svals) -> None:
+    """reduce_sparse_schur! scatter (schur.jl:1229-1318), lower-only form: S[ids[r], ids[p]] +=
+    Svals over p <= r, mirrored here so S stays the full symmetric matrix."""
+    na, q = len(ids), 0
+    for p in range(na):
+        for r in range(p, na):
+            S[ids[r], ids[p]] += Svals[q]
+            if r != p:
+                S[ids[p], ids[r]] += Svals[q]
+            q += 1
 
 
 def schur_mul(S: np.ndarray, x) -> np.ndarray:
@@ -1014,7 +1126,7 @@ def newton_step(prob: Problem, o_iter: Options, o: Options, x, X, Y, mu, ctrl: C
     p_res, d_res, ok = compute_residuals(prob, st, x, X, Y, mu, o_iter, factor=True)
     if not ok:
         return dict(status="breakdown", reason="Cholesky factorization of X or Y failed", p_res=p_res, d_res=d_res)
-    st.S = schur_build(prob, st.LX, st.MY)
+    st.S = schur_build(prob, st.LX, st.MY, Y)
     if trace is not None:
         trace.setdefault("S", []).append(st.S.copy())
         trace.setdefault("X", []).append([b.copy() for b in X])

@@ -38,6 +38,10 @@ struct Block {
   double *dinvX = nullptr, *dinvY = nullptr;
   double *P = nullptr, *R = nullptr, *dX = nullptr, *dY = nullptr, *W1 = nullptr, *W2 = nullptr;
   double *trial = nullptr, *trial_dinv = nullptr;
+  // sparse block (sdpx_dense_set_block_sparse): SparseBlockCOO over na positions, 0-based
+  int na = 0;
+  int *ids = nullptr, *sptr = nullptr, *srow = nullptr, *scol = nullptr, *slin = nullptr;
+  double *sval = nullptr, *W1T = nullptr;
 };
 
 // closed forms for blocks of side <= 2 (kernels/generic.jl:133-154, 292-340, 350-420)
@@ -261,6 +265,67 @@ __global__ void __launch_bounds__(256) ruiz_colmax_kernel(const double* Av, long
     __syncthreads();
   }
   if (threadIdx.x == 0) maxima[i] = first ? red[0] : fmax(maxima[i], red[0]);
+}
+
+// ---- sparse blocks (schur.jl:566-640, 936-1032, 1229-1318) --------------------------
+// A block given as the reference's SparseBlockCOO (0-based here): positions 0..na-1 map
+// to constraints ids[p]; entries ptr[p]..ptr[p+1]-1 with row, col, lin = row + col k, val,
+// every stored entry of the symmetric A_p (both triangles).
+//
+// densify: Av[:, ids[p]] = sum of the position's entries (one thread per position, in order)
+__global__ void sparse_densify_kernel(int na, const int* ids, const int* ptr, const int* lin, const double* val,
+                                      long long k2, double* Av) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= na) return;
+  double* a = Av + (long long)ids[p] * k2;
+  for (int t = ptr[p]; t < ptr[p + 1]; ++t) a[lin[t]] += val[t];
+}
+
+// W1_p = Y A_p W2 (_two_sided_coo_product!, schur.jl:601-640): entry (i, j) of W1_p is
+// sum_t Y[i, row_t] * (val_t * W2[col_t, j]), the first entry writing and the rest
+// accumulating in stored order.  Stored lin-major, W1T[lin * na + p] (lin = i + j k), so the
+// pair kernel reads consecutive positions.  Grid: x over lin, y over groups of 256 positions.
+__global__ void sparse_w1_kernel(int k, int na, const int* ptr, const int* row, const int* col, const double* val,
+                                 const double* Y, const double* W2, double* W1T) {
+  const int p = blockIdx.y * 256 + threadIdx.x;
+  if (p >= na) return;
+  const int li = blockIdx.x;
+  const int i = li % k, j = li / k;
+  double acc = 0.0;
+  const int t0 = ptr[p], t1 = ptr[p + 1];
+  for (int t = t0; t < t1; ++t) {
+    const double scaled = val[t] * W2[col[t] + (long long)j * k];
+    const double term = Y[i + (long long)row[t] * k] * scaled;
+    acc = t == t0 ? term : acc + term;
+  }
+  W1T[(long long)li * na + p] = acc;
+}
+
+// S_pr = <W1_p, A_r> for p <= r (_dot_dense_coo, schur.jl:566-579: the sum over r's entries
+// of W1_p[lin_t] * val_t in stored order), accumulated into the m x m buffer Acc at
+// (ids[p], ids[r]) (upper triangle: ids are sorted).  One work-group per r, threads over p.
+__global__ void __launch_bounds__(256) sparse_pair_kernel(int na, const int* ids, const int* ptr, const int* lin,
+                                                          const double* val, const double* W1T, double* Acc,
+                                                          long long m) {
+  const int r = blockIdx.x;
+  const int t0 = ptr[r], t1 = ptr[r + 1];
+  const long long colr = (long long)ids[r] * m;
+  for (int p = threadIdx.x; p <= r; p += 256) {
+    double acc = 0.0;
+    for (int t = t0; t < t1; ++t) acc += W1T[(long long)lin[t] * na + p] * val[t];
+    Acc[ids[p] + colr] += acc;
+  }
+}
+
+// S(i, j) = (dense ? S(i, j) : 0) + Acc(j, i) for i >= j: the sparse blocks' sums, held in the
+// upper triangle of Acc, added to the lower triangle of S
+__global__ void add_upper_to_lower_kernel(double* S, const double* Acc, int n, int has_dense) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  if (i < j) return;
+  const double v = Acc[j + (long long)i * n];
+  S[idx] = has_dense ? S[idx] + v : v;
 }
 
 // ---- equality-constrained KKT (kkt.jl:1466-1665, 2570-2595) ----------------------
@@ -966,6 +1031,10 @@ int sdpx_dense_set_block(sdpx_dense_t* h, int l, const double* Av, const double*
     return kInvalidArgument;
   }
   Block& b = h->blk[l];
+  if (b.na != 0) {
+    set_last_error("sdpx_dense_set_block: the block was given as sparse coordinates");
+    return kInvalidArgument;
+  }
   SDPX_HIP(hipMemcpyAsync(b.Av, Av, sizeof(double) * b.k2 * h->m, hipMemcpyHostToDevice, h->stream));
   SDPX_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
   SDPX_HIP(hipStreamSynchronize(h->stream));
@@ -1125,9 +1194,16 @@ int sdpx_dense_residuals(sdpx_dense_t* h, const double* mu, int affine, int fact
 int sdpx_dense_schur_build(sdpx_dense_t* h) {
   SDPX_CHECK(check_handle(h));
   const int m = h->m;
+  bool first_dense = true, any_sparse = false;
   for (int l = 0; l < h->L; ++l) {
     Block& b = h->blk[l];
     const int k = b.k;
+    if (b.na > 0) {
+      any_sparse = true;
+      continue;
+    }
+    const double beta0 = first_dense ? 0.0 : 1.0;
+    first_dense = false;
     if (h->prof) SDPX_HIP(hipEventRecord(h->ev[0], h->stream));
     // T_i = A_i M_Y for every i as one product over the stacked panel
     // [A_1; ...; A_m] = Av' (each A_i is symmetric), M = km rows, op(B) lower;
@@ -1142,7 +1218,7 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
                              kGemmALower | kGemmRaggedFirst | h->panel_tiles[1]),
                         h->stream));
     // S (+)= Phat' Phat, lower triangle only (schur.jl:469-475 syrk 'L','T')
-    GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, l == 0 ? 0.0 : 1.0, h->S, m,
+    GemmArgs s = gemm(m, m, (int)b.k2, 1.0, h->Ph, b.k2, h->Ph, b.k2, beta0, h->S, m,
                       kGemmTransA | kGemmLowerOut | kGemmSchurGram);
     s.tile_order = h->tile_order;
     s.item_order = h->gram_items;
@@ -1152,13 +1228,13 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
       const long long kp = (b.k2 + 31) / 32 * 32;
       SDPX_CHECK(split_panel_bf16(h->Ph, b.k2, h->Ph_hi, h->Ph_lo, h->ldPb, b.k2, m, h->stream));
       if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
-      SDPX_CHECK(gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, l == 0 ? 0.0 : 1.0, h->split,
+      SDPX_CHECK(gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, beta0, h->split,
                              h->splitws, h->tile_order, h->stream, h->prof ? h->ev[2] : nullptr));
     } else if (h->precision == 1) {
       // mixed precision: Phat rounded to FP32, Gram product on FP32 MFMA, S kept in FP64
       SDPX_CHECK(convert_panel_f32(h->Ph, b.k2, h->Pf, h->ldPf, b.k2, m, h->stream));
       if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
-      SDPX_CHECK(gram_f32(h->Pf, h->ldPf, m, (int)b.k2, h->S, m, l == 0 ? 0.0 : 1.0, h->split, h->splitws,
+      SDPX_CHECK(gram_f32(h->Pf, h->ldPf, m, (int)b.k2, h->S, m, beta0, h->split, h->splitws,
                           h->tile_order, h->stream, h->prof ? h->ev[2] : nullptr));
     } else {
       if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
@@ -1176,6 +1252,25 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
       h->t_reduce += r;
       h->n_gram += 1;
     }
+  }
+  if (any_sparse) {
+    // sparse blocks (sparse_schur_block!, schur.jl:936-1032, then the scatter of
+    // reduce_sparse_schur!, :1229-1318): pair sums accumulated in the upper triangle of the
+    // factor buffer (free until the factor copies S), then added to the lower triangle of S
+    SDPX_HIP(hipMemsetAsync(h->Sf, 0, sizeof(double) * (size_t)m * m, h->stream));
+    for (int l = 0; l < h->L; ++l) {
+      Block& b = h->blk[l];
+      if (b.na == 0) continue;
+      hipLaunchKernelGGL(sparse_w1_kernel, dim3((unsigned)b.k2, (unsigned)ceil_div(b.na, 256)), dim3(256), 0,
+                         h->stream, b.k, b.na, b.sptr, b.srow, b.scol, b.sval, b.Y, b.Xinv, b.W1T);
+      SDPX_HIP(hipGetLastError());
+      hipLaunchKernelGGL(sparse_pair_kernel, dim3((unsigned)b.na), dim3(256), 0, h->stream, b.na, b.ids, b.sptr,
+                         b.slin, b.sval, b.W1T, h->Sf, (long long)m);
+      SDPX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(add_upper_to_lower_kernel, dim3((unsigned)ceil_div((long long)m * m, 256)), dim3(256), 0,
+                       h->stream, h->S, h->Sf, m, first_dense ? 0 : 1);
+    SDPX_HIP(hipGetLastError());
   }
   return kOk;
 }
@@ -1566,6 +1661,12 @@ int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
     set_last_error("sdpx_dense_set_precision: precision must be 0 (FP64), 1 (FP32 Schur Gram) or 2 (BF16x3 Schur Gram)");
     return kInvalidArgument;
   }
+  for (int l = 0; l < h->L && precision != 0; ++l) {
+    if (h->blk[l].na > 0) {
+      set_last_error("sdpx_dense_set_precision: sparse blocks assemble in FP64 only (precision 0)");
+      return kInvalidArgument;
+    }
+  }
   if (precision == 1 && !h->Pf) {
     h->ldPf = (h->maxk2 + 3) / 4 * 4;
     int s = dalloc(h, &h->Pf, (size_t)(h->ldPf * h->m));
@@ -1796,6 +1897,12 @@ int sdpx_dense_equilibrate(sdpx_dense_t* h, int ruiz_iters, double* E, double* s
                            int* passes) {
   SDPX_CHECK(check_handle(h));
   const int L = h->L, m = h->m;
+  for (int l = 0; l < L; ++l) {
+    if (h->blk[l].na > 0) {
+      set_last_error("sdpx_dense_equilibrate: sparse blocks are taken as given (equilibrate before packing)");
+      return kInvalidArgument;
+    }
+  }
   // _ruiz_control (ingest.jl:1141-1160): :auto = 2..8 passes, log tolerance 1/20
   const bool adaptive = ruiz_iters < 0;
   const int min_it = adaptive ? 2 : ruiz_iters, max_it = adaptive ? 8 : ruiz_iters;
@@ -2192,6 +2299,77 @@ int sdpx_dense_get_equality(sdpx_dense_t* h, double* Bt, double* Q, double* scal
   if (Q) SDPX_HIP(hipMemcpyAsync(Q, h->Q, sizeof(double) * n * n, hipMemcpyDeviceToHost, h->stream));
   if (scale) SDPX_HIP(hipMemcpyAsync(scale, h->eqscale, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_dense_set_block_sparse(sdpx_dense_t* h, int l, int na, const int* ids, const int* ptr, const int* lin,
+                                const int* row, const int* col, const double* val, const double* C) {
+  SDPX_CHECK(check_handle(h));
+  if (l < 0 || l >= h->L || na < 1 || na > h->m || !ids || !ptr || !C) {
+    set_last_error("sdpx_dense_set_block_sparse: block index, 1 <= na <= m, ids, ptr and C required");
+    return kInvalidArgument;
+  }
+  Block& b = h->blk[l];
+  if (b.na != 0) {
+    set_last_error("sdpx_dense_set_block_sparse: the block is already sparse (set once)");
+    return kInvalidArgument;
+  }
+  if (h->precision != 0) {
+    set_last_error("sdpx_dense_set_block_sparse: sparse blocks assemble in FP64 only");
+    return kInvalidArgument;
+  }
+  const int k = b.k;
+  // SparseBlockCOO as stored (types/constraints.jl:59-109): 1-based ptr / row / col / lin
+  if (ptr[0] != 1) {
+    set_last_error("sdpx_dense_set_block_sparse: ptr[1] must be 1 (1-based positions)");
+    return kInvalidArgument;
+  }
+  for (int p = 0; p < na; ++p) {
+    if (ptr[p + 1] < ptr[p] || ids[p] < 0 || ids[p] >= h->m || (p > 0 && ids[p] <= ids[p - 1])) {
+      set_last_error("sdpx_dense_set_block_sparse: ptr non-decreasing, ids strictly increasing in [0, m)");
+      return kInvalidArgument;
+    }
+  }
+  const int nnz = ptr[na] - 1;
+  if (nnz > 0 && (!lin || !row || !col || !val)) {
+    set_last_error("sdpx_dense_set_block_sparse: null lin, row, col or val with entries present");
+    return kInvalidArgument;
+  }
+  std::vector<int> p0(na + 1), r0(nnz), c0(nnz), l0(nnz);
+  for (int p = 0; p <= na; ++p) p0[p] = ptr[p] - 1;
+  for (int t = 0; t < nnz; ++t) {
+    if (row[t] < 1 || row[t] > k || col[t] < 1 || col[t] > k || lin[t] != (col[t] - 1) * k + row[t]) {
+      set_last_error("sdpx_dense_set_block_sparse: entry " + std::to_string(t + 1) +
+                     " outside the block or lin != (col-1)k+row");
+      return kInvalidArgument;
+    }
+    r0[t] = row[t] - 1;
+    c0[t] = col[t] - 1;
+    l0[t] = lin[t] - 1;
+  }
+  SDPX_CHECK(dalloc(h, &b.ids, (size_t)na));
+  SDPX_CHECK(dalloc(h, &b.sptr, (size_t)na + 1));
+  SDPX_CHECK(dalloc(h, &b.srow, (size_t)std::max(nnz, 1)));
+  SDPX_CHECK(dalloc(h, &b.scol, (size_t)std::max(nnz, 1)));
+  SDPX_CHECK(dalloc(h, &b.slin, (size_t)std::max(nnz, 1)));
+  SDPX_CHECK(dalloc(h, &b.sval, (size_t)std::max(nnz, 1)));
+  SDPX_CHECK(dalloc(h, &b.W1T, (size_t)b.k2 * na));
+  SDPX_HIP(hipMemcpyAsync(b.ids, ids, sizeof(int) * na, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(b.sptr, p0.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice, h->stream));
+  if (nnz > 0) {
+    SDPX_HIP(hipMemcpyAsync(b.srow, r0.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(b.scol, c0.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(b.slin, l0.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(b.sval, val, sizeof(double) * nnz, hipMemcpyHostToDevice, h->stream));
+  }
+  // the dense panel the Av passes (residuals, right-hand sides, direction recovery) read
+  SDPX_CHECK(fill_f64(b.k2 * h->m, 0.0, b.Av, h->stream));
+  hipLaunchKernelGGL(sparse_densify_kernel, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, h->stream, na, b.ids,
+                     b.sptr, b.slin, b.sval, b.k2, b.Av);
+  SDPX_HIP(hipGetLastError());
+  SDPX_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  b.na = na;
   return kOk;
 }
 

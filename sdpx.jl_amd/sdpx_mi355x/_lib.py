@@ -24,6 +24,7 @@ SIGNATURES = {
     "sdpx_dense_destroy": [_h],
     "sdpx_dense_memory": [_h, C.POINTER(C.c_longlong)],
     "sdpx_dense_set_block": [_h, C.c_int, _dp, _dp],
+    "sdpx_dense_set_block_sparse": [_h, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp],
     "sdpx_dense_set_objective": [_h, _dp],
     "sdpx_dense_set_iterate": [_h, _dp, _dp, _dp],
     "sdpx_dense_get_iterate": [_h, _dp, _dp, _dp],

@@ -10,7 +10,7 @@ matrix and factor stay resident in HBM between calls.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -30,6 +30,8 @@ class SDPProblem:
     C: list
     Av: list
     k: list
+    # blocks on the sparse Schur route: l -> (ids, SparseBlockCOO); Av[l] is None for them
+    sparse: dict = field(default_factory=dict)
 
     @property
     def m(self) -> int:
@@ -40,19 +42,62 @@ class SDPProblem:
         return len(self.k)
 
 
-def ingest(c, A, C) -> SDPProblem:
-    """Dense ingest (ingest.jl:680-694): ``A[l]`` is (m, k, k) with A[l][i] = A_i^(l)."""
+@dataclass
+class SparseBlockCOO:
+    """SparseBlockCOO (types/constraints.jl:59-65) as the reference stores it: 1-based Int32
+    ``ptr`` over positions, ``lin = (col-1) k + row``, ``row``, ``col`` and ``val``, every
+    stored entry of each symmetric coefficient (both triangles)."""
+
+    ptr: np.ndarray
+    lin: np.ndarray
+    row: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @classmethod
+    def build(cls, Al: np.ndarray, order) -> "SparseBlockCOO":
+        """build_block_coo (types/constraints.jl:79-109): ``Al`` (m, k, k), positions in
+        ``order``, each coefficient's nonzeros in column-major order."""
+        Al = np.asarray(Al, dtype=np.float64)
+        k = Al.shape[1]
+        order = np.asarray(order, dtype=np.int64)
+        sub = Al[order].transpose(0, 2, 1).reshape(len(order), k * k)  # row p = vec(A_order[p])
+        pos, lin0 = np.nonzero(sub)                                       # position-major, lin ascending
+        ptr = np.searchsorted(pos, np.arange(len(order) + 1)) + 1
+        i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)  # noqa: E731
+        return cls(i32(ptr), i32(lin0 + 1), i32(lin0 % k + 1), i32(lin0 // k + 1), sub[pos, lin0].copy())
+
+
+def block_active(Al: np.ndarray) -> np.ndarray:
+    """Variables with a structurally nonzero coefficient on the block, ascending (the
+    reference's ``active`` / ``schur_order``, test/sparse.jl:206-216)."""
+    Al = np.asarray(Al)
+    return np.flatnonzero(np.any(Al.reshape(Al.shape[0], -1) != 0.0, axis=1))
+
+
+def ingest(c, A, C, sparse=()) -> SDPProblem:
+    """Dense ingest (ingest.jl:680-694): ``A[l]`` is (m, k, k) with A[l][i] = A_i^(l).
+
+    Blocks listed in ``sparse`` are packed as SparseBlockCOO over their active variables
+    (the ingest's sparse route, types/constraints.jl:79-109) instead of a dense panel."""
     c = np.ascontiguousarray(c, dtype=np.float64)
-    Av, Cs, ks = [], [], []
-    for Al, Cl in zip(A, C):
+    Av, Cs, ks, sp = [], [], [], {}
+    for l, (Al, Cl) in enumerate(zip(A, C)):
         Al = np.asarray(Al, dtype=np.float64)
         m, k, k2 = Al.shape
         if k != k2 or m != c.shape[0]:
             raise ValueError("A blocks must be (m, k, k) with m = len(c)")
-        Av.append(np.ascontiguousarray(Al.transpose(0, 2, 1).reshape(m, k * k)))
+        if l in sparse:
+            ids = block_active(Al)
+            if ids.size == 0:
+                raise ValueError(f"ingest: sparse block {l} has no active variable")
+            sp[l] = (ids, SparseBlockCOO.build(Al, ids))
+            Av.append(None)
+        else:
+            Av.append(np.ascontiguousarray(Al.transpose(0, 2, 1).reshape(m, k * k)))
         Cs.append(np.asfortranarray(np.asarray(Cl, dtype=np.float64)))
         ks.append(int(k))
-    return SDPProblem(c=c, C=Cs, Av=Av, k=ks)
+    return SDPProblem(c=c, C=Cs, Av=Av, k=ks, sparse=sp)
 
 
 COLD_START_REASONS = {0: "none", 1: "identity_kkt_factorization", 2: "kkt_cold_start_solve",
@@ -99,11 +144,25 @@ class MI355XDenseBackend:
         check(self.lib.sdpx_dense_create(device, self.L, karr, self.m, C.byref(h)), "sdpx_dense_create")
         self.h = h
         for l in range(self.L):
+            Cl = np.ascontiguousarray(np.asarray(prob.C[l], dtype=np.float64).ravel(order="F"))
+            if l in prob.sparse:
+                self._set_block_sparse(l, *prob.sparse[l], Cl)
+                continue
             Av = np.ascontiguousarray(prob.Av[l], dtype=np.float64)
-            Cl = np.asfortranarray(prob.C[l], dtype=np.float64)
-            check(self.lib.sdpx_dense_set_block(h, l, dptr(Av), dptr(Cl.ravel(order="F"))), "set_block")
+            check(self.lib.sdpx_dense_set_block(h, l, dptr(Av), dptr(Cl)), "set_block")
         check(self.lib.sdpx_dense_set_objective(h, dptr(np.ascontiguousarray(prob.c))), "set_objective")
         self._o = np.zeros(4 + 4 * self.L)
+
+    def _set_block_sparse(self, l, ids, coo, Cl):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        arrs = [np.ascontiguousarray(a, dtype=np.int32) for a in (coo.ptr, coo.lin, coo.row, coo.col)]
+        val = np.ascontiguousarray(coo.val, dtype=np.float64)
+        if arrs[0].size != ids.size + 1 or not (arrs[1].size == arrs[2].size == arrs[3].size == val.size):
+            raise ValueError(f"sparse block {l}: ptr needs na+1 entries and lin/row/col/val one per entry")
+        ip = _lib._ip
+        check(self.lib.sdpx_dense_set_block_sparse(self.h, l, int(ids.size), ids.ctypes.data_as(ip),
+                                                   *(a.ctypes.data_as(ip) for a in arrs), dptr(val), dptr(Cl)),
+              "set_block_sparse")
 
     # ---- lifecycle -----------------------------------------------------
     def close(self):
