@@ -237,6 +237,24 @@ int sdpx_dense_get_data(sdpx_dense_t* h, double* C, double* c);
  * floor / degree. */
 int sdpx_dense_cold_start(sdpx_dense_t* h, double beta, double eps_primal, double eps_dual, double* mu, double* out);
 
+/* ---- block-arrow elimination ----------------------------------------------------
+ * Replaces factorize!(::ArrowBackend) / solve!(::ArrowBackend) (kkt_backend.jl:235-258,
+ * 330-333 -> factor_arrow_kkt!, kkt.jl:2132-2349, and solve_arrow_kkt!, kkt.jl:2977-3096)
+ * for a handle whose Schur matrix has the block-arrow structure of ArrowWorkspace
+ * (workspace.jl:318-400): owner[i] = the block whose active set alone holds variable i
+ * (a local variable), or -1 for a variable that touches several blocks (global).  After
+ * this call sdpx_dense_schur_factor eliminates the local variables block by block (each
+ * local block factored with its own sqrt(eps)*10^(a-1) ladder; W = D^-1 C through the
+ * coupling rows), factors Sred = S_GG - C'D^-1 C with the ladder of
+ * _factor_with_relative_regularization! (kkt.jl:1695-1716) and reports the sum of the
+ * attempts; every solve with the factor (direction, refinement, cold start) takes the
+ * arrow solve.  The unfactored S stays the refinement operator.  At most 128 local
+ * variables per block, at least one local variable, no equality border, precision 0;
+ * once per handle. */
+int sdpx_dense_set_arrow(sdpx_dense_t* h, const int* owner);
+/* Global variable count (-1 when the route is not set), local blocks, largest local block. */
+int sdpx_dense_get_arrow(sdpx_dense_t* h, int* ng, int* nloc, int* max_q);
+
 /* ---- equality-constrained KKT (n_eq > 0) ---------------------------------------
  * The KKT system  S dx - B dy = r,  B' dx = p  of the dense route with n
  * equality constraints (B: m x n column-major, uploaded once).

@@ -19,6 +19,10 @@ SDPX.jl repository root).  Parity pins (see ``tests/test_oracle.py``):
 * ``schur_build`` -- brute-force tr(A_i X^-1 A_j Y) in 50-digit mpmath and the
   X=Y=I Gram closed form.
 * ``solve_dense_kkt``/``refine_direction`` -- S dx = r identities.
+* ``factor_arrow_kkt``/``solve_arrow_kkt`` -- the dense solve of the same S (the elimination
+  is exact), the reference's three-block case (test/sparse.jl:8-26, 141-188) and its
+  at-scale arrow-vs-dense check (test/sparse.jl:437-495) on the same structure, the
+  per-block ladder on a planted indefinite local block.
 * ``sparse_schur_block``/``scatter_sparse_schur`` -- the dense Gram route on the same
   blocks (the identity tr(A_p X^-1 A_r Y)), the reference's own "general sparse
   outer-product Schur equals dense path" case (test/sparse.jl:191-258) and its analytic
@@ -137,6 +141,9 @@ class Options:
     refine_tol: float = 0.0
     parameter_strategy: str = "adaptive"
     adaptive_sigma_max: float = 0.0
+    # KKT route of the Schur system: "dense_cholesky" or "block_arrow" (the exact
+    # block-arrow elimination, kkt.jl:2132-2349, 2977-3096)
+    kkt_route: str = "dense_cholesky"
 
 
 @dataclass
@@ -879,10 +886,136 @@ def _factor_dense_kkt(S: np.ndarray):
     return (np.tril(c) if ok else None), ok, attempts
 
 
-def solve_dense_kkt(Lf: np.ndarray, r) -> np.ndarray:
-    """dx = L^-T (L^-1 r) (kkt.jl:2570-2595, n_eq = 0)."""
+def solve_dense_kkt(Lf, r) -> np.ndarray:
+    """dx = L^-T (L^-1 r) (kkt.jl:2570-2595, n_eq = 0); an ArrowFactor takes solve_arrow_kkt."""
+    if isinstance(Lf, ArrowFactor):
+        return solve_arrow_kkt(Lf, r)
     t = sla.solve_triangular(Lf, r, lower=True, check_finite=False)
     return sla.solve_triangular(Lf, t, lower=True, trans="T", check_finite=False)
+
+
+# --------------------------------------------------------------------------
+# Block-arrow elimination (workspace.jl:318-400, kkt.jl:1695-1716, 2101-2349, 2977-3096)
+# --------------------------------------------------------------------------
+def block_active_sets(prob: Problem) -> list:
+    """Per block, the variables with a structurally nonzero coefficient, ascending (``cons.active``)."""
+    out = []
+    for l in range(prob.L):
+        if l in prob.sparse:
+            out.append(np.asarray(prob.sparse[l][0], dtype=np.int64))
+        else:
+            out.append(np.flatnonzero(np.any(prob.Av[l] != 0.0, axis=0)))
+    return out
+
+
+def arrow_partition(prob: Problem):
+    """ArrowWorkspace (workspace.jl:318-400): a variable touching more than one block is global, one
+    touching exactly one block is local to it.  None (the general route) when a variable touches no
+    block or every variable is global.  Returns (global_ids, local_ids per block)."""
+    m = prob.m
+    freq = np.zeros(m, dtype=np.int64)
+    owner = np.zeros(m, dtype=np.int64)
+    for l, act in enumerate(block_active_sets(prob)):
+        freq[act] += 1
+        owner[act] = l
+    if not np.all(freq > 0):
+        return None
+    gids = np.flatnonzero(freq > 1)
+    if len(gids) >= m:
+        return None
+    lids = [np.flatnonzero((freq == 1) & (owner == l)) for l in range(prob.L)]
+    return gids, lids
+
+
+@dataclass
+class ArrowFactor:
+    gids: np.ndarray
+    lids: list
+    D: list        # lower Cholesky factor of each local block S[U_l, U_l]
+    C: list        # coupling S[U_l, G]
+    W: list        # D_l^-1 C_l
+    Lred: np.ndarray  # lower factor of S[G, G] - sum_l C_l' W_l
+
+
+def _chol_ladder(src: np.ndarray):
+    """The relative-shift ladder of factor_arrow_kkt!'s local blocks (kkt.jl:2232-2260) and of
+    _factor_with_relative_regularization! (kkt.jl:1695-1716): reg = sqrt(eps), x10, <= 6 attempts,
+    diagonal += reg max(|src_ii|, 1).  Returns (L, ok, attempts)."""
+    Lf, ok = kchol(src)
+    attempts, reg = 0, SQRT_EPS
+    n = src.shape[0]
+    while not ok and attempts < 6:
+        attempts += 1
+        A = src.copy()
+        A[np.arange(n), np.arange(n)] += reg * np.maximum(np.abs(np.diag(src)), 1.0)
+        Lf, ok = kchol(A)
+        reg *= 10
+    return Lf, ok, attempts
+
+
+def factor_arrow_kkt(S: np.ndarray, part):
+    """factor_arrow_kkt! serial branch (kkt.jl:2232-2349): per local block chol(D_l) with its own
+    ladder, W_l = D_l^-1 C_l (_prepare_arrow_coupling_solve!, :2101-2122), Sred = S_GG - sum_l
+    C_l' W_l (_arrow_rank_sub_lower!), then the ladder on Sred.  Returns (factor, ok, attempts)."""
+    gids, lids = part
+    Ssym = np.tril(S) + np.tril(S, -1).T
+    total = 0
+    D, Cs, Ws = [], [], []
+    Sred = Ssym[np.ix_(gids, gids)].copy()
+    for ids in lids:
+        if len(ids) == 0:
+            D.append(None), Cs.append(None), Ws.append(None)
+            continue
+        Lf, ok, att = _chol_ladder(Ssym[np.ix_(ids, ids)])
+        total += att
+        if not ok:
+            return None, False, total
+        Cl = Ssym[np.ix_(ids, gids)]
+        Wl = kcholsolve(Lf, Cl) if len(gids) else np.zeros((len(ids), 0))
+        Sred -= Cl.T @ Wl
+        D.append(Lf), Cs.append(Cl), Ws.append(Wl)
+    Lred, ok, att = (np.zeros((0, 0)), True, 0) if len(gids) == 0 else _chol_ladder(np.tril(Sred) + np.tril(Sred, -1).T)
+    total += att
+    if not ok:
+        return None, False, total
+    return ArrowFactor(gids, lids, D, Cs, Ws, Lred), True, total
+
+
+def solve_arrow_kkt(f: ArrowFactor, r) -> np.ndarray:
+    """solve_arrow_kkt! serial branch (kkt.jl:2977-3096): r_G -= sum_l C_l' D_l^-1 r_U_l,
+    x_G = Sred^-1 r_G, x_U_l = D_l^-1 r_U_l - W_l x_G."""
+    r = np.asarray(r, dtype=np.float64)
+    rg = r[f.gids].copy()
+    tl = []
+    for ids, Lf, Cl in zip(f.lids, f.D, f.C):
+        if len(ids) == 0:
+            tl.append(None)
+            continue
+        t = kcholsolve(Lf, r[ids])
+        tl.append(t)
+        rg -= Cl.T @ t
+    out = np.zeros_like(r)
+    if len(f.gids):
+        xg = sla.solve_triangular(f.Lred, rg, lower=True, check_finite=False)
+        xg = sla.solve_triangular(f.Lred, xg, lower=True, trans="T", check_finite=False)
+        out[f.gids] = xg
+    else:
+        xg = np.zeros(0)
+    for ids, t, Wl in zip(f.lids, tl, f.W):
+        if len(ids):
+            out[ids] = t - Wl @ xg
+    return out
+
+
+def factor_kkt(prob: Problem, S: np.ndarray, o: Options):
+    """The Schur factor of the selected KKT route: dense Cholesky with the ladder, or the block-arrow
+    elimination (an incompatible problem raises, as the reference's workspace does, workspace.jl:938-940)."""
+    if o.kkt_route == "block_arrow":
+        part = arrow_partition(prob)
+        if part is None:
+            raise ValueError("block_arrow route selected, but the problem is incompatible")
+        return factor_arrow_kkt(S, part)
+    return factor_dense_kkt(S)
 
 
 def equality_rank_tau(m, n, requested):
@@ -1131,7 +1264,7 @@ def newton_step(prob: Problem, o_iter: Options, o: Options, x, X, Y, mu, ctrl: C
         trace.setdefault("S", []).append(st.S.copy())
         trace.setdefault("X", []).append([b.copy() for b in X])
         trace.setdefault("Y", []).append([b.copy() for b in Y])
-    st.Lf, fok, reg_attempts = factor_dense_kkt(st.S)
+    st.Lf, fok, reg_attempts = factor_kkt(prob, st.S, o)
     if not fok:
         return dict(status="breakdown", reason="Schur complement not positive definite", p_res=p_res, d_res=d_res,
                     reg_attempts=reg_attempts)
@@ -1199,7 +1332,10 @@ def newton_step(prob: Problem, o_iter: Options, o: Options, x, X, Y, mu, ctrl: C
 
 
 def _diag_quality(Lm) -> float:
-    """_cholesky_diagonal_quality (step.jl:233-245)."""
+    """_cholesky_diagonal_quality (step.jl:233-245); an arrow factor is rated by its reduced factor
+    alone (_kkt_factorization_quality, step.jl:266-300)."""
+    if isinstance(Lm, ArrowFactor):
+        Lm = Lm.Lred
     if Lm is None or Lm.size == 0:
         return 1.0
     dg = np.abs(np.diag(Lm))
@@ -1401,7 +1537,7 @@ def cold_start(prob: Problem, o: Options):
     degree = sum(k)
     eye = [np.eye(kl) for kl in k]
     S = schur_build(prob, eye, eye)
-    Lf, ok, reg = factor_dense_kkt(S)
+    Lf, ok, reg = factor_kkt(prob, S, o)
     rec = {"regularization_attempts": reg}
     if not ok:
         return False, "identity_kkt_factorization", None, None, None, None, rec

@@ -328,6 +328,127 @@ __global__ void add_upper_to_lower_kernel(double* S, const double* Acc, int n, i
   S[idx] = has_dense ? S[idx] + v : v;
 }
 
+// ---- block-arrow elimination (kkt.jl:2101-2349, 2977-3096) ----------------------
+constexpr int kArrowMaxLocal = 128;  // local block side held in LDS
+
+// S(u, v) of the symmetric Schur matrix from its lower triangle
+__device__ __forceinline__ double sym_lower(const double* S, long long m, int u, int v) {
+  return u >= v ? S[u + v * m] : S[v + u * m];
+}
+
+// One work-group per listed local block b (q <= kArrowMaxLocal): D_b = S[U_b, U_b] gathered
+// through perm, its diagonal shifted by reg[b] max(|D_ii|, 1) when reg[b] > 0 (the ladder of
+// factor_arrow_kkt!, kkt.jl:2232-2260), Cholesky in LDS (info[b] = first failing column + 1,
+// a non-positive, infinite or NaN pivot failing), L_b stored to Lloc; then the coupling rows
+// AT[g, U_b] = C_b L_b^-T (x L_b' = C[g, U_b] by forward substitution, one thread per global g;
+// the reference's W_b = D_b^-1 C_b is L_b^-T of these rows).
+__global__ void __launch_bounds__(256) arrow_local_factor_kernel(const double* S, int m, const int* perm,
+                                                                 const int* loc_off, const long long* loc_foff,
+                                                                 const int* list, const double* reg, double* Lloc,
+                                                                 int* info, double* AT, int ng, int mloc) {
+  extern __shared__ double A[];
+  __shared__ int fail;
+  const int b = list[blockIdx.x];
+  const int o = loc_off[b], q = loc_off[b + 1] - o, pq = q + 1;
+  const double shift = reg[b];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < q * q; idx += 256) {
+    const int i = idx % q, j = idx / q;
+    if (i < j) continue;
+    double a = sym_lower(S, m, perm[o + i], perm[o + j]);
+    if (i == j && shift > 0.0) a += shift * fmax(fabs(a), 1.0);
+    A[i + j * pq] = a;
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  for (int j = 0; j < q; ++j) {
+    if (tid == 0) {
+      const double d = A[j + j * pq];
+      if (!(d > 0.0) || isinf(d)) {
+        fail = j + 1;
+      } else {
+        A[j + j * pq] = sqrt(d);
+      }
+    }
+    __syncthreads();
+    if (fail) break;
+    const double djj = A[j + j * pq];
+    for (int i = j + 1 + tid; i < q; i += 256) A[i + j * pq] /= djj;
+    __syncthreads();
+    const int r = q - j - 1;  // trailing rows / columns j+1 .. q-1, lower entries
+    for (int idx = tid; idx < r * r; idx += 256) {
+      const int i = j + 1 + idx % r, c = j + 1 + idx / r;
+      if (i >= c) A[i + c * pq] -= A[i + j * pq] * A[c + j * pq];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) info[b] = fail;
+  if (fail) return;
+  double* Lb = Lloc + loc_foff[b];
+  for (int idx = tid; idx < q * q; idx += 256) {
+    const int i = idx % q, j = idx / q;
+    Lb[idx] = i >= j ? A[i + j * pq] : 0.0;
+  }
+  for (int g = tid; g < ng; g += 256) {
+    const int vg = perm[mloc + g];
+    double* row = AT + g;  // row g of AT: entries (g, o + j) at row[(o + j) * ng]
+    for (int j = 0; j < q; ++j) {
+      double t = sym_lower(S, m, vg, perm[o + j]);
+      for (int i = 0; i < j; ++i) t -= row[(long long)(o + i) * ng] * A[j + i * pq];
+      row[(long long)(o + j) * ng] = t / A[j + j * pq];
+    }
+  }
+}
+
+// Sred(a, c) = S(G_a, G_c) for a >= c (ld ng)
+__global__ void arrow_gather_global_kernel(const double* S, int m, const int* perm, int mloc, int ng,
+                                           double* Sred) {
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)ng * ng) return;
+  const int a = (int)(idx % ng), c = (int)(idx / ng);
+  if (a < c) return;
+  Sred[idx] = sym_lower(S, m, perm[mloc + a], perm[mloc + c]);
+}
+
+// xp[i] = x[perm[i]] (dir 0) or x[perm[i]] = xp[i] (dir 1)
+__global__ void arrow_permute_kernel(double* x, const int* perm, double* xp, int m, int dir) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  if (dir == 0)
+    xp[i] = x[perm[i]];
+  else
+    x[perm[i]] = xp[i];
+}
+
+// One work-group per local block: xp[U_b] <- L_b^-1 xp[U_b] (trans 0) or L_b^-T xp[U_b] (trans 1),
+// the vector in LDS, one column (or row) of L_b per step
+__global__ void __launch_bounds__(256) arrow_local_solve_kernel(const double* Lloc, const int* loc_off,
+                                                                const long long* loc_foff, double* xp, int trans) {
+  __shared__ double xs[kArrowMaxLocal];
+  const int b = blockIdx.x;
+  const int o = loc_off[b], q = loc_off[b + 1] - o;
+  const double* L = Lloc + loc_foff[b];
+  const int tid = threadIdx.x;
+  if (tid < q) xs[tid] = xp[o + tid];
+  __syncthreads();
+  if (trans == 0) {
+    for (int j = 0; j < q; ++j) {
+      if (tid == 0) xs[j] /= L[j + (long long)j * q];
+      __syncthreads();
+      if (tid > j && tid < q) xs[tid] -= L[tid + (long long)j * q] * xs[j];
+      __syncthreads();
+    }
+  } else {
+    for (int j = q - 1; j >= 0; --j) {
+      if (tid == 0) xs[j] /= L[j + (long long)j * q];
+      __syncthreads();
+      if (tid < j) xs[tid] -= L[j + (long long)tid * q] * xs[j];
+      __syncthreads();
+    }
+  }
+  if (tid < q) xp[o + tid] = xs[tid];
+}
+
 // ---- equality-constrained KKT (kkt.jl:1466-1665, 2570-2595) ----------------------
 // dst(r, c) = src(r, c) for an rows x cols block
 __global__ void copy_block_kernel(const double* src, long long lds, double* dst, long long ldd, int rows, int cols) {
@@ -430,6 +551,17 @@ struct sdpx_dense {
   bool eq_factored = false;
   double *B = nullptr, *Bt = nullptr, *Btmp = nullptr, *Q = nullptr, *Qf = nullptr, *dinvQ = nullptr;
   double *eqscale = nullptr, *qrhs = nullptr, *qy = nullptr, *rtil = nullptr, *peq = nullptr;
+  // block-arrow elimination (sdpx_dense_set_arrow): permuted order = the local variables of each
+  // block (ascending, block by block), then the global ones.  Lloc: local factors packed
+  // (q_b x q_b column-major each); AT = C L_U^-T (ng x mloc, ld ng); Sred = S_GG - AT AT' (ng x ng,
+  // ld ng), factored into Sf with ld ng
+  bool arrow = false;
+  int ng = 0, mloc = 0, nloc = 0, max_q = 0;
+  int *perm = nullptr, *loc_off = nullptr, *loc_list = nullptr, *loc_info = nullptr;
+  long long* loc_foff = nullptr;
+  double *loc_reg = nullptr, *Lloc = nullptr, *AT = nullptr, *Sred = nullptr, *xp = nullptr;
+  double* arrow_part = nullptr;
+  long long arrow_part_cap = 0;
   // device-event kernel timing (sdpx_dense_set_profiling)
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -543,8 +675,98 @@ int direction_residual_dev(sdpx_dense_t* h, int slot) {
   return kOk;
 }
 
+// solve_arrow_kkt! (kkt.jl:2977-3096) with the factors of arrow_factor: x = P r; x_U <- L_U^-1 x_U;
+// x_G -= AT x_U; x_G <- Sred^-1 x_G; x_U -= AT' x_G; x_U <- L_U^-T x_U; out = P' x
+int arrow_solve(sdpx_dense_t* h, double* out) {
+  const int m = h->m, mloc = h->mloc, ng = h->ng;
+  hipLaunchKernelGGL(arrow_permute_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, h->stream, out, h->perm, h->xp, m, 0);
+  SDPX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(arrow_local_solve_kernel, dim3(h->nloc), dim3(256), 0, h->stream, h->Lloc, h->loc_off,
+                     h->loc_foff, h->xp, 0);
+  SDPX_HIP(hipGetLastError());
+  if (ng > 0) {
+    double* xg = h->xp + mloc;
+    SDPX_CHECK(avx_f64(h->AT, ng, mloc, h->xp, -1.0, xg, 1.0, xg, 0.0, nullptr, h->arrow_part, h->arrow_part_cap,
+                       h->stream));
+    SDPX_CHECK(trsv_f64_steps(h->Sf, ng, ng, h->dinvS, xg, h->trsv_y, 0, h->stream));
+    SDPX_CHECK(trsv_f64_steps(h->Sf, ng, ng, h->dinvS, xg, h->trsv_y, 1, h->stream));
+    SDPX_CHECK(avt_f64(h->AT, ng, mloc, xg, -1.0, 1.0, h->xp, h->stream));
+  }
+  hipLaunchKernelGGL(arrow_local_solve_kernel, dim3(h->nloc), dim3(256), 0, h->stream, h->Lloc, h->loc_off,
+                     h->loc_foff, h->xp, 1);
+  SDPX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(arrow_permute_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, h->stream, out, h->perm, h->xp, m, 1);
+  SDPX_HIP(hipGetLastError());
+  return kOk;
+}
+
+// factor_arrow_kkt! (kkt.jl:2132-2349): every local block factored with its own ladder (the
+// failing ones re-launched with the next shift), the coupling rows solved in the same launch,
+// Sred = S_GG - AT AT' on MFMA, then the ladder of _factor_with_relative_regularization!
+// (kkt.jl:1695-1716) on Sred.  *attempts = the sum of every ladder's attempts.
+int arrow_factor(sdpx_dense_t* h, int* attempts_out, bool* ok_out) {
+  const int m = h->m, nloc = h->nloc, ng = h->ng, mloc = h->mloc;
+  std::vector<int> list(nloc), attempts(nloc, 0), info(nloc, 0);
+  std::vector<double> reg(nloc, 0.0);
+  for (int b = 0; b < nloc; ++b) list[b] = b;
+  int total = 0;
+  bool ok = true;
+  const size_t lds = sizeof(double) * (size_t)h->max_q * (h->max_q + 1);
+  while (!list.empty()) {
+    SDPX_HIP(hipMemcpyAsync(h->loc_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, h->stream));
+    SDPX_HIP(hipMemcpyAsync(h->loc_reg, reg.data(), sizeof(double) * nloc, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(arrow_local_factor_kernel, dim3((unsigned)list.size()), dim3(256), lds, h->stream, h->S, m,
+                       h->perm, h->loc_off, h->loc_foff, h->loc_list, h->loc_reg, h->Lloc, h->loc_info, h->AT, ng,
+                       mloc);
+    SDPX_HIP(hipGetLastError());
+    SDPX_HIP(hipMemcpyAsync(info.data(), h->loc_info, sizeof(int) * nloc, hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    std::vector<int> next;
+    for (int b : list) {
+      if (info[b] == 0) continue;
+      if (attempts[b] >= 6) {
+        ok = false;
+        continue;
+      }
+      attempts[b] += 1;
+      total += 1;
+      reg[b] = attempts[b] == 1 ? std::sqrt(kEps) : reg[b] * 10.0;
+      next.push_back(b);
+    }
+    if (!ok) break;
+    list.swap(next);
+  }
+  if (ok && ng > 0) {
+    hipLaunchKernelGGL(arrow_gather_global_kernel, dim3(ceil_div((long long)ng * ng, 256)), dim3(256), 0, h->stream,
+                       h->S, m, h->perm, mloc, ng, h->Sred);
+    SDPX_HIP(hipGetLastError());
+    SDPX_CHECK(gemm_f64(gemm(ng, ng, mloc, -1.0, h->AT, ng, h->AT, ng, 1.0, h->Sred, ng,
+                             kGemmTransB | kGemmLowerOut),
+                        h->stream));
+    int att = 0;
+    double shift = 0.0;
+    while (true) {
+      SDPX_CHECK(copy_lower_shift_f64(h->Sred, h->Sf, ng, att == 0 ? 0.0 : shift, h->stream));
+      SDPX_CHECK(potrf_f64(h->Sf, ng, 0, ng, 1, h->dinvS, 0, h->dinfo, h->stream));
+      SDPX_CHECK(sync_scalars(h, 0, 1));
+      if (h->hinfo[0] == 0) break;
+      if (att >= 6) {
+        ok = false;
+        break;
+      }
+      att += 1;
+      shift = att == 1 ? std::sqrt(kEps) : shift * 10.0;
+    }
+    total += att;
+  }
+  *attempts_out = total;
+  *ok_out = ok;
+  return kOk;
+}
+
 int solve_into(sdpx_dense_t* h, const double* rhs, double* out) {
   if (rhs != out) SDPX_HIP(hipMemcpyAsync(out, rhs, sizeof(double) * h->m, hipMemcpyDeviceToDevice, h->stream));
+  if (h->arrow) return arrow_solve(h, out);
   SDPX_CHECK(trsv_f64_steps(h->Sf, h->m, h->m, h->dinvS, out, h->trsv_y, 0, h->stream));
   SDPX_CHECK(trsv_f64_steps(h->Sf, h->m, h->m, h->dinvS, out, h->trsv_y, 1, h->stream));
   return kOk;
@@ -1282,14 +1504,18 @@ int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts) {
   double reg = 0.0;
   bool ok = false;
   if (h->prof) SDPX_HIP(hipEventRecord(h->ev[4], h->stream));
-  while (true) {
-    SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
-    SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
-    SDPX_CHECK(sync_scalars(h, 0, 1));
-    ok = h->hinfo[0] == 0;
-    if (ok || attempts >= 6) break;
-    attempts += 1;
-    reg = attempts == 1 ? std::sqrt(kEps) : reg * 10.0;
+  if (h->arrow) {
+    SDPX_CHECK(arrow_factor(h, &attempts, &ok));
+  } else {
+    while (true) {
+      SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
+      SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
+      SDPX_CHECK(sync_scalars(h, 0, 1));
+      ok = h->hinfo[0] == 0;
+      if (ok || attempts >= 6) break;
+      attempts += 1;
+      reg = attempts == 1 ? std::sqrt(kEps) : reg * 10.0;
+    }
   }
   h->last_reg_attempts = attempts;
   if (reg_attempts) *reg_attempts = attempts;
@@ -1661,6 +1887,10 @@ int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
     set_last_error("sdpx_dense_set_precision: precision must be 0 (FP64), 1 (FP32 Schur Gram) or 2 (BF16x3 Schur Gram)");
     return kInvalidArgument;
   }
+  if (precision != 0 && h->arrow) {
+    set_last_error("sdpx_dense_set_precision: the block-arrow route factors in FP64 only (precision 0)");
+    return kInvalidArgument;
+  }
   for (int l = 0; l < h->L && precision != 0; ++l) {
     if (h->blk[l].na > 0) {
       set_last_error("sdpx_dense_set_precision: sparse blocks assemble in FP64 only (precision 0)");
@@ -1877,7 +2107,16 @@ int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out) {
     SDPX_CHECK(diag_extrema_f64(b.LX, b.k, 0, b.k, 1, h->dsc + 4 * l, h->stream));
     SDPX_CHECK(diag_extrema_f64(b.MY, b.k, 0, b.k, 1, h->dsc + 4 * l + 2, h->stream));
   }
-  SDPX_CHECK(diag_extrema_f64(h->Sf, h->m, 0, h->m, 1, h->dsc + 4 * L, h->stream));
+  if (h->arrow) {
+    // _kkt_factorization_quality with an arrow workspace (step.jl:266-300): the reduced factor
+    // alone, 1 when there is no global variable
+    if (h->ng > 0)
+      SDPX_CHECK(diag_extrema_f64(h->Sf, h->ng, 0, h->ng, 1, h->dsc + 4 * L, h->stream));
+    else
+      SDPX_CHECK(fill_f64(2, 1.0, h->dsc + 4 * L, h->stream));
+  } else {
+    SDPX_CHECK(diag_extrema_f64(h->Sf, h->m, 0, h->m, 1, h->dsc + 4 * L, h->stream));
+  }
   SDPX_CHECK(sync_scalars(h, 4 * L + 2));
   auto quality = [](double mn, double mx) {
     return (mx > 0.0 && std::isfinite(mx)) ? std::min(std::max(mn / mx, 0.0), 1.0) : 0.0;
@@ -2174,6 +2413,10 @@ int sdpx_dense_set_equalities(sdpx_dense_t* h, int n, const double* B) {
     set_last_error("sdpx_dense_set_equalities: n >= 1 and B (m x n) required, once per handle");
     return kInvalidArgument;
   }
+  if (h->arrow) {
+    set_last_error("sdpx_dense_set_equalities: the block-arrow route takes no equality border");
+    return kInvalidArgument;
+  }
   const long long m = h->m;
   const long long nbq = (long long)ceil_div(n, kCholNb) * kCholNb * kCholNb;
   SDPX_CHECK(dalloc(h, &h->B, (size_t)(m * n)));
@@ -2370,6 +2613,87 @@ int sdpx_dense_set_block_sparse(sdpx_dense_t* h, int l, int na, const int* ids, 
   SDPX_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
   SDPX_HIP(hipStreamSynchronize(h->stream));
   b.na = na;
+  return kOk;
+}
+
+int sdpx_dense_set_arrow(sdpx_dense_t* h, const int* owner) {
+  SDPX_CHECK(check_handle(h));
+  const int m = h->m, L = h->L;
+  if (!owner || h->arrow || h->neq != 0 || h->precision != 0) {
+    set_last_error("sdpx_dense_set_arrow: owner (m) required, once per handle, with no equality border and "
+                   "precision 0");
+    return kInvalidArgument;
+  }
+  // permuted order: the local variables of block 0, 1, ... (ascending within a block), then the globals
+  std::vector<std::vector<int>> loc(L);
+  std::vector<int> glob;
+  for (int i = 0; i < m; ++i) {
+    if (owner[i] < -1 || owner[i] >= L) {
+      set_last_error("sdpx_dense_set_arrow: owner[" + std::to_string(i) + "] outside [-1, L)");
+      return kInvalidArgument;
+    }
+    if (owner[i] < 0)
+      glob.push_back(i);
+    else
+      loc[owner[i]].push_back(i);
+  }
+  std::vector<int> perm, off{0};
+  std::vector<long long> foff;
+  long long fsz = 0;
+  int max_q = 0;
+  for (int l = 0; l < L; ++l) {
+    const int q = (int)loc[l].size();
+    if (q == 0) continue;
+    if (q > kArrowMaxLocal) {
+      set_last_error("sdpx_dense_set_arrow: block " + std::to_string(l) + " has " + std::to_string(q) +
+                     " local variables; at most " + std::to_string(kArrowMaxLocal) + " are supported");
+      return kInvalidArgument;
+    }
+    perm.insert(perm.end(), loc[l].begin(), loc[l].end());
+    off.push_back((int)perm.size());
+    foff.push_back(fsz);
+    fsz += (long long)q * q;
+    max_q = std::max(max_q, q);
+  }
+  const int mloc = (int)perm.size(), ng = (int)glob.size(), nloc = (int)foff.size();
+  if (mloc == 0) {
+    set_last_error("sdpx_dense_set_arrow: no local variable (every variable touches several blocks)");
+    return kInvalidArgument;
+  }
+  perm.insert(perm.end(), glob.begin(), glob.end());
+  const size_t lds = sizeof(double) * (size_t)max_q * (max_q + 1);
+  if (lds > 64 * 1024)
+    SDPX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(arrow_local_factor_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  SDPX_CHECK(dalloc(h, &h->perm, (size_t)m));
+  SDPX_CHECK(dalloc(h, &h->loc_off, (size_t)nloc + 1));
+  SDPX_CHECK(dalloc(h, &h->loc_foff, (size_t)nloc));
+  SDPX_CHECK(dalloc(h, &h->loc_list, (size_t)nloc));
+  SDPX_CHECK(dalloc(h, &h->loc_info, (size_t)nloc));
+  SDPX_CHECK(dalloc(h, &h->loc_reg, (size_t)nloc));
+  SDPX_CHECK(dalloc(h, &h->Lloc, (size_t)fsz));
+  SDPX_CHECK(dalloc(h, &h->AT, (size_t)std::max<long long>((long long)ng * mloc, 1)));
+  SDPX_CHECK(dalloc(h, &h->Sred, (size_t)std::max<long long>((long long)ng * ng, 1)));
+  SDPX_CHECK(dalloc(h, &h->xp, (size_t)m));
+  h->arrow_part_cap = std::max<long long>((long long)ng * 64, 4096);
+  SDPX_CHECK(dalloc(h, &h->arrow_part, (size_t)h->arrow_part_cap));
+  SDPX_HIP(hipMemcpyAsync(h->perm, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->loc_off, off.data(), sizeof(int) * (nloc + 1), hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipMemcpyAsync(h->loc_foff, foff.data(), sizeof(long long) * nloc, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipStreamSynchronize(h->stream));
+  h->ng = ng;
+  h->mloc = mloc;
+  h->nloc = nloc;
+  h->max_q = max_q;
+  h->arrow = true;
+  return kOk;
+}
+
+int sdpx_dense_get_arrow(sdpx_dense_t* h, int* ng, int* nloc, int* max_q) {
+  SDPX_CHECK(check_handle(h));
+  if (ng) *ng = h->arrow ? h->ng : -1;
+  if (nloc) *nloc = h->nloc;
+  if (max_q) *max_q = h->max_q;
   return kOk;
 }
 

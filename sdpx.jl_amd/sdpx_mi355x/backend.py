@@ -75,6 +75,30 @@ def block_active(Al: np.ndarray) -> np.ndarray:
     return np.flatnonzero(np.any(Al.reshape(Al.shape[0], -1) != 0.0, axis=1))
 
 
+def arrow_owner(prob: SDPProblem, max_local: int = 128):
+    """ArrowWorkspace's partition (workspace.jl:318-400) as the C-ABI takes it: owner[i] = the one
+    block whose active set holds variable i, -1 for a variable on several blocks.  None when the
+    block-arrow route does not apply: a variable on no block, every variable global, or a block
+    with more than ``max_local`` local variables (the device route's limit)."""
+    m = prob.m
+    freq = np.zeros(m, dtype=np.int64)
+    owner = np.full(m, -1, dtype=np.int32)
+    for l in range(prob.L):
+        if l in prob.sparse:
+            act = np.asarray(prob.sparse[l][0], dtype=np.int64)
+        else:
+            act = np.flatnonzero(np.any(np.asarray(prob.Av[l]) != 0.0, axis=1))
+        freq[act] += 1
+        owner[act] = l
+    if not np.all(freq > 0) or not np.any(freq == 1):
+        return None
+    owner[freq > 1] = -1
+    counts = np.bincount(owner[owner >= 0], minlength=prob.L)
+    if counts.max(initial=0) > max_local:
+        return None
+    return owner
+
+
 def ingest(c, A, C, sparse=()) -> SDPProblem:
     """Dense ingest (ingest.jl:680-694): ``A[l]`` is (m, k, k) with A[l][i] = A_i^(l).
 
@@ -163,6 +187,24 @@ class MI355XDenseBackend:
         check(self.lib.sdpx_dense_set_block_sparse(self.h, l, int(ids.size), ids.ctypes.data_as(ip),
                                                    *(a.ctypes.data_as(ip) for a in arrs), dptr(val), dptr(Cl)),
               "set_block_sparse")
+
+    def set_arrow(self, owner=None):
+        """Switch the Schur factor and solves to the block-arrow elimination (factor_arrow_kkt!,
+        kkt.jl:2132-2349; solve_arrow_kkt!, :2977-3096).  ``owner`` defaults to arrow_owner(prob)."""
+        if owner is None:
+            owner = arrow_owner(self.prob)
+            if owner is None:
+                raise ValueError("block_arrow route selected, but the problem is incompatible")
+        owner = np.ascontiguousarray(owner, dtype=np.int32)
+        if owner.shape != (self.m,):
+            raise ValueError(f"set_arrow: owner needs {self.m} entries")
+        check(self.lib.sdpx_dense_set_arrow(self.h, owner.ctypes.data_as(_lib._ip)), "set_arrow")
+
+    def arrow_info(self):
+        """(global variables or -1, local blocks, largest local block)."""
+        v = [C.c_int(0), C.c_int(0), C.c_int(0)]
+        check(self.lib.sdpx_dense_get_arrow(self.h, *(C.byref(x) for x in v)), "get_arrow")
+        return tuple(x.value for x in v)
 
     # ---- lifecycle -----------------------------------------------------
     def close(self):

@@ -216,6 +216,8 @@ def solve(prob: SDPProblem, o: SolverOptions = SolverOptions(), x0=None, X0=None
     own = backend is None
     b = backend if backend is not None else MI355XDenseBackend(prob, device)
     try:
+        if o.kkt_route == "block_arrow" and b.arrow_info()[0] < 0:
+            b.set_arrow()
         if not equilibrate:
             return _solve(b, prob, o, x0, X0, Y0, timing, trace_schur, start)
         eq = b.equilibrate(ruiz_iters)

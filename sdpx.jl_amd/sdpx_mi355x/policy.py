@@ -65,6 +65,10 @@ class SolverOptions:
     bf16x3_cond_limit: float = 1e4
     mixed_refine_target: float = 1e-10
     mixed_max_refine: int = 32
+    # KKT route: "dense_cholesky" or "block_arrow" (the exact block-arrow elimination of
+    # factor_arrow_kkt! / solve_arrow_kkt!, kkt.jl:2132-2349, 2977-3096; an incompatible problem
+    # raises, as the reference's workspace does, workspace.jl:938-940)
+    kkt_route: str = "dense_cholesky"
 
     def __post_init__(self):
         if not 0 < self.beta < 1:
@@ -77,6 +81,8 @@ class SolverOptions:
             raise ValueError("step_rule must be 'backtrack', 'fraction_to_boundary', or 'auto'")
         if self.refine_policy not in ("fixed", "adaptive", "auto"):
             raise ValueError("refine_policy must be 'fixed', 'adaptive', or 'auto'")
+        if self.kkt_route not in ("dense_cholesky", "block_arrow"):
+            raise ValueError("kkt_route must be 'dense_cholesky' or 'block_arrow'")
         if self.precision not in ("fp64", "mixed", "mixed_bf16x3"):
             raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_bf16x3'")
 
