@@ -84,12 +84,31 @@ function DenseHandle(prob::SDPX.SDPProblem{Float64}; device::Int=0)
         x.ptr == C_NULL || ccall(_sym(:sdpx_dense_destroy), Cint, (Ptr{Cvoid},), x.ptr)
         x.ptr = C_NULL
     end
-    cons = prob.cons::SDPX.DenseCons{Float64}
-    for l in 1:L
-        # DenseCons.Av[l] is the k^2 x m column-major panel the ABI expects
-        _check(ccall(_sym(:sdpx_dense_set_block), Cint,
-                     (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}),
-                     h.ptr, l - 1, cons.Av[l], prob.C[l]), "sdpx_dense_set_block")
+    if prob.cons isa SDPX.DenseCons{Float64}
+        cons = prob.cons::SDPX.DenseCons{Float64}
+        for l in 1:L
+            # DenseCons.Av[l] is the k^2 x m column-major panel the ABI expects
+            _check(ccall(_sym(:sdpx_dense_set_block), Cint,
+                         (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}),
+                         h.ptr, l - 1, cons.Av[l], prob.C[l]), "sdpx_dense_set_block")
+        end
+    else
+        # SparseCons: every block on the sparse pair route, its SparseBlockCOO passed as stored
+        # (2x2 blocks keep packed2 instead of a COO; their COO is packed here the same way)
+        cons = prob.cons::SDPX.SparseCons{Float64}
+        for l in 1:L
+            ids = cons.schur_order[l]
+            isempty(ids) && throw(ArgumentError("block $l has no active variable"))
+            coo = cons.coo[l]
+            if length(coo.ptr) != length(ids) + 1
+                coo = SDPX.build_block_coo(collect(cons.Asp[l]), ids, k[l])
+            end
+            _check(ccall(_sym(:sdpx_dense_set_block_sparse), Cint,
+                         (Ptr{Cvoid}, Cint, Cint, Ptr{Cint}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32},
+                          Ptr{Float64}, Ptr{Float64}),
+                         h.ptr, l - 1, length(ids), Cint.(ids .- 1), coo.ptr, coo.lin, coo.row, coo.col, coo.val,
+                         prob.C[l]), "sdpx_dense_set_block_sparse")
+        end
     end
     _check(ccall(_sym(:sdpx_dense_set_objective), Cint, (Ptr{Cvoid}, Ptr{Float64}),
                  h.ptr, prob.c), "sdpx_dense_set_objective")
@@ -429,6 +448,19 @@ mutable struct MI355XDenseBackend <: SDPX.KKTBackend
 end
 
 SDPX.backend_name(::MI355XDenseBackend) = :mi355x_dense_cholesky
+
+"""Block-arrow KKT backend (the :block_arrow route, factor_arrow_kkt! / solve_arrow_kkt!): the
+same handle with its factor and solves switched to the arrow elimination over the partition of
+the workspace's ArrowWorkspace (local_ids per block, global_ids)."""
+function arrow_backend(h::DenseHandle, arrow::SDPX.ArrowWorkspace{Float64})
+    owner = fill(Cint(-1), h.m)
+    for (l, ids) in pairs(arrow.local_ids), i in ids
+        owner[i] = Cint(l - 1)
+    end
+    _check(ccall(_sym(:sdpx_dense_set_arrow), Cint, (Ptr{Cvoid}, Ptr{Cint}), h.ptr, owner),
+           "sdpx_dense_set_arrow")
+    return MI355XDenseBackend(h)
+end
 
 function SDPX.factorize!(b::MI355XDenseBackend, ws::SDPX.Workspace{Float64},
                          prob::SDPX.SDPProblem{Float64}, opts::SDPX.SolverOptions{Float64})
