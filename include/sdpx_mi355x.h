@@ -278,6 +278,16 @@ int sdpx_dense_solve_equality(sdpx_dense_t* h, const double* p, double* dy);
 /* B~ D (m x n), Q (n x n, lower) and D (n) of the last equality factor. */
 int sdpx_dense_get_equality(sdpx_dense_t* h, double* Bt, double* Q, double* scale);
 
+/* Condition estimates from the last Schur factor L (S = L L'); which: 1 = [0], 2 = [1..3],
+ * 3 = both (the others are left 0 / infinite); out (4):
+ * [0] cond_1(S) as xPOCON estimates it (||S||_1 of the unfactored S times Higham's
+ *     estimate of ||S^-1||_1 from triangular solves on the device);
+ * [1] _triangular_condition_estimate's (mixed_precision_kkt.jl:371-392) xTRCON product
+ *     1 / (rcond_1(L) rcond_inf(L)), an upper bound of cond_1(S);
+ * [2], [3] rcond_1(L), rcond_inf(L).
+ * Infinite (or 0) when an estimate is not positive and finite.  Dense factor only. */
+int sdpx_dense_factor_condition(sdpx_dense_t* h, int which, double* out);
+
 /* ---- a13: mixed precision ----------------------------------------------------
  * The reference's mixed-precision template (kernels/mixed_precision_kkt.jl:
  * low-precision factor, target-precision residuals and refinement,

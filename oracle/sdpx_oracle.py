@@ -1018,6 +1018,30 @@ def factor_kkt(prob: Problem, S: np.ndarray, o: Options):
     return factor_dense_kkt(S)
 
 
+def triangular_condition_estimate(L: np.ndarray) -> float:
+    """_triangular_condition_estimate (mixed_precision_kkt.jl:371-392): LAPACK dtrcon of the lower
+    factor in the one and infinity norms, cond = 1 / (rcond_1 rcond_inf) (inf when either is 0)."""
+    a = np.asfortranarray(np.tril(L))
+    r1, info1 = lapack.dtrcon(a, norm="1", uplo="L", diag="N")
+    ri, info2 = lapack.dtrcon(a, norm="I", uplo="L", diag="N")
+    if info1 != 0 or info2 != 0 or not (r1 > 0.0 and ri > 0.0):
+        return math.inf
+    est = 1.0 / (r1 * ri)
+    return est if math.isfinite(est) else math.inf
+
+
+def schur_condition_estimate(S: np.ndarray, L: np.ndarray) -> float:
+    """cond_1(S) as LAPACK dpocon estimates it from the Cholesky factor: ||S||_1 of the symmetric S
+    (from its lower triangle) over dpocon's reciprocal condition number."""
+    Sl = np.tril(S)
+    anorm = float(np.max(np.sum(np.abs(Sl + np.tril(Sl, -1).T), axis=0)))
+    rc, info = lapack.dpocon(np.asfortranarray(np.tril(L)), anorm, uplo="L")
+    if info != 0 or not rc > 0.0:
+        return math.inf
+    est = 1.0 / rc
+    return est if math.isfinite(est) else math.inf
+
+
 def equality_rank_tau(m, n, requested):
     """_equality_cholesky_rank_tau, Float64 branch (kkt.jl:103-112)."""
     req = requested if requested > 0 else SQRT_EPS

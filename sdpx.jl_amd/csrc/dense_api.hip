@@ -328,6 +328,35 @@ __global__ void add_upper_to_lower_kernel(double* S, const double* Acc, int n, i
   S[idx] = has_dense ? S[idx] + v : v;
 }
 
+// ---- triangular condition estimate (mixed_precision_kkt.jl:371-392) ----------------
+// colsum[j] = sum_{i >= j} |L(i, j)| (one work-group per column)
+__global__ void __launch_bounds__(256) tri_colsum_kernel(const double* L, long long ld, int n, double* colsum) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int i = j + threadIdx.x; i < n; i += 256) acc += fabs(L[i + j * ld]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) colsum[j] = red[0];
+}
+
+// part[s * n + i] = sum over the columns j <= i of chunk s of |L(i, j)| (one thread per row:
+// the reads of one column are coalesced across the threads)
+__global__ void __launch_bounds__(256) tri_rowsum_kernel(const double* L, long long ld, int n, int chunk,
+                                                        double* part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int s = blockIdx.y;
+  const int j0 = s * chunk, j1 = min(min(n, j0 + chunk), i + 1);
+  double acc = 0.0;
+  for (int j = j0; j < j1; ++j) acc += fabs(L[i + (long long)j * ld]);
+  part[(long long)s * n + i] = acc;
+}
+
 // ---- block-arrow elimination (kkt.jl:2101-2349, 2977-3096) ----------------------
 constexpr int kArrowMaxLocal = 128;  // local block side held in LDS
 
@@ -562,6 +591,7 @@ struct sdpx_dense {
   double *loc_reg = nullptr, *Lloc = nullptr, *AT = nullptr, *Sred = nullptr, *xp = nullptr;
   double* arrow_part = nullptr;
   long long arrow_part_cap = 0;
+  double* cond_part = nullptr;  // row-sum partials of the condition estimate (32 x m)
   // device-event kernel timing (sdpx_dense_set_profiling)
   bool prof = false;
   hipEvent_t ev[6] = {};
@@ -761,6 +791,67 @@ int arrow_factor(sdpx_dense_t* h, int* attempts_out, bool* ok_out) {
   }
   *attempts_out = total;
   *ok_out = ok;
+  return kOk;
+}
+
+// Higham's one-norm estimator (the LAPACK xLACN2 iteration behind xTRCON; N. J. Higham,
+// ACM TOMS 14 (1988) 381-396, Algorithm 4.1): an estimate of ||B||_1 from products with B and
+// B' only, at most five rounds plus the alternating-sign test vector.  apply(x, t) overwrites
+// the host vector x with B x (t = 0) or B' x (t = 1).
+template <typename Apply>
+int onenorm_estimate(int n, Apply apply, double* est_out) {
+  std::vector<double> x(n, 1.0 / n), v;
+  std::vector<int> sgn(n);
+  auto sign = [](double a) { return a >= 0.0 ? 1 : -1; };
+  auto asum = [](const std::vector<double>& a) {
+    double t = 0.0;
+    for (double e : a) t += std::fabs(e);
+    return t;
+  };
+  auto argmax = [&](const std::vector<double>& a) {
+    int j = 0;
+    for (int i = 1; i < n; ++i)
+      if (std::fabs(a[i]) > std::fabs(a[j])) j = i;
+    return j;
+  };
+  SDPX_CHECK(apply(x, 0));
+  if (n == 1) {
+    *est_out = std::fabs(x[0]);
+    return kOk;
+  }
+  double est = asum(x);
+  for (int i = 0; i < n; ++i) {
+    sgn[i] = sign(x[i]);
+    x[i] = sgn[i];
+  }
+  SDPX_CHECK(apply(x, 1));
+  int j = argmax(x);
+  for (int iter = 2;; ++iter) {
+    std::fill(x.begin(), x.end(), 0.0);
+    x[j] = 1.0;
+    SDPX_CHECK(apply(x, 0));
+    const double old = est;
+    est = asum(x);
+    bool same = true;
+    for (int i = 0; i < n && same; ++i) same = sign(x[i]) == sgn[i];
+    if (same || est <= old) break;
+    for (int i = 0; i < n; ++i) {
+      sgn[i] = sign(x[i]);
+      x[i] = sgn[i];
+    }
+    SDPX_CHECK(apply(x, 1));
+    const int jlast = j;
+    j = argmax(x);
+    if (!(x[jlast] != std::fabs(x[j]) && iter < 5)) break;
+  }
+  double alt = 1.0;
+  for (int i = 0; i < n; ++i) {
+    x[i] = alt * (1.0 + (double)i / (n - 1));
+    alt = -alt;
+  }
+  SDPX_CHECK(apply(x, 0));
+  const double temp = 2.0 * asum(x) / (3.0 * n);
+  *est_out = std::max(est, temp);
   return kOk;
 }
 
@@ -2096,6 +2187,93 @@ int sdpx_dense_scale_blocks(sdpx_dense_t* h, int which, double factor) {
     if (which == 2) SDPX_CHECK(axpbyz_f64(b.k2, factor, b.Y, 0.0, nullptr, b.Y, h->stream));
   }
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+// Condition estimates of the Schur system from its factor L (S = L L'):
+//  * xPOCON's: ||S||_1 of the unfactored S (its column sums from the lower triangle) times
+//    Higham's estimate of ||S^-1||_1, S^-1 x applied as two triangular solves;
+//  * _triangular_condition_estimate's (mixed_precision_kkt.jl:371-392): xTRCON's reciprocal
+//    condition numbers of L in the one and infinity norms, 1 / (rcond_1 rcond_inf), an upper
+//    bound of cond_1(S) that the reference calls deliberately conservative.
+// out (4) = cond_1(S) estimate, the xTRCON product, rcond_1(L), rcond_inf(L) (infinite / 0 when an
+// estimate is not positive and finite).
+int sdpx_dense_factor_condition(sdpx_dense_t* h, int which, double* out) {
+  SDPX_CHECK(check_handle(h));
+  if (h->arrow || !out || which < 1 || which > 3) {
+    set_last_error("sdpx_dense_factor_condition: which in 1..3 and out (4) required; not defined for the "
+                   "block-arrow factor");
+    return kInvalidArgument;
+  }
+  const bool want_s = (which & 1) != 0, want_l = (which & 2) != 0;
+  const int m = h->m;
+  const int nsplit = std::max(1, std::min(32, ceil_div(m, 256)));
+  const int chunk = ceil_div(m, nsplit);
+  if (!h->cond_part) SDPX_CHECK(dalloc(h, &h->cond_part, (size_t)33 * m));
+  // column sums of the lower triangle (rows >= j) and row sums (columns <= i), for L and for S
+  auto norms = [&](const double* A, std::vector<double>& col, std::vector<double>& row) -> int {
+    hipLaunchKernelGGL(tri_colsum_kernel, dim3(m), dim3(256), 0, h->stream, A, (long long)m, m,
+                       h->cond_part + (size_t)32 * m);
+    SDPX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(tri_rowsum_kernel, dim3(ceil_div(m, 256), nsplit), dim3(256), 0, h->stream, A, (long long)m, m,
+                       chunk, h->cond_part);
+    SDPX_HIP(hipGetLastError());
+    std::vector<double> part((size_t)nsplit * m);
+    col.resize(m);
+    row.assign(m, 0.0);
+    SDPX_HIP(hipMemcpyAsync(col.data(), h->cond_part + (size_t)32 * m, sizeof(double) * m, hipMemcpyDeviceToHost,
+                            h->stream));
+    SDPX_HIP(hipMemcpyAsync(part.data(), h->cond_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    for (int sp = 0; sp < nsplit; ++sp)
+      for (int i = 0; i < m; ++i) row[i] += part[(size_t)sp * m + i];
+    return kOk;
+  };
+  double l1 = 0.0, linf = 0.0, s1 = 0.0;
+  if (want_l) {
+    std::vector<double> lc, lr;
+    SDPX_CHECK(norms(h->Sf, lc, lr));
+    for (int j = 0; j < m; ++j) {
+      l1 = std::max(l1, lc[j]);
+      linf = std::max(linf, lr[j]);
+    }
+  }
+  if (want_s) {
+    std::vector<double> sc, sr, sdiag(m);
+    SDPX_CHECK(norms(h->S, sc, sr));
+    SDPX_HIP(hipMemcpy2DAsync(sdiag.data(), sizeof(double), h->S, sizeof(double) * (m + 1), sizeof(double), m,
+                              hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    for (int j = 0; j < m; ++j) s1 = std::max(s1, sc[j] + sr[j] - std::fabs(sdiag[j]));  // column j of S
+  }
+  // one triangular solve with L (trans 0) or L' (trans 1) on a host vector, through the device
+  auto solve = [&](std::vector<double>& x, int trans) -> int {
+    SDPX_HIP(hipMemcpyAsync(h->delta, x.data(), sizeof(double) * m, hipMemcpyHostToDevice, h->stream));
+    SDPX_CHECK(trsv_f64_steps(h->Sf, m, m, h->dinvS, h->delta, h->trsv_y, trans, h->stream));
+    SDPX_HIP(hipMemcpyAsync(x.data(), h->delta, sizeof(double) * m, hipMemcpyDeviceToHost, h->stream));
+    SDPX_HIP(hipStreamSynchronize(h->stream));
+    return kOk;
+  };
+  auto solve_s = [&](std::vector<double>& x, int) -> int {
+    SDPX_CHECK(solve(x, 0));
+    return solve(x, 1);
+  };
+  double sinv = 0.0, inv1 = 0.0, invinf = 0.0;
+  if (want_s) SDPX_CHECK(onenorm_estimate(m, solve_s, &sinv));
+  if (want_l) {
+    // ||L^-1||_1: B = L^-1; ||L^-1||_inf = ||L^-T||_1: B = L^-T
+    SDPX_CHECK(onenorm_estimate(m, [&](std::vector<double>& x, int t) { return solve(x, t); }, &inv1));
+    SDPX_CHECK(onenorm_estimate(m, [&](std::vector<double>& x, int t) { return solve(x, 1 - t); }, &invinf));
+  }
+  const double rcs = (s1 > 0.0 && sinv > 0.0) ? (1.0 / sinv) / s1 : 0.0;
+  const double rc1 = (l1 > 0.0 && inv1 > 0.0) ? (1.0 / l1) / inv1 : 0.0;
+  const double rcinf = (linf > 0.0 && invinf > 0.0) ? (1.0 / linf) / invinf : 0.0;
+  const double conds = rcs > 0.0 ? 1.0 / rcs : INFINITY;
+  const double condl = (rc1 > 0.0 && rcinf > 0.0) ? 1.0 / (rc1 * rcinf) : INFINITY;
+  out[0] = std::isfinite(conds) ? conds : INFINITY;
+  out[1] = std::isfinite(condl) ? condl : INFINITY;
+  out[2] = rc1;
+  out[3] = rcinf;
   return kOk;
 }
 

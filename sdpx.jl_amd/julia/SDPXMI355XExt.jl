@@ -231,6 +231,15 @@ set_precision!(h::DenseHandle, precision::Integer) =
            "sdpx_dense_set_precision")
 
 """rho = r - S dx against the matrix-free FP64 Schur operator -> (||rho||_inf, ||r||_inf)."""
+# Condition estimates from the last Schur factor: cond_1(S) (xPOCON), the xTRCON product of
+# _triangular_condition_estimate (mixed_precision_kkt.jl:371-392), rcond_1(L), rcond_inf(L)
+function factor_condition(h::DenseHandle; which::Integer=3)
+    out = zeros(Float64, 4)
+    _check(ccall(_sym(:sdpx_dense_factor_condition), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), h.ptr, which, out),
+           "sdpx_dense_factor_condition")
+    return (cond_schur=out[1], cond_trcon=out[2], rcond_1=out[3], rcond_inf=out[4])
+end
+
 function exact_residual(h::DenseHandle)
     out = zeros(2)
     _check(ccall(_sym(:sdpx_dense_exact_residual), Cint, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, out),

@@ -26,6 +26,7 @@ SIGNATURES = {
     "sdpx_dense_set_block": [_h, C.c_int, _dp, _dp],
     "sdpx_dense_set_block_sparse": [_h, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp],
     "sdpx_dense_set_arrow": [_h, _ip],
+    "sdpx_dense_factor_condition": [_h, C.c_int, _dp],
     "sdpx_dense_get_arrow": [_h, _ip, _ip, _ip],
     "sdpx_dense_set_objective": [_h, _dp],
     "sdpx_dense_set_iterate": [_h, _dp, _dp, _dp],

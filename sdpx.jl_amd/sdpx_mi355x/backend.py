@@ -419,6 +419,14 @@ class MI355XDenseBackend:
     def scale_blocks(self, which: int, factor: float):
         check(self.lib.sdpx_dense_scale_blocks(self.h, which, factor), "scale_blocks")
 
+    def factor_condition(self, which: int = 3):
+        """(cond_1(S) estimate as xPOCON, the xTRCON product of _triangular_condition_estimate
+        (mixed_precision_kkt.jl:371-392), rcond_1(L), rcond_inf(L)) from the last Schur factor;
+        which = 1 computes the first only, 2 the others only."""
+        out = np.zeros(4)
+        check(self.lib.sdpx_dense_factor_condition(self.h, int(which), dptr(out)), "factor_condition")
+        return tuple(float(v) for v in out)
+
     def factor_quality(self):
         out = np.zeros(3)
         check(self.lib.sdpx_dense_factor_quality(self.h, dptr(out)), "factor_quality")

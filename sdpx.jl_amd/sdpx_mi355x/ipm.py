@@ -103,11 +103,16 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
                     reason=f"Schur complement not positive definite after {reg_attempts} regularization attempt(s)",
                     p_res=p_res, d_res=d_res, reg_attempts=reg_attempts)
     if use32:
-        # _triangular_condition_estimate (mixed_precision_kkt.jl:371-392): squared diagonal ratio of the factor
-        _, _, q = b.factor_quality()
-        cond_est = 1.0 / (q * q) if q > 0 else math.inf
-        if not cond_est <= (o.mixed_cond_limit if o.precision == "mixed" else o.bf16x3_cond_limit):
-            return dict(status="retry_fp64", reason=f"Schur condition estimate {cond_est:.2e} above the reduced-precision limit")
+        # the static condition cutoff of _try_factor_mixed_kkt! (mixed_precision_kkt.jl:1049-1059): the
+        # Schur matrix's condition estimate from its factor (cond_1(S) as xPOCON estimates it; the
+        # reference's xTRCON product of :371-392 bounds it from above and is reported beside it) under
+        # the reduced-precision limit.  The predicted-refinement-step budget (:1079-1093) is left to
+        # the measured refinement guards below, as the reference's :on mode does (:990-1000).
+        cond_est = b.factor_condition(1)[0]
+        limit = o.mixed_cond_limit if o.precision == "mixed" else o.bf16x3_cond_limit
+        if not cond_est <= limit:
+            return dict(status="retry_fp64", reason=f"Schur condition estimate {cond_est:.2e} above the reduced-precision limit",
+                        cond_estimate=cond_est)
     # predictor (step.jl:699-726)
     b.predictor_rhs()
     b.solve_direction()

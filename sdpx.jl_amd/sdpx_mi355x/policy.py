@@ -62,7 +62,7 @@ class SolverOptions:
     # iteration is redone in FP64 and FP64 is kept from then on.
     precision: str = "fp64"
     mixed_cond_limit: float = 1e6
-    bf16x3_cond_limit: float = 1e4
+    bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10
     mixed_max_refine: int = 32
     # KKT route: "dense_cholesky" or "block_arrow" (the exact block-arrow elimination of

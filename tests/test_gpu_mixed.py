@@ -141,3 +141,31 @@ def test_mixed_bf16x3_solve_matches_oracle(gpu_lib):
     cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
                                          res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
     assert cert["valid"], cert
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["random", "graded"])
+def test_condition_estimates_match_lapack(gpu_lib, case):
+    """sdpx_dense_factor_condition against LAPACK on the oracle's factor of the same S: dpocon's
+    cond_1(S) and the dtrcon product of _triangular_condition_estimate (mixed_precision_kkt.jl:371-392).
+    Higham's estimator takes the same steps as LAPACK's, so the two agree unless a rounding-level tie
+    changes a step: measured equal to 1e-12; the bar is 1e-8."""
+    m = 300
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((m, m)))
+    ev = np.exp(rng.uniform(0, np.log(1e3 if case == "random" else 1e9), m))
+    S = (Q * ev) @ Q.T
+    _, prob, _, X, Y = _state(8, m, 2)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_schur(S)
+        assert b.factorize() == (True, 0)
+        cs, ct, r1, ri = b.factor_condition()
+    finally:
+        b.close()
+    Lf, ok, _ = O.factor_dense_kkt(S)
+    cs_o, ct_o = O.schur_condition_estimate(S, Lf), O.triangular_condition_estimate(Lf)
+    print(f"{case}: cond_1(S) gpu {cs:.6e} lapack {cs_o:.6e} exact {np.linalg.cond(S, 1):.6e}; "
+          f"trcon gpu {ct:.6e} lapack {ct_o:.6e}", flush=True)
+    assert abs(cs - cs_o) <= 1e-8 * cs_o and abs(ct - ct_o) <= 1e-8 * ct_o
+    assert cs <= ct * (1 + 1e-12)

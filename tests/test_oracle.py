@@ -363,3 +363,25 @@ def test_equality_rank_gate():
     prop[:, 3] = -2.5 * prop[:, 0]
     assert O.factor_equality(Lf, prop, 1e-8)[0] in (1, 2)
     assert O.factor_equality(Lf, B, 1e-8)[0] == 0
+
+
+def test_condition_estimates_bounds_and_closed_forms():
+    """schur_condition_estimate (dpocon) and triangular_condition_estimate (dtrcon product): the exact
+    cond_1 of a diagonal S (max/min ratio) and of L = diag, a lower bound of the exact cond_1(S) within
+    the estimator's factor 3 on random S, and cond_1(S) <= cond_1(L) cond_inf(L)."""
+    d = np.array([4.0, 1.0, 0.25, 9.0])
+    S = np.diag(d)
+    L = np.diag(np.sqrt(d))
+    assert O.schur_condition_estimate(S, L) == pytest.approx(36.0, rel=1e-14)
+    assert O.triangular_condition_estimate(L) == pytest.approx(36.0, rel=1e-14)
+    rng = np.random.default_rng(3)
+    for n, cond in ((20, 1e2), (150, 1e6)):
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        S = (Q * np.geomspace(1.0, cond, n)) @ Q.T
+        Lf = np.linalg.cholesky(S)
+        exact = np.linalg.cond(S, 1)
+        est = O.schur_condition_estimate(S, Lf)
+        assert exact / 3 <= est <= exact * (1 + 1e-10)
+        prod = np.linalg.cond(Lf, 1) * np.linalg.cond(Lf, np.inf)
+        tr = O.triangular_condition_estimate(Lf)
+        assert tr <= prod * (1 + 1e-10) and exact <= prod * (1 + 1e-10)
