@@ -425,6 +425,9 @@ int sdpx_batch_adjoint(sdpx_batch_t* h, const double* G, const double* hv, doubl
  * iterate onto the central path, where the adjoint above is the exact
  * derivative of the central-path point. */
 int sdpx_batch_center(sdpx_batch_t* h, int iterations);
+/* The same Newton steps with mu fixed at mu[p] per member (NaN: its current <X,Y>/k); mu null
+ * is sdpx_batch_center.  Used to take the adjoint at two points of the central path. */
+int sdpx_batch_center_at(sdpx_batch_t* h, int iterations, const double* mu);
 /* Replace each member's iterate by its differentiation checkpoint (members
  * without one keep their iterate); *restored = members restored, mu[p] = the
  * checkpoint's <X, Y>/k (NaN without one).  The adjoint is taken at the

@@ -1891,13 +1891,23 @@ int sdpx_batch_restore_checkpoint(sdpx_batch_t* h, int* restored, double* mu) {
   return kOk;
 }
 
-int sdpx_batch_center(sdpx_batch_t* h, int iterations) {
+int sdpx_batch_center(sdpx_batch_t* h, int iterations) { return sdpx_batch_center_at(h, iterations, nullptr); }
+
+int sdpx_batch_center_at(sdpx_batch_t* h, int iterations, const double* target) {
   SDPX_CHECK(check_batch(h));
-  if (!h->have_data || !h->have_start) {
-    set_last_error("sdpx_batch_center: data and an iterate are required");
+  if (!h->have_data || !h->have_start || iterations < 0) {
+    set_last_error("sdpx_batch_center: data and an iterate are required, iterations >= 0");
     return kInvalidArgument;
   }
-  // mu fixed at its value on entry
+  if (target) {
+    for (int p = 0; p < h->B; ++p) {
+      if (!(std::isnan(target[p]) || (target[p] > 0.0 && std::isfinite(target[p])))) {
+        set_last_error("sdpx_batch_center_at: mu[" + std::to_string(p) + "] must be positive and finite, or NaN");
+        return kInvalidArgument;
+      }
+    }
+  }
+  // mu fixed at the member's target, or at its value on entry
   std::vector<int> all(h->B);
   for (int p = 0; p < h->B; ++p) {
     all[p] = p;
@@ -1907,7 +1917,8 @@ int sdpx_batch_center(sdpx_batch_t* h, int iterations) {
   SDPX_CHECK(launch_comp(h->arr, put_list(h, all), h->B, h->stream));
   SDPX_CHECK(pull_scalars(h));
   std::vector<double> mu(h->B);
-  for (int p = 0; p < h->B; ++p) mu[p] = sc(h, p, S_COMP) / h->mem[p].k;
+  for (int p = 0; p < h->B; ++p)
+    mu[p] = (target && !std::isnan(target[p])) ? target[p] : sc(h, p, S_COMP) / h->mem[p].k;
   for (int it = 0; it < iterations; ++it) {
     std::vector<int> ok;
     SDPX_CHECK(factor_at_iterate(h, 1.0, ok));

@@ -373,17 +373,29 @@ end
 """Backward pass of the layer at the solved batch: restore the differentiation checkpoints,
 centre them (`iterations` Newton steps at fixed mu) and run the adjoint for cotangents
 G (of X*) and hv (of y*).  Returns (dC, db) per member (NaN where no checkpoint)."""
-function batch_adjoint!(h::BatchHandle, Gs, hs; iterations::Integer=4)
+# The optimum's derivative as the mu -> 0 limit of the central-path adjoint: the checkpoint
+# re-centred at its own mu (D1) and at ratio * mu (D2), combined as (D2 - ratio D1) / (1 - ratio)
+function batch_adjoint!(h::BatchHandle, Gs, hs; iterations::Integer=6, ratio::Float64=0.1)
     B = length(h.k)
     restored = Ref{Cint}(0)
     mu = zeros(B)
     _check(ccall(_sym(:sdpx_batch_restore_checkpoint), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Float64}),
                  h.ptr, restored, mu), "sdpx_batch_restore_checkpoint")
+    G, hv = reduce(vcat, vec.(Gs)), reduce(vcat, hs)
+    function one_adjoint()
+        dC, db = zeros(sum(abs2, h.k)), zeros(sum(h.m))
+        _check(ccall(_sym(:sdpx_batch_adjoint), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                     h.ptr, G, hv, dC, db), "sdpx_batch_adjoint")
+        return dC, db
+    end
     _check(ccall(_sym(:sdpx_batch_center), Cint, (Ptr{Cvoid}, Cint), h.ptr, iterations), "sdpx_batch_center")
-    dC, db = zeros(sum(abs2, h.k)), zeros(sum(h.m))
-    _check(ccall(_sym(:sdpx_batch_adjoint), Cint,
-                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                 h.ptr, reduce(vcat, vec.(Gs)), reduce(vcat, hs), dC, db), "sdpx_batch_adjoint")
+    dC1, db1 = one_adjoint()
+    _check(ccall(_sym(:sdpx_batch_center_at), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}), h.ptr, iterations,
+                 ratio .* mu), "sdpx_batch_center_at")
+    dC2, db2 = one_adjoint()
+    dC = (dC2 .- ratio .* dC1) ./ (1 - ratio)
+    db = (db2 .- ratio .* db1) ./ (1 - ratio)
     dCs = [reshape(b, k, k) for (b, k) in zip(_split(dC, abs2.(h.k)), h.k)]
     dbs = _split(db, h.m)
     for p in 1:B

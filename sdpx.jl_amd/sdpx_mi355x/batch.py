@@ -225,6 +225,30 @@ class BatchSolver:
         """Newton steps on XY = mu I (mu fixed at entry) from the current iterate."""
         check(self._lib.sdpx_batch_center(self._h, int(iterations)), "sdpx_batch_center")
 
+    def center_at(self, mu, iterations: int = 4) -> None:
+        """Newton steps on XY = mu[p] I per member (NaN: the member's current <X,Y>/k)."""
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if mu.shape != (self.B,):
+            raise ValueError(f"center_at: mu needs {self.B} entries")
+        check(self._lib.sdpx_batch_center_at(self._h, int(iterations), dptr(mu)), "sdpx_batch_center_at")
+
+    def adjoint_limit(self, G_list, h_list, ratio: float = 0.1, iterations: int = 6):
+        """The optimum's derivative as the mu -> 0 limit of the central-path adjoint: from each member's
+        checkpoint (mu_ck), the adjoint D1 at the central-path point of mu_ck and D2 at ratio * mu_ck,
+        combined by one Richardson step for a derivative linear in mu, (D2 - ratio D1) / (1 - ratio).
+        Leaves the iterate at the second point.  Returns (dC list, db list, mu_ck)."""
+        if not 0.0 < ratio < 1.0:
+            raise ValueError("adjoint_limit: ratio must lie in (0, 1)")
+        _, mu = self.restore_checkpoint()
+        self.center(iterations)
+        dC1, db1 = self.adjoint(G_list, h_list)
+        self.center_at(np.where(np.isnan(mu), np.nan, ratio * mu), iterations)
+        dC2, db2 = self.adjoint(G_list, h_list)
+        w = 1.0 / (1.0 - ratio)
+        dC = [(b - ratio * a) * w for a, b in zip(dC1, dC2)]
+        db = [(b - ratio * a) * w for a, b in zip(db1, db2)]
+        return dC, db, mu
+
     def adjoint(self, G_list, h_list):
         """dL/dC and dL/db of L = sum_p <G_p, X*_p> + h_p'y*_p (user's dual-form model) at the current iterate.
 

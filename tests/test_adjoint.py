@@ -159,3 +159,101 @@ def test_central_path_gradient_approaches_the_optimum(p):
     assert 0.05 <= b_changes[1] / b_changes[0] <= 0.2 and 0.05 <= b_changes[2] / b_changes[1] <= 0.2
     assert b_changes[2] <= 1e-3
     assert c_changes[1] <= 3e-3 and c_changes[1] < c_changes[0]
+
+
+def _t1():
+    A = np.zeros((2, 2, 2))
+    A[0, 0, 0] = A[1, 1, 1] = 1.0
+    return O.ingest([2.0, 3.0], [A], [np.array([[0.0, 1.0], [1.0, 0.0]])])
+
+
+def test_optimum_adjoint_t1_closed_form():
+    """The optimum of T1 (test/correctness.jl:71-85) in closed form: x* = (sqrt(c2/c1), sqrt(c1/c2)) times
+    C12, x*_1 += C11, x*_2 += C22, Y* = [[c1, sqrt(c1 c2)], [., c2]] (complementary slackness); the
+    derivative of L = <G, Y*> + h'x* from these formulas against optimum_adjoint (the implicit function
+    theorem on the optimality conditions), to the accuracy of the solve (gap 1e-9)."""
+    prob = _t1()
+    r = O.solve(prob, O.Options(eps_gap=1e-9, eps_primal=1e-9, eps_dual=1e-9))
+    assert r.status == "Optimal"
+    rng = np.random.default_rng(0)
+    Gm, h = _unit_sym(rng, 2), _unit(rng, 2)
+    c1, c2 = 2.0, 3.0
+    x1, x2 = np.sqrt(c2 / c1), np.sqrt(c1 / c2)
+    dL_dc = np.array([Gm[0, 0] + Gm[0, 1] * np.sqrt(c2 / c1) - h[0] * x1 / (2 * c1) + h[1] * x2 / (2 * c1),
+                      Gm[1, 1] + Gm[0, 1] * np.sqrt(c1 / c2) + h[0] * x1 / (2 * c2) - h[1] * x2 / (2 * c2)])
+    hx = h[0] * x1 + h[1] * x2
+    dL_dC = np.array([[h[0], hx / 2], [hx / 2, h[1]]])
+    dC, db = AO.optimum_adjoint(prob, r.x, r.X[0], r.Y[0], Gm, h)
+    # the user's model: dL/dC_std = -dL/dC (core), dL/db = -dL/dc (core)
+    assert np.max(np.abs(db + dL_dc)) <= 1e-6 and np.max(np.abs(dC + dL_dC)) <= 1e-6
+    # a planted slip (Y's block on the null space of X transposed into X's range) is caught
+    bad = AO.optimum_adjoint(prob, r.x, r.Y[0], r.X[0], Gm, h)
+    assert bad is None or np.max(np.abs(bad[1] + dL_dc)) > 1e-3
+
+
+def test_central_path_adjoint_converges_linearly_to_the_optimum():
+    """On T1 the central-path adjoint approaches optimum_adjoint linearly in mu (one decade of mu, one
+    decade of distance), and one Richardson step removes the linear term."""
+    prob = _t1()
+    r = O.solve(prob, O.Options(eps_gap=1e-9, eps_primal=1e-9, eps_dual=1e-9))
+    rng = np.random.default_rng(1)
+    Gm, h = _unit_sym(rng, 2), _unit(rng, 2)
+    ref = AO.optimum_adjoint(prob, r.x, r.X[0], r.Y[0], Gm, h)
+    dist = []
+    for mu in (1e-2, 1e-3):
+        x, X, Y = AO.center(prob, r.x, r.X[0], r.Y[0], mu, iterations=20)
+        D = AO.adjoint(prob, x, X, Y, Gm, h)
+        dist.append(np.linalg.norm(D[0] - ref[0]) / np.linalg.norm(ref[0]))
+    assert 7 <= dist[0] / dist[1] <= 13
+    ex = AO.extrapolated_adjoint(prob, r.x, r.X[0], r.Y[0], Gm, h, 1e-2)
+    assert np.linalg.norm(ex[0] - ref[0]) / np.linalg.norm(ref[0]) <= 1e-2 * dist[1]
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_limit_matches_the_optimum(gpu_lib):
+    """The differentiable-layer shape (16 problems n=32, m=128, gap 1e-9): the device's adjoint_limit
+    (central-path adjoints at the checkpoint's mu ~ 1e-5 and at a tenth of it, one Richardson step)
+    against optimum_adjoint at the device's solution.  Measured ~1e-5 relative; the plain adjoint at the
+    checkpoint sits ~1e-3 away."""
+    from sdpx_mi355x.batch import BatchOptions, BatchSolver
+    from sdpx_mi355x.policy import SolverOptions
+
+    probs = G.batched_dense_layer(16, seed=11)
+    B = len(probs)
+    rng = np.random.default_rng(5)
+    Gs = [_unit_sym(rng, 32) for _ in range(B)]
+    hs = [_unit(rng, 128) for _ in range(B)]
+    tol = 1e-9
+    bo = BatchOptions.from_solver_options(SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol))
+    bo.adjoint_mu = 1e-5
+    with BatchSolver([32] * B, [128] * B) as bs:
+        bs.set_dense([g.Av_rows for g in probs], [g.C for g in probs], [g.c for g in probs])
+        bs.set_start([g.x0 for g in probs], [np.eye(32)] * B, [np.eye(32)] * B)
+        res = bs.solve(bo)
+        bs.restore_checkpoint()
+        bs.center(6)
+        dCp, dbp = bs.adjoint(Gs, hs)
+        dC, db, mu = bs.adjoint_limit(Gs, hs)
+    errs, plain = [], []
+    for p, g in enumerate(probs):
+        if res.status[p] != "Optimal" or np.isnan(mu[p]):
+            continue
+        prob = O.ingest(g.c, [g.A_blocks()], [g.C])
+        ref = AO.optimum_adjoint(prob, res.x[p], res.X[p], res.Y[p], Gs[p], hs[p])
+        if ref is None:
+            continue
+        eC = np.linalg.norm(dC[p] - ref[0]) / np.linalg.norm(ref[0])
+        eb = np.linalg.norm(db[p] - ref[1]) / np.linalg.norm(ref[1])
+        pC = np.linalg.norm(dCp[p] - ref[0]) / np.linalg.norm(ref[0])
+        # strict-complementarity margin: the smallest nonzero eigenvalue of X* and of Y*
+        ex, ey = np.linalg.eigvalsh(res.X[p]), np.linalg.eigvalsh(res.Y[p])
+        margin = min(ex[ex > 1e-5 * ex[-1]].min(), ey[ey > 1e-5 * ey[-1]].min())
+        print(f"problem {p}: mu_ck {mu[p]:.1e}, margin {margin:.1e}: limit dC {eC:.2e} db {eb:.2e} | plain dC "
+              f"{pC:.2e}", flush=True)
+        errs.append(max(eC, eb))
+        plain.append(pC)
+    errs, plain = np.array(errs), np.array(plain)
+    # the limit beats the plain adjoint on every member; near-degenerate members (small margin) converge
+    # more slowly in mu, so the bar is on the median and the bulk
+    assert len(errs) >= 12 and np.all(errs < plain)
+    assert np.median(errs) <= 1e-5 and np.mean(errs <= 3e-5) >= 0.75
