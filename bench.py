@@ -246,6 +246,7 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
         dev_gb = bs.memory_bytes() / 1e9
         bs.set_start([g.x0 for g in probs], eyes, eyes)
         bs.solve(o)  # warm-up
+        time.sleep(0.2)  # an idle gap that separates the timed solves in a kernel trace
         wall, res = 0.0, None
         for _ in range(steps):
             bs.set_start([g.x0 for g in probs], eyes, eyes)
