@@ -518,6 +518,36 @@ int sdpx_soc_hkm_metric(sdpx_soc_t* h, int primal, int dual, int dst, int* nfail
  * subtracts the affine product of slots affine_primal / affine_dual. */
 int sdpx_soc_hkm_contract(sdpx_soc_t* h, int mode, int primal, int dual, int v, int affine_primal,
                           int affine_dual, double target, int include_affine, int dst);
+/* ---- LA provider seam, host-buffer operations (la_backend.jl:1529-1769) -------
+ * The la_* methods of the provider's LA backend on caller-owned host arrays
+ * (Float64, column-major, leading dimensions), computed on the device: each call
+ * uploads its operands to the handle's scratch, runs the device kernels and
+ * writes the result back before returning.  One handle per LA backend, called
+ * sequentially.  Status 1 = not positive definite (la_chol! returning false). */
+typedef struct sdpx_la sdpx_la_t;
+int sdpx_la_create(int device, sdpx_la_t** out);
+int sdpx_la_destroy(sdpx_la_t* h);
+/* la_chol! (la_backend.jl:1612-1622): cholesky!(Symmetric(A, :L)) in place; the lower triangle
+ * becomes L, the upper one is kept.  Returns 1 (A untouched) when A is not positive definite or
+ * a lower entry of A or L is not finite. */
+int sdpx_la_chol(sdpx_la_t* h, int n, double* A, long long lda);
+/* la_trsm! (:1634): X <- L^-1 X, L lower n x n (non-unit), X n x nrhs. */
+int sdpx_la_trsm(sdpx_la_t* h, int n, int nrhs, const double* L, long long ldl, double* X, long long ldx);
+/* la_trsv_lower! / la_trsv_transpose! (:1642-1653): x <- L^-1 x (trans 0) or L^-T x (trans 1). */
+int sdpx_la_trsv(sdpx_la_t* h, int n, const double* L, long long ldl, double* x, int trans);
+/* la_syrk! (:1595-1604): BLAS syrk('L', 'T'), the lower triangle of S (n x n) <- alpha P'P +
+ * beta S with P k x n; the upper triangle is kept. */
+int sdpx_la_syrk(sdpx_la_t* h, int n, int k, double alpha, const double* P, long long ldp, double beta, double* S,
+                 long long lds);
+/* la_mul_owned! (:1556-1577): C <- alpha op(A) op(B) + beta C (op = transpose when ta / tb). */
+int sdpx_la_gemm(sdpx_la_t* h, int ta, int tb, int m, int n, int k, double alpha, const double* A, long long lda,
+                 const double* B, long long ldb, double beta, double* C, long long ldc);
+/* la_axpby_owned! (:1659-1686): y <- alpha x + beta y. */
+int sdpx_la_axpby(sdpx_la_t* h, long long n, double alpha, const double* x, double beta, double* y);
+/* la_dot, la_norminf (:1529-1541); an empty vector gives 0. */
+int sdpx_la_dot(sdpx_la_t* h, long long n, const double* x, const double* y, double* out);
+int sdpx_la_norminf(sdpx_la_t* h, long long n, const double* x, double* out);
+
 /* ---- SOC metric assembly: the dense branch of _native_soc_add_metric!
  * (soc_native.jl:1119-1218) over every cone at once.  A: (sum_c q_c) x n
  * column-major (cone c's coefficient rows A_c, q_c x n, at its coordinate

@@ -205,20 +205,129 @@ end
 instantiate_mi355x_la_backend(::Type, ::SDPX.LABackendConfiguration, ::Int=1) = nothing
 
 """The instantiated :mi355x LA backend.  On the dense Schur route its operations run inside
-MI355XDenseBackend on the device (ws.S, its factor and the directions never leave HBM), so the
-host-buffer la_* entry points are fail-closed here (la_backend.jl:1695-1745)."""
-struct MI355XLABackend <: SDPX.AbstractLABackend
+MI355XDenseBackend on the device (ws.S, its factor and the directions never leave HBM); the
+host-buffer la_* methods below serve every other caller, each computed on the device through
+the sdpx_la_* entry points (operands uploaded, result written back, la_backend.jl:1529-1769)."""
+mutable struct MI355XLABackend <: SDPX.AbstractLABackend
     arithmetic::Symbol
     provider::MI355XProvider
+    handle::Ptr{Cvoid}
 end
+MI355XLABackend(arithmetic::Symbol, provider::MI355XProvider) = MI355XLABackend(arithmetic, provider, C_NULL)
 
 SDPX.la_backend_capabilities(b::MI355XLABackend) = _CAPABILITY_MODEL
 
+function _la(b::MI355XLABackend)
+    if b.handle == C_NULL
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        _check(ccall(_sym(:sdpx_la_create), Cint, (Cint, Ref{Ptr{Cvoid}}), b.provider.device, out), "sdpx_la_create")
+        b.handle = out[]
+        finalizer(b) do x
+            x.handle == C_NULL || ccall(_sym(:sdpx_la_destroy), Cint, (Ptr{Cvoid},), x.handle)
+            x.handle = C_NULL
+        end
+    end
+    return b.handle
+end
+
+# Strided column-major Float64 storage: (pointer, leading dimension)
+function _colmajor(A::StridedMatrix{Float64})
+    stride(A, 1) == 1 || throw(ArgumentError("MI355X provider: column-major operands with unit row stride only"))
+    return pointer(A), max(stride(A, 2), size(A, 1), 1)
+end
+_colmajor(A::StridedVector{Float64}) = (stride(A, 1) == 1 ? pointer(A) :
+    throw(ArgumentError("MI355X provider: contiguous vectors only")), max(length(A), 1))
+
+function SDPX.la_chol!(b::MI355XLABackend, A::StridedMatrix{Float64})
+    p, ld = _colmajor(A)
+    status = GC.@preserve A _check(ccall(_sym(:sdpx_la_chol), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Clonglong),
+                                        _la(b), size(A, 1), p, ld), "sdpx_la_chol")
+    return status == 0
+end
+
+SDPX.la_cholesky_factor!(b::MI355XLABackend, A::StridedMatrix{Float64}) =
+    SDPX.la_chol!(b, A) ? SDPX.ProviderLACholeskyFactor(b.provider, A) : nothing
+
+function SDPX.la_trsm!(b::MI355XLABackend, L::StridedMatrix{Float64}, X::StridedVecOrMat{Float64})
+    pl, ldl = _colmajor(L)
+    px, ldx = X isa StridedVector ? (_colmajor(X)[1], max(length(X), 1)) : _colmajor(X)
+    GC.@preserve L X _check(ccall(_sym(:sdpx_la_trsm), Cint,
+                                  (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Clonglong, Ptr{Float64}, Clonglong),
+                                  _la(b), size(L, 1), size(X, 2), pl, ldl, px, ldx), "sdpx_la_trsm")
+    return X
+end
+
+function _la_trsv!(b::MI355XLABackend, L::StridedMatrix{Float64}, x::StridedVector{Float64}, trans::Integer)
+    pl, ldl = _colmajor(L)
+    px, _ = _colmajor(x)
+    GC.@preserve L x _check(ccall(_sym(:sdpx_la_trsv), Cint,
+                                  (Ptr{Cvoid}, Cint, Ptr{Float64}, Clonglong, Ptr{Float64}, Cint),
+                                  _la(b), size(L, 1), pl, ldl, px, trans), "sdpx_la_trsv")
+    return x
+end
+SDPX.la_trsv_lower!(b::MI355XLABackend, L, x) = _la_trsv!(b, L, x, 0)
+SDPX.la_trsv_transpose!(b::MI355XLABackend, L, x) = _la_trsv!(b, L, x, 1)
+
+function SDPX.la_syrk!(b::MI355XLABackend, S::StridedMatrix{Float64}, P::StridedMatrix{Float64}, α, β)
+    ps, lds = _colmajor(S)
+    pp, ldp = _colmajor(P)
+    GC.@preserve S P _check(ccall(_sym(:sdpx_la_syrk), Cint,
+                                  (Ptr{Cvoid}, Cint, Cint, Float64, Ptr{Float64}, Clonglong, Float64, Ptr{Float64},
+                                   Clonglong),
+                                  _la(b), size(S, 1), size(P, 1), Float64(α), pp, ldp, Float64(β), ps, lds),
+                            "sdpx_la_syrk")
+    return S
+end
+
+# op(A): a Transpose / Adjoint wrapper of a real matrix is passed as its parent with the flag set
+_op(A::Union{Transpose{Float64,<:StridedMatrix{Float64}},Adjoint{Float64,<:StridedMatrix{Float64}}}) = (parent(A), 1)
+_op(A::StridedMatrix{Float64}) = (A, 0)
+
+function SDPX.la_mul_owned!(b::MI355XLABackend, C::StridedMatrix{Float64}, A, B, α=1.0, β=0.0)
+    Ap, ta = _op(A)
+    Bp, tb = _op(B)
+    pa, lda = _colmajor(Ap)
+    pb, ldb = _colmajor(Bp)
+    pc, ldc = _colmajor(C)
+    m, n = size(C)
+    k = size(A, 2)
+    GC.@preserve Ap Bp C _check(ccall(_sym(:sdpx_la_gemm), Cint,
+                                      (Ptr{Cvoid}, Cint, Cint, Cint, Cint, Cint, Float64, Ptr{Float64}, Clonglong,
+                                       Ptr{Float64}, Clonglong, Float64, Ptr{Float64}, Clonglong),
+                                      _la(b), ta, tb, m, n, k, Float64(α), pa, lda, pb, ldb, Float64(β), pc, ldc),
+                                "sdpx_la_gemm")
+    return C
+end
+
+function SDPX.la_axpby_owned!(b::MI355XLABackend, α, X::StridedArray{Float64}, β, Y::StridedArray{Float64})
+    size(X) == size(Y) || throw(DimensionMismatch("la_axpby_owned!: X and Y differ in size"))
+    x, y = vec(X), vec(Y)
+    GC.@preserve x y _check(ccall(_sym(:sdpx_la_axpby), Cint,
+                                  (Ptr{Cvoid}, Clonglong, Float64, Ptr{Float64}, Float64, Ptr{Float64}),
+                                  _la(b), length(y), Float64(α), x, Float64(β), y), "sdpx_la_axpby")
+    return Y
+end
+
+function SDPX.la_dot(b::MI355XLABackend, x::StridedVector{Float64}, y::StridedVector{Float64})
+    out = Ref{Float64}(0.0)
+    GC.@preserve x y _check(ccall(_sym(:sdpx_la_dot), Cint, (Ptr{Cvoid}, Clonglong, Ptr{Float64}, Ptr{Float64},
+                                                             Ref{Float64}),
+                                  _la(b), length(x), x, y, out), "sdpx_la_dot")
+    return out[]
+end
+
+function SDPX.la_norminf(b::MI355XLABackend, x::StridedVector{Float64})
+    out = Ref{Float64}(0.0)
+    GC.@preserve x _check(ccall(_sym(:sdpx_la_norminf), Cint, (Ptr{Cvoid}, Clonglong, Ptr{Float64}, Ref{Float64}),
+                                _la(b), length(x), x, out), "sdpx_la_norminf")
+    return out[]
+end
+
+# any other operand type (BigFloat, non-strided views) stays fail-closed (la_backend.jl:1695-1745)
 for op in (:la_chol!, :la_cholesky_factor!, :la_trsm!, :la_trsv_lower!, :la_trsv_transpose!, :la_syrk!,
            :la_mul_owned!, :la_axpby_owned!, :la_dot, :la_norminf)
     @eval SDPX.$op(::MI355XLABackend, args...) = throw(ArgumentError(
-        "MI355X provider: $($(QuoteNode(op))) on host buffers is not provided; the dense Schur route runs it " *
-        "on the device inside MI355XDenseBackend"))
+        "MI355X provider: $($(QuoteNode(op))) takes Float64 strided column-major operands only"))
 end
 
 # ---------------------------------------------------------------------------
@@ -230,7 +339,6 @@ set_precision!(h::DenseHandle, precision::Integer) =
     _check(ccall(_sym(:sdpx_dense_set_precision), Cint, (Ptr{Cvoid}, Cint), h.ptr, precision),
            "sdpx_dense_set_precision")
 
-"""rho = r - S dx against the matrix-free FP64 Schur operator -> (||rho||_inf, ||r||_inf)."""
 # Condition estimates from the last Schur factor: cond_1(S) (xPOCON), the xTRCON product of
 # _triangular_condition_estimate (mixed_precision_kkt.jl:371-392), rcond_1(L), rcond_inf(L)
 function factor_condition(h::DenseHandle; which::Integer=3)
@@ -240,6 +348,7 @@ function factor_condition(h::DenseHandle; which::Integer=3)
     return (cond_schur=out[1], cond_trcon=out[2], rcond_1=out[3], rcond_inf=out[4])
 end
 
+"""rho = r - S dx against the matrix-free FP64 Schur operator -> (||rho||_inf, ||r||_inf)."""
 function exact_residual(h::DenseHandle)
     out = zeros(2)
     _check(ccall(_sym(:sdpx_dense_exact_residual), Cint, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, out),

@@ -26,6 +26,17 @@ SIGNATURES = {
     "sdpx_dense_set_block": [_h, C.c_int, _dp, _dp],
     "sdpx_dense_set_block_sparse": [_h, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, _dp],
     "sdpx_dense_set_arrow": [_h, _ip],
+    "sdpx_la_create": [C.c_int, C.POINTER(_h)],
+    "sdpx_la_destroy": [_h],
+    "sdpx_la_chol": [_h, C.c_int, _dp, C.c_longlong],
+    "sdpx_la_trsm": [_h, C.c_int, C.c_int, _dp, C.c_longlong, _dp, C.c_longlong],
+    "sdpx_la_trsv": [_h, C.c_int, _dp, C.c_longlong, _dp, C.c_int],
+    "sdpx_la_syrk": [_h, C.c_int, C.c_int, C.c_double, _dp, C.c_longlong, C.c_double, _dp, C.c_longlong],
+    "sdpx_la_gemm": [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_longlong, _dp,
+                     C.c_longlong, C.c_double, _dp, C.c_longlong],
+    "sdpx_la_axpby": [_h, C.c_longlong, C.c_double, _dp, C.c_double, _dp],
+    "sdpx_la_dot": [_h, C.c_longlong, _dp, _dp, _dp],
+    "sdpx_la_norminf": [_h, C.c_longlong, _dp, _dp],
     "sdpx_dense_factor_condition": [_h, C.c_int, _dp],
     "sdpx_dense_get_arrow": [_h, _ip, _ip, _ip],
     "sdpx_dense_set_objective": [_h, _dp],
