@@ -139,7 +139,8 @@ int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long l
              double* ws, const int2* tile_order, hipStream_t stream, hipEvent_t after_main = nullptr);
 // BF16x3 Schur Gram product (gemm_f32.hip): S (lower, FP64) = P' P + beta S
 // with P ~= H + L given as two bf16 planes (ld ldp, Kp rows, both multiples of
-// 32, padding rows zero), three bf16 MFMA products H'H + H'L + L'H per step.
+// kBf16Align, padding rows zero), three bf16 MFMA products H'H + H'L + L'H per step.
+constexpr int kBf16Align = 64;  // the deepest K slice of the kernel divides it
 int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, double* S, long long lds, double beta,
                 int split, double* ws, const int2* tile_order, hipStream_t stream, hipEvent_t after_main = nullptr);
 // H = bf16(src), L = bf16(src - H) (ld ld_dst, rows >= `rows` zeroed), rows x cols

@@ -1538,7 +1538,7 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     if (h->precision == 2) {
       // mixed precision: Phat split into bf16 hi + lo planes, Gram product as three
       // bf16 MFMA products accumulated in FP32, S kept in FP64
-      const long long kp = (b.k2 + 31) / 32 * 32;
+      const long long kp = (b.k2 + kBf16Align - 1) / kBf16Align * kBf16Align;
       SDPX_CHECK(split_panel_bf16(h->Ph, b.k2, h->Ph_hi, h->Ph_lo, h->ldPb, b.k2, m, h->stream));
       if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
       SDPX_CHECK(gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, beta0, h->split,
@@ -1755,7 +1755,7 @@ int sdpx_dense_autotune(sdpx_dense_t* h, double* out) {
       h->tune_panel_ms += best;
     }
     // Gram product in the handle's precision: split-K factor and tile order
-    const long long kp = (b.k2 + 31) / 32 * 32;
+    const long long kp = (b.k2 + kBf16Align - 1) / kBf16Align * kBf16Align;
     if (h->precision == 2)
       SDPX_CHECK(split_panel_bf16(h->Ph, b.k2, h->Ph_hi, h->Ph_lo, h->ldPb, b.k2, m, h->stream));
     else if (h->precision == 1)
@@ -1994,7 +1994,7 @@ int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
     if (s) return s;
   }
   if (precision == 2 && !h->Ph_hi) {
-    h->ldPb = (h->maxk2 + 31) / 32 * 32;
+    h->ldPb = (h->maxk2 + kBf16Align - 1) / kBf16Align * kBf16Align;
     int s = dalloc(h, &h->Ph_hi, (size_t)(h->ldPb * h->m));
     if (!s) s = dalloc(h, &h->Ph_lo, (size_t)(h->ldPb * h->m));
     if (s) return s;

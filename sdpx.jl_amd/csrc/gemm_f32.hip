@@ -183,8 +183,27 @@ __global__ void panel_f32_kernel(const double* __restrict__ src, long long ld_sr
 }
 
 // ---- BF16x3 ------------------------------------------------------------------
-constexpr int BKB = 32;       // K per slice: one bf16 MFMA step
-constexpr int LDB = BKB + 8;  // LDS row pitch in bf16 (80 bytes: 16-byte aligned, 2-way banks)
+// K per LDS slice (32 or 64) and the slice rows' LDS layout: rows hold KB / 8 16-byte chunks
+// of 8 bf16; with kBf16Swizzle < 0 the pitch is KB + 8 bf16 and chunks stay in place (KB = 32
+// only: the padded 64-deep tiles exceed 64 KiB of LDS), otherwise the pitch is KB and chunk c of
+// row r sits at chunk c ^ ((r >> kBf16Swizzle) & (KB / 8 - 1)).
+// Measured at k = 200, m = 4000 (10 launches): 25.2 ms for 32-deep slices (either layout),
+// 21.1 ms for 64-deep slices with swizzle 0 (21.3 with 1, 25.4 with 2): the deeper slice
+// doubles the MFMA work between the prefetch and its use, which hides more of the load latency.
+#ifndef SDPX_BF16_KB
+#define SDPX_BF16_KB 64
+#endif
+#ifndef SDPX_BF16_SWIZZLE
+#define SDPX_BF16_SWIZZLE 0
+#endif
+constexpr int BKB = SDPX_BF16_KB;
+constexpr int kBf16Swizzle = SDPX_BF16_SWIZZLE;
+static_assert(BKB == 32 || (BKB == 64 && kBf16Swizzle >= 0), "64-deep slices need the swizzled layout");
+constexpr int kChunks = BKB / 8;
+constexpr int LDB = kBf16Swizzle < 0 ? BKB + 8 : BKB;
+__device__ __forceinline__ int bf16_col(int row, int chunk) {
+  return 8 * (kBf16Swizzle < 0 ? chunk : (chunk ^ ((row >> (kBf16Swizzle < 0 ? 0 : kBf16Swizzle)) & (kChunks - 1))));
+}
 
 // H = bf16(p), L = bf16(p - H) for every entry; rows >= rows of each column are zero
 __global__ void split_bf16_kernel(const double* __restrict__ src, long long ld_src, __bf16* __restrict__ H,
@@ -200,19 +219,21 @@ __global__ void split_bf16_kernel(const double* __restrict__ src, long long ld_s
 }
 
 // One 128 x 128 lower tile of S (or of a split-K partial).  256 threads =
-// 2 x 2 waves of 64 x 64 (4 x 4 blocks of 16 x 16); the next 32-deep slice of
+// 2 x 2 waves of 64 x 64 (4 x 4 blocks of 16 x 16); the next BKB-deep slice of
 // both planes of both operand tiles is prefetched into registers while the
-// current one, staged in LDS, feeds 48 MFMAs per wave.  Operand layout of
-// v_mfma_f32_16x16x32_bf16: lane l holds rows (l % 16) with K entries
+// current one, staged in LDS, feeds 48 MFMAs per wave per 32 of K.  Operand
+// layout of v_mfma_f32_16x16x32_bf16: lane l holds rows (l % 16) with K entries
 // 8 (l / 16) .. +7 of both operands (one 16-byte LDS read each); the
 // accumulator holds D[4 (l / 16) + r][l % 16].  Operands are fed (B, A) so the
 // accumulator holds C' and stores are contiguous.  The planes are zero padded
-// to a multiple of 32 rows, so no slice needs masking.
+// to a multiple of BKB rows, so no slice needs masking.
 __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16* __restrict__ H,
                                                                    const __bf16* __restrict__ L, long long ldp,
                                                                    int m, int Kp, double* S, long long lds,
                                                                    double beta, int split, double* ws,
                                                                    const int2* tile_order) {
+  constexpr int RPT = kChunks / 2;      // slice rows each thread loads per operand plane
+  constexpr int ROW_STEP = 256 / kChunks;
   __shared__ __align__(16) __bf16 As[2][BM][LDB];
   __shared__ __align__(16) __bf16 Bs[2][BM][LDB];
   const int tid = threadIdx.x;
@@ -248,10 +269,17 @@ __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16*
     kEnd = min(Kp, kBegin + per * BKB);
   }
 
-  // thread loads: rows lr and lr + 64 of each operand tile, K entries kc .. kc + 7, both planes
-  const int lr = tid >> 2, kc = (tid & 3) * 8;
-  const long long ra0 = (long long)min(m0 + lr, m - 1) * ldp, ra1 = (long long)min(m0 + lr + 64, m - 1) * ldp;
-  const long long rb0 = (long long)min(n0 + lr, m - 1) * ldp, rb1 = (long long)min(n0 + lr + 64, m - 1) * ldp;
+  // thread loads: slice rows lr + ROW_STEP t of each operand tile, K entries kc .. kc + 7, both planes
+  const int lr = tid / kChunks, kc = (tid % kChunks) * 8;
+  long long ra[RPT], rb[RPT];
+  int sc[RPT];
+#pragma unroll
+  for (int t = 0; t < RPT; ++t) {
+    const int r = lr + ROW_STEP * t;
+    ra[t] = (long long)min(m0 + r, m - 1) * ldp;
+    rb[t] = (long long)min(n0 + r, m - 1) * ldp;
+    sc[t] = bf16_col(r, kc / 8);
+  }
 
   float4_t acc[4][4];
 #pragma unroll
@@ -259,40 +287,41 @@ __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16*
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
 
-  bf16x8_t v[8];
+  bf16x8_t v[4 * RPT];
   auto load = [&](int k0) {
     const long long o = k0 + kc;
-    v[0] = *reinterpret_cast<const bf16x8_t*>(H + ra0 + o);
-    v[1] = *reinterpret_cast<const bf16x8_t*>(H + ra1 + o);
-    v[2] = *reinterpret_cast<const bf16x8_t*>(L + ra0 + o);
-    v[3] = *reinterpret_cast<const bf16x8_t*>(L + ra1 + o);
-    v[4] = *reinterpret_cast<const bf16x8_t*>(H + rb0 + o);
-    v[5] = *reinterpret_cast<const bf16x8_t*>(H + rb1 + o);
-    v[6] = *reinterpret_cast<const bf16x8_t*>(L + rb0 + o);
-    v[7] = *reinterpret_cast<const bf16x8_t*>(L + rb1 + o);
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+      v[4 * t + 0] = *reinterpret_cast<const bf16x8_t*>(H + ra[t] + o);
+      v[4 * t + 1] = *reinterpret_cast<const bf16x8_t*>(L + ra[t] + o);
+      v[4 * t + 2] = *reinterpret_cast<const bf16x8_t*>(H + rb[t] + o);
+      v[4 * t + 3] = *reinterpret_cast<const bf16x8_t*>(L + rb[t] + o);
+    }
   };
   auto store = [&]() {
-    *reinterpret_cast<bf16x8_t*>(&As[0][lr][kc]) = v[0];
-    *reinterpret_cast<bf16x8_t*>(&As[0][lr + 64][kc]) = v[1];
-    *reinterpret_cast<bf16x8_t*>(&As[1][lr][kc]) = v[2];
-    *reinterpret_cast<bf16x8_t*>(&As[1][lr + 64][kc]) = v[3];
-    *reinterpret_cast<bf16x8_t*>(&Bs[0][lr][kc]) = v[4];
-    *reinterpret_cast<bf16x8_t*>(&Bs[0][lr + 64][kc]) = v[5];
-    *reinterpret_cast<bf16x8_t*>(&Bs[1][lr][kc]) = v[6];
-    *reinterpret_cast<bf16x8_t*>(&Bs[1][lr + 64][kc]) = v[7];
+#pragma unroll
+    for (int t = 0; t < RPT; ++t) {
+      const int r = lr + ROW_STEP * t;
+      *reinterpret_cast<bf16x8_t*>(&As[0][r][sc[t]]) = v[4 * t + 0];
+      *reinterpret_cast<bf16x8_t*>(&As[1][r][sc[t]]) = v[4 * t + 1];
+      *reinterpret_cast<bf16x8_t*>(&Bs[0][r][sc[t]]) = v[4 * t + 2];
+      *reinterpret_cast<bf16x8_t*>(&Bs[1][r][sc[t]]) = v[4 * t + 3];
+    }
   };
-  auto compute = [&]() {
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
+  auto compute = [&](int ks) {
+    const int fr = lane & 15, fq = 4 * ks + (lane >> 4);
     bf16x8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      ah[i] = *reinterpret_cast<const bf16x8_t*>(&As[0][wm0 + 16 * i + fr][fk]);
-      al[i] = *reinterpret_cast<const bf16x8_t*>(&As[1][wm0 + 16 * i + fr][fk]);
+      const int r = wm0 + 16 * i + fr, c = bf16_col(r, fq);
+      ah[i] = *reinterpret_cast<const bf16x8_t*>(&As[0][r][c]);
+      al[i] = *reinterpret_cast<const bf16x8_t*>(&As[1][r][c]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][wn0 + 16 * j + fr][fk]);
-      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][wn0 + 16 * j + fr][fk]);
+      const int r = wn0 + 16 * j + fr, c = bf16_col(r, fq);
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[0][r][c]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[1][r][c]);
     }
     // the three products of one accumulator are issued 16 MFMAs apart, so that
     // consecutive MFMAs never depend on each other (same order per accumulator)
@@ -315,7 +344,8 @@ __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16*
       store();
       __syncthreads();
       if (k0 + BKB < kEnd) load(k0 + BKB);
-      compute();
+#pragma unroll
+      for (int ks = 0; ks < BKB / 32; ++ks) compute(ks);
       __syncthreads();
     }
   }
@@ -380,7 +410,7 @@ int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, 
   if (m <= 0) return kOk;
   if (ldp % BKB != 0 || Kp % BKB != 0 || Kp > ldp || (reinterpret_cast<uintptr_t>(H) & 15) != 0 ||
       (reinterpret_cast<uintptr_t>(L) & 15) != 0) {
-    set_last_error("gram_bf16x3: the bf16 planes need 16-byte aligned bases and K, ld multiples of 32");
+    set_last_error("gram_bf16x3: the bf16 planes need 16-byte aligned bases and K, ld multiples of the slice depth");
     return kInvalidArgument;
   }
   if (split < 1) split = 1;
