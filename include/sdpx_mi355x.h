@@ -434,6 +434,16 @@ int sdpx_batch_center_at(sdpx_batch_t* h, int iterations, const double* mu);
  * checkpoint re-centred by sdpx_batch_center: the final iterate of a solve is
  * too far from the central path, relative to its mu, for the linearisation. */
 int sdpx_batch_restore_checkpoint(sdpx_batch_t* h, int* restored, double* mu);
+/* Plan-time autotuner of the batched route (no counterpart in the reference): the
+ * grouped Schur Gram launch timed at the start point (set_start) under each
+ * visiting order of its tiles -- 0 the largest K = k^2 first, 1 member-major, 2
+ * members dealt to the 8 XCDs by work so that a member's tiles share one L2 --
+ * `repeats` launches each after a warm-up; the fastest order is kept.  out (4) =
+ * chosen order, median milliseconds of orders 0, 1, 2.  The Schur matrices of
+ * the start point are left in the handle. */
+int sdpx_batch_autotune(sdpx_batch_t* h, int repeats, double* out);
+/* Re-applies a saved order (0, 1 or 2). */
+int sdpx_batch_set_gram_order(sdpx_batch_t* h, int mode);
 /* Schur matrix of one member (m_p x m_p, lower triangle authoritative). */
 int sdpx_batch_get_schur(sdpx_batch_t* h, int member, double* S);
 /* Schur assembly of every member at the uploaded start point (parity hook of

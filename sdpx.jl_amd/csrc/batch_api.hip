@@ -397,6 +397,8 @@ struct sdpx_batch {
   GroupGemm* dgram = nullptr;
   GroupTile* dgram_tiles = nullptr;
   int ngram_tiles = 0;
+  std::vector<GroupTile> gram_tiles;  // member-major tile list (the autotuner's candidates reorder it)
+  int gram_order = 0;                 // 0 K-descending, 1 member-major, 2 XCD bins (sdpx_batch_autotune)
   // grouped Schur Cholesky: per 64-column step, the members it touches and the
   // panel / trailing GEMM tiles
   CholDesc* dchol = nullptr;
@@ -620,6 +622,67 @@ int pull_scalars(sdpx_batch_t* h, bool ints = false) {
 inline double& par(sdpx_batch_t* h, int p, int slot) { return h->hpar[(size_t)p * NPARAM + slot]; }
 inline double sc(sdpx_batch_t* h, int p, int slot) { return h->hsc[(size_t)p * NSLOT + slot]; }
 inline int si(sdpx_batch_t* h, int p, int slot) { return h->hsi[(size_t)p * NISLOT + slot]; }
+
+// Visiting orders of the grouped Schur Gram launch (work-groups are dispatched round-robin over
+// the 8 XCDs in launch order):
+//   0  tiles of members with the largest K = k^2 first (stable: member order, then tile order);
+//   1  member-major (the creation order);
+//   2  members dealt to 8 XCD bins, largest work first (work = tiles x K, each member to the
+//      lightest bin), every bin in K-descending order, bins interleaved so that launch position
+//      8 j + x holds the j-th tile of bin x while every bin lasts: a member's tiles, which share
+//      its P-hat panels, run on one XCD and its L2.
+std::vector<GroupTile> gram_tile_order(const sdpx_batch_t* h, int mode) {
+  std::vector<GroupTile> t = h->gram_tiles;
+  auto K = [&](int p) { return (long long)h->mem[p].k * h->mem[p].k; };
+  if (mode == 0) {
+    std::stable_sort(t.begin(), t.end(), [&](const GroupTile& x, const GroupTile& y) { return K(x.g) > K(y.g); });
+    return t;
+  }
+  if (mode == 1) return t;
+  const int B = h->B;
+  std::vector<long long> ntile(B, 0);
+  for (const GroupTile& x : t) ntile[x.g] += 1;
+  std::vector<int> order(B);
+  for (int p = 0; p < B; ++p) order[p] = p;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ntile[x] * K(x) > ntile[y] * K(y); });
+  std::vector<long long> load(kNumXcd, 0);
+  std::vector<int> bin_of(B, 0);
+  for (int p : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    bin_of[p] = x;
+    load[x] += ntile[p] * K(p);
+  }
+  std::stable_sort(t.begin(), t.end(), [&](const GroupTile& x, const GroupTile& y) { return K(x.g) > K(y.g); });
+  std::vector<std::vector<GroupTile>> bins(kNumXcd);
+  for (const GroupTile& x : t) bins[bin_of[x.g]].push_back(x);
+  std::vector<GroupTile> out;
+  out.reserve(t.size());
+  std::vector<size_t> pos(kNumXcd, 0);
+  while (out.size() < t.size()) {
+    for (int x = 0; x < kNumXcd; ++x) {
+      int src = x;
+      if (pos[src] >= bins[src].size()) {  // an exhausted bin's slot takes the fullest remaining bin's tile
+        size_t best = 0;
+        for (int y = 0; y < kNumXcd; ++y)
+          if (bins[y].size() - pos[y] > best) {
+            best = bins[y].size() - pos[y];
+            src = y;
+          }
+        if (best == 0) break;
+      }
+      out.push_back(bins[src][pos[src]++]);
+    }
+  }
+  return out;
+}
+
+int set_gram_order(sdpx_batch_t* h, int mode) {
+  const std::vector<GroupTile> t = gram_tile_order(h, mode);
+  SDPX_HIP(hipStreamSynchronize(h->stream));  // no launch in flight reads the old list
+  SDPX_HIP(hipMemcpy(h->dgram_tiles, t.data(), sizeof(GroupTile) * t.size(), hipMemcpyHostToDevice));
+  h->gram_order = mode;
+  return kOk;
+}
 
 // I_SKIP = 0 for the listed members, 1 for every other member
 int set_active(sdpx_batch_t* h, const std::vector<int>& l) {
@@ -1386,9 +1449,8 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
       for (int ti = 0; ti < t; ++ti)
         for (int tj = 0; tj <= ti; ++tj) tiles.push_back(GroupTile{p, ti, tj, 0});
     }
-    std::stable_sort(tiles.begin(), tiles.end(), [&](const GroupTile& x, const GroupTile& y) {
-      return k[x.g] > k[y.g];
-    });
+    h->gram_tiles = tiles;
+    tiles = gram_tile_order(h, 0);
     h->ngram_tiles = (int)tiles.size();
     if ((s = balloc_n(h, &h->dgram, B))) return fail(s);
     if ((s = balloc_n(h, &h->dgram_tiles, h->ngram_tiles))) return fail(s);
@@ -2036,6 +2098,60 @@ int sdpx_batch_schur_at_start(sdpx_batch_t* h) {
     }
   SDPX_CHECK(schur_build(h, all));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_batch_set_gram_order(sdpx_batch_t* h, int mode) {
+  SDPX_CHECK(check_batch(h));
+  if (mode < 0 || mode > 2) {
+    set_last_error("sdpx_batch_set_gram_order: mode 0 (K-descending), 1 (member-major) or 2 (XCD bins)");
+    return kInvalidArgument;
+  }
+  return set_gram_order(h, mode);
+}
+
+int sdpx_batch_autotune(sdpx_batch_t* h, int repeats, double* out) {
+  SDPX_CHECK(check_batch(h));
+  if (repeats < 1 || !out) {
+    set_last_error("sdpx_batch_autotune: repeats >= 1 and out (4) required");
+    return kInvalidArgument;
+  }
+  // panels of every member at the start point, then each candidate order's Gram launch timed
+  SDPX_CHECK(sdpx_batch_schur_at_start(h));
+  hipEvent_t e0, e1;
+  SDPX_HIP(hipEventCreate(&e0));
+  SDPX_HIP(hipEventCreate(&e1));
+  double best = INFINITY;
+  int best_mode = 0;
+  int rc = kOk;
+  for (int mode = 0; mode < 3 && rc == kOk; ++mode) {
+    rc = set_gram_order(h, mode);
+    std::vector<float> ms;
+    for (int r = 0; r <= repeats && rc == kOk; ++r) {  // the first launch of an order is a warm-up
+      if (hipEventRecord(e0, h->stream) != hipSuccess) rc = kHipError;
+      if (rc == kOk) rc = gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream);
+      if (rc == kOk && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = kHipError;
+      float f = 0.f;
+      if (rc == kOk && hipEventElapsedTime(&f, e0, e1) != hipSuccess) rc = kHipError;
+      if (rc == kOk && r > 0) ms.push_back(f);
+    }
+    if (rc != kOk) break;
+    std::sort(ms.begin(), ms.end());
+    out[1 + mode] = ms[ms.size() / 2];
+    if (out[1 + mode] < best) {
+      best = out[1 + mode];
+      best_mode = mode;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != kOk) {
+    if (rc == kHipError) set_last_error("sdpx_batch_autotune: timing the Gram launch failed");
+    return rc;
+  }
+  SDPX_CHECK(set_gram_order(h, best_mode));
+  out[0] = best_mode;
   return kOk;
 }
 

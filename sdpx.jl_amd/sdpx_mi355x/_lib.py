@@ -100,6 +100,8 @@ SIGNATURES = {
     "sdpx_batch_adjoint": [_h, _dp, _dp, _dp, _dp],
     "sdpx_batch_center": [_h, C.c_int],
     "sdpx_batch_center_at": [_h, C.c_int, _dp],
+    "sdpx_batch_autotune": [_h, C.c_int, _dp],
+    "sdpx_batch_set_gram_order": [_h, C.c_int],
     "sdpx_batch_restore_checkpoint": [_h, _ip, _dp],
     "sdpx_batch_schur_at_start": [_h],
     "sdpx_batch_policy_select": [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp],

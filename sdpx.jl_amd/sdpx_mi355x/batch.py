@@ -225,6 +225,19 @@ class BatchSolver:
         """Newton steps on XY = mu I (mu fixed at entry) from the current iterate."""
         check(self._lib.sdpx_batch_center(self._h, int(iterations)), "sdpx_batch_center")
 
+    GRAM_ORDERS = ("k_descending", "member_major", "xcd_bins")
+
+    def autotune(self, repeats: int = 3) -> dict:
+        """Time the grouped Schur Gram launch at the start point under each tile order and keep the
+        fastest (sdpx_batch_autotune); needs set_start first."""
+        out = np.zeros(4)
+        check(self._lib.sdpx_batch_autotune(self._h, int(repeats), dptr(out)), "sdpx_batch_autotune")
+        return {"gram_order": self.GRAM_ORDERS[int(out[0])],
+                "gram_ms": {name: round(float(v), 4) for name, v in zip(self.GRAM_ORDERS, out[1:])}}
+
+    def set_gram_order(self, order: str) -> None:
+        check(self._lib.sdpx_batch_set_gram_order(self._h, self.GRAM_ORDERS.index(order)), "sdpx_batch_set_gram_order")
+
     def center_at(self, mu, iterations: int = 4) -> None:
         """Newton steps on XY = mu[p] I per member (NaN: the member's current <X,Y>/k)."""
         mu = np.ascontiguousarray(mu, dtype=np.float64)

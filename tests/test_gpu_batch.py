@@ -65,6 +65,33 @@ def test_batch_schur_matches_oracle(gpu_lib, kind):
             assert rel_fro_lower(got, ref) <= 1e-12
 
 
+@pytest.mark.gpu
+def test_batch_autotune_orders_give_identical_schur(gpu_lib):
+    """The autotuner's candidate tile orders of the grouped Gram (K-descending, member-major, XCD
+    bins) only change which work-group computes a tile: every member's Schur matrix is bitwise the
+    same, and the chosen order is one of the timed ones."""
+    rng = np.random.default_rng(8)
+    ks = [3, 8, 17, 33, 64, 40, 12, 25, 5, 50]
+    ms = [7, 20, 50, 70, 130, 300, 40, 90, 12, 200]
+    probs = [G.dense_feasible(k, m, seed=40 + i) for i, (k, m) in enumerate(zip(ks, ms))]
+    Xs = [_random_pd(rng, k) for k in ks]
+    Ys = [_random_pd(rng, k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], Xs, Ys)
+        tuned = bs.autotune(repeats=2)
+        print(tuned, flush=True)
+        assert tuned["gram_order"] in BatchSolver.GRAM_ORDERS and all(v > 0 for v in tuned["gram_ms"].values())
+        ref = None
+        for order in BatchSolver.GRAM_ORDERS:
+            bs.set_gram_order(order)
+            assert bs.schur_at_start() == 0
+            S = [bs.schur(p) for p in range(len(ks))]
+            if ref is None:
+                ref = S
+            assert all(np.array_equal(np.tril(a), np.tril(b)) for a, b in zip(S, ref)), order
+
+
 def _check_members(res, probs, o, oo, max_iter_diff=2):
     for p, g in enumerate(probs):
         prob = _oracle_problem(g)
@@ -167,6 +194,16 @@ def test_batch_matches_single_problem_gpu_path(gpu_lib):
     # tolerance is the one each route meets against the oracle (+-2)
     assert abs(int(res.iterations[0]) - single.iterations) <= 2
     assert abs(res.pobj[0] - single.pobj) <= 1e-8 * max(1.0, abs(single.pobj))
+    # with the fixed strategy no decision sits at rounding level: the routes take the same steps
+    of = SolverOptions(eps_gap=1e-8, eps_primal=1e-8, eps_dual=1e-8, parameter_strategy="fixed")
+    single_f = solve(ingest(g.c, [g.A_blocks()], [g.C]), of, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    with BatchSolver([24], [90]) as bs:
+        _load(bs, [g])
+        bs.set_start([g.x0], [g.X0], [g.Y0])
+        rf = bs.solve(of)
+    assert rf.status[0] == single_f.status == "Optimal"
+    assert int(rf.iterations[0]) == single_f.iterations
+    assert abs(rf.pobj[0] - single_f.pobj) <= 1e-10 * max(1.0, abs(single_f.pobj))
 
 
 @pytest.mark.gpu
