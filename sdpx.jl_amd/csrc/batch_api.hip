@@ -33,6 +33,7 @@ constexpr double kProjectionSlack = 10.0;     // stagnation.jl:26
 constexpr int kPanelChunk = 16;               // constraints per Schur panel work-group
 constexpr int kAlign = 32;                    // doubles: every member offset is 256-byte aligned
 constexpr int kListRing = 64;                 // member-list staging slots between stream drains
+constexpr int kParRing = 8;                   // parameter staging buffers (copies in flight without a drain)
 constexpr int kTraceCols = SDPX_BATCH_TRACE_COLS;
 
 long long align_up(long long v) { return (v + kAlign - 1) / kAlign * kAlign; }
@@ -425,6 +426,10 @@ struct sdpx_batch {
   double* hsc = nullptr;
   int* hsi = nullptr;
   double* hpar = nullptr;
+  double* hpar_ring = nullptr;  // kParRing pinned copies of hpar, one per copy in flight
+  hipEvent_t par_done[kParRing] = {};
+  bool par_busy[kParRing] = {};
+  int par_pos = 0;
   std::vector<double> scale_p, scale_d;
   std::vector<MemberState> st;
   bool have_data = false, have_start = false, solved = false;
@@ -599,12 +604,20 @@ const int* put_list(sdpx_batch_t* h, const std::vector<int>& l) {
   return h->dlist + off;
 }
 
-// parameters are copied synchronously: the host edits them right afterwards
+// The host keeps editing hpar after a push, so each push stages a snapshot in the next of
+// kParRing pinned buffers and copies that one: the stream is not drained (kernels launched before
+// the copy read the earlier values, stream order); only a buffer whose copy is still in flight is
+// waited for before it is reused.
 int push_params(sdpx_batch_t* h) {
-  SDPX_HIP(hipMemcpyAsync(h->arr.par, h->hpar, sizeof(double) * (size_t)h->B * NPARAM, hipMemcpyHostToDevice,
-                          h->stream));
-  SDPX_HIP(hipStreamSynchronize(h->stream));
-  h->ring_pos = 0;
+  const size_t n = (size_t)h->B * NPARAM;
+  const int pos = h->par_pos;
+  h->par_pos = (pos + 1) % kParRing;
+  if (h->par_busy[pos]) SDPX_HIP(hipEventSynchronize(h->par_done[pos]));
+  double* stage = h->hpar_ring + (size_t)pos * n;
+  std::copy(h->hpar, h->hpar + n, stage);
+  SDPX_HIP(hipMemcpyAsync(h->arr.par, stage, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  SDPX_HIP(hipEventRecord(h->par_done[pos], h->stream));
+  h->par_busy[pos] = true;
   return kOk;
 }
 
@@ -682,6 +695,13 @@ int set_gram_order(sdpx_batch_t* h, int mode) {
   SDPX_HIP(hipMemcpy(h->dgram_tiles, t.data(), sizeof(GroupTile) * t.size(), hipMemcpyHostToDevice));
   h->gram_order = mode;
   return kOk;
+}
+
+// the line-search kernels the listed members need (one per block-size class)
+unsigned step_classes(const sdpx_batch_t* h, const std::vector<int>& l) {
+  unsigned c = 0;
+  for (int p : l) c |= step_class(h->mem[p].k);
+  return c;
 }
 
 // I_SKIP = 0 for the listed members, 1 for every other member
@@ -920,7 +940,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
     }
     SDPX_CHECK(push_params(h));
     if (!adapt.empty()) {
-      SDPX_CHECK(launch_ftb(h->arr, put_list(h, adapt), (int)adapt.size(), h->stream));
+      SDPX_CHECK(launch_ftb(h->arr, put_list(h, adapt), (int)adapt.size(), step_classes(h, adapt), h->stream));
       SDPX_CHECK(pull_scalars(h));
     }
     for (int p : ok) {
@@ -1107,8 +1127,9 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       }
     }
     SDPX_CHECK(push_params(h));
-    if (!ftb.empty()) SDPX_CHECK(launch_ftb(h->arr, put_list(h, ftb), (int)ftb.size(), h->stream));
-    if (!bt.empty()) SDPX_CHECK(launch_backtrack(h->arr, put_list(h, bt), (int)bt.size(), h->stream));
+    if (!ftb.empty()) SDPX_CHECK(launch_ftb(h->arr, put_list(h, ftb), (int)ftb.size(), step_classes(h, ftb), h->stream));
+    if (!bt.empty())
+      SDPX_CHECK(launch_backtrack(h->arr, put_list(h, bt), (int)bt.size(), step_classes(h, bt), h->stream));
     SDPX_CHECK(pull_scalars(h));
     // collapse handling (interior_point.jl:2010-2217)
     std::vector<int> upd, center, restart;
@@ -1388,10 +1409,16 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
   if (hipHostMalloc(reinterpret_cast<void**>(&h->hlist), sizeof(int) * B * kListRing) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&h->hsc), sizeof(double) * B * NSLOT) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&h->hsi), sizeof(int) * B * NISLOT) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&h->hpar), sizeof(double) * B * NPARAM) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&h->hpar), sizeof(double) * B * NPARAM) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&h->hpar_ring), sizeof(double) * B * NPARAM * kParRing) != hipSuccess) {
     set_last_error("hipHostMalloc failed");
     return fail(kOutOfMemory);
   }
+  for (auto& e : h->par_done)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      set_last_error("hipEventCreate failed");
+      return fail(kHipError);
+    }
   std::fill(h->hpar, h->hpar + (size_t)B * NPARAM, 0.0);
   // zero the panels once: padding stays finite, Av starts empty for COO input
   if (hipMemset(a.Av, 0, sizeof(double) * h->nAv) != hipSuccess ||
@@ -1556,6 +1583,9 @@ int sdpx_batch_destroy(sdpx_batch_t* h) {
   if (h->hsc) hipHostFree(h->hsc);
   if (h->hsi) hipHostFree(h->hsi);
   if (h->hpar) hipHostFree(h->hpar);
+  if (h->hpar_ring) hipHostFree(h->hpar_ring);
+  for (auto& e : h->par_done)
+    if (e) hipEventDestroy(e);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return kOk;
@@ -2001,7 +2031,7 @@ int sdpx_batch_center_at(sdpx_batch_t* h, int iterations, const double* target) 
       par(h, p, P_RATIO) = 0.0;
     }
     SDPX_CHECK(push_params(h));
-    SDPX_CHECK(launch_backtrack(h->arr, put_list(h, ok), n, h->stream));
+    SDPX_CHECK(launch_backtrack(h->arr, put_list(h, ok), n, step_classes(h, ok), h->stream));
     SDPX_CHECK(pull_scalars(h));
     for (int p : ok) {
       par(h, p, P_A) = sc(h, p, S_TX);

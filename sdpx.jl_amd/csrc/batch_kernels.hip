@@ -1628,20 +1628,26 @@ int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t 
   SDPX_LAUNCH(corrector_block_kernel, n, a, list);
 }
 // one launch per register width; members outside a width's range exit at once
-int launch_step_waves(const Arrays& a, const int* list, int n, int op, hipStream_t st) {
+// one kernel per block-size class present in the list (bit c of `classes`, step_class())
+int launch_step_waves(const Arrays& a, const int* list, int n, int op, unsigned classes, hipStream_t st) {
   if (n <= 0) return kOk;
   const dim3 block(NT);
-  hipLaunchKernelGGL(step_small_kernel, dim3((unsigned)ceil_div(n, NT)), block, 0, st, a, list, n, op);  // k <= 2
-  hipLaunchKernelGGL(step_quad_kernel<16>, dim3((unsigned)n), block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_quad_kernel<32>, dim3((unsigned)n), block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_quad_wide_kernel<48>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
-  hipLaunchKernelGGL(step_quad_wide_kernel<64>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
+  if (classes & 1u)
+    hipLaunchKernelGGL(step_small_kernel, dim3((unsigned)ceil_div(n, NT)), block, 0, st, a, list, n, op);  // k <= 2
+  if (classes & 2u) hipLaunchKernelGGL(step_quad_kernel<16>, dim3((unsigned)n), block, 0, st, a, list, n, op);
+  if (classes & 4u) hipLaunchKernelGGL(step_quad_kernel<32>, dim3((unsigned)n), block, 0, st, a, list, n, op);
+  if (classes & 8u)
+    hipLaunchKernelGGL(step_quad_wide_kernel<48>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
+  if (classes & 16u)
+    hipLaunchKernelGGL(step_quad_wide_kernel<64>, dim3((unsigned)(2 * n)), block, 0, st, a, list, n, op);
   SDPX_HIP(hipGetLastError());
   return kOk;
 }
-int launch_ftb(const Arrays& a, const int* list, int n, hipStream_t st) { return launch_step_waves(a, list, n, 0, st); }
-int launch_backtrack(const Arrays& a, const int* list, int n, hipStream_t st) {
-  return launch_step_waves(a, list, n, 1, st);
+int launch_ftb(const Arrays& a, const int* list, int n, unsigned classes, hipStream_t st) {
+  return launch_step_waves(a, list, n, 0, classes, st);
+}
+int launch_backtrack(const Arrays& a, const int* list, int n, unsigned classes, hipStream_t st) {
+  return launch_step_waves(a, list, n, 1, classes, st);
 }
 int launch_update(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(update_kernel, n, a, list);

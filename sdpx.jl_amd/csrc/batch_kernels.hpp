@@ -110,9 +110,11 @@ int launch_comp(const Arrays& a, const int* list, int n, hipStream_t st);
 // R = -X Y - dX dY + target I (P_MODE 0 fixed (P_MU, P_BETA), 1 local (P_SIGMA), 2 global (P_SIGMA, P_MUG))
 int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t st);
 // fraction to boundary: S_TX = P_PF * bX (if bX < 1), S_TY likewise
-int launch_ftb(const Arrays& a, const int* list, int n, hipStream_t st);
+// Size class of a member's line-search kernel (a bit of the launch mask): k <= 2, <= 16, <= 32, <= 48, <= 64.
+inline unsigned step_class(int k) { return k <= 2 ? 1u : k <= 16 ? 2u : k <= 32 ? 4u : k <= 48 ? 8u : 16u; }
+int launch_ftb(const Arrays& a, const int* list, int n, unsigned classes, hipStream_t st);
 // backtracking with P_GAMMA, P_MINSTEP, P_RATIO: S_TX, S_TY
-int launch_backtrack(const Arrays& a, const int* list, int n, hipStream_t st);
+int launch_backtrack(const Arrays& a, const int* list, int n, unsigned classes, hipStream_t st);
 // X += P_A dX, Y += P_B dY, x += P_A dx; S_COMP, S_XMAX, S_XBMAX, S_YBMAX
 int launch_update(const Arrays& a, const int* list, int n, hipStream_t st);
 // d = (1 - P_B) d - P_B rho, S_DACC
