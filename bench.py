@@ -229,7 +229,8 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
     ks = [g.k for g in probs]
     ms = [g.m for g in probs]
     log(f"{which}: generated {B} problems in {time.perf_counter() - t0:.1f}s")
-    o = BatchOptions.from_solver_options(SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol))
+    so = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    o = BatchOptions.from_solver_options(so)
     if which == "layer":
         o.adjoint_mu = 1e-5  # the differentiation checkpoint: the first iterate with mu <= 1e-5
     eyes = [np.eye(k) for k in ks]
@@ -314,10 +315,10 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
         prob = ingest(g.c, [A], [g.C])
         b = MI355XDenseBackend(prob)
         try:
-            solve(prob, o, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
+            solve(prob, so, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
             b.synchronize()
             t0 = time.perf_counter()
-            solve(prob, o, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
+            solve(prob, so, x0=g.x0, X0=[np.eye(g.k)], Y0=[np.eye(g.k)], backend=b)
             b.synchronize()
             tt += time.perf_counter() - t0
             n += 1
