@@ -211,6 +211,26 @@ def sparse_reference_flops(k: int, m: int) -> float:
     return 2.0 * k ** 3 + 12.0 * k ** 3 * m + 6.0 * k * m * (m + 1) + m ** 3 / 3.0
 
 
+def mixed_summary(res_m, res, mixed_it, per_m: float, per: float) -> dict:
+    """The batch solved with FP32 Schur assembly and FP64 refinement against the same batch in FP64."""
+    import numpy as np
+
+    B = len(res.status)
+    opt_m = np.array([s == "Optimal" for s in res_m.status])
+    both = opt_m & np.array([s == "Optimal" for s in res.status])
+    rel = np.abs(res_m.pobj - res.pobj) / np.maximum(1.0, np.abs(res.pobj))
+    its = np.asarray(res_m.iterations, dtype=np.float64)
+    return {"solves_per_s": round(B / per_m, 1), "seconds_per_batch": round(per_m, 4),
+            "speedup_vs_fp64_batched": round(per / per_m, 3), "optimal": int(opt_m.sum()),
+            "statuses": {s: res_m.status.count(s) for s in sorted(set(res_m.status))},
+            "reduced_precision_steps_fraction": round(float(np.sum(mixed_it)) / max(float(its.sum()), 1.0), 4),
+            "members_never_reduced": int(np.sum(mixed_it == 0)),
+            "objective_max_rel_diff_vs_fp64_batched": float(np.max(rel[both])) if both.any() else None,
+            "iterations_mean": round(float(its.mean()), 3),
+            "iterations_mean_fp64": round(float(np.mean(res.iterations)), 3),
+            "batched_steps_launched": res_m.launched}
+
+
 def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
     """Many independent SDPs in one batch (sdpx_batch_*), against the plain FP64 single-problem GPU route
     run member by member on a sample of the same problems."""
@@ -285,6 +305,19 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
                                "measured against the implicit-function-theorem derivative at the optimum "
                                "(oracle/adjoint_oracle.py optimum_adjoint): median ~3e-6 relative, "
                                "near-degenerate members up to ~1e-2 (tests/test_adjoint.py)"}
+        # mixed precision on the same batch: FP32 Schur Gram products, FP64 matrix-free refinement
+        om = BatchOptions.from_solver_options(so, precision=1)
+        om.adjoint_mu = 0.0
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        bs.solve(om)  # warm-up
+        wall_m, res_m = 0.0, None
+        for _ in range(steps):
+            bs.set_start([g.x0 for g in probs], eyes, eyes)
+            t0 = time.perf_counter()
+            res_m = bs.solve(om)
+            wall_m += time.perf_counter() - t0
+        mixed_it = bs.mixed_iterations()
+    mixed = mixed_summary(res_m, res, mixed_it, wall_m / steps, wall / steps)
     it = np.asarray(res.iterations, dtype=np.float64)
     opt = np.array([s == "Optimal" for s in res.status])
     dense_F = sum(it[p] * newton_flops(ks[p], ms[p]) for p in range(B))
@@ -305,6 +338,7 @@ def batched_leg(which: str, B: int, steps: int, seq_sample: int) -> dict:
         out["schur_tflops_reference_sparse_count"] = round(sparse_F / per / 1e12, 3)
     if adjoint is not None:
         out["adjoint"] = adjoint
+    out["mixed_precision"] = mixed
     # the plain FP64 single-problem GPU route, member by member, on a sample
     rng = np.random.default_rng(1)
     idx = rng.choice(B, size=min(seq_sample, B), replace=False)

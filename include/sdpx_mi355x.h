@@ -324,11 +324,23 @@ typedef struct sdpx_batch_options {
   int refine_steps, refine_max_steps;
   int adaptive;           /* parameter_strategy: 1 :adaptive, 0 :fixed */
   int step_rule;          /* 0 :auto, 1 :backtrack, 2 :fraction_to_boundary */
-  int precision;          /* 0: plain FP64 */
+  /* 0: plain FP64; 1: mixed precision, the Schur Gram product assembled from
+   * FP32 Phat panels on FP32 MFMA (FP64 S and factor) and every direction
+   * refined against the matrix-free FP64 Schur operator, the template of
+   * kernels/mixed_precision_kkt.jl:972-1111, 1475-1713 with FP64 as the target:
+   * the predictor to 1e-8 max(||r||, 1), the corrector to refine_target
+   * max(||r||, 1), at most mixed_max_refine steps each.  A member whose
+   * reduced-precision Schur matrix does not factor, whose Cholesky pivot ratio
+   * (max L_ii / min L_ii)^2 (a lower bound of cond_2(S)) exceeds cond_limit,
+   * or whose refinement misses its target redoes the step in FP64 and keeps
+   * FP64 (per member).  cond_limit <= 0 disables the pivot-ratio guard;
+   * sdpx_batch_default_options sets 1e6, 1e-10 and 32. */
+  int precision;
   double cond_limit, refine_target;
   /* differentiation checkpoint: the first iterate with <X, Y>/k <= adjoint_mu
    * is kept (0 disables; see sdpx_batch_restore_checkpoint) */
   double adjoint_mu;
+  int mixed_max_refine;
 } sdpx_batch_options_t;
 
 /* Fill `o` with the reference defaults (SolverOptions{Float64}()) and tolerance
@@ -449,6 +461,14 @@ int sdpx_batch_get_schur(sdpx_batch_t* h, int member, double* S);
 /* Schur assembly of every member at the uploaded start point (parity hook of
  * schur_build!, schur.jl:511-547); returns 1 when a start block is not PD. */
 int sdpx_batch_schur_at_start(sdpx_batch_t* h);
+/* Assembly precision of sdpx_batch_schur_at_start and sdpx_batch_autotune: 0
+ * FP64, 1 the FP32 Gram product of the mixed-precision solve (the precision of
+ * a solve is its options' field).  Replaces the precision switch of the
+ * reference's mixed-precision KKT template (kernels/mixed_precision_kkt.jl:58-120). */
+int sdpx_batch_set_precision(sdpx_batch_t* h, int precision);
+/* Per member (B ints): Newton steps of the last solve whose Schur matrix was
+ * assembled in reduced precision (precision 1). */
+int sdpx_batch_get_mixed_iterations(sdpx_batch_t* h, int* out);
 
 /* ==== a14: Lorentz (second-order) cone kernels ===============================
  * Many cones Q_q = {(t, u): t >= ||u||} of arbitrary dimensions resident in

@@ -362,7 +362,11 @@ struct MemberState {
   double ap = 1, ad = 1;
   int refine_steps = 0, refine_cap = 0;
   bool refine_fixed = false, refine_active = false;
-  double refine_abstol = 0, refine_residual = 0;
+  double refine_abstol = 0, refine_residual = 0, refine_rnorm = 0;
+  // mixed precision: the Schur Gram product is assembled in FP32 while `mixed` holds; a step
+  // whose reduced-precision direction fails a guard is redone in FP64 and FP64 is kept
+  bool mixed = false;
+  int mixed_iterations = 0;  // Newton steps taken with the reduced-precision Schur matrix
   double tX = 1, tY = 1;
   // one row per Newton step: iteration, pobj, dobj, p_res, d_res (of the step), tX, tY, sigma,
   // affine steps, regularization attempts, refinement steps, outcome (0 update, 1 centering,
@@ -396,6 +400,7 @@ struct sdpx_batch {
   double* sp_pval = nullptr;
   // grouped Schur Gram product
   GroupGemm* dgram = nullptr;
+  GroupGramF32* dgram32 = nullptr;  // the same products on FP32 panels (mixed precision)
   GroupTile* dgram_tiles = nullptr;
   int ngram_tiles = 0;
   std::vector<GroupTile> gram_tiles;  // member-major tile list (the autotuner's candidates reorder it)
@@ -433,6 +438,7 @@ struct sdpx_batch {
   std::vector<double> scale_p, scale_d;
   std::vector<MemberState> st;
   bool have_data = false, have_start = false, solved = false;
+  int precision = 0;  // assembly precision of sdpx_batch_schur_at_start / autotune (sdpx_batch_set_precision)
   long long bytes = 0;
   std::vector<void*> allocs;
   // phase timing (device events around whole phases, summed over a solve)
@@ -704,19 +710,89 @@ unsigned step_classes(const sdpx_batch_t* h, const std::vector<int>& l) {
   return c;
 }
 
-// I_SKIP = 0 for the listed members, 1 for every other member
-int set_active(sdpx_batch_t* h, const std::vector<int>& l) {
+// I_SKIP = 0 for the members of l64 (FP64 assembly), I_SKIP32 = 0 for those of l32 (FP32
+// assembly), 1 otherwise
+int set_active(sdpx_batch_t* h, const std::vector<int>& l64, const std::vector<int>& l32) {
   SDPX_CHECK(launch_set_int(h->arr, h->dall, h->B, I_SKIP, 1, h->stream));
-  SDPX_CHECK(launch_set_int(h->arr, put_list(h, l), (int)l.size(), I_SKIP, 0, h->stream));
+  SDPX_CHECK(launch_set_int(h->arr, h->dall, h->B, I_SKIP32, 1, h->stream));
+  if (!l64.empty()) SDPX_CHECK(launch_set_int(h->arr, put_list(h, l64), (int)l64.size(), I_SKIP, 0, h->stream));
+  if (!l32.empty()) SDPX_CHECK(launch_set_int(h->arr, put_list(h, l32), (int)l32.size(), I_SKIP32, 0, h->stream));
   return kOk;
 }
 
-// Schur assembly for the listed members: panels then the grouped Gram product
+// Schur assembly for the listed members: panels then the grouped Gram product, in FP32 for the
+// members whose state is `mixed` and in FP64 for the others
 int schur_build(sdpx_batch_t* h, const std::vector<int>& l) {
   if (l.empty()) return kOk;
-  SDPX_CHECK(set_active(h, l));
+  std::vector<int> l64, l32;
+  for (int p : l) (h->st[p].mixed ? l32 : l64).push_back(p);
+  SDPX_CHECK(set_active(h, l64, l32));
   SDPX_CHECK(launch_panel(h->arr, h->dpanel, h->panel_off, kPanelChunk, h->stream));
-  SDPX_CHECK(gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream));
+  if (!l64.empty())
+    SDPX_CHECK(gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream));
+  if (!l32.empty()) SDPX_CHECK(gram_f32_grouped(h->dgram32, h->dgram_tiles, h->ngram_tiles, h->stream));
+  return kOk;
+}
+
+// rho = r - S dx with S applied matrix-free in FP64, (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y>;
+// S_RHO = ||rho||_inf, S_RNORM = ||r||_inf (dX and W2 are scratch)
+int exact_residual(sdpx_batch_t* h, const std::vector<int>& l) {
+  if (l.empty()) return kOk;
+  const int* dl = put_list(h, l);
+  const int n = (int)l.size();
+  SDPX_CHECK(launch_gemv(h->arr, dl, n, 3, h->stream));
+  SDPX_CHECK(launch_exact_block(h->arr, dl, n, h->stream));
+  SDPX_CHECK(launch_gemvt(h->arr, dl, n, 3, h->stream));
+  return kOk;
+}
+
+// Refinement of dx against the matrix-free FP64 operator (the mixed-precision direction's
+// _refine_mixed_direction!, mixed_precision_kkt.jl:1634-1713): delta = Sf^-1 rho with the factor of
+// the reduced-precision Schur matrix, dx += delta, until ||rho||_inf <= rel * max(||r||_inf, 1) or
+// `cap` steps; a step that raises the residual is rolled back and ends the member's refinement.
+// Sets refine_residual, refine_rnorm, refine_abstol and refine_steps of the listed members.
+int refine_exact(sdpx_batch_t* h, const std::vector<int>& l, double rel, int cap) {
+  if (l.empty()) return kOk;
+  SDPX_CHECK(exact_residual(h, l));
+  SDPX_CHECK(pull_scalars(h));
+  for (int p : l) {
+    MemberState& s = h->st[p];
+    s.refine_residual = sc(h, p, S_RHO);
+    s.refine_rnorm = sc(h, p, S_RNORM);
+    s.refine_abstol = rel * std::max(s.refine_rnorm, 1.0);
+    s.refine_steps = 0;
+    s.refine_active = !(s.refine_residual <= s.refine_abstol);
+  }
+  for (int it = 0; it < cap; ++it) {
+    std::vector<int> act;
+    for (int p : l)
+      if (h->st[p].refine_active) act.push_back(p);
+    if (act.empty()) break;
+    const int* dl = put_list(h, act);
+    const int na = (int)act.size();
+    SDPX_CHECK(launch_vec_op(h->arr, dl, na, 0, h->stream));
+    SDPX_CHECK(launch_solve(h->arr, dl, na, 1, h->stream));
+    SDPX_CHECK(launch_vec_op(h->arr, dl, na, 1, h->stream));
+    SDPX_CHECK(exact_residual(h, act));
+    SDPX_CHECK(pull_scalars(h));
+    std::vector<int> back;
+    for (int p : act) {
+      MemberState& s = h->st[p];
+      const double corrected = sc(h, p, S_RHO);
+      if (!std::isfinite(corrected) || corrected > s.refine_residual) {
+        back.push_back(p);
+        s.refine_active = false;
+        continue;
+      }
+      s.refine_steps += 1;
+      s.refine_residual = corrected;
+      if (corrected <= s.refine_abstol) s.refine_active = false;
+    }
+    if (!back.empty()) {
+      SDPX_CHECK(launch_vec_op(h->arr, put_list(h, back), (int)back.size(), 2, h->stream));
+      SDPX_CHECK(exact_residual(h, back));
+    }
+  }
   return kOk;
 }
 
@@ -781,6 +857,290 @@ bool at_floor(double p_res, double d_res, double gap, double sp, double sd) {
   return p_res / sp <= f && d_res / sd <= f && gap <= f;
 }
 
+// The Newton direction of the listed members (newton_step!, step.jl:643-1006, from the Schur
+// build to dX, dY): members that end here in breakdown get their status, members whose
+// reduced-precision step failed a guard move to `retry` (the caller redoes them in FP64), the
+// others stay in `ok` with their direction and step parameters.
+int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector<int>& ok, std::vector<int>& retry) {
+  auto& S = h->st;
+  SDPX_CHECK(schur_build(h, ok));
+  SDPX_CHECK(schur_factor(h, ok));
+  // members whose reduced-precision step is abandoned move from ok to retry
+  auto to_retry = [&](const std::vector<int>& bad) {
+    if (bad.empty()) return;
+    std::vector<int> keep;
+    for (int p : ok)
+      if (std::find(bad.begin(), bad.end(), p) == bad.end()) keep.push_back(p);
+    ok.swap(keep);
+    retry.insert(retry.end(), bad.begin(), bad.end());
+  };
+  {
+    std::vector<int> keep, bad;
+    for (int p : ok) {
+      if (!S[p].factor_ok) {
+        if (S[p].mixed) {
+          bad.push_back(p);  // reduced-precision Schur matrix not positive definite
+          continue;
+        }
+        S[p].status = kBreakdown;
+        S[p].p_res = S[p].p_it;
+        S[p].d_res = S[p].d_it;
+      } else {
+        keep.push_back(p);
+      }
+    }
+    ok.swap(keep);
+    retry.insert(retry.end(), bad.begin(), bad.end());
+  }
+  if (ok.empty()) return kOk;
+  std::vector<int> mixed;
+  for (int p : ok)
+    if (S[p].mixed) mixed.push_back(p);
+  if (!mixed.empty() && o.cond_limit > 0) {
+    // condition guard: the pivots d_i = L_ii^2 of a Cholesky factor lie in [lambda_min, lambda_max],
+    // so (max L_ii / min L_ii)^2 bounds cond_2(S) from below; a member above the limit cannot
+    // keep the reduced-precision Schur matrix (the refinement guards below catch the rest)
+    SDPX_CHECK(launch_factor_extrema(h->arr, put_list(h, mixed), (int)mixed.size(), h->stream));
+    SDPX_CHECK(pull_scalars(h));
+    std::vector<int> bad, keep;
+    for (int p : mixed) {
+      const double mn = sc(h, p, S_SFMIN), mx = sc(h, p, S_SFMAX);
+      const double est = (mx / mn) * (mx / mn);
+      if (!(est <= o.cond_limit)) bad.push_back(p);
+      else keep.push_back(p);
+    }
+    mixed.swap(keep);
+    to_retry(bad);
+    if (ok.empty()) return kOk;
+  }
+  const int mixed_cap = o.mixed_max_refine;
+  // predictor (step.jl:699-726)
+  {
+    const int* l = put_list(h, ok);
+    const int n0 = (int)ok.size();
+    SDPX_CHECK(launch_rhs_block(h->arr, l, n0, h->stream));
+    SDPX_CHECK(launch_gemvt(h->arr, l, n0, 1, h->stream));
+    SDPX_CHECK(launch_solve(h->arr, l, n0, 0, h->stream));
+    if (!mixed.empty()) {
+      // predictor guard (mixed_precision_kkt.jl:1475-1586): residual <= 1e-8 max(||r||, 1)
+      SDPX_CHECK(refine_exact(h, mixed, 1e-8, mixed_cap));
+      std::vector<int> bad, keep;
+      for (int p : mixed) (S[p].refine_residual <= S[p].refine_abstol ? keep : bad).push_back(p);
+      mixed.swap(keep);
+      to_retry(bad);
+      if (ok.empty()) return kOk;
+    }
+    l = put_list(h, ok);
+    const int n1 = (int)ok.size();
+    SDPX_CHECK(launch_gemv(h->arr, l, n1, 1, h->stream));
+    SDPX_CHECK(launch_dir_block(h->arr, l, n1, h->stream));
+  }
+  const int n = (int)ok.size();
+  std::vector<int> fp64;
+  for (int p : ok)
+    if (!S[p].mixed) fp64.push_back(p);
+  // affine predictor diagnostics (step.jl:546-625)
+  std::vector<int> adapt;
+  for (int p : ok) {
+    if (S[p].iter_adaptive) {
+      adapt.push_back(p);
+      par(h, p, P_PF) = 1.0;
+      par(h, p, P_DF) = 1.0;
+    }
+  }
+  SDPX_CHECK(push_params(h));
+  if (!adapt.empty()) {
+    SDPX_CHECK(launch_ftb(h->arr, put_list(h, adapt), (int)adapt.size(), step_classes(h, adapt), h->stream));
+    SDPX_CHECK(pull_scalars(h));
+  }
+  for (int p : ok) {
+    MemberState& s = S[p];
+    if (s.iter_adaptive) {
+      s.ap = sc(h, p, S_TX);
+      s.ad = sc(h, p, S_TY);
+    } else {
+      s.ap = s.ad = 1.0;
+    }
+    par(h, p, P_A) = s.ap;
+    par(h, p, P_B) = s.ad;
+  }
+  SDPX_CHECK(push_params(h));
+  {
+    const int* l = put_list(h, ok);
+    SDPX_CHECK(launch_comp(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_factor_extrema(h->arr, l, n, h->stream));
+    SDPX_CHECK(pull_scalars(h));
+  }
+  auto quality = [](double mn, double mx) {
+    return (mx > 0.0 && std::isfinite(mx)) ? std::min(std::max(mn / mx, 0.0), 1.0) : 0.0;
+  };
+  for (int p : ok) {
+    MemberState& s = S[p];
+    const int k = h->mem[p].k;
+    const double comp = sc(h, p, S_COMP), compt = sc(h, p, S_COMPT);
+    const bool uniform = s.iter_adaptive && std::isfinite(comp);
+    s.comp = comp;
+    s.mu_g = comp / k;
+    s.mu_aff = std::max(0.0, compt / k);
+    const double cur = std::max(s.p_it / std::max(s.scale_p, 1.0), s.d_it / std::max(s.scale_d, 1.0));
+    double achieved = 1.0;
+    if (!s.ctrl.history.empty()) {
+      const Hist& row = s.ctrl.history.back();
+      const double prev = std::max(std::max(row.primal_residual / std::max(s.scale_p, 1.0),
+                                            row.dual_residual / std::max(s.scale_d, 1.0)),
+                                   kEps);
+      achieved = cur / prev;
+    }
+    Diag d;
+    d.iteration = s.iteration + 1;
+    d.primal_residual = s.p_it / std::max(s.scale_p, 1.0);
+    d.dual_residual = s.d_it / std::max(s.scale_d, 1.0);
+    d.relative_gap = s.gap_rel;
+    d.mu = s.mu_g;
+    d.mu_aff = s.mu_aff;
+    d.affine_primal_step = s.ap;
+    d.affine_dual_step = s.ad;
+    d.previous_primal_step = s.ctrl.last_primal_step();
+    d.previous_dual_step = s.ctrl.last_dual_step();
+    d.backtracking_count = s.ctrl.last_backtracking();
+    d.regularization = s.reg_attempts <= 0 ? 0.0 : kSqrtEps * std::pow(10.0, s.reg_attempts - 1);
+    d.refinement_count = s.ctrl.last_refinement();
+    d.factorization_quality = quality(sc(h, p, S_SFMIN), sc(h, p, S_SFMAX));
+    d.achieved_residual_reduction = achieved;
+    d.primal_psd_margin = std::min(1.0, quality(sc(h, p, S_LXMIN), sc(h, p, S_LXMAX)));
+    d.dual_psd_margin = std::min(1.0, quality(sc(h, p, S_LYMIN), sc(h, p, S_LYMAX)));
+    s.prm = s.ctrl.select(d);
+    // corrector target (threaded.jl:839-854, 943-1003)
+    if (s.iter_adaptive && !s.prm.fallback) {
+      par(h, p, P_MODE) = uniform ? 2.0 : 1.0;
+      par(h, p, P_SIGMA) = s.prm.sigma;
+      par(h, p, P_MUG) = s.mu_g;
+    } else {
+      par(h, p, P_MODE) = 0.0;
+      par(h, p, P_MU) = s.mu;
+      par(h, p, P_BETA) = s.o_beta;
+    }
+    // refinement policy
+    s.refine_fixed = o.refine_policy == 1;
+    double rtol;
+    if (s.iter_adaptive && !s.prm.fallback) {
+      rtol = s.prm.rtol;
+      s.refine_cap = s.prm.rmax;
+    } else {
+      rtol = o.refine_tol;
+      s.refine_cap = o.refine_max_steps;
+    }
+    if (s.refine_fixed) s.refine_cap = o.refine_steps;
+    s.refine_abstol = rtol > 0 ? rtol : kRefineTolUlps * kEps;  // scaled by ||r|| below
+  }
+  SDPX_CHECK(push_params(h));
+  // corrector (step.jl:846-936)
+  {
+    const int* l = put_list(h, ok);
+    SDPX_CHECK(launch_corrector_block(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_rhs_block(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_gemvt(h->arr, l, n, 1, h->stream));
+    SDPX_CHECK(launch_solve(h->arr, l, n, 0, h->stream));
+    if (!fp64.empty()) SDPX_CHECK(launch_residual(h->arr, put_list(h, fp64), (int)fp64.size(), h->stream));
+    SDPX_CHECK(pull_scalars(h));
+  }
+  if (!mixed.empty()) {
+    // _refine_mixed_direction! (mixed_precision_kkt.jl:1634-1713) to the FP64 target
+    SDPX_CHECK(refine_exact(h, mixed, o.refine_target > 0 ? o.refine_target : 1e-10, mixed_cap));
+    std::vector<int> bad, keep;
+    for (int p : mixed) {
+      MemberState& s = S[p];
+      if (s.refine_residual <= s.refine_abstol) {
+        s.refine_steps = std::min(s.refine_steps, 1);  // the controller sees a native-equivalent count
+        keep.push_back(p);
+      } else {
+        bad.push_back(p);
+      }
+    }
+    mixed.swap(keep);
+    to_retry(bad);
+    if (ok.empty()) return kOk;
+  }
+  // refine! (kkt_backend.jl:476-485 -> kkt.jl:3891-3963), member by member in rounds
+  int max_cap = 0;
+  for (int p : fp64) {
+    MemberState& s = S[p];
+    s.refine_steps = 0;
+    s.refine_residual = sc(h, p, S_RHO);
+    s.refine_abstol = s.refine_fixed ? 0.0 : s.refine_abstol * std::max(sc(h, p, S_RNORM), 1.0);
+    s.refine_active = s.refine_cap > 0;
+    max_cap = std::max(max_cap, s.refine_cap);
+  }
+  for (int it = 0; it < max_cap; ++it) {
+    std::vector<int> act;
+    for (int p : fp64) {
+      MemberState& s = S[p];
+      if (!s.refine_active) continue;
+      if (it >= s.refine_cap || (!s.refine_fixed && s.refine_residual <= s.refine_abstol)) {
+        s.refine_active = false;
+        continue;
+      }
+      act.push_back(p);
+    }
+    if (act.empty()) break;
+    const int* l = put_list(h, act);
+    const int na = (int)act.size();
+    SDPX_CHECK(launch_vec_op(h->arr, l, na, 0, h->stream));
+    SDPX_CHECK(launch_solve(h->arr, l, na, 1, h->stream));
+    SDPX_CHECK(launch_vec_op(h->arr, l, na, 1, h->stream));
+    SDPX_CHECK(launch_residual(h->arr, l, na, h->stream));
+    SDPX_CHECK(pull_scalars(h));
+    std::vector<int> back;
+    for (int p : act) {
+      MemberState& s = S[p];
+      const double corrected = sc(h, p, S_RHO);
+      if (!std::isfinite(corrected) || corrected > s.refine_residual) {
+        back.push_back(p);
+        s.refine_active = false;
+        continue;
+      }
+      s.refine_steps += 1;
+      const bool small = corrected > s.refine_residual * kRefineMinDecrease;
+      s.refine_residual = corrected;
+      if (!s.refine_fixed && small) s.refine_active = false;
+    }
+    if (!back.empty()) {
+      const int* lb = put_list(h, back);
+      SDPX_CHECK(launch_vec_op(h->arr, lb, (int)back.size(), 2, h->stream));
+      SDPX_CHECK(launch_residual(h->arr, lb, (int)back.size(), h->stream));
+    }
+  }
+  if (!fp64.empty()) {
+    SDPX_CHECK(launch_residual(h->arr, put_list(h, fp64), (int)fp64.size(), h->stream));
+    SDPX_CHECK(pull_scalars(h));
+  }
+  {
+    std::vector<int> keep;
+    for (int p : ok) {
+      MemberState& s = S[p];
+      const double res = s.mixed ? s.refine_residual : sc(h, p, S_RHO);
+      const double r_norm = s.mixed ? s.refine_rnorm : sc(h, p, S_RNORM);
+      const double tol = std::max(kSqrtEps, std::max(o.eps_primal, o.eps_dual)) * std::max(r_norm, 1.0);
+      if (!std::isfinite(res) || res > tol) {
+        s.status = kBreakdown;
+        s.p_res = s.p_it;
+        s.d_res = s.d_it;
+      } else {
+        keep.push_back(p);
+      }
+    }
+    ok.swap(keep);
+  }
+  if (ok.empty()) return kOk;
+  const int n2 = (int)ok.size();
+  {
+    const int* l = put_list(h, ok);
+    SDPX_CHECK(launch_gemv(h->arr, l, n2, 1, h->stream));
+    SDPX_CHECK(launch_dir_block(h->arr, l, n2, h->stream));
+  }
+  return kOk;
+}
+
 int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_out) {
   const int B = h->B;
   std::vector<int> all(B);
@@ -793,6 +1153,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
     S[p].stag.init(o.stall_iterations, o.stall_tolerance);
     S[p].scale_p = h->scale_p[p];
     S[p].scale_d = h->scale_d[p];
+    S[p].mixed = o.precision != 0;
     par(h, p, P_A) = 0.0;
     par(h, p, P_B) = 0.0;
   }
@@ -903,211 +1264,24 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
         ok.push_back(p);
       }
     }
-    SDPX_CHECK(schur_build(h, ok));
-    SDPX_CHECK(schur_factor(h, ok));
     {
-      std::vector<int> keep;
-      for (int p : ok) {
-        if (!S[p].factor_ok) {
-          S[p].status = kBreakdown;
-          S[p].p_res = S[p].p_it;
-          S[p].d_res = S[p].d_it;
-        } else {
-          keep.push_back(p);
-        }
+      std::vector<int> retry;
+      SDPX_CHECK(newton_direction(h, o, ok, retry));
+      if (!retry.empty()) {
+        // the reduced-precision direction failed a guard: redo the step in FP64 and stay there
+        // (R = -X Y (+ mu I) and the block factors are recomputed, bitwise as before)
+        for (int p : retry) S[p].mixed = false;
+        SDPX_CHECK(push_params(h));
+        SDPX_CHECK(launch_block_factor(h->arr, put_list(h, retry), (int)retry.size(), h->stream));
+        std::vector<int> none;
+        SDPX_CHECK(newton_direction(h, o, retry, none));
+        ok.insert(ok.end(), retry.begin(), retry.end());
+        std::sort(ok.begin(), ok.end());
       }
-      ok.swap(keep);
     }
     if (ok.empty()) continue;
-    const int n = (int)ok.size();
-    // predictor (step.jl:699-726)
-    {
-      const int* l = put_list(h, ok);
-      SDPX_CHECK(launch_rhs_block(h->arr, l, n, h->stream));
-      SDPX_CHECK(launch_gemvt(h->arr, l, n, 1, h->stream));
-      SDPX_CHECK(launch_solve(h->arr, l, n, 0, h->stream));
-      SDPX_CHECK(launch_gemv(h->arr, l, n, 1, h->stream));
-      SDPX_CHECK(launch_dir_block(h->arr, l, n, h->stream));
-    }
-    // affine predictor diagnostics (step.jl:546-625)
-    std::vector<int> adapt;
-    for (int p : ok) {
-      if (S[p].iter_adaptive) {
-        adapt.push_back(p);
-        par(h, p, P_PF) = 1.0;
-        par(h, p, P_DF) = 1.0;
-      }
-    }
-    SDPX_CHECK(push_params(h));
-    if (!adapt.empty()) {
-      SDPX_CHECK(launch_ftb(h->arr, put_list(h, adapt), (int)adapt.size(), step_classes(h, adapt), h->stream));
-      SDPX_CHECK(pull_scalars(h));
-    }
-    for (int p : ok) {
-      MemberState& s = S[p];
-      if (s.iter_adaptive) {
-        s.ap = sc(h, p, S_TX);
-        s.ad = sc(h, p, S_TY);
-      } else {
-        s.ap = s.ad = 1.0;
-      }
-      par(h, p, P_A) = s.ap;
-      par(h, p, P_B) = s.ad;
-    }
-    SDPX_CHECK(push_params(h));
-    {
-      const int* l = put_list(h, ok);
-      SDPX_CHECK(launch_comp(h->arr, l, n, h->stream));
-      SDPX_CHECK(launch_factor_extrema(h->arr, l, n, h->stream));
-      SDPX_CHECK(pull_scalars(h));
-    }
-    auto quality = [](double mn, double mx) {
-      return (mx > 0.0 && std::isfinite(mx)) ? std::min(std::max(mn / mx, 0.0), 1.0) : 0.0;
-    };
-    for (int p : ok) {
-      MemberState& s = S[p];
-      const int k = h->mem[p].k;
-      const double comp = sc(h, p, S_COMP), compt = sc(h, p, S_COMPT);
-      const bool uniform = s.iter_adaptive && std::isfinite(comp);
-      s.comp = comp;
-      s.mu_g = comp / k;
-      s.mu_aff = std::max(0.0, compt / k);
-      const double cur = std::max(s.p_it / std::max(s.scale_p, 1.0), s.d_it / std::max(s.scale_d, 1.0));
-      double achieved = 1.0;
-      if (!s.ctrl.history.empty()) {
-        const Hist& row = s.ctrl.history.back();
-        const double prev = std::max(std::max(row.primal_residual / std::max(s.scale_p, 1.0),
-                                              row.dual_residual / std::max(s.scale_d, 1.0)),
-                                     kEps);
-        achieved = cur / prev;
-      }
-      Diag d;
-      d.iteration = s.iteration + 1;
-      d.primal_residual = s.p_it / std::max(s.scale_p, 1.0);
-      d.dual_residual = s.d_it / std::max(s.scale_d, 1.0);
-      d.relative_gap = s.gap_rel;
-      d.mu = s.mu_g;
-      d.mu_aff = s.mu_aff;
-      d.affine_primal_step = s.ap;
-      d.affine_dual_step = s.ad;
-      d.previous_primal_step = s.ctrl.last_primal_step();
-      d.previous_dual_step = s.ctrl.last_dual_step();
-      d.backtracking_count = s.ctrl.last_backtracking();
-      d.regularization = s.reg_attempts <= 0 ? 0.0 : kSqrtEps * std::pow(10.0, s.reg_attempts - 1);
-      d.refinement_count = s.ctrl.last_refinement();
-      d.factorization_quality = quality(sc(h, p, S_SFMIN), sc(h, p, S_SFMAX));
-      d.achieved_residual_reduction = achieved;
-      d.primal_psd_margin = std::min(1.0, quality(sc(h, p, S_LXMIN), sc(h, p, S_LXMAX)));
-      d.dual_psd_margin = std::min(1.0, quality(sc(h, p, S_LYMIN), sc(h, p, S_LYMAX)));
-      s.prm = s.ctrl.select(d);
-      // corrector target (threaded.jl:839-854, 943-1003)
-      if (s.iter_adaptive && !s.prm.fallback) {
-        par(h, p, P_MODE) = uniform ? 2.0 : 1.0;
-        par(h, p, P_SIGMA) = s.prm.sigma;
-        par(h, p, P_MUG) = s.mu_g;
-      } else {
-        par(h, p, P_MODE) = 0.0;
-        par(h, p, P_MU) = s.mu;
-        par(h, p, P_BETA) = s.o_beta;
-      }
-      // refinement policy
-      s.refine_fixed = o.refine_policy == 1;
-      double rtol;
-      if (s.iter_adaptive && !s.prm.fallback) {
-        rtol = s.prm.rtol;
-        s.refine_cap = s.prm.rmax;
-      } else {
-        rtol = o.refine_tol;
-        s.refine_cap = o.refine_max_steps;
-      }
-      if (s.refine_fixed) s.refine_cap = o.refine_steps;
-      s.refine_abstol = rtol > 0 ? rtol : kRefineTolUlps * kEps;  // scaled by ||r|| below
-    }
-    SDPX_CHECK(push_params(h));
-    // corrector (step.jl:846-936)
-    {
-      const int* l = put_list(h, ok);
-      SDPX_CHECK(launch_corrector_block(h->arr, l, n, h->stream));
-      SDPX_CHECK(launch_rhs_block(h->arr, l, n, h->stream));
-      SDPX_CHECK(launch_gemvt(h->arr, l, n, 1, h->stream));
-      SDPX_CHECK(launch_solve(h->arr, l, n, 0, h->stream));
-      SDPX_CHECK(launch_residual(h->arr, l, n, h->stream));
-      SDPX_CHECK(pull_scalars(h));
-    }
-    // refine! (kkt_backend.jl:476-485 -> kkt.jl:3891-3963), member by member in rounds
-    int max_cap = 0;
-    for (int p : ok) {
-      MemberState& s = S[p];
-      s.refine_steps = 0;
-      s.refine_residual = sc(h, p, S_RHO);
-      s.refine_abstol = s.refine_fixed ? 0.0 : s.refine_abstol * std::max(sc(h, p, S_RNORM), 1.0);
-      s.refine_active = s.refine_cap > 0;
-      max_cap = std::max(max_cap, s.refine_cap);
-    }
-    for (int it = 0; it < max_cap; ++it) {
-      std::vector<int> act;
-      for (int p : ok) {
-        MemberState& s = S[p];
-        if (!s.refine_active) continue;
-        if (it >= s.refine_cap || (!s.refine_fixed && s.refine_residual <= s.refine_abstol)) {
-          s.refine_active = false;
-          continue;
-        }
-        act.push_back(p);
-      }
-      if (act.empty()) break;
-      const int* l = put_list(h, act);
-      const int na = (int)act.size();
-      SDPX_CHECK(launch_vec_op(h->arr, l, na, 0, h->stream));
-      SDPX_CHECK(launch_solve(h->arr, l, na, 1, h->stream));
-      SDPX_CHECK(launch_vec_op(h->arr, l, na, 1, h->stream));
-      SDPX_CHECK(launch_residual(h->arr, l, na, h->stream));
-      SDPX_CHECK(pull_scalars(h));
-      std::vector<int> back;
-      for (int p : act) {
-        MemberState& s = S[p];
-        const double corrected = sc(h, p, S_RHO);
-        if (!std::isfinite(corrected) || corrected > s.refine_residual) {
-          back.push_back(p);
-          s.refine_active = false;
-          continue;
-        }
-        s.refine_steps += 1;
-        const bool small = corrected > s.refine_residual * kRefineMinDecrease;
-        s.refine_residual = corrected;
-        if (!s.refine_fixed && small) s.refine_active = false;
-      }
-      if (!back.empty()) {
-        const int* lb = put_list(h, back);
-        SDPX_CHECK(launch_vec_op(h->arr, lb, (int)back.size(), 2, h->stream));
-        SDPX_CHECK(launch_residual(h->arr, lb, (int)back.size(), h->stream));
-      }
-    }
-    SDPX_CHECK(launch_residual(h->arr, put_list(h, ok), n, h->stream));
-    SDPX_CHECK(pull_scalars(h));
-    {
-      std::vector<int> keep;
-      for (int p : ok) {
-        MemberState& s = S[p];
-        const double res = sc(h, p, S_RHO), r_norm = sc(h, p, S_RNORM);
-        const double tol = std::max(kSqrtEps, std::max(o.eps_primal, o.eps_dual)) * std::max(r_norm, 1.0);
-        if (!std::isfinite(res) || res > tol) {
-          s.status = kBreakdown;
-          s.p_res = s.p_it;
-          s.d_res = s.d_it;
-        } else {
-          keep.push_back(p);
-        }
-      }
-      ok.swap(keep);
-    }
-    if (ok.empty()) continue;
-    const int n2 = (int)ok.size();
-    {
-      const int* l = put_list(h, ok);
-      SDPX_CHECK(launch_gemv(h->arr, l, n2, 1, h->stream));
-      SDPX_CHECK(launch_dir_block(h->arr, l, n2, h->stream));
-    }
+    for (int p : ok)
+      if (S[p].mixed) S[p].mixed_iterations += 1;
     // ---- line search (interior_point.jl:1962; threaded.jl:1589-1600) -------------------------------
     std::vector<int> ftb, bt;
     for (int p : ok) {
@@ -1308,6 +1482,9 @@ int sdpx_batch_default_options(sdpx_batch_options_t* o, double tol) {
   o->adaptive = 1;
   o->step_rule = 0;
   o->precision = 0;
+  o->cond_limit = 1e6;
+  o->refine_target = 1e-10;
+  o->mixed_max_refine = 32;
   o->adjoint_mu = 1e-6;
   return kOk;
 }
@@ -1350,6 +1527,7 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
     M.kp = (k[p] + 15) / 16 * 16;
     const long long k2 = (long long)k[p] * k[p];
     M.ldP = k2 + (k2 & 1);
+    M.ldPf = (k2 + 3) / 4 * 4;  // the FP32 panel (4 bytes each) fits in the FP64 one: ldPf <= 2 ldP
     M.oAv = h->nAv;
     h->nAv += align_up(k2 * m[p]);
     M.oPh = h->nPh;
@@ -1479,9 +1657,17 @@ int sdpx_batch_create(int device, int B, const int* k, const int* m, sdpx_batch_
     h->gram_tiles = tiles;
     tiles = gram_tile_order(h, 0);
     h->ngram_tiles = (int)tiles.size();
+    std::vector<GroupGramF32> g32(B);
+    for (int p = 0; p < B; ++p) {
+      const Member& M = h->mem[p];
+      g32[p] = GroupGramF32{reinterpret_cast<const float*>(a.Ph + M.oPh), a.S + M.oS, M.ldPf, (long long)M.m, M.m,
+                            (int)M.ldPf, a.si + (long long)p * NISLOT + I_SKIP32};
+    }
     if ((s = balloc_n(h, &h->dgram, B))) return fail(s);
+    if ((s = balloc_n(h, &h->dgram32, B))) return fail(s);
     if ((s = balloc_n(h, &h->dgram_tiles, h->ngram_tiles))) return fail(s);
-    if (hipMemcpy(h->dgram, g.data(), sizeof(GroupGemm) * B, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(h->dgram32, g32.data(), sizeof(GroupGramF32) * B, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->dgram, g.data(), sizeof(GroupGemm) * B, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->dgram_tiles, tiles.data(), sizeof(GroupTile) * tiles.size(), hipMemcpyHostToDevice) !=
             hipSuccess)
       return fail(kHipError);
@@ -1905,8 +2091,9 @@ int sdpx_batch_solve(sdpx_batch_t* h, const sdpx_batch_options_t* o, int* iterat
     set_last_error("sdpx_batch_solve: options, data (set_dense / set_coo) and a start point are required");
     return kInvalidArgument;
   }
-  if (o->precision != 0) {
-    set_last_error("sdpx_batch_solve: only precision 0 (plain FP64) is available");
+  if ((o->precision != 0 && o->precision != 1) || o->mixed_max_refine < 0) {
+    set_last_error("sdpx_batch_solve: precision 0 (plain FP64) or 1 (FP32 Schur assembly, FP64 refinement), "
+                   "mixed_max_refine >= 0");
     return kInvalidArgument;
   }
   if (!(o->beta > 0 && o->beta < 1) || !(o->gamma > 0 && o->gamma < 1) || o->step_rule < 0 || o->step_rule > 2) {
@@ -2117,6 +2304,7 @@ int sdpx_batch_schur_at_start(sdpx_batch_t* h) {
     all[p] = p;
     par(h, p, P_MU) = 0.0;
     par(h, p, P_AFF) = 1.0;
+    h->st[p].mixed = h->precision != 0;
   }
   SDPX_CHECK(push_params(h));
   SDPX_CHECK(launch_block_factor(h->arr, put_list(h, all), h->B, h->stream));
@@ -2128,6 +2316,26 @@ int sdpx_batch_schur_at_start(sdpx_batch_t* h) {
     }
   SDPX_CHECK(schur_build(h, all));
   SDPX_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int sdpx_batch_set_precision(sdpx_batch_t* h, int precision) {
+  SDPX_CHECK(check_batch(h));
+  if (precision != 0 && precision != 1) {
+    set_last_error("sdpx_batch_set_precision: precision 0 (FP64) or 1 (FP32)");
+    return kInvalidArgument;
+  }
+  h->precision = precision;
+  return kOk;
+}
+
+int sdpx_batch_get_mixed_iterations(sdpx_batch_t* h, int* out) {
+  SDPX_CHECK(check_batch(h));
+  if (!h->solved || !out) {
+    set_last_error("sdpx_batch_get_mixed_iterations: no solve has completed, or null output");
+    return kInvalidArgument;
+  }
+  for (int p = 0; p < h->B; ++p) out[p] = h->st[p].mixed_iterations;
   return kOk;
 }
 
@@ -2159,7 +2367,9 @@ int sdpx_batch_autotune(sdpx_batch_t* h, int repeats, double* out) {
     std::vector<float> ms;
     for (int r = 0; r <= repeats && rc == kOk; ++r) {  // the first launch of an order is a warm-up
       if (hipEventRecord(e0, h->stream) != hipSuccess) rc = kHipError;
-      if (rc == kOk) rc = gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream);
+      if (rc == kOk)
+        rc = h->precision != 0 ? gram_f32_grouped(h->dgram32, h->dgram_tiles, h->ngram_tiles, h->stream)
+                               : gemm_f64_grouped(h->dgram, h->dgram_tiles, h->ngram_tiles, true, false, true, h->stream);
       if (rc == kOk && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
         rc = kHipError;
       float f = 0.f;

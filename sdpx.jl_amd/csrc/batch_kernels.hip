@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NT) void sp_gemvt_kernel(Arrays a, const int* list,
   const int k2 = M.k * M.k;
   const int i0 = blockIdx.y * kSpConstraints, i1 = min(M.m, i0 + kSpConstraints);
   if (i0 >= M.m) return;  // whole work-group
-  const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
+  const double* src = (mode == 1 || mode == 3 ? a.W2 : a.Y) + M.oK;
   for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -247,6 +247,8 @@ __global__ __launch_bounds__(NT) void sp_gemvt_kernel(Arrays a, const int* list,
       const long long o = M.oM + i;
       if (mode == 1)
         a.r[o] = -(a.d[o] + s);
+      else if (mode == 3)
+        a.rho[o] = a.r[o] - s;
       else
         (mode == 0 ? a.d : a.v)[o] = a.c[o] - s;
     }
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(NT) void gemvt_kernel(Arrays a, const int* list, in
   if (i0 >= M.m) return;
   const int i1 = min(M.m, i0 + kColChunk);
   const int k2 = M.k * M.k;
-  const double* src = (mode == 1 ? a.W2 : a.Y) + M.oK;
+  const double* src = (mode == 1 || mode == 3 ? a.W2 : a.Y) + M.oK;
   for (int e = threadIdx.x; e < k2; e += NT) W[e] = src[e];
   __syncthreads();
   const double* Av = a.Av + M.oAv;
@@ -312,6 +314,8 @@ __global__ __launch_bounds__(NT) void gemvt_kernel(Arrays a, const int* list, in
         const double s = h == 0 ? s0 : s1;
         if (mode == 1)
           a.r[o] = -(a.d[o] + s);
+        else if (mode == 3)
+          a.rho[o] = a.r[o] - s;
         else
           (mode == 0 ? a.d : a.v)[o] = a.c[o] - s;
       }
@@ -319,15 +323,26 @@ __global__ __launch_bounds__(NT) void gemvt_kernel(Arrays a, const int* list, in
   }
 }
 
-// ||d||_inf (mode 0, S_DRES) or ||v||_inf (mode 2, S_DSOL) after gemvt
+// ||d||_inf (mode 0, S_DRES) or ||v||_inf (mode 2, S_DSOL) after gemvt; mode 3: ||rho||_inf
+// (S_RHO) and ||r||_inf (S_RNORM)
 __global__ __launch_bounds__(NT) void vec_norm_kernel(Arrays a, const int* list, int mode) {
   __shared__ double red[NT];
   const int p = list[blockIdx.x];
   const Member M = a.mem[p];
-  const double* v = (mode == 0 ? a.d : a.v) + M.oM;
+  const double* v = (mode == 0 ? a.d : mode == 3 ? a.rho : a.v) + M.oM;
   double amax = 0.0;
   for (int i = threadIdx.x; i < M.m; i += NT) amax = nan_max(amax, fabs(v[i]));
   amax = block_max(amax, red);
+  if (mode == 3) {
+    double rmax = 0.0;
+    for (int i = threadIdx.x; i < M.m; i += NT) rmax = nan_max(rmax, fabs(a.r[M.oM + i]));
+    rmax = block_max(rmax, red);
+    if (threadIdx.x == 0) {
+      a.sc[(long long)p * NSLOT + S_RHO] = amax;
+      a.sc[(long long)p * NSLOT + S_RNORM] = rmax;
+    }
+    return;
+  }
   if (threadIdx.x == 0) a.sc[(long long)p * NSLOT + (mode == 0 ? S_DRES : S_DSOL)] = amax;
 }
 
@@ -406,6 +421,26 @@ __global__ __launch_bounds__(NT) void rhs_block_kernel(Arrays a, const int* list
   lds_load(sP, a.Xinv + o, k);
   __syncthreads();
   lds_mm(sY, sP, false, sA, false, k, 1.0, 0.0);
+  __syncthreads();
+  lds_store(a.W2 + o, sY, k);
+}
+
+// W2 = X^-1 dX Y, dX = sum_j dx_j A_j (gemv mode 3): the k x k part of the matrix-free FP64 Schur
+// operator (S dx)_i = <A_i, X^-1 dX Y> (HKM: S_ij = tr(A_i X^-1 A_j Y))
+__global__ __launch_bounds__(NT) void exact_block_kernel(Arrays a, const int* list) {
+  __shared__ double sD[KMAX * LP], sY[KMAX * LP], sA[KMAX * LP];
+  const int p = list[blockIdx.x];
+  const Member M = a.mem[p];
+  const int k = M.k;
+  const long long o = M.oK;
+  lds_load(sD, a.dX + o, k);
+  lds_load(sY, a.Y + o, k);
+  __syncthreads();
+  lds_mm(sA, sD, false, sY, false, k, 1.0, 0.0);  // dX Y
+  __syncthreads();
+  lds_load(sD, a.Xinv + o, k);
+  __syncthreads();
+  lds_mm(sY, sD, false, sA, false, k, 1.0, 0.0);
   __syncthreads();
   lds_store(a.W2 + o, sY, k);
 }
@@ -1264,7 +1299,7 @@ constexpr PanelTiles panel_tiles(int w) {
 // them is taken by every wave of the work-group: one instance per wave).
 template <int KP, int W>
 __device__ __forceinline__ void panel_products(const double* mA, double* mT, const double* sLi, const double* sLy,
-                                               double* Pi, int k, int lr, int lq) {
+                                               double* Pi, float* Pf, int k, int lr, int lq) {
   using S = PanelShape<KP>;
   constexpr int LPA = S::LPA, LPB = S::LPB;
   constexpr PanelTiles ts = panel_tiles<KP>(W);
@@ -1310,6 +1345,16 @@ __device__ __forceinline__ void panel_products(const double* mA, double* mT, con
           if (row < k && col < k) Pi[row + col * k] = acc[u][rr];
         }
       }
+    } else if (Pf) {
+#pragma unroll
+      for (int u = 0; u < ts.n; ++u) {
+        const int row = ts.r0[u] + lr;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int col = ts.c0[u] + lq + 4 * rr;
+          if (row < k && col < k) Pf[row + col * k] = (float)acc[u][rr];
+        }
+      }
     }
   }
 }
@@ -1322,7 +1367,9 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
   const int2 w = work[blockIdx.x];
   const int p = w.x;
   const Member M = a.mem[p];
-  if (a.si[(long long)p * NISLOT + I_SKIP] != 0) return;
+  const bool skip64 = a.si[(long long)p * NISLOT + I_SKIP] != 0;
+  if (skip64 && a.si[(long long)p * NISLOT + I_SKIP32] != 0) return;
+  const bool f32 = skip64;  // FP32 Phat for the mixed-precision Gram product
   const int k = M.k;
   const int i0 = w.y, i1 = min(M.m, w.y + chunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1363,16 +1410,18 @@ __global__ __launch_bounds__(PT) void panel_kernel(Arrays a, const int2* work, i
       if (loff[j] >= 0) mA[loff[j]] = pf[j];
     prefetch(i + CP);
     __syncthreads();
-    double* Pi = i < i1 ? a.Ph + M.oPh + (long long)i * M.ldP : nullptr;
+    double* Pi = (!f32 && i < i1) ? a.Ph + M.oPh + (long long)i * M.ldP : nullptr;
+    float* Pf = (f32 && i < i1) ? reinterpret_cast<float*>(a.Ph + M.oPh) + (long long)i * M.ldPf : nullptr;
+    if (Pf && st < M.ldPf - k2) Pf[k2 + st] = 0.f;  // padding rows read by the FP32 Gram product
     switch (lw) {
-      case 0: panel_products<KP, 0>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 1: panel_products<KP, 1>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 2: panel_products<KP, 2>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 3: panel_products<KP, 3>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 4: panel_products<KP, 4>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 5: panel_products<KP, 5>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      case 6: panel_products<KP, 6>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
-      default: panel_products<KP, 7>(mA, mT, sLi, sLy, Pi, k, lr, lq); break;
+      case 0: panel_products<KP, 0>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 1: panel_products<KP, 1>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 2: panel_products<KP, 2>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 3: panel_products<KP, 3>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 4: panel_products<KP, 4>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 5: panel_products<KP, 5>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      case 6: panel_products<KP, 6>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
+      default: panel_products<KP, 7>(mA, mT, sLi, sLy, Pi, Pf, k, lr, lq); break;
     }
     // no barrier here: the next pass writes mA after every wave has passed the
     // barrier between the products (its product-1 reads are done) and mT after
@@ -1603,6 +1652,9 @@ int launch_shift_copy(const Arrays& a, const int* list, int n, hipStream_t st) {
 }
 int launch_factor_extrema(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(factor_extrema_kernel, n, a, list);
+}
+int launch_exact_block(const Arrays& a, const int* list, int n, hipStream_t st) {
+  SDPX_LAUNCH(exact_block_kernel, n, a, list);
 }
 int launch_rhs_block(const Arrays& a, const int* list, int n, hipStream_t st) {
   SDPX_LAUNCH(rhs_block_kernel, n, a, list);

@@ -31,7 +31,9 @@ enum Slot : int {
   NSLOT = 24
 };
 // per-member integer slots
-enum ISlot : int { I_INFOX = 0, I_INFOY, I_INFOS, I_SKIP, NISLOT = 4 };
+// I_SKIP: the member's FP64 Schur assembly is skipped; I_SKIP32: its FP32 assembly (mixed
+// precision: the panel pass writes Phat in FP32 and the FP32 grouped Gram product reads it)
+enum ISlot : int { I_INFOX = 0, I_INFOY, I_INFOS, I_SKIP, I_SKIP32, NISLOT = 5 };
 // per-member parameters (double), written by the host
 enum Param : int {
   P_MU = 0, P_A, P_B, P_PF, P_DF, P_GAMMA, P_MINSTEP, P_RATIO, P_SIGMA, P_MUG, P_MODE, P_BETA, P_REG, P_SCALE,
@@ -44,6 +46,7 @@ struct Member {
   long long oAv;           // Av: k^2 x m, column i = vec(A_i)
   long long oPh;           // Phat: ldP x m
   long long ldP;           // k^2 rounded up to even
+  long long ldPf;          // k^2 rounded up to a multiple of 4: the FP32 Phat (floats at the same offset)
   long long oK;            // offset of this member's k x k blocks
   long long oM;            // offset of its m-vectors
   long long oS;            // offset of its m x m matrices
@@ -85,16 +88,20 @@ struct Arrays {
 // ---- launchers (batch_kernels.hip) ------------------------------------------
 // mode 0: P = Av x - X - C, S_PRES;  1: dX = Av dx + P;  2: W1 = Av x - X - C, S_PSOL;  3: dX = Av dx
 int launch_gemv(const Arrays& a, const int* list, int n, int mode, hipStream_t st);
-// mode 0: d = c - Av' vec(Y), S_DRES;  1: r = -(d + Av' vec(W2));  2: v = c - Av' vec(Y), S_DSOL
+// mode 0: d = c - Av' vec(Y), S_DRES;  1: r = -(d + Av' vec(W2));  2: v = c - Av' vec(Y), S_DSOL;
+// 3: rho = r - Av' vec(W2), S_RHO = ||rho||_inf, S_RNORM = ||r||_inf
 int launch_gemvt(const Arrays& a, const int* list, int n, int mode, hipStream_t st);
 // R = -X Y (+ P_MU I unless P_AFF); L_X, L_Y, their |diag| extrema and info; L_X^-1, X^-1
 int launch_block_factor(const Arrays& a, const int* list, int n, hipStream_t st);
-// Phat_i = L_X^-1 A_i L_Y for the (member, first constraint) items of `work`
+// Phat_i = L_X^-1 A_i L_Y for the (member, first constraint) items of `work`: in FP64 (ldP) for
+// members with I_SKIP == 0, in FP32 (ldPf, padding rows zeroed) for members with I_SKIP32 == 0
 int launch_panel(const Arrays& a, const int2* work, const int* off, int chunk, hipStream_t st);
 // Sf = lower(S) + P_REG * max(|S_ii|, 1) on the diagonal; info S reset
 int launch_shift_copy(const Arrays& a, const int* list, int n, hipStream_t st);
 // |diag| extrema of the Schur factor
 int launch_factor_extrema(const Arrays& a, const int* list, int n, hipStream_t st);
+// W2 = X^-1 dX Y (the matrix-free FP64 Schur operator: (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y>)
+int launch_exact_block(const Arrays& a, const int* list, int n, hipStream_t st);
 // W2 = X^-1 (P Y - R)
 int launch_rhs_block(const Arrays& a, const int* list, int n, hipStream_t st);
 // mode 0: dx = S^-1 r;  1: delta = S^-1 rho   (two triangular solves on Sf)

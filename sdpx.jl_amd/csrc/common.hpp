@@ -146,6 +146,19 @@ int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, 
 // H = bf16(src), L = bf16(src - H) (ld ld_dst, rows >= `rows` zeroed), rows x cols
 int split_panel_bf16(const double* src, long long ld_src, __bf16* H, __bf16* L, long long ld_dst, long long rows,
                      int cols, hipStream_t stream);
+// Grouped FP32 Schur Gram products (gemm_f32.hip): for each listed GEMM g,
+// C_g (lower, FP64, leading dim ldc) = P_g' P_g over 64 x 64 lower tiles (the
+// GroupTile list: g, ti, tj), P_g an FP32 panel of M columns of K floats
+// (K and ldp multiples of 4, rows K..ldp-1 unread, 16-byte aligned base).
+// A GEMM whose *skip != 0 is skipped.
+struct GroupGramF32 {
+  const float* P;
+  double* C;
+  long long ldp, ldc;
+  int M, K;
+  const int* skip;
+};
+int gram_f32_grouped(const GroupGramF32* groups, const GroupTile* tiles, int ntiles, hipStream_t stream);
 // dst (ld_dst >= rows, padding rows zeroed) = (float) src (ld_src), rows x cols
 int convert_panel_f32(const double* src, long long ld_src, float* dst, long long ld_dst, long long rows, int cols,
                       hipStream_t stream);

@@ -173,6 +173,99 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* ws, int 
   }
 }
 
+// ---- grouped FP32 Gram (batched route) ------------------------------------------
+// One work-group per (GEMM, 64 x 64 lower tile) item: 4 waves in 2 x 2, each a
+// 32 x 32 quadrant (2 x 2 blocks of v_mfma_f32_16x16x4_f32), BK = 16 slices
+// double-buffered through LDS, the next slice prefetched into registers as one
+// float4 per thread and operand.  LDS rows (one K value, 64 tile rows) have a
+// pitch of 84 floats: the transposing stores (4 K values x 8 rows per 32
+// lanes) and the fragment reads (2 K values x 16 rows per 32 lanes) both land
+// on distinct banks.  A diagonal tile stages one operand (A == B) and its
+// upper-right quadrant wave skips its MFMAs (nothing of it is stored).
+constexpr int GBM = 64;
+constexpr int GPITCH = GBM + 20;
+
+__global__ __launch_bounds__(256, 2) void gram_grouped_f32_kernel(const GroupGramF32* __restrict__ G,
+                                                                  const GroupTile* __restrict__ T) {
+  const GroupTile t = T[blockIdx.x];
+  const GroupGramF32 g = G[t.g];
+  if (g.skip && *g.skip != 0) return;
+  __shared__ float As[2][BK][GPITCH], Bs[2][BK][GPITCH];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
+  const int m0 = t.ti * GBM, n0 = t.tj * GBM;
+  const bool diag = t.ti == t.tj;
+  const bool idle = diag && wn0 > wm0;  // quadrant above the diagonal of a diagonal tile
+  const int M = g.M, K = g.K;
+  const float* P = g.P;
+  const int lr = tid >> 2, lk = (tid & 3) * 4;
+  const long long ra = (long long)min(m0 + lr, M - 1) * g.ldp;
+  const long long rb = (long long)min(n0 + lr, M - 1) * g.ldp;
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  float4_t va, vb;
+  auto load = [&](int k0) {
+    const int kk = k0 + lk;
+    const bool in = kk < K;  // K is a multiple of 4: a float4 is wholly in or out
+    va = in ? *reinterpret_cast<const float4_t*>(P + ra + kk) : float4_t{0.f, 0.f, 0.f, 0.f};
+    if (!diag) vb = in ? *reinterpret_cast<const float4_t*>(P + rb + kk) : float4_t{0.f, 0.f, 0.f, 0.f};
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      As[buf][lk + e][lr] = va[e];
+      if (!diag) Bs[buf][lk + e][lr] = vb[e];
+    }
+  };
+  auto compute = [&](int cur) {
+    if (idle) return;
+    const float(*Bt)[GPITCH] = diag ? As[cur] : Bs[cur];
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      float fa[2], fb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) fa[i] = As[cur][kr][wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) fb[j] = Bt[kr][wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j], fa[i], acc[i][j], 0, 0, 0);
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  int cur = 0;
+  int k0 = 0;
+  for (; k0 + BK < K; k0 += BK) {
+    load(k0 + BK);
+    compute(cur);
+    store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
+  }
+  compute(cur);
+  if (idle) return;
+  // acc[i][j][r] = C(row = m0 + wm0 + 16 i + lane % 16, col = n0 + wn0 + 16 j + 4 (lane / 16) + r)
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int row = m0 + wm0 + i * 16 + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + 4 * (lane >> 4) + r;
+        if (row < M && row >= col) g.C[row + (long long)col * g.ldc] = (double)acc[i][j][r];
+      }
+  }
+}
+
 __global__ void panel_f32_kernel(const double* __restrict__ src, long long ld_src, float* __restrict__ dst,
                                  long long ld_dst, long long rows, int cols) {
   const long long n = ld_dst * cols;
@@ -391,6 +484,13 @@ int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long l
                        stream, ws, split, m, S, lds, beta);
     SDPX_HIP(hipGetLastError());
   }
+  return kOk;
+}
+
+int gram_f32_grouped(const GroupGramF32* groups, const GroupTile* tiles, int ntiles, hipStream_t stream) {
+  if (ntiles <= 0) return kOk;
+  hipLaunchKernelGGL(gram_grouped_f32_kernel, dim3((unsigned)ntiles), dim3(256), 0, stream, groups, tiles);
+  SDPX_HIP(hipGetLastError());
   return kOk;
 }
 

@@ -378,11 +378,14 @@ mutable struct BatchOptions
     refine_policy::Cint; refine_steps::Cint; refine_max_steps::Cint; adaptive::Cint; step_rule::Cint
     precision::Cint
     cond_limit::Float64; refine_target::Float64; adjoint_mu::Float64
+    mixed_max_refine::Cint
     BatchOptions() = new()
 end
 
 """Reference defaults (SolverOptions{Float64}(), types/workspaces.jl:101-228) at tolerance tol,
 overridden by the options SDPX passes."""
+"""precision 1: FP32 Schur assembly with FP64 refinement (kernels/mixed_precision_kkt.jl template);
+a member whose reduced-precision step fails a guard continues in FP64."""
 function BatchOptions(opts::SDPX.SolverOptions{Float64}; precision::Integer=0)
     o = BatchOptions()
     _check(ccall(_sym(:sdpx_batch_default_options), Cint, (Ref{BatchOptions}, Float64), o, opts.ϵ_gap),

@@ -33,11 +33,13 @@ class BatchOptions(C.Structure):
         ("stall_iterations", C.c_int), ("restart", C.c_int), ("refine_policy", C.c_int),
         ("refine_steps", C.c_int), ("refine_max_steps", C.c_int), ("adaptive", C.c_int), ("step_rule", C.c_int),
         ("precision", C.c_int), ("cond_limit", C.c_double), ("refine_target", C.c_double),
-        ("adjoint_mu", C.c_double),
+        ("adjoint_mu", C.c_double), ("mixed_max_refine", C.c_int),
     ]
 
     @classmethod
-    def from_solver_options(cls, o: SolverOptions, precision: int = 0) -> "BatchOptions":
+    def from_solver_options(cls, o: SolverOptions, precision: int | None = None) -> "BatchOptions":
+        """precision None follows o.precision ("fp64" -> 0, "mixed" -> 1); the mixed route takes
+        o.mixed_cond_limit, o.mixed_refine_target and o.mixed_max_refine."""
         b = cls()
         check(_lib.load().sdpx_batch_default_options(C.addressof(b), o.eps_gap), "sdpx_batch_default_options")
         for name in ("beta", "gamma", "eps_gap", "eps_primal", "eps_dual", "min_step", "omega_step",
@@ -48,7 +50,14 @@ class BatchOptions(C.Structure):
         b.refine_policy = 1 if o.refine_policy == "fixed" else 0
         b.adaptive = 1 if o.parameter_strategy == "adaptive" else 0
         b.step_rule = {"auto": 0, "backtrack": 1, "fraction_to_boundary": 2}[o.step_rule]
+        if precision is None:
+            if o.precision == "mixed_bf16x3":
+                raise ValueError("the batched route assembles in FP64 or FP32 (precision 'fp64' or 'mixed')")
+            precision = 1 if o.precision == "mixed" else 0
         b.precision = precision
+        b.cond_limit = o.mixed_cond_limit
+        b.refine_target = o.mixed_refine_target
+        b.mixed_max_refine = o.mixed_max_refine
         return b
 
 
@@ -296,6 +305,17 @@ class BatchSolver:
             d = dict(zip(self.TRACE_COLUMNS, r.tolist()))
             d["outcome"] = names[int(d["outcome"])]
             out.append(d)
+        return out
+
+    def set_precision(self, precision: int) -> None:
+        """Assembly precision of schur_at_start and autotune: 0 FP64, 1 FP32 (mixed precision)."""
+        check(self._lib.sdpx_batch_set_precision(self._h, int(precision)), "sdpx_batch_set_precision")
+
+    def mixed_iterations(self) -> np.ndarray:
+        """Per member: Newton steps of the last solve taken with the reduced-precision Schur matrix."""
+        out = np.zeros(self.B, dtype=np.int32)
+        check(self._lib.sdpx_batch_get_mixed_iterations(self._h, out.ctypes.data_as(_lib._ip)),
+              "sdpx_batch_get_mixed_iterations")
         return out
 
     def schur_at_start(self) -> int:

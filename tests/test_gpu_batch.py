@@ -241,6 +241,115 @@ def test_batch_coo_gather_matches_dense_panel(gpu_lib):
         assert abs(res_s.pobj[p] - res_d.pobj[p]) <= 1e-9 * max(1.0, abs(res_d.pobj[p]))
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["dense", "coo"])
+def test_batch_fp32_schur_close_to_oracle(gpu_lib, kind):
+    """Mixed-precision assembly: Phat rounded to FP32 by the panel pass and the grouped FP32 Gram
+    product on FP32 MFMA.  The Schur matrix differs from the oracle's FP64 one by FP32 rounding only
+    (relative Frobenius <= 2e-6 here; the same members assemble to 1e-12 in FP64), and it is not
+    the FP64 matrix (the FP32 route really ran)."""
+    rng = np.random.default_rng(5)
+    ks = [3, 8, 17, 33, 64, 1, 2, 40]
+    ms = [7, 20, 50, 70, 130, 3, 5, 300]
+    probs = [G.dense_feasible(k, m, seed=10 + i) if kind == "dense" else G.sparse_feasible(max(k, 1), m, seed=10 + i)
+             for i, (k, m) in enumerate(zip(ks, ms))]
+    Xs = [_random_pd(rng, k) for k in ks]
+    Ys = [_random_pd(rng, k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], Xs, Ys)
+        bs.set_precision(1)
+        assert bs.schur_at_start() == 0
+        S32 = [bs.schur(p) for p in range(len(ks))]
+        bs.set_precision(0)
+        assert bs.schur_at_start() == 0
+        S64 = [bs.schur(p) for p in range(len(ks))]
+    for p, g in enumerate(probs):
+        prob = _oracle_problem(g)
+        LX, _ = O.kchol(Xs[p])
+        MY, _ = O.kchol(Ys[p])
+        ref = O.schur_build(prob, [LX], [MY])
+        e32, e64 = rel_fro_lower(S32[p], ref), rel_fro_lower(S64[p], ref)
+        print(f"member {p}: k={ks[p]} m={ms[p]} fp32 rel={e32:.2e} fp64 rel={e64:.2e}", flush=True)
+        assert e32 <= 2e-6 and e64 <= 1e-12
+        if ks[p] > 2:
+            assert e32 > 1e-12  # rounded to FP32 (k <= 2 panels can be exact in FP32)
+
+
+def _mixed_options(tol, **kw):
+    from sdpx_mi355x.batch import BatchOptions
+
+    return BatchOptions.from_solver_options(SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol,
+                                                          precision="mixed", **kw))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workload", ["dense", "sparse", "layer"])
+def test_batch_mixed_members_match_oracle(gpu_lib, workload):
+    """Mixed precision in the batched route: FP32 Schur assembly, every direction refined against the
+    matrix-free FP64 operator.  Each member reaches the FP64 oracle's optimum (status, objective 1e-8
+    relative, iterations +-2, certificate), and the reduced-precision Schur matrix carries most steps."""
+    if workload == "dense":
+        ks = [5, 12, 20, 32, 9, 3, 2, 1]
+        ms = [12, 40, 60, 128, 30, 6, 3, 2]
+        probs = [G.dense_feasible(k, m, seed=100 + i) for i, (k, m) in enumerate(zip(ks, ms))]
+        tol = 1e-8
+    elif workload == "sparse":
+        probs = G.batched_sparse(6, seed=5, kmin=8, kmax=24)
+        tol = 1e-8
+    else:
+        probs = G.batched_dense_layer(4, seed=2)
+        tol = 1e-9
+    ks = [g.k for g in probs]
+    ms = [g.m for g in probs]
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    oo = O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol)
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], [np.eye(k) for k in ks], [np.eye(k) for k in ks])
+        res = bs.solve(_mixed_options(tol))
+        reduced = bs.mixed_iterations()
+    print(f"reduced-precision steps per member: {reduced.tolist()} of {res.iterations.tolist()}", flush=True)
+    _check_members(res, probs, o, oo)
+    big = [p for p in range(len(ks)) if ks[p] >= 5]
+    assert all(reduced[p] >= 1 for p in big)
+    assert sum(int(reduced[p]) for p in big) >= 0.5 * sum(int(res.iterations[p]) for p in big)
+
+
+@pytest.mark.gpu
+def test_batch_mixed_guard_fallback_is_the_fp64_solve(gpu_lib):
+    """A pivot-ratio limit no Schur matrix meets sends every member back to FP64 at its first step,
+    before any reduced-precision direction is used: the solve is then the FP64 solve bit for bit
+    (the redone step recomputes the block factors and rebuilds S in FP64)."""
+    probs = G.batched_sparse(5, seed=11, kmin=6, kmax=20)
+    ks = [g.k for g in probs]
+    ms = [g.m for g in probs]
+    tol = 1e-8
+    eyes = [np.eye(k) for k in ks]
+    with BatchSolver(ks, ms) as bs:
+        _load(bs, probs)
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        r64 = bs.solve(SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol))
+        om = _mixed_options(tol)
+        om.cond_limit = 0.5  # below every (max L_ii / min L_ii)^2 >= 1
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        rm = bs.solve(om)
+        reduced = bs.mixed_iterations()
+        # no refinement steps allowed: every member fails the predictor or corrector guard
+        oz = _mixed_options(tol)
+        oz.mixed_max_refine = 0
+        bs.set_start([g.x0 for g in probs], eyes, eyes)
+        rz = bs.solve(oz)
+        reduced_z = bs.mixed_iterations()
+    assert reduced.tolist() == [0] * len(ks)
+    assert rm.status == r64.status
+    assert rm.iterations.tolist() == r64.iterations.tolist()
+    assert np.array_equal(rm.pobj, r64.pobj) and np.array_equal(rm.dobj, r64.dobj)
+    assert reduced_z.tolist() == [0] * len(ks)
+    assert all(s == "Optimal" for s in rz.status)
+    assert np.allclose(rz.pobj, r64.pobj, rtol=1e-8, atol=1e-8)
+
+
 def _full_batch(workload):
     if workload == "layer":
         probs, tol = G.batched_dense_layer(1024, seed=0), 1e-9

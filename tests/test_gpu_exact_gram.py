@@ -93,10 +93,15 @@ def test_gpu_reduced_precision_cold_start_gram_is_bit_exact(gpu_lib, k, m, preci
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kind", ["dense", "coo"])
-def test_gpu_batched_cold_start_gram_is_bit_exact(gpu_lib, kind):
+def test_gpu_batched_cold_start_gram_is_bit_exact(gpu_lib, kind, precision):
     """Every padded block size of the batched panel kernels (k = 3 .. 64) and both
-    input forms; members use the same integer pattern with their own block index."""
+    input forms; members use the same integer pattern with their own block index.
+    Precision 1 runs the FP32 panel output and the grouped FP32 Gram product: the
+    entries are integers of at most 20 in magnitude, so every FP32 partial sum
+    (at most 64^2 * 400 < 2^24) is exact and the FP32 route must give the same
+    integers."""
     from sdpx_mi355x.batch import BatchSolver
 
     ks = [3, 9, 16, 17, 30, 32, 33, 47, 48, 49, 63, 64]
@@ -118,11 +123,13 @@ def test_gpu_batched_cold_start_gram_is_bit_exact(gpu_lib, kind):
                     vals.append(Ai[iu])
             bs.set_coo(rows, cols, vals, [np.zeros((k, k)) for k in ks], [np.ones(m) for m in ms])
         bs.set_start([np.zeros(m) for m in ms], [np.eye(k) for k in ks], [np.eye(k) for k in ks])
+        bs.set_precision(precision)
         assert bs.schur_at_start() == 0
         for p, A in enumerate(panels):
             V = A.reshape(A.shape[0], -1)
             ref = np.tril(V @ V.T)
             S = np.tril(bs.schur(p))
             bad = np.argwhere(S != ref)
-            print(f"{kind} member {p}: k={ks[p]} m={ms[p]}: {len(bad)} entries differ", flush=True)
+            print(f"{kind} precision {precision} member {p}: k={ks[p]} m={ms[p]}: {len(bad)} entries differ",
+                  flush=True)
             assert len(bad) == 0, (p, bad[:5])
