@@ -334,7 +334,9 @@ typedef struct sdpx_batch_options {
    * (max L_ii / min L_ii)^2 (a lower bound of cond_2(S)) exceeds cond_limit,
    * or whose refinement misses its target redoes the step in FP64 and keeps
    * FP64 (per member).  cond_limit <= 0 disables the pivot-ratio guard;
-   * sdpx_batch_default_options sets 1e6, 1e-10 and 32. */
+   * sdpx_batch_default_options sets 1e6, 1e-10 and 2 (a refinement round is
+   * one launch over every member still refining, so the budget is small: a
+   * member that needs more rounds continues in FP64). */
   int precision;
   double cond_limit, refine_target;
   /* differentiation checkpoint: the first iterate with <X, Y>/k <= adjoint_mu

@@ -39,7 +39,7 @@ class BatchOptions(C.Structure):
     @classmethod
     def from_solver_options(cls, o: SolverOptions, precision: int | None = None) -> "BatchOptions":
         """precision None follows o.precision ("fp64" -> 0, "mixed" -> 1); the mixed route takes
-        o.mixed_cond_limit, o.mixed_refine_target and o.mixed_max_refine."""
+        o.mixed_cond_limit, o.mixed_refine_target and o.mixed_batch_max_refine."""
         b = cls()
         check(_lib.load().sdpx_batch_default_options(C.addressof(b), o.eps_gap), "sdpx_batch_default_options")
         for name in ("beta", "gamma", "eps_gap", "eps_primal", "eps_dual", "min_step", "omega_step",
@@ -57,7 +57,7 @@ class BatchOptions(C.Structure):
         b.precision = precision
         b.cond_limit = o.mixed_cond_limit
         b.refine_target = o.mixed_refine_target
-        b.mixed_max_refine = o.mixed_max_refine
+        b.mixed_max_refine = o.mixed_batch_max_refine
         return b
 
 

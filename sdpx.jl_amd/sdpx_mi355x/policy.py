@@ -65,6 +65,11 @@ class SolverOptions:
     bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10
     mixed_max_refine: int = 32
+    # the batched route's refinement budget per direction and member: one refinement round is a
+    # launch over every member still refining, so a slow member holds the whole batch; a member
+    # that would need more rounds redoes its step in FP64 (2 measured fastest on the 4096-member
+    # sparse and the 1024-member layer batches among 2, 3, 4, 6, 8, 12 and 32)
+    mixed_batch_max_refine: int = 2
     # KKT route: "dense_cholesky" or "block_arrow" (the exact block-arrow elimination of
     # factor_arrow_kkt! / solve_arrow_kkt!, kkt.jl:2132-2349, 2977-3096; an incompatible problem
     # raises, as the reference's workspace does, workspace.jl:938-940)

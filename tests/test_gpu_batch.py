@@ -284,11 +284,13 @@ def _mixed_options(tol, **kw):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("budget", [2, 32])
 @pytest.mark.parametrize("workload", ["dense", "sparse", "layer"])
-def test_batch_mixed_members_match_oracle(gpu_lib, workload):
+def test_batch_mixed_members_match_oracle(gpu_lib, workload, budget):
     """Mixed precision in the batched route: FP32 Schur assembly, every direction refined against the
     matrix-free FP64 operator.  Each member reaches the FP64 oracle's optimum (status, objective 1e-8
-    relative, iterations +-2, certificate), and the reduced-precision Schur matrix carries most steps."""
+    relative, iterations +-2, certificate) with the batched default refinement budget (2) and the
+    reference's (32); with 32 the reduced-precision Schur matrix carries most steps."""
     if workload == "dense":
         ks = [5, 12, 20, 32, 9, 3, 2, 1]
         ms = [12, 40, 60, 128, 30, 6, 3, 2]
@@ -307,13 +309,16 @@ def test_batch_mixed_members_match_oracle(gpu_lib, workload):
     with BatchSolver(ks, ms) as bs:
         _load(bs, probs)
         bs.set_start([g.x0 for g in probs], [np.eye(k) for k in ks], [np.eye(k) for k in ks])
-        res = bs.solve(_mixed_options(tol))
+        om = _mixed_options(tol)
+        om.mixed_max_refine = budget
+        res = bs.solve(om)
         reduced = bs.mixed_iterations()
     print(f"reduced-precision steps per member: {reduced.tolist()} of {res.iterations.tolist()}", flush=True)
     _check_members(res, probs, o, oo)
     big = [p for p in range(len(ks)) if ks[p] >= 5]
     assert all(reduced[p] >= 1 for p in big)
-    assert sum(int(reduced[p]) for p in big) >= 0.5 * sum(int(res.iterations[p]) for p in big)
+    if budget == 32:
+        assert sum(int(reduced[p]) for p in big) >= 0.5 * sum(int(res.iterations[p]) for p in big)
 
 
 @pytest.mark.gpu
