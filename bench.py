@@ -398,13 +398,17 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         b.set_precision(0)
         out["schur_gram"] = gram
         res = {}
-        for prec in ("fp64", "mixed", "mixed_bf16x3", "fp64", "mixed", "mixed_bf16x3"):  # first round warms up
-            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec)
+        # "auto": the static condition cutoff before each reduced-precision step; "on": the reference's
+        # measured trial (the residual and refinement guards alone)
+        variants = (("fp64", "fp64", "auto"), ("mixed", "mixed", "auto"), ("mixed_bf16x3", "mixed_bf16x3", "auto"),
+                    ("mixed_on", "mixed", "on"), ("mixed_bf16x3_on", "mixed_bf16x3", "on"))
+        for name, prec, mode in variants + variants:  # the first round warms up
+            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec, mixed_mode=mode)
             b.synchronize()
             t0 = time.perf_counter()
             r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
             b.synchronize()
-            res[prec] = (r, time.perf_counter() - t0)
+            res[name] = (r, time.perf_counter() - t0)
         F = newton_flops(k, m)
         for prec, (r, dt) in res.items():
             out[prec] = {"status": r.status, "iterations": r.iterations, "fp32_iterations": r.fp32_iterations,
@@ -417,6 +421,8 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         out["objective_rel_diff_bf16x3"] = abs(pb - p64) / max(1.0, abs(p64))
         out["speedup_mixed_vs_fp64"] = round(res["fp64"][1] / res["mixed"][1], 3)
         out["speedup_mixed_bf16x3_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3"][1], 3)
+        out["speedup_mixed_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_on"][1], 3)
+        out["speedup_mixed_bf16x3_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3_on"][1], 3)
     finally:
         b.close()
     return out

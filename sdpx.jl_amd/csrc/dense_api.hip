@@ -2050,8 +2050,16 @@ int sdpx_dense_refine_exact(sdpx_dense_t* h, double abstol, int max_steps, int* 
       SDPX_HIP(hipStreamSynchronize(h->stream));
       break;
     }
+    const double contraction = corrected / residual;
     steps += 1;
     residual = corrected;
+    if (!(residual <= abstol) && contraction > 0.0) {
+      // the refinement budget of _predicted_mixed_refinement_steps (mixed_precision_kkt.jl:394-415)
+      // with the measured contraction in place of cond * eps * 8: stop as soon as the target
+      // cannot be reached within max_steps
+      const double more = std::ceil(std::log(abstol / residual) / std::log(contraction));
+      if (!(steps + more <= max_steps)) break;
+    }
   }
   if (steps_out) *steps_out = steps;
   if (residual_out) *residual_out = residual;

@@ -102,7 +102,7 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
         return dict(status="breakdown",
                     reason=f"Schur complement not positive definite after {reg_attempts} regularization attempt(s)",
                     p_res=p_res, d_res=d_res, reg_attempts=reg_attempts)
-    if use32:
+    if use32 and o.mixed_mode == "auto":
         # the static condition cutoff of _try_factor_mixed_kkt! (mixed_precision_kkt.jl:1049-1059): the
         # Schur matrix's condition estimate from its factor (cond_1(S) as xPOCON estimates it; the
         # reference's xTRCON product of :371-392 bounds it from above and is reported beside it) under

@@ -61,6 +61,11 @@ class SolverOptions:
     # to mixed_refine_target * max(||r||_inf, 1) in at most mixed_max_refine steps; otherwise the
     # iteration is redone in FP64 and FP64 is kept from then on.
     precision: str = "fp64"
+    # "auto": the static condition cutoff (the xPOCON estimate of cond_1(S) against
+    # mixed_cond_limit / bf16x3_cond_limit) before a reduced-precision step; "on": the reference's
+    # measured trial (mixed_precision_kkt.jl:1008-1017, mode :on): no estimate, the predictor
+    # residual guard and the monotone refinement within its budget decide
+    mixed_mode: str = "auto"
     mixed_cond_limit: float = 1e6
     bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10
@@ -90,6 +95,8 @@ class SolverOptions:
             raise ValueError("kkt_route must be 'dense_cholesky' or 'block_arrow'")
         if self.precision not in ("fp64", "mixed", "mixed_bf16x3"):
             raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_bf16x3'")
+        if self.mixed_mode not in ("auto", "on"):
+            raise ValueError("mixed_mode must be 'auto' or 'on'")
 
 
 def settings(tolerance: float = 1e-8, **kw) -> SolverOptions:

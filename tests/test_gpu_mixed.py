@@ -78,12 +78,15 @@ def test_fp32_schur_within_fp32_accuracy(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["auto", "on"])
 @pytest.mark.parametrize("k,m", [(50, 200), (64, 500)])
-def test_mixed_solve_matches_oracle(gpu_lib, k, m):
+def test_mixed_solve_matches_oracle(gpu_lib, k, m, mode):
+    """mode "auto": the condition estimate gates each reduced-precision step; "on": the measured
+    trial, where the residual and refinement guards alone decide."""
     g = generators.dense_feasible(k, m, seed=3)
     prob = ingest(g.c, [g.A_blocks()], [g.C])
     tol = 1e-8
-    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed")
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed", mixed_mode=mode)
     res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
     oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
     ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
@@ -123,12 +126,13 @@ def test_bf16x3_schur_within_split_accuracy(gpu_lib):
 
 
 @pytest.mark.gpu
-def test_mixed_bf16x3_solve_matches_oracle(gpu_lib):
+@pytest.mark.parametrize("mode", ["auto", "on"])
+def test_mixed_bf16x3_solve_matches_oracle(gpu_lib, mode):
     k, m = 50, 200
     g = generators.dense_feasible(k, m, seed=3)
     prob = ingest(g.c, [g.A_blocks()], [g.C])
     tol = 1e-8
-    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3")
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3", mixed_mode=mode)
     res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
     oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
     ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])

@@ -1,7 +1,8 @@
 """The mixed-precision configuration at its own size (one PSD block n = 500, m = 6000, rel gap 1e-8).
 
 Each precision route -- FP64, FP32-assembled Schur Gram and BF16x3-assembled Schur Gram, both refined
-against the matrix-free FP64 operator -- is compared with the oracle's own solve of the same problem
+against the matrix-free FP64 operator, with the static condition cutoff ("auto") and as the measured
+trial ("on") -- is compared with the oracle's own solve of the same problem
 (tests/golden/oracle_solves.json, written by make_solve_fixtures.py): same status, iterations within
 2, objective within 1e-8 relative; the reduced-precision routes must have run some iterations in
 reduced precision, and the three GPU objectives agree with each other to 1e-8.
@@ -29,10 +30,13 @@ def test_mixed_fullsize_matches_oracle(gpu_lib):
     try:
         b.autotune()
         results = {}
-        for prec in ("fp64", "mixed", "mixed_bf16x3"):
-            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec)
+        for name, prec, mode in (("fp64", "fp64", "auto"), ("mixed", "mixed", "auto"),
+                                 ("mixed_bf16x3", "mixed_bf16x3", "auto"), ("mixed_on", "mixed", "on"),
+                                 ("mixed_bf16x3_on", "mixed_bf16x3", "on")):
+            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec, mixed_mode=mode)
             r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
-            results[prec] = r
+            results[name] = r
+            prec = name
             print(f"{prec}: {r.status} iters={r.iterations} reduced={r.fp32_iterations} pobj={r.pobj:.12e} "
                   f"| oracle {ref['status']} iters={ref['iterations']} pobj={ref['pobj']:.12e}", flush=True)
     finally:
@@ -41,6 +45,6 @@ def test_mixed_fullsize_matches_oracle(gpu_lib):
         assert r.status == ref["status"] == "Optimal" and r.gap_rel <= tol, prec
         assert abs(r.iterations - ref["iterations"]) <= 2, prec
         assert abs(r.pobj - ref["pobj"]) <= 1e-8 * max(1.0, abs(ref["pobj"])), prec
-    assert results["mixed"].fp32_iterations > 0 and results["mixed_bf16x3"].fp32_iterations > 0
+    assert all(r.fp32_iterations > 0 for name, r in results.items() if name != "fp64")
     p64 = results["fp64"].pobj
     assert all(abs(r.pobj - p64) <= 1e-8 * max(1.0, abs(p64)) for r in results.values())
