@@ -471,6 +471,16 @@ int sdpx_batch_set_precision(sdpx_batch_t* h, int precision);
 /* Per member (B ints): Newton steps of the last solve whose Schur matrix was
  * assembled in reduced precision (precision 1). */
 int sdpx_batch_get_mixed_iterations(sdpx_batch_t* h, int* out);
+/* Per member (B ints): why the last solve stopped, the reference's termination
+ * messages as codes (interior_point.jl:1742-2217, step.jl:665-936): 1 optimal;
+ * 2 Cholesky of X or Y failed (the iterate left the interior); 3 Schur
+ * complement not positive definite after the regularization ladder; 4 final
+ * direction residual above max(sqrt(eps), eps_p, eps_d) max(||r||, 1); 5 step
+ * collapse near the solution; 6 step collapse with insufficient progress; 7
+ * step collapse with no centering or restart left; 8 non-finite iterate after
+ * the update; 9 iteration limit; 10 + the stagnation detector's reason (1 no
+ * progress, 2 too slow, 3 precision floor). */
+int sdpx_batch_get_reasons(sdpx_batch_t* h, int* out);
 
 /* ==== a14: Lorentz (second-order) cone kernels ===============================
  * Many cones Q_q = {(t, u): t >= ||u||} of arbitrary dimensions resident in

@@ -367,6 +367,7 @@ struct MemberState {
   // whose reduced-precision direction fails a guard is redone in FP64 and FP64 is kept
   bool mixed = false;
   int mixed_iterations = 0;  // Newton steps taken with the reduced-precision Schur matrix
+  int reason = 0;            // why the member stopped (sdpx_batch_get_reasons)
   double tX = 1, tY = 1;
   // one row per Newton step: iteration, pobj, dobj, p_res, d_res (of the step), tX, tY, sigma,
   // affine steps, regularization attempts, refinement steps, outcome (0 update, 1 centering,
@@ -893,6 +894,7 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
           continue;
         }
         S[p].status = kBreakdown;
+        S[p].reason = 3;
         S[p].p_res = S[p].p_it;
         S[p].d_res = S[p].d_it;
       } else {
@@ -1133,6 +1135,7 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
       const double tol = std::max(kSqrtEps, std::max(o.eps_primal, o.eps_dual)) * std::max(r_norm, 1.0);
       if (!std::isfinite(res) || res > tol) {
         s.status = kBreakdown;
+        s.reason = 4;
         s.p_res = s.p_it;
         s.d_res = s.d_it;
       } else {
@@ -1232,10 +1235,13 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       }
       if (term_ok && gap_ok) {
         s.status = kOptimal;
+        s.reason = 1;
       } else if (s.iteration >= o.iter_max) {
         s.status = kIterLimit;
+        s.reason = 9;
       } else if (stagnated) {
         s.status = s.stag.reason == 3 ? kInsufficient : kStalled;
+        s.reason = 10 + s.stag.reason;
       } else {
         step.push_back(p);
       }
@@ -1268,6 +1274,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
       s.d_it = sc(h, p, S_DRES);
       if (si(h, p, I_INFOX) != 0 || si(h, p, I_INFOY) != 0) {
         s.status = kBreakdown;
+        s.reason = 2;
         s.p_res = s.p_it;
         s.d_res = s.d_it;
       } else {
@@ -1342,12 +1349,14 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
         if (s.gap_rel < near * o.eps_gap && s.p_res / s.scale_p < near * o.eps_primal &&
             s.d_res / s.scale_d < near * o.eps_dual) {
           s.status = kStalled;
+          s.reason = 5;
           s.log(3);
           continue;
         }
         if (o.stall_iterations > 0 && std::isfinite(s.initial_merit) && s.best_merit < s.initial_merit * 1e-3 &&
             s.best_merit < kNearSolutionMerit) {
           s.status = kInsufficient;
+          s.reason = 6;
           s.log(3);
           continue;
         }
@@ -1373,6 +1382,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
           s.status = kStalled;
         else
           s.status = s.restarts >= o.max_restarts ? kMaxRestarts : kBreakdown;
+        s.reason = 7;
         s.log(3);
         continue;
       }
@@ -1426,6 +1436,7 @@ int run_solve(sdpx_batch_t* h, const sdpx_batch_options_t& o, int* iterations_ou
                           std::isfinite(sc(h, p, S_YBMAX));
       if (!(finite && std::isfinite(s.mu))) {
         s.status = kBreakdown;
+        s.reason = 8;
         continue;
       }
       const double p_after = std::fabs(1.0 - s.tX) * s.p_it;
@@ -2346,6 +2357,16 @@ int sdpx_batch_get_mixed_iterations(sdpx_batch_t* h, int* out) {
     return kInvalidArgument;
   }
   for (int p = 0; p < h->B; ++p) out[p] = h->st[p].mixed_iterations;
+  return kOk;
+}
+
+int sdpx_batch_get_reasons(sdpx_batch_t* h, int* out) {
+  SDPX_CHECK(check_batch(h));
+  if (!h->solved || !out) {
+    set_last_error("sdpx_batch_get_reasons: no solve has completed, or null output");
+    return kInvalidArgument;
+  }
+  for (int p = 0; p < h->B; ++p) out[p] = h->st[p].reason;
   return kOk;
 }
 

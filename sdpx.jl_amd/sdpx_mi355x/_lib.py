@@ -106,6 +106,7 @@ SIGNATURES = {
     "sdpx_batch_schur_at_start": [_h],
     "sdpx_batch_set_precision": [_h, C.c_int],
     "sdpx_batch_get_mixed_iterations": [_h, _ip],
+    "sdpx_batch_get_reasons": [_h, _ip],
     "sdpx_batch_policy_select": [C.c_void_p, _dp, C.c_int, _dp, C.c_int, _dp],
     "sdpx_batch_policy_record": [C.c_void_p, _dp, _dp],
     "sdpx_soc_create": [C.c_int, C.c_int, _ip, C.POINTER(_h)],

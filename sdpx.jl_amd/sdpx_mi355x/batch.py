@@ -318,6 +318,17 @@ class BatchSolver:
               "sdpx_batch_get_mixed_iterations")
         return out
 
+    REASONS = {0: "none", 1: "optimal", 2: "block_factor_failed", 3: "schur_not_positive_definite",
+               4: "direction_residual", 5: "collapse_near_solution", 6: "collapse_insufficient_progress",
+               7: "collapse_exhausted", 8: "nonfinite_iterate", 9: "iteration_limit", 11: "stagnation_no_progress",
+               12: "stagnation_too_slow", 13: "stagnation_precision_floor"}
+
+    def reasons(self) -> list:
+        """Per member: why the last solve stopped (sdpx_batch_get_reasons)."""
+        out = np.zeros(self.B, dtype=np.int32)
+        check(self._lib.sdpx_batch_get_reasons(self._h, out.ctypes.data_as(_lib._ip)), "sdpx_batch_get_reasons")
+        return [self.REASONS.get(int(v), str(int(v))) for v in out]
+
     def schur_at_start(self) -> int:
         return check(self._lib.sdpx_batch_schur_at_start(self._h), "sdpx_batch_schur_at_start")
 

@@ -168,8 +168,10 @@ def test_batch_member_failure_is_isolated(gpu_lib):
         _load(bs, probs)
         bs.set_start([g.x0 for g in probs], X0, [np.eye(k) for k in ks])
         res = bs.solve(o)
+        reasons = bs.reasons()
     assert res.status[1] == "NumericalBreakdown"
     assert res.status[0] == res.status[2] == "Optimal"
+    assert reasons == ["optimal", "block_factor_failed", "optimal"]
 
 
 @pytest.mark.gpu
