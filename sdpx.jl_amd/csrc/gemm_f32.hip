@@ -462,6 +462,197 @@ __global__ __launch_bounds__(256, 2) void schur_gram_bf16x3_kernel(const __bf16*
   }
 }
 
+// The same product on 256 x 256 lower tiles: 512 threads = 2 x 4 waves of
+// 128 x 64 (8 x 4 blocks of 16 x 16), 32-deep slices double-buffered through
+// LDS (128 KiB: one group per CU), the next slice prefetched into registers
+// while the current one feeds 96 MFMAs per wave.  With 16-cycle bf16 MFMAs the
+// 128 x 128 kernel above needs about the whole LDS bandwidth of a CU (its
+// fragment reads, 1/32 byte per MFMA flop per plane pair, plus the slice
+// stores); the 128 x 64 wave tile reads 3/256 and the 256-row slice is stored
+// once for eight waves, about 35% less LDS traffic per flop.  Rows hold 4
+// chunks of 8 bf16 (64 bytes), chunk c of row r at chunk c ^ ((r >> 1) & 3):
+// the eight lanes that share a 128-byte LDS cycle (fragment reads: rows
+// r..r+7 of one chunk; slice stores: rows r, r+1 of four chunks) hit eight
+// distinct 16-byte slots.  Each accumulator sees its products in the same
+// order as in the 128 x 128 kernel (per 32 of K: L'H, H'L, H'H), so the two
+// kernels give the same S for the same split.  A diagonal tile stages one
+// operand, and the two waves whose blocks lie wholly above the diagonal skip
+// their MFMAs.
+constexpr int BMW = 256;
+constexpr int KSW = 32;
+#ifndef SDPX_WIDE_GROUP
+#define SDPX_WIDE_GROUP 4
+#endif
+constexpr int kWideGroup = SDPX_WIDE_GROUP;
+__device__ __forceinline__ int bf16w_col(int row, int chunk) { return 8 * (chunk ^ ((row >> 1) & 3)); }
+
+__global__ __launch_bounds__(512, 1) void schur_gram_bf16x3_wide_kernel(const __bf16* __restrict__ H,
+                                                                       const __bf16* __restrict__ L, long long ldp,
+                                                                       int m, int Kp, double* S, long long lds,
+                                                                       double beta, int split, double* ws) {
+  // [buffer][operand A / B][plane H / L][row][K]
+  __shared__ __align__(16) __bf16 T[2][2][2][BMW][KSW];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 64;
+
+  const int total = gridDim.x;
+  const int bid = blockIdx.x;
+  const int xcd = bid % kNumXcd, slot = bid / kNumXcd;
+  const int q = total / kNumXcd, rq = total % kNumXcd;
+  const int lin = xcd * q + (xcd < rq ? xcd : rq) + slot;
+  const int per_split = total / split;
+  const int sp = lin / per_split;
+  const int tile = lin % per_split;
+  // lower tiles in kWideGroup x kWideGroup super-blocks (row-major over the blocks and inside each):
+  // the groups of an XCD that run together share few Phat panels through its L2
+  const int nt = (m + BMW - 1) / BMW;
+  int ti = 0, tj = 0;
+  {
+    int rest = tile;
+    const int nb = (nt + kWideGroup - 1) / kWideGroup;
+    for (int I = 0; I < nb; ++I) {
+      const int r0 = I * kWideGroup, r1 = min(nt, r0 + kWideGroup);
+      bool found = false;
+      for (int J = 0; J <= I; ++J) {
+        const int c0 = J * kWideGroup;
+        int cnt = 0;  // tiles of block (I, J): rows r0..r1-1, columns c0..min(c0 + G, row + 1) - 1
+        for (int r = r0; r < r1; ++r) cnt += min(c0 + kWideGroup, r + 1) - c0;
+        if (rest < cnt) {
+          for (int r = r0; r < r1; ++r) {
+            const int w = min(c0 + kWideGroup, r + 1) - c0;
+            if (rest < w) {
+              ti = r;
+              tj = c0 + rest;
+              break;
+            }
+            rest -= w;
+          }
+          found = true;
+          break;
+        }
+        rest -= cnt;
+      }
+      if (found) break;
+    }
+  }
+  const int m0 = ti * BMW, n0 = tj * BMW;
+  const bool diag = ti == tj;
+  const bool idle = diag && wn0 >= wm0 + 128;
+
+  int kBegin = 0, kEnd = Kp;
+  if (split > 1) {
+    const int stages = Kp / KSW;
+    const int per = (stages + split - 1) / split;
+    kBegin = min(Kp, sp * per * KSW);
+    kEnd = min(Kp, kBegin + per * KSW);
+  }
+
+  // thread loads: slice rows lr and lr + 128 of each operand tile, chunk ch (K entries 8 ch .. 8 ch + 7), both planes
+  const int lr = tid >> 2, ch = tid & 3;
+  long long ga[2], gb[2];
+  int sc[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int r = lr + 128 * t;
+    ga[t] = (long long)min(m0 + r, m - 1) * ldp + 8 * ch;
+    gb[t] = (long long)min(n0 + r, m - 1) * ldp + 8 * ch;
+    sc[t] = bf16w_col(r, ch);
+  }
+  bf16x8_t v[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      v[4 * t + 0] = *reinterpret_cast<const bf16x8_t*>(H + ga[t] + k0);
+      v[4 * t + 1] = *reinterpret_cast<const bf16x8_t*>(L + ga[t] + k0);
+      if (!diag) {
+        v[4 * t + 2] = *reinterpret_cast<const bf16x8_t*>(H + gb[t] + k0);
+        v[4 * t + 3] = *reinterpret_cast<const bf16x8_t*>(L + gb[t] + k0);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int r = lr + 128 * t;
+      *reinterpret_cast<bf16x8_t*>(&T[buf][0][0][r][sc[t]]) = v[4 * t + 0];
+      *reinterpret_cast<bf16x8_t*>(&T[buf][0][1][r][sc[t]]) = v[4 * t + 1];
+      if (!diag) {
+        *reinterpret_cast<bf16x8_t*>(&T[buf][1][0][r][sc[t]]) = v[4 * t + 2];
+        *reinterpret_cast<bf16x8_t*>(&T[buf][1][1][r][sc[t]]) = v[4 * t + 3];
+      }
+    }
+  };
+  float4_t acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fq = lane >> 4;
+  const int bop = diag ? 0 : 1;
+  bf16x8_t bh[4], bl[4];
+  auto load_b = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = wn0 + 16 * j + fr, c = bf16w_col(r, fq);
+      bh[j] = *reinterpret_cast<const bf16x8_t*>(&T[buf][bop][0][r][c]);
+      bl[j] = *reinterpret_cast<const bf16x8_t*>(&T[buf][bop][1][r][c]);
+    }
+  };
+  // row blocks i0 .. i0 + 3 of the wave tile
+  auto compute = [&](int buf, int i0) {
+#pragma unroll
+    for (int i = i0; i < i0 + 4; ++i) {
+      const int r = wm0 + 16 * i + fr, c = bf16w_col(r, fq);
+      const bf16x8_t ah = *reinterpret_cast<const bf16x8_t*>(&T[buf][0][0][r][c]);
+      const bf16x8_t al = *reinterpret_cast<const bf16x8_t*>(&T[buf][0][1][r][c]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[j], ah, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], al, acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[j], ah, acc[i][j], 0, 0, 0);
+    }
+  };
+  if (kBegin < kEnd) {
+    load(kBegin);
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kBegin; k0 < kEnd; k0 += KSW) {
+      const bool next = k0 + KSW < kEnd;
+      if (next) load(k0 + KSW);
+      if (!idle) {
+        load_b(cur);
+        compute(cur, 0);
+        compute(cur, 4);
+      }
+      if (next) store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+  if (idle) return;
+  const bool part = split > 1;
+  double* out = part ? ws + (long long)sp * m * m : S;
+  const long long ldo = part ? (long long)m : lds;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int row = m0 + wm0 + i * 16 + fr;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = n0 + wn0 + j * 16 + 4 * fq + r;
+        if (row < m && row >= col) {
+          double* cp = out + row + (long long)col * ldo;
+          const double v2 = (double)acc[i][j][r];
+          *cp = (part || beta == 0.0) ? v2 : v2 + beta * (*cp);
+        }
+      }
+  }
+}
+
 }  // namespace
 
 int gram_f32(const float* P, long long ldp, int m, int K, double* S, long long lds, double beta, int split,
@@ -514,10 +705,17 @@ int gram_bf16x3(const __bf16* H, const __bf16* L, long long ldp, int m, int Kp, 
     return kInvalidArgument;
   }
   if (split < 1) split = 1;
-  const int nt = ceil_div(m, BM);
-  const long long tiles = (long long)nt * (nt + 1) / 2;
-  hipLaunchKernelGGL(schur_gram_bf16x3_kernel, dim3((unsigned)(tiles * split)), dim3(256), 0, stream, H, L, ldp, m,
-                     Kp, S, lds, beta, split, ws, tile_order);
+  if (m >= BMW) {
+    const int nt = ceil_div(m, BMW);
+    const long long tiles = (long long)nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(schur_gram_bf16x3_wide_kernel, dim3((unsigned)(tiles * split)), dim3(512), 0, stream, H, L, ldp,
+                       m, Kp, S, lds, beta, split, ws);
+  } else {
+    const int nt = ceil_div(m, BM);
+    const long long tiles = (long long)nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(schur_gram_bf16x3_kernel, dim3((unsigned)(tiles * split)), dim3(256), 0, stream, H, L, ldp, m,
+                       Kp, S, lds, beta, split, ws, tile_order);
+  }
   SDPX_HIP(hipGetLastError());
   if (after_main) SDPX_HIP(hipEventRecord(after_main, stream));
   if (split > 1) {
