@@ -951,18 +951,29 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
   std::vector<int> fp64;
   for (int p : ok)
     if (!S[p].mixed) fp64.push_back(p);
-  // affine predictor diagnostics (step.jl:546-625)
+  // affine predictor diagnostics (step.jl:546-625): the fraction to the boundary of the affine
+  // direction, then the complementarity at those steps (read by the comp pass from the
+  // line-search slots: no read-back in between) and the factor's diagonal extrema
   std::vector<int> adapt;
   for (int p : ok) {
     if (S[p].iter_adaptive) {
       adapt.push_back(p);
       par(h, p, P_PF) = 1.0;
       par(h, p, P_DF) = 1.0;
+      par(h, p, P_A) = -1.0;
+      par(h, p, P_B) = -1.0;
+    } else {
+      par(h, p, P_A) = 1.0;
+      par(h, p, P_B) = 1.0;
     }
   }
   SDPX_CHECK(push_params(h));
-  if (!adapt.empty()) {
+  if (!adapt.empty())
     SDPX_CHECK(launch_ftb(h->arr, put_list(h, adapt), (int)adapt.size(), step_classes(h, adapt), h->stream));
+  {
+    const int* l = put_list(h, ok);
+    SDPX_CHECK(launch_comp(h->arr, l, n, h->stream));
+    SDPX_CHECK(launch_factor_extrema(h->arr, l, n, h->stream));
     SDPX_CHECK(pull_scalars(h));
   }
   for (int p : ok) {
@@ -975,13 +986,6 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
     }
     par(h, p, P_A) = s.ap;
     par(h, p, P_B) = s.ad;
-  }
-  SDPX_CHECK(push_params(h));
-  {
-    const int* l = put_list(h, ok);
-    SDPX_CHECK(launch_comp(h->arr, l, n, h->stream));
-    SDPX_CHECK(launch_factor_extrema(h->arr, l, n, h->stream));
-    SDPX_CHECK(pull_scalars(h));
   }
   auto quality = [](double mn, double mx) {
     return (mx > 0.0 && std::isfinite(mx)) ? std::min(std::max(mn / mx, 0.0), 1.0) : 0.0;
@@ -1083,6 +1087,7 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
     s.refine_active = s.refine_cap > 0;
     max_cap = std::max(max_cap, s.refine_cap);
   }
+  bool stale = false;  // a roll-back in the last round recomputed S_RHO after its read-back
   for (int it = 0; it < max_cap; ++it) {
     std::vector<int> act;
     for (int p : fp64) {
@@ -1116,16 +1121,16 @@ int newton_direction(sdpx_batch_t* h, const sdpx_batch_options_t& o, std::vector
       s.refine_residual = corrected;
       if (!s.refine_fixed && small) s.refine_active = false;
     }
-    if (!back.empty()) {
+    stale = !back.empty();
+    if (stale) {
       const int* lb = put_list(h, back);
       SDPX_CHECK(launch_vec_op(h->arr, lb, (int)back.size(), 2, h->stream));
       SDPX_CHECK(launch_residual(h->arr, lb, (int)back.size(), h->stream));
     }
   }
-  if (!fp64.empty()) {
-    SDPX_CHECK(launch_residual(h->arr, put_list(h, fp64), (int)fp64.size(), h->stream));
-    SDPX_CHECK(pull_scalars(h));
-  }
+  // S_RHO / S_RNORM on the host are the residuals of every member's final dx: from the corrector's
+  // pass, or from the last refinement round it took part in, unless that round rolled it back
+  if (stale) SDPX_CHECK(pull_scalars(h));
   {
     std::vector<int> keep;
     for (int p : ok) {

@@ -475,7 +475,10 @@ __global__ __launch_bounds__(NT) void comp_kernel(Arrays a, const int* list) {
   const Member M = a.mem[p];
   const int k2 = M.k * M.k;
   const long long o = M.oK;
-  const double ta = a.par[(long long)p * NPARAM + P_A], tb = a.par[(long long)p * NPARAM + P_B];
+  double ta = a.par[(long long)p * NPARAM + P_A], tb = a.par[(long long)p * NPARAM + P_B];
+  // a negative step means: the affine step the fraction-to-boundary pass just left in S_TX / S_TY
+  if (ta < 0.0) ta = a.sc[(long long)p * NSLOT + S_TX];
+  if (tb < 0.0) tb = a.sc[(long long)p * NSLOT + S_TY];
   double s0 = 0.0, s1 = 0.0;
   for (int e = threadIdx.x; e < k2; e += NT) {
     const double x = a.X[o + e], y = a.Y[o + e];

@@ -112,7 +112,7 @@ int launch_residual(const Arrays& a, const int* list, int n, hipStream_t st);
 int launch_vec_op(const Arrays& a, const int* list, int n, int mode, hipStream_t st);
 // dY = sym(X^-1 (R - dX Y))
 int launch_dir_block(const Arrays& a, const int* list, int n, hipStream_t st);
-// S_COMP = <X,Y>, S_COMPT = <X + P_A dX, Y + P_B dY>
+// S_COMP = <X,Y>, S_COMPT = <X + P_A dX, Y + P_B dY> (a negative P_A / P_B takes S_TX / S_TY)
 int launch_comp(const Arrays& a, const int* list, int n, hipStream_t st);
 // R = -X Y - dX dY + target I (P_MODE 0 fixed (P_MU, P_BETA), 1 local (P_SIGMA), 2 global (P_SIGMA, P_MUG))
 int launch_corrector_block(const Arrays& a, const int* list, int n, hipStream_t st);
