@@ -396,14 +396,32 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
             gram[name] = {"avg_launch_ms": round(ms, 3),
                           "tflops_fp64_equivalent": round(float(k) ** 2 * m * (m + 1) / ms / 1e9, 1)}
         b.set_precision(0)
+        b.schur_build()
         out["schur_gram"] = gram
+        # Schur factor time per factor precision on the same matrix (HIP events)
+        fac = {}
+        for code, name in ((0, "fp64"), (1, "fp32")):
+            b.set_factor_precision(code)
+            b.factorize()
+            b.set_profiling(True)
+            b.kernel_times(reset=True)
+            for _ in range(3):
+                b.factorize()
+            kt = b.kernel_times(reset=True)
+            b.set_profiling(False)
+            ms = kt["factor_ms"] / max(kt["factor_calls"], 1)
+            fac[name] = {"avg_ms": round(ms, 3), "tflops": round(float(m) ** 3 / 3 / ms / 1e9, 1)}
+        b.set_factor_precision(0)
+        out["schur_factor"] = fac
         res = {}
         # "auto": the static condition cutoff before each reduced-precision step; "on": the reference's
         # measured trial (the residual and refinement guards alone)
         variants = (("fp64", "fp64", "auto"), ("mixed", "mixed", "auto"), ("mixed_bf16x3", "mixed_bf16x3", "auto"),
-                    ("mixed_on", "mixed", "on"), ("mixed_bf16x3_on", "mixed_bf16x3", "on"))
+                    ("mixed_on", "mixed", "on"), ("mixed_bf16x3_on", "mixed_bf16x3", "on"),
+                    ("mixed_bf16x3_on_fp32_factor", "mixed_bf16x3", "on+f32"))
         for name, prec, mode in variants + variants:  # the first round warms up
-            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec, mixed_mode=mode)
+            o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec,
+                              mixed_mode=mode.split("+")[0], mixed_factor=mode.endswith("+f32"))
             b.synchronize()
             t0 = time.perf_counter()
             r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
@@ -423,6 +441,8 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         out["speedup_mixed_bf16x3_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3"][1], 3)
         out["speedup_mixed_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_on"][1], 3)
         out["speedup_mixed_bf16x3_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3_on"][1], 3)
+        out["speedup_mixed_bf16x3_on_fp32_factor_vs_fp64"] = round(
+            res["fp64"][1] / res["mixed_bf16x3_on_fp32_factor"][1], 3)
     finally:
         b.close()
     return out

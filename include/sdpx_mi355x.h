@@ -297,6 +297,14 @@ int sdpx_dense_factor_condition(sdpx_dense_t* h, int which, double* out);
  * Phat as three bf16 MFMA products accumulated in FP32 (BF16x3); S stays FP64
  * (lower) in both; 0 restores FP64. */
 int sdpx_dense_set_precision(sdpx_dense_t* h, int precision);
+/* Precision of the Schur factor (the template's low-precision factor,
+ * _factor_float64_preconditioner! in _try_factor_mixed_kkt!,
+ * kernels/mixed_precision_kkt.jl:1010-1030, with FP32 under the FP64 target):
+ * 1 factors S in FP32 -- panels and trailing updates on FP32 MFMA, the 64 x 64
+ * diagonal blocks in FP64 and rounded -- and leaves the factor in FP64 storage
+ * for the solves, the condition estimate and the refinement; 0 restores FP64.
+ * The plain dense route only (no block-arrow elimination, no equality border). */
+int sdpx_dense_set_factor_precision(sdpx_dense_t* h, int precision);
 /* rho = r - S dx with S applied matrix-free in FP64,
  * (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y> (the exact operator the FP32
  * assembly approximates); out[0] = ||rho||_inf, out[1] = ||r||_inf. */

@@ -205,6 +205,17 @@ int trsv_f64(const double* L, long long lda, int n, const double* dinv, double* 
 int trsv_f64_steps(const double* L, long long lda, int n, const double* dinv, double* x, double* y, int trans,
                    hipStream_t stream);
 
+// FP32 Cholesky (chol_f32.hip): A (FP32, n x n, ld lda, lower) factored in place, the 64 x 64
+// diagonal blocks in FP64 (their inverses into dinv, ceil(n/64) * 64 * 64 doubles; blk is a
+// 64 x 64 FP64 scratch block, dinv32 a 64 x 64 FP32 one), panels and trailing updates on FP32
+// MFMA; *info (device) = 1-based column of the first bad pivot, 0 on success.
+int potrf_f32(float* A, long long lda, int n, double* blk, double* dinv, float* dinv32, int* info,
+              hipStream_t stream);
+// dst (FP32, ld ldf) = lower(src) (FP64, ld n) with diagonal += shift max(|src_ii|, 1)
+int lower_to_f32(const double* src, int n, double shift, float* dst, long long ldf, hipStream_t stream);
+// dst (FP64, ld n) = lower(src) (FP32, ld ldf)
+int lower_to_f64(const float* src, long long ldf, int n, double* dst, hipStream_t stream);
+
 // Inverse of a lower-triangular n x n factor (Linv written to out, ld n),
 // blocked on the diagonal-block inverses written by potrf_f64.
 int trtri_lower_f64(const double* L, long long lda, int n, const double* dinv, double* out,

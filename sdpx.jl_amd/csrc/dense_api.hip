@@ -558,6 +558,13 @@ struct sdpx_dense {
   float tune_ftb_ms[4] = {};  // modelled fraction-to-the-boundary search, D = 4..7 levels per round
   float* Pf = nullptr;
   long long ldPf = 0, maxk2 = 0;
+  // FP32 factor of the Schur matrix (sdpx_dense_set_factor_precision): the FP32 matrix (ld ldS32),
+  // a 64 x 64 FP64 block and a 64 x 64 FP32 inverse for the diagonal-block steps
+  int factor_precision = 0;
+  float* S32 = nullptr;
+  long long ldS32 = 0;
+  double* blk64 = nullptr;
+  float* dinv32 = nullptr;
   __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
   long long ldPb = 0;
   double* gemv_part = nullptr;
@@ -1599,8 +1606,15 @@ int sdpx_dense_schur_factor(sdpx_dense_t* h, int* reg_attempts) {
     SDPX_CHECK(arrow_factor(h, &attempts, &ok));
   } else {
     while (true) {
-      SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
-      SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
+      if (h->factor_precision == 1) {
+        // FP32 factor (panels and trailing updates on FP32 MFMA), returned in FP64 storage
+        SDPX_CHECK(lower_to_f32(h->S, m, attempts == 0 ? 0.0 : reg, h->S32, h->ldS32, h->stream));
+        SDPX_CHECK(potrf_f32(h->S32, h->ldS32, m, h->blk64, h->dinvS, h->dinv32, h->dinfo, h->stream));
+        SDPX_CHECK(lower_to_f64(h->S32, h->ldS32, m, h->Sf, h->stream));
+      } else {
+        SDPX_CHECK(copy_lower_shift_f64(h->S, h->Sf, m, attempts == 0 ? 0.0 : reg, h->stream));
+        SDPX_CHECK(potrf_f64(h->Sf, m, 0, m, 1, h->dinvS, 0, h->dinfo, h->stream));
+      }
       SDPX_CHECK(sync_scalars(h, 0, 1));
       ok = h->hinfo[0] == 0;
       if (ok || attempts >= 6) break;
@@ -1969,6 +1983,28 @@ int sdpx_dense_refine(sdpx_dense_t* h, double abstol, int max_steps, int fixed_p
   }
   if (steps_out) *steps_out = steps;
   if (residual_out) *residual_out = residual;
+  return kOk;
+}
+
+int sdpx_dense_set_factor_precision(sdpx_dense_t* h, int precision) {
+  SDPX_CHECK(check_handle(h));
+  if (precision != 0 && precision != 1) {
+    set_last_error("sdpx_dense_set_factor_precision: precision must be 0 (FP64) or 1 (FP32)");
+    return kInvalidArgument;
+  }
+  if (precision == 1 && (h->arrow || h->neq > 0)) {
+    set_last_error("sdpx_dense_set_factor_precision: the FP32 factor serves the plain dense route only "
+                   "(no block-arrow elimination, no equality border)");
+    return kInvalidArgument;
+  }
+  if (precision == 1 && !h->S32) {
+    h->ldS32 = ((long long)h->m + 3) / 4 * 4;
+    int s = dalloc(h, &h->S32, (size_t)(h->ldS32 * h->m));
+    if (!s) s = dalloc(h, &h->blk64, (size_t)kCholNb * kCholNb);
+    if (!s) s = dalloc(h, &h->dinv32, (size_t)kCholNb * kCholNb);
+    if (s) return s;
+  }
+  h->factor_precision = precision;
   return kOk;
 }
 

@@ -89,6 +89,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
                     p_res=p_res, d_res=d_res)
     if o.precision != "fp64":
         b.set_precision((1 if o.precision == "mixed" else 2) if use32 else 0)
+        if o.mixed_factor:
+            b.set_factor_precision(1 if use32 else 0)
     b.schur_build()
     clock.lap("schur_assembly")
     if trace_schur is not None:

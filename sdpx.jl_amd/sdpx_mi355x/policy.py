@@ -66,6 +66,10 @@ class SolverOptions:
     # measured trial (mixed_precision_kkt.jl:1008-1017, mode :on): no estimate, the predictor
     # residual guard and the monotone refinement within its budget decide
     mixed_mode: str = "auto"
+    # factor the reduced-precision Schur matrix in FP32 too (FP32 MFMA panels and trailing
+    # updates), under the same guards: the condition estimate of the FP32 factor against the
+    # limit ("auto") or the refinement guards ("on"); a failing step is redone in FP64
+    mixed_factor: bool = False
     mixed_cond_limit: float = 1e6
     bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10

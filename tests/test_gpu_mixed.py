@@ -78,15 +78,18 @@ def test_fp32_schur_within_fp32_accuracy(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("factor32", [False, True])
 @pytest.mark.parametrize("mode", ["auto", "on"])
 @pytest.mark.parametrize("k,m", [(50, 200), (64, 500)])
-def test_mixed_solve_matches_oracle(gpu_lib, k, m, mode):
+def test_mixed_solve_matches_oracle(gpu_lib, k, m, mode, factor32):
     """mode "auto": the condition estimate gates each reduced-precision step; "on": the measured
-    trial, where the residual and refinement guards alone decide."""
+    trial, where the residual and refinement guards alone decide.  factor32: the reduced-precision
+    steps also factor S in FP32."""
     g = generators.dense_feasible(k, m, seed=3)
     prob = ingest(g.c, [g.A_blocks()], [g.C])
     tol = 1e-8
-    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed", mixed_mode=mode)
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed", mixed_mode=mode,
+                      mixed_factor=factor32)
     res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
     oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
     ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
@@ -99,6 +102,54 @@ def test_mixed_solve_matches_oracle(gpu_lib, k, m, mode):
     cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
                                          res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
     assert cert["valid"], cert
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,m", [(40, 130), (30, 300), (20, 64)])
+def test_fp32_factor_solves_to_fp32_accuracy(gpu_lib, k, m):
+    """The FP32 Schur factor (FP32 MFMA panels and trailing updates, FP64 diagonal blocks): the
+    solve's backward error sits at FP32 level (it is not the FP64 factor), refinement against the
+    FP64 operator recovers 1e-10, and the condition estimate from the FP32 factor is LAPACK's
+    dpocon estimate of cond_1(S) from the FP64 factor to within FP32 rounding."""
+    from scipy.linalg import lapack
+
+    g, prob, oprob, X, Y = _state(k, m, 11)
+    LX, _ = O.kchol(X)
+    MY, _ = O.kchol(Y)
+    S = O.schur_build(oprob, [LX], [MY])
+    S = np.tril(S) + np.tril(S, -1).T
+    r = np.random.default_rng(3).standard_normal(m)
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(g.x0, [X], [Y])
+        b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        b.set_schur(S)
+        out = {}
+        for prec in (0, 1):
+            b.set_factor_precision(prec)
+            ok, attempts = b.factorize()
+            assert ok and attempts == 0
+            b.set_rhs(r)
+            b.solve_direction()
+            dx = b.get_direction()[0]
+            out[prec] = (np.linalg.norm(S @ dx - r) / (np.linalg.norm(S) * np.linalg.norm(dx)),
+                         b.factor_condition(1)[0])
+        b.set_factor_precision(1)
+        b.factorize()
+        b.set_rhs(r)
+        b.solve_direction()
+        steps, res = b.refine_exact(1e-10 * max(np.max(np.abs(r)), 1.0), 32)
+    finally:
+        b.close()
+    Lc, info = lapack.dpotrf(S, lower=1)
+    assert info == 0
+    rcond, info = lapack.dpocon(Lc, np.max(np.sum(np.abs(S), axis=0)), uplo="L")
+    print(f"k={k} m={m}: backward error fp64 {out[0][0]:.2e} fp32 {out[1][0]:.2e}; cond est fp64 "
+          f"{out[0][1]:.4e} fp32 {out[1][1]:.4e} lapack {1 / rcond:.4e}; refined in {steps} to {res:.2e}", flush=True)
+    assert out[0][0] <= 1e-14
+    assert 1e-12 < out[1][0] <= 1e-5
+    assert abs(out[1][1] - 1 / rcond) <= 1e-3 * (1 / rcond)
+    assert res <= 1e-10 * max(np.max(np.abs(r)), 1.0)
 
 
 @pytest.mark.gpu
