@@ -383,8 +383,9 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         b.set_iterate(g.x0, [g.X0], [g.Y0])
         b.compute_residuals(np.array([0.0]), affine=True, factor=True)
         gram = {}
-        for code, name in ((0, "fp64"), (1, "fp32"), (2, "bf16x3")):
+        for code, name, p32 in ((0, "fp64", 0), (1, "fp32", 0), (2, "bf16x3", 0), (2, "bf16x3_fp32_products", 1)):
             b.set_precision(code)
+            b.set_panel_precision(p32)
             b.schur_build()
             b.set_profiling(True)
             b.kernel_times(reset=True)
@@ -393,8 +394,11 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
             kt = b.kernel_times(reset=True)
             b.set_profiling(False)
             ms = kt["gram_ms"] / max(kt["gram_launches"], 1)
+            pms = kt["panels_ms"] / max(kt["gram_launches"], 1)
             gram[name] = {"avg_launch_ms": round(ms, 3),
-                          "tflops_fp64_equivalent": round(float(k) ** 2 * m * (m + 1) / ms / 1e9, 1)}
+                          "tflops_fp64_equivalent": round(float(k) ** 2 * m * (m + 1) / ms / 1e9, 1),
+                          "phat_ms": round(pms, 3)}
+        b.set_panel_precision(0)
         b.set_precision(0)
         b.schur_build()
         out["schur_gram"] = gram
@@ -418,10 +422,12 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         # measured trial (the residual and refinement guards alone)
         variants = (("fp64", "fp64", "auto"), ("mixed", "mixed", "auto"), ("mixed_bf16x3", "mixed_bf16x3", "auto"),
                     ("mixed_on", "mixed", "on"), ("mixed_bf16x3_on", "mixed_bf16x3", "on"),
-                    ("mixed_bf16x3_on_fp32_factor", "mixed_bf16x3", "on+f32"))
+                    ("mixed_bf16x3_on_fp32_factor", "mixed_bf16x3", "on+f32"),
+                    ("mixed_bf16x3_on_fp32_products", "mixed_bf16x3", "on+p32"))
         for name, prec, mode in variants + variants:  # the first round warms up
             o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec,
-                              mixed_mode=mode.split("+")[0], mixed_factor=mode.endswith("+f32"))
+                              mixed_mode=mode.split("+")[0], mixed_factor=mode.endswith("+f32"),
+                              mixed_panels32=mode.endswith("+p32"))
             b.synchronize()
             t0 = time.perf_counter()
             r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
@@ -443,6 +449,8 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         out["speedup_mixed_bf16x3_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3_on"][1], 3)
         out["speedup_mixed_bf16x3_on_fp32_factor_vs_fp64"] = round(
             res["fp64"][1] / res["mixed_bf16x3_on_fp32_factor"][1], 3)
+        out["speedup_mixed_bf16x3_on_fp32_products_vs_fp64"] = round(
+            res["fp64"][1] / res["mixed_bf16x3_on_fp32_products"][1], 3)
     finally:
         b.close()
     return out

@@ -305,6 +305,14 @@ int sdpx_dense_set_precision(sdpx_dense_t* h, int precision);
  * for the solves, the condition estimate and the refinement; 0 restores FP64.
  * The plain dense route only (no block-arrow elimination, no equality border). */
 int sdpx_dense_set_factor_precision(sdpx_dense_t* h, int precision);
+/* Precision of the P-hat products that feed the BF16x3 Schur assembly
+ * (precision 2 of sdpx_dense_set_precision; the low-precision bulk assembly of
+ * kernels/mixed_precision_kkt.jl:972-1111): 1 forms P-hat_i = L_X^-1 A_i M_Y on
+ * FP32 MFMA from an FP32 copy of each dense block's Av (k^2 x m floats, kept
+ * until the handle is destroyed, refreshed after set_block / equilibrate) and
+ * writes the bf16 planes directly; 0 (default) forms P-hat in FP64 and splits
+ * it.  Blocks whose side is not a multiple of 4 keep the FP64 products. */
+int sdpx_dense_set_panel_precision(sdpx_dense_t* h, int precision);
 /* rho = r - S dx with S applied matrix-free in FP64,
  * (S dx)_i = <A_i, X^-1 (sum_j dx_j A_j) Y> (the exact operator the FP32
  * assembly approximates); out[0] = ||rho||_inf, out[1] = ||r||_inf. */

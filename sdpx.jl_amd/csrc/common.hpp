@@ -159,6 +159,12 @@ struct GroupGramF32 {
   const int* skip;
 };
 int gram_f32_grouped(const GroupGramF32* groups, const GroupTile* tiles, int ntiles, hipStream_t stream);
+// FP32 P-hat chain for the BF16x3 assembly (panel_f32.hip): P-hat_i = L_X^-1 A_i M_Y from FP32
+// operands (Av32: k^2 x m, MY32 / LXinv32: k x k lower with explicit zeros above, all ld k),
+// T32 (k^2 x m floats) scratch, written as bf16 planes H = bf16(P-hat), L = bf16(P-hat - H)
+// (ld ldo >= k^2, rows k^2 .. ldo - 1 zeroed); k % 4 == 0.
+int panel_chain_bf16(const float* Av32, const float* MY32, const float* LXinv32, int k, int m, float* T32,
+                     __bf16* H, __bf16* L, long long ldo, hipStream_t stream);
 // dst (ld_dst >= rows, padding rows zeroed) = (float) src (ld_src), rows x cols
 int convert_panel_f32(const double* src, long long ld_src, float* dst, long long ld_dst, long long rows, int cols,
                       hipStream_t stream);

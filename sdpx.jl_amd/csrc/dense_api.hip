@@ -38,6 +38,10 @@ struct Block {
   double *dinvX = nullptr, *dinvY = nullptr;
   double *P = nullptr, *R = nullptr, *dX = nullptr, *dY = nullptr, *W1 = nullptr, *W2 = nullptr;
   double *trial = nullptr, *trial_dinv = nullptr;
+  // FP32 copy of Av for the FP32 P-hat chain (sdpx_dense_set_panel_precision) and the
+  // version of Av it was converted from
+  float* Av32 = nullptr;
+  long long av32_ver = -1;
   // sparse block (sdpx_dense_set_block_sparse): SparseBlockCOO over na positions, 0-based
   int na = 0;
   int *ids = nullptr, *sptr = nullptr, *srow = nullptr, *scol = nullptr, *slin = nullptr;
@@ -567,6 +571,11 @@ struct sdpx_dense {
   float* dinv32 = nullptr;
   __bf16 *Ph_hi = nullptr, *Ph_lo = nullptr;
   long long ldPb = 0;
+  // FP32 P-hat chain of the BF16x3 assembly (sdpx_dense_set_panel_precision): FP32 copies of
+  // M_Y and L_X^-1 (max k^2 floats each); av_ver counts the writes to any block's Av
+  int panel_precision = 0;
+  float *MY32 = nullptr, *LX32 = nullptr;
+  long long av_ver = 0;
   double* gemv_part = nullptr;
   long long gemv_part_cap = 0;
   double* symv_part = nullptr;
@@ -1356,6 +1365,7 @@ int sdpx_dense_set_block(sdpx_dense_t* h, int l, const double* Av, const double*
     return kInvalidArgument;
   }
   SDPX_HIP(hipMemcpyAsync(b.Av, Av, sizeof(double) * b.k2 * h->m, hipMemcpyHostToDevice, h->stream));
+  h->av_ver += 1;
   SDPX_HIP(hipMemcpyAsync(b.C, C, sizeof(double) * b.k2, hipMemcpyHostToDevice, h->stream));
   SDPX_HIP(hipStreamSynchronize(h->stream));
   return kOk;
@@ -1525,6 +1535,34 @@ int sdpx_dense_schur_build(sdpx_dense_t* h) {
     const double beta0 = first_dense ? 0.0 : 1.0;
     first_dense = false;
     if (h->prof) SDPX_HIP(hipEventRecord(h->ev[0], h->stream));
+    if (h->precision == 2 && h->panel_precision == 1 && b.Av32) {
+      // BF16x3 assembly from the FP32 P-hat chain: the bf16 planes straight from FP32 MFMA products
+      if (b.av32_ver != h->av_ver) {
+        SDPX_CHECK(convert_panel_f32(b.Av, b.k2, b.Av32, b.k2, b.k2, m, h->stream));
+        b.av32_ver = h->av_ver;
+      }
+      SDPX_CHECK(convert_panel_f32(b.MY, k, h->MY32, k, k, k, h->stream));
+      SDPX_CHECK(convert_panel_f32(b.LXinv, k, h->LX32, k, k, k, h->stream));
+      SDPX_CHECK(panel_chain_bf16(b.Av32, h->MY32, h->LX32, k, m, reinterpret_cast<float*>(h->T), h->Ph_hi,
+                                  h->Ph_lo, h->ldPb, h->stream));
+      const long long kp = (b.k2 + kBf16Align - 1) / kBf16Align * kBf16Align;
+      if (h->prof) SDPX_HIP(hipEventRecord(h->ev[1], h->stream));
+      SDPX_CHECK(gram_bf16x3(h->Ph_hi, h->Ph_lo, h->ldPb, m, (int)kp, h->S, m, beta0, h->split, h->splitws,
+                             h->tile_order, h->stream, h->prof ? h->ev[2] : nullptr));
+      if (h->prof) {
+        SDPX_HIP(hipEventRecord(h->ev[3], h->stream));
+        SDPX_HIP(hipEventSynchronize(h->ev[3]));
+        float a = 0.f, g = 0.f, r = 0.f;
+        SDPX_HIP(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+        SDPX_HIP(hipEventElapsedTime(&g, h->ev[1], h->ev[2]));
+        SDPX_HIP(hipEventElapsedTime(&r, h->ev[2], h->ev[3]));
+        h->t_panels += a;
+        h->t_gram += g;
+        h->t_reduce += r;
+        h->n_gram += 1;
+      }
+      continue;
+    }
     // T_i = A_i M_Y for every i as one product over the stacked panel
     // [A_1; ...; A_m] = Av' (each A_i is symmetric), M = km rows, op(B) lower;
     // rows (i, r) are written back to the k^2 x m panel layout (block remap)
@@ -2008,6 +2046,31 @@ int sdpx_dense_set_factor_precision(sdpx_dense_t* h, int precision) {
   return kOk;
 }
 
+int sdpx_dense_set_panel_precision(sdpx_dense_t* h, int precision) {
+  SDPX_CHECK(check_handle(h));
+  if (precision != 0 && precision != 1) {
+    set_last_error("sdpx_dense_set_panel_precision: precision must be 0 (FP64) or 1 (FP32)");
+    return kInvalidArgument;
+  }
+  if (precision == 1) {
+    // FP32 copies: M_Y and L_X^-1 (max k^2 floats each) and every dense block's Av (k^2 x m floats)
+    if (!h->MY32) {
+      int s = dalloc(h, &h->MY32, (size_t)h->maxk2);
+      if (!s) s = dalloc(h, &h->LX32, (size_t)h->maxk2);
+      if (s) return s;
+    }
+    for (int l = 0; l < h->L; ++l) {
+      Block& b = h->blk[l];
+      if (b.na > 0 || b.k % 4 != 0 || b.Av32) continue;
+      int s = dalloc(h, &b.Av32, (size_t)(b.k2 * h->m));
+      if (s) return s;
+      b.av32_ver = -1;
+    }
+  }
+  h->panel_precision = precision;
+  return kOk;
+}
+
 int sdpx_dense_set_precision(sdpx_dense_t* h, int precision) {
   SDPX_CHECK(check_handle(h));
   if (precision < 0 || precision > 2) {
@@ -2357,6 +2420,7 @@ int sdpx_dense_factor_quality(sdpx_dense_t* h, double* out) {
 int sdpx_dense_equilibrate(sdpx_dense_t* h, int ruiz_iters, double* E, double* s_out, double* objective_scale,
                            int* passes) {
   SDPX_CHECK(check_handle(h));
+  h->av_ver += 1;  // the Ruiz passes rescale Av
   const int L = h->L, m = h->m;
   for (int l = 0; l < L; ++l) {
     if (h->blk[l].na > 0) {
@@ -2829,6 +2893,7 @@ int sdpx_dense_set_block_sparse(sdpx_dense_t* h, int l, int na, const int* ids, 
   }
   // the dense panel the Av passes (residuals, right-hand sides, direction recovery) read
   SDPX_CHECK(fill_f64(b.k2 * h->m, 0.0, b.Av, h->stream));
+  h->av_ver += 1;
   hipLaunchKernelGGL(sparse_densify_kernel, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, h->stream, na, b.ids,
                      b.sptr, b.slin, b.sval, b.k2, b.Av);
   SDPX_HIP(hipGetLastError());

@@ -339,6 +339,17 @@ set_precision!(h::DenseHandle, precision::Integer) =
     _check(ccall(_sym(:sdpx_dense_set_precision), Cint, (Ptr{Cvoid}, Cint), h.ptr, precision),
            "sdpx_dense_set_precision")
 
+"""Schur factor precision: 0 FP64, 1 FP32 (FP32 MFMA panels and trailing updates)."""
+set_factor_precision!(h::DenseHandle, precision::Integer) =
+    _check(ccall(_sym(:sdpx_dense_set_factor_precision), Cint, (Ptr{Cvoid}, Cint), h.ptr, precision),
+           "sdpx_dense_set_factor_precision")
+
+"""Precision of the P-hat products feeding the BF16x3 assembly: 0 FP64, 1 FP32 MFMA (bf16
+planes written by the product, FP32 copy of the constraint panel kept on the device)."""
+set_panel_precision!(h::DenseHandle, precision::Integer) =
+    _check(ccall(_sym(:sdpx_dense_set_panel_precision), Cint, (Ptr{Cvoid}, Cint), h.ptr, precision),
+           "sdpx_dense_set_panel_precision")
+
 # Condition estimates from the last Schur factor: cond_1(S) (xPOCON), the xTRCON product of
 # _triangular_condition_estimate (mixed_precision_kkt.jl:371-392), rcond_1(L), rcond_inf(L)
 function factor_condition(h::DenseHandle; which::Integer=3)

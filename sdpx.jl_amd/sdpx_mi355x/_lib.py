@@ -75,6 +75,7 @@ SIGNATURES = {
     "sdpx_gram_work_list": [C.c_int, C.c_int, C.c_int, _ip],
     "sdpx_dense_set_precision": [_h, C.c_int],
     "sdpx_dense_set_factor_precision": [_h, C.c_int],
+    "sdpx_dense_set_panel_precision": [_h, C.c_int],
     "sdpx_dense_equilibrate": [_h, C.c_int, _dp, _dp, _dp, _ip],
     "sdpx_dense_set_equalities": [_h, C.c_int, _dp],
     "sdpx_dense_equality_factor": [_h, C.c_double, _dp],

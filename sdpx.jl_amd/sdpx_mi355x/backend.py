@@ -472,6 +472,10 @@ class MI355XDenseBackend:
         """Schur factor in FP64 (0) or FP32 (1: FP32 MFMA panels and trailing updates)."""
         check(self.lib.sdpx_dense_set_factor_precision(self.h, int(precision)), "set_factor_precision")
 
+    def set_panel_precision(self, precision: int):
+        """P-hat products of the BF16x3 assembly in FP64 (0) or on FP32 MFMA (1)."""
+        check(self.lib.sdpx_dense_set_panel_precision(self.h, int(precision)), "set_panel_precision")
+
     def set_precision(self, precision: int):
         """0: FP64 Schur assembly; 1: FP32 Schur Gram product; 2: BF16x3 Schur Gram product (S kept in FP64)."""
         check(self.lib.sdpx_dense_set_precision(self.h, int(precision)), "set_precision")

@@ -91,6 +91,8 @@ def newton_step(b: MI355XDenseBackend, o_iter: SolverOptions, o: SolverOptions, 
         b.set_precision((1 if o.precision == "mixed" else 2) if use32 else 0)
         if o.mixed_factor:
             b.set_factor_precision(1 if use32 else 0)
+        if o.precision == "mixed_bf16x3":
+            b.set_panel_precision(1 if o.mixed_panels32 else 0)
     b.schur_build()
     clock.lap("schur_assembly")
     if trace_schur is not None:

@@ -70,6 +70,9 @@ class SolverOptions:
     # updates), under the same guards: the condition estimate of the FP32 factor against the
     # limit ("auto") or the refinement guards ("on"); a failing step is redone in FP64
     mixed_factor: bool = False
+    # form P-hat on FP32 MFMA for the BF16x3 assembly (FP32 copy of the constraint panel, bf16
+    # planes written by the product) instead of in FP64 followed by the split
+    mixed_panels32: bool = False
     mixed_cond_limit: float = 1e6
     bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10

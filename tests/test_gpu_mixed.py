@@ -177,6 +177,66 @@ def test_bf16x3_schur_within_split_accuracy(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("k", [64, 50])
+def test_bf16x3_fp32_panels_within_split_accuracy(gpu_lib, k):
+    """P-hat on FP32 MFMA (bf16 planes written by the product): the BF16x3-assembled S stays
+    within the split's accuracy of the oracle's; before and after equilibration (the FP32 copy
+    of Av follows the rescaled panel).  k = 50 is not a multiple of 4 and keeps the FP64
+    products: its S equals the FP64-product one bit for bit."""
+    g, prob, oprob, X, Y = _state(k, 300, 2)
+    b = MI355XDenseBackend(prob)
+    try:
+        for equilibrated in (False, True):
+            if equilibrated:
+                b.equilibrate(2)
+                oprob, _ = O.equilibrate(oprob, 2)  # bitwise the device's scaling
+            b.set_iterate(g.x0, [X], [Y])
+            b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+            b.set_precision(2)
+            b.set_panel_precision(0)
+            b.schur_build()
+            S64p = b.get_schur()
+            b.set_panel_precision(1)
+            b.schur_build()
+            S32p = b.get_schur()
+            LX, _ = O.kchol(X)
+            MY, _ = O.kchol(Y)
+            ref = O.schur_build(oprob, [LX], [MY])
+            e64, e32 = rel_fro_lower(S64p, ref), rel_fro_lower(S32p, ref)
+            print(f"k={k} equilibrated={equilibrated}: BF16x3 Schur rel error, FP64 products {e64:.2e}, "
+                  f"FP32 products {e32:.2e}", flush=True)
+            if k % 4:
+                assert np.array_equal(np.tril(S32p), np.tril(S64p))
+            assert e32 <= 2.0 * e64 and e32 <= 1e-4
+    finally:
+        b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["auto", "on"])
+def test_mixed_bf16x3_fp32_panels_solve_matches_oracle(gpu_lib, mode):
+    k, m = 48, 200
+    g = generators.dense_feasible(k, m, seed=3)
+    prob = ingest(g.c, [g.A_blocks()], [g.C])
+    tol = 1e-8
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3", mixed_mode=mode,
+                      mixed_panels32=True)
+    res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
+    ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
+    print(f"mixed_bf16x3 (FP32 products): {res.status} it={res.iterations} low={res.fp32_iterations} "
+          f"fp64={res.fp64_iterations} pobj={res.pobj:.12e} | oracle {ref.status} it={ref.iterations} "
+          f"pobj={ref.pobj:.12e}", flush=True)
+    assert res.status == ref.status == "Optimal"
+    assert res.fp32_iterations >= 1
+    assert abs(res.pobj - ref.pobj) <= 1e-8 * max(1.0, abs(ref.pobj))
+    assert abs(res.iterations - ref.iterations) <= 3
+    cert = O.certificate(oprob, O.Result(res.status, res.x, res.X, res.Y, res.pobj, res.dobj, res.iterations,
+                                         res.p_res, res.d_res, res.gap_rel), tol, tol, tol)
+    assert cert["valid"], cert
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["auto", "on"])
 def test_mixed_bf16x3_solve_matches_oracle(gpu_lib, mode):
     k, m = 50, 200
