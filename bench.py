@@ -423,11 +423,11 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         variants = (("fp64", "fp64", "auto"), ("mixed", "mixed", "auto"), ("mixed_bf16x3", "mixed_bf16x3", "auto"),
                     ("mixed_on", "mixed", "on"), ("mixed_bf16x3_on", "mixed_bf16x3", "on"),
                     ("mixed_bf16x3_on_fp32_factor", "mixed_bf16x3", "on+f32"),
-                    ("mixed_bf16x3_on_fp32_products", "mixed_bf16x3", "on+p32"))
+                    ("mixed_bf16x3_on_fp64_products", "mixed_bf16x3", "on+p64"))
         for name, prec, mode in variants + variants:  # the first round warms up
             o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision=prec,
                               mixed_mode=mode.split("+")[0], mixed_factor=mode.endswith("+f32"),
-                              mixed_panels32=mode.endswith("+p32"))
+                              mixed_panels32=not mode.endswith("+p64"))
             b.synchronize()
             t0 = time.perf_counter()
             r = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0], backend=b)
@@ -449,8 +449,8 @@ def mixed_leg(k: int = 500, m: int = 6000, tol: float = 1e-8) -> dict:
         out["speedup_mixed_bf16x3_on_vs_fp64"] = round(res["fp64"][1] / res["mixed_bf16x3_on"][1], 3)
         out["speedup_mixed_bf16x3_on_fp32_factor_vs_fp64"] = round(
             res["fp64"][1] / res["mixed_bf16x3_on_fp32_factor"][1], 3)
-        out["speedup_mixed_bf16x3_on_fp32_products_vs_fp64"] = round(
-            res["fp64"][1] / res["mixed_bf16x3_on_fp32_products"][1], 3)
+        out["speedup_mixed_bf16x3_on_fp64_products_vs_fp64"] = round(
+            res["fp64"][1] / res["mixed_bf16x3_on_fp64_products"][1], 3)
     finally:
         b.close()
     return out

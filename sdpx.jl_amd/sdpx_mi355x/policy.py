@@ -72,7 +72,7 @@ class SolverOptions:
     mixed_factor: bool = False
     # form P-hat on FP32 MFMA for the BF16x3 assembly (FP32 copy of the constraint panel, bf16
     # planes written by the product) instead of in FP64 followed by the split
-    mixed_panels32: bool = False
+    mixed_panels32: bool = True
     mixed_cond_limit: float = 1e6
     bf16x3_cond_limit: float = 1e5
     mixed_refine_target: float = 1e-10

@@ -243,7 +243,8 @@ def test_mixed_bf16x3_solve_matches_oracle(gpu_lib, mode):
     g = generators.dense_feasible(k, m, seed=3)
     prob = ingest(g.c, [g.A_blocks()], [g.C])
     tol = 1e-8
-    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3", mixed_mode=mode)
+    o = SolverOptions(eps_gap=tol, eps_primal=tol, eps_dual=tol, precision="mixed_bf16x3", mixed_mode=mode,
+                      mixed_panels32=False)
     res = solve(prob, o, x0=g.x0, X0=[g.X0], Y0=[g.Y0])
     oprob = O.ingest(g.c, [g.A_blocks()], [g.C])
     ref = O.solve(oprob, O.Options(eps_gap=tol, eps_primal=tol, eps_dual=tol), x0=g.x0, X0=[g.X0], Y0=[g.Y0])
