@@ -93,6 +93,33 @@ def test_gpu_reduced_precision_cold_start_gram_is_bit_exact(gpu_lib, k, m, preci
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("k,m", [(64, 300), (128, 700)])
+def test_gpu_bf16x3_fp32_products_cold_start_gram_is_bit_exact(gpu_lib, k, m):
+    """The BF16x3 assembly from the FP32-MFMA P-hat products (block sides divisible by 4): with
+    X = Y = I the products reproduce the integer panel exactly and the Gram stays exact."""
+    A = scheduler_panel(k, m)
+    assert 400.0 * k * k < 2.0 ** 24  # every FP32 partial sum is an exact integer
+    prob = ingest(np.ones(m), [A], [np.zeros((k, k))])
+    b = MI355XDenseBackend(prob)
+    try:
+        I = np.eye(k)
+        b.set_iterate(np.zeros(m), [I], [I])
+        _, _, ok = b.compute_residuals(np.array([0.0]), affine=True, factor=True)
+        assert ok
+        b.set_precision(2)
+        b.set_panel_precision(1)
+        b.schur_build()
+        S = np.tril(b.get_schur())
+    finally:
+        b.close()
+    V = A.reshape(m, k * k)
+    ref = np.tril(V @ V.T)
+    bad = np.argwhere(S != ref)
+    print(f"BF16x3 from FP32 products k={k} m={m}: {len(bad)} entries differ", flush=True)
+    assert len(bad) == 0, bad[:5]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kind", ["dense", "coo"])
 def test_gpu_batched_cold_start_gram_is_bit_exact(gpu_lib, kind, precision):
