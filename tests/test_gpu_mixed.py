@@ -213,6 +213,48 @@ def test_bf16x3_fp32_panels_within_split_accuracy(gpu_lib, k):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("k2nd", [48, 50])
+def test_bf16x3_fp32_panels_two_blocks(gpu_lib, k2nd):
+    """Two dense blocks (64 and 48 or 50) share the bf16 planes: the second block's shorter
+    panels leave no stale rows, and with k = 50 one block takes the FP32 products and the
+    other the FP64 ones within the same Schur matrix."""
+    m = 180
+    blocks = [(64, 5), (k2nd, 6)]
+    gs = [generators.dense_feasible(k, m, seed=sd) for k, sd in blocks]
+    c = sum(g.c for g in gs)
+    A = [g.A_blocks() for g in gs]
+    Cs = [g.C for g in gs]
+    prob = ingest(c, A, Cs)
+    oprob = O.ingest(c, A, Cs)
+    rng = np.random.default_rng(7)
+    Xs, Ys = [], []
+    for k, _ in blocks:
+        Q = rng.standard_normal((k, k))
+        Xs.append(Q @ Q.T / k + 0.5 * np.eye(k))
+        Q = rng.standard_normal((k, k))
+        Ys.append(Q @ Q.T / k + 0.5 * np.eye(k))
+    b = MI355XDenseBackend(prob)
+    try:
+        b.set_iterate(np.zeros(m), Xs, Ys)
+        b.compute_residuals(np.array([0.0, 0.0]), affine=True, factor=True)
+        b.set_precision(2)
+        b.set_panel_precision(0)
+        b.schur_build()
+        S64p = b.get_schur()
+        b.set_panel_precision(1)
+        b.schur_build()
+        S32p = b.get_schur()
+    finally:
+        b.close()
+    LX = [O.kchol(X)[0] for X in Xs]
+    MY = [O.kchol(Y)[0] for Y in Ys]
+    ref = O.schur_build(oprob, LX, MY)
+    e64, e32 = rel_fro_lower(S64p, ref), rel_fro_lower(S32p, ref)
+    print(f"blocks 64 + {k2nd}: BF16x3 Schur rel error, FP64 products {e64:.2e}, FP32 products {e32:.2e}", flush=True)
+    assert e32 <= 2.0 * e64 and e32 <= 1e-4
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["auto", "on"])
 def test_mixed_bf16x3_fp32_panels_solve_matches_oracle(gpu_lib, mode):
     k, m = 48, 200
